@@ -1,0 +1,220 @@
+/*
+ * zq_smoke3d.h -- C ABI of the MI355X (gfx950) 3D smoke step.
+ *
+ * Part 1 replaces, symbol for symbol, the extern "C" entry points the reference's stage
+ * functions call (ZQ_SmokeSimulationUtils3D.cpp:2088,2112,2157,2163,3835): same names, argument
+ * order and meaning, enum values, host pointers, synchronous, elapsed milliseconds returned.
+ * Under C++ they are declared inside the reference's namespaces, so `ZQ_CUDA_Advection3D::
+ * Advect_Velocity(...)` in caller code resolves to the same unmangled symbol.
+ * Part 2 is the red-black SOR pressure solve the reference takes from ZQlibCuda (absent from
+ * the reference tree; signatures inferred from the calls at ZQ_SmokeSimulationUtils3D.cpp:2354-2363,
+ * 2424-2437) plus an extended variant.
+ * Part 3 keeps a whole simulation resident in HBM behind an opaque handle (no reference
+ * counterpart: the reference copies every field host<->device at every stage).
+ *
+ * Errors: the reference aborts the process on a CUDA error (checkCudaErrors).  Here the Part 1/2
+ * functions return a negative value instead and Part 3 returns a nonzero ZQ3D_ERR_*; the message
+ * is available from zq3d_last_error() on the calling thread.
+ */
+#ifndef ZQ_SMOKE3D_H
+#define ZQ_SMOKE3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+/* ====================== Part 1: reference stage entry points ====================== */
+
+#ifdef __cplusplus
+namespace ZQ_CUDA_Advection3D {
+#endif
+/* ZQ_CUDA_Advection3D.h:7-21 (values differ from ZQ_SmokeSimulationPara3D's enums) */
+enum AdvectVelocityType {
+    ADV_VEL_INREG_OUTREG = 0,
+    ADV_VEL_INREG_OUTREG_BFECC,
+    ADV_VEL_INMAC_OUTMAC,
+    ADV_VEL_INMAC_OUTMAC_BFECC,
+    ADV_VEL_INREG_OUTMAC,
+    ADV_VEL_INREG_OUTMAC_BFECC
+};
+enum AdvectScalarType { ADV_SCA_MAC, ADV_SCA_MAC_BFECC, ADV_SCA_REG, ADV_SCA_REG_BFECC };
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* replaces ZQ_CUDA_Advection3D.h:23 -- stores the grid and step constants used by the next calls */
+void ZQ_Cuda_Prepare_Advection(const unsigned int width, const unsigned int height, const unsigned int depth,
+                               const float voxelSize, const unsigned int steps, const float deltatt);
+/* replaces ZQ_CUDA_Advection3D.h:31 -- u,v,w updated in place.  BFECC modes return -1 (not built). */
+float Advect_Velocity(float* u, float* v, float* w, const bool* occupy, bool is_open, enum AdvectVelocityType type);
+/* replaces ZQ_CUDA_Advection3D.h:43-44.  The BFECC modes leave the outputs zero, as the
+ * reference's switch does (ZQ_CUDA_Advection3D.cu:1782-1794). */
+float Advect_Scalar(const float* u, const float* v, const float* w, const bool* occupy,
+                    const float* input_temperature, const float* input_density, float* output_temperature,
+                    float* output_density, enum AdvectScalarType type);
+#ifdef __cplusplus
+}
+}
+namespace ZQ_CUDA_AddForce3D {
+#endif
+/* ZQ_CUDA_AddForce3D.h:6-10 */
+enum AddForceType { ADD_FORCE_ENTIRE, ADD_FORCE_SLICE };
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* replaces ZQ_CUDA_AddForce3D.h:12-14 -- buoyancy beta*(T-Tamb) on y plus vorticity confinement;
+ * both types run the same computation, as in the reference (ZQ_CUDA_AddForce3D.cu:633-641). */
+float AddForce3D(float* mac_u, float* mac_v, float* mac_w, const bool* occupy, const float* temperature,
+                 const float deltah, const float deltat, const float buoyCoeff, const float confineCoeff,
+                 const float Tamb, const int width, const int height, const int depth, enum AddForceType type);
+#ifdef __cplusplus
+}
+}
+namespace ZQ_CUDA_Attenuation3D {
+extern "C" {
+#endif
+/* replaces ZQ_CUDA_Attenuation3D.h:6-8 */
+float Attenuation3D(float* mac_u, float* mac_v, float* mac_w, float* temperature, float* density, const bool* occupy,
+                    const float velAtten, const float tempAtten, const float densityAtten, const int width,
+                    const int height, const int depth);
+#ifdef __cplusplus
+}
+}
+#endif
+
+/* ====================== Part 2: red-black SOR projection ====================== */
+
+#ifdef __cplusplus
+namespace ZQ_CUDA_PoissonSolver3D {
+extern "C" {
+#endif
+/* replaces the ZQlibCuda call at ZQ_SmokeSimulationUtils3D.cpp:2354-2363: p = 0 outside all six
+ * domain faces, maxIter red-black Gauss-Seidel iterations (omega = 1) from p = 0, then u -= grad p. */
+float SolveOpenPoissonRedBlackwithOccupy3D_MAC(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
+                                               const int width, const int height, const int depth, const int maxIter);
+/* replaces the ZQlibCuda call at ZQ_SmokeSimulationUtils3D.cpp:2424-2437: all domain faces are
+ * walls; div_per_volume (the caller's mean divergence over fluid cells) is subtracted from every
+ * fluid cell's divergence so the Neumann problem is compatible. */
+float SolveClosedPoissonRedBlackwithOccupy3D_MAC(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
+                                                 const float div_per_volume, const int width, const int height,
+                                                 const int depth, const int maxIter);
+#ifdef __cplusplus
+}
+}
+extern "C" {
+#endif
+
+/* domain faces across which p = 0 (open); the others are walls */
+enum { ZQ3D_OPEN_XLO = 1, ZQ3D_OPEN_XHI = 2, ZQ3D_OPEN_YLO = 4, ZQ3D_OPEN_YHI = 8, ZQ3D_OPEN_ZLO = 16,
+       ZQ3D_OPEN_ZHI = 32, ZQ3D_OPEN_ALL = 63 };
+
+typedef struct zq3d_sor_opts {
+    int open_mask;        /* ZQ3D_OPEN_* bits */
+    int closed_mean;      /* subtract the mean divergence over fluid cells (FP64 on the device) */
+    float omega;          /* SOR relaxation, 1 = Gauss-Seidel */
+    int iters;            /* plain: iterations; mixed: FP32 sweeps per refinement step */
+    int mixed_precision;  /* 0: FP32 SOR from p = 0.  1: FP64 residual + pressure, FP32 sweeps */
+    int max_outer;        /* mixed: refinement steps at most */
+    double tol;           /* mixed: stop when max|r| <= tol */
+} zq3d_sor_opts;
+
+typedef struct zq3d_proj_stats {
+    int outer;            /* refinement steps run (mixed) */
+    int sweeps;           /* red-black iterations run in total */
+    double max_residual;  /* mixed: final max|r| (FP64) */
+    double max_div;       /* max |div - mean| over counted cells after the correction, FP64 */
+    double mean;          /* subtracted mean divergence */
+} zq3d_proj_stats;
+
+/* No reference counterpart (extends the two calls above): host pointers, omega, per-face open
+ * boundaries and the mixed-precision mode.  stats may be NULL. */
+float ZQ3D_SolvePoissonRedBlack_ex(float* mac_u, float* mac_v, float* mac_w, const bool* occupy, const int width,
+                                   const int height, const int depth, const zq3d_sor_opts* opts,
+                                   zq3d_proj_stats* stats);
+
+/* replaces the reference's MAC->cell conversion (ZQ_CUDA_MACtoRegular.cu:9-30 on the device, CPU
+ * twin MACtoRegular4 at ZQ_SmokeSimulationUtils3D.cpp:355-374): vel4 gets (u,v,w,0) per cell. */
+float ZQ3D_MACtoRegular4(const float* mac_u, const float* mac_v, const float* mac_w, float* vel4, const int width,
+                         const int height, const int depth);
+
+/* ====================== Part 3: device-resident simulation ====================== */
+
+enum {
+    ZQ3D_OK = 0,
+    ZQ3D_ERR_ARG = 1,      /* bad argument (sizes, NULL, mode) */
+    ZQ3D_ERR_HIP = 2,      /* HIP runtime error */
+    ZQ3D_ERR_STATE = 3     /* call not valid in the current state */
+};
+
+/* field ids for upload / download / device pointers */
+enum {
+    ZQ3D_FIELD_U = 0,      /* (W+1)*H*D float */
+    ZQ3D_FIELD_V = 1,      /* W*(H+1)*D float */
+    ZQ3D_FIELD_W = 2,      /* W*H*(D+1) float */
+    ZQ3D_FIELD_T = 3,      /* W*H*D float */
+    ZQ3D_FIELD_RHO = 4,    /* W*H*D float */
+    ZQ3D_FIELD_OCC = 5,    /* W*H*D bytes, nonzero = solid */
+    ZQ3D_FIELD_P64 = 6,    /* (W+2)*(H+2)*(D+2) double, padded pressure of the mixed-precision solve */
+    ZQ3D_FIELD_VEL4 = 7    /* W*H*D float4: last MAC->cell conversion */
+};
+
+typedef struct zq3d_config {
+    int struct_size;       /* sizeof(zq3d_config) */
+    int width, height, depth;
+    float voxel_size;      /* h; the reference uses 1/width (ZQ_SmokeSimulationUtils3D.cpp:323) */
+    float step_time;       /* dt (stepTime) */
+    int steps;             /* advection substeps (steps) */
+    float Tamb;            /* ambient temperature */
+    float buoyancy;        /* beta (buoyancyCoeff) */
+    float gravity;         /* alpha: f_y -= alpha*rho (gravityCoeff, unused by the reference) */
+    float confine;         /* eps (confineCoeff) */
+    float vel_atten, temp_atten, dens_atten;   /* per-step factors exp(-dt*coeff) */
+    int adv_vel_type;      /* enum AdvectVelocityType value */
+    int adv_sca_type;      /* enum AdvectScalarType value */
+    zq3d_sor_opts sor;     /* projection */
+    int warm_start;        /* mixed: keep the pressure between steps */
+} zq3d_config;
+
+typedef struct zq3d_sim zq3d_sim;
+
+/* fills defaults equal to ZQ_SmokeSimulationPara3D's constructor (ZQ_SmokeSimulationUtils3D.cpp:18-72)
+ * mapped to the TOTAL_OPEN_POISSON_SOR solver */
+int zq3d_config_default(zq3d_config* cfg, int width, int height, int depth);
+int zq3d_create(const zq3d_config* cfg, zq3d_sim** out);
+void zq3d_destroy(zq3d_sim* sim);
+/* stream is a hipStream_t (NULL = the default stream) */
+int zq3d_set_stream(zq3d_sim* sim, void* stream);
+int zq3d_synchronize(zq3d_sim* sim);
+int zq3d_upload(zq3d_sim* sim, int field, const void* host, size_t bytes);
+int zq3d_download(zq3d_sim* sim, int field, void* host, size_t bytes);
+int zq3d_device_ptr(zq3d_sim* sim, int field, void** dev, size_t* bytes);
+/* box sources: cx,cy,cz,half_x,half_y,half_z (normalized), rho, T per source (ZQ_SmokeSource3D) */
+int zq3d_set_box_sources(zq3d_sim* sim, int count, const float* boxes8);
+/* sphere sources: cx,cy,cz (normalized), radius (units of width), rho, T */
+int zq3d_set_sphere_sources(zq3d_sim* sim, int count, const float* spheres6);
+/* solid sphere obstacle: occupancy = shell_mask walls + sphere, faces of solid voxels = velocity */
+int zq3d_stamp_sphere_obstacle(zq3d_sim* sim, float cx, float cy, float cz, float radius, float su, float sv,
+                               float sw, int shell_mask);
+
+/* stages, in the order of ZQ_SmokeSimulation3D::UpdateOneFrame (ZQ_SmokeSimulation3D.cpp:156-243).
+ * extra_force: NULL or a device pointer to 3 floats per cell added to the cell force. */
+int zq3d_add_force_attenuate(zq3d_sim* sim, const float* extra_force_dev);
+int zq3d_advect_velocity(zq3d_sim* sim);
+int zq3d_project(zq3d_sim* sim, zq3d_proj_stats* stats);
+int zq3d_advect_scalar(zq3d_sim* sim);
+int zq3d_reinject(zq3d_sim* sim);
+/* all stages of one step */
+int zq3d_step(zq3d_sim* sim, const float* extra_force_dev, zq3d_proj_stats* stats);
+/* MAC -> cell float4 into ZQ3D_FIELD_VEL4 (the reference's output conversion) */
+int zq3d_mac_to_regular(zq3d_sim* sim);
+
+/* message of the last failed call on this thread ("" if none) */
+const char* zq3d_last_error(void);
+/* frees the device workspace cached by the Part 1/2 host-pointer functions */
+void zq3d_release_workspace(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""CPU: the C-ABI library loads and exports every function include/zq_smoke3d.h declares (no GPU
+calls), and the oracle library loads."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set()
+    for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+        name = m.group(1)
+        prev = text[:m.start()].rstrip()
+        if name in ("if", "sizeof", "defined") or prev.endswith("#define"):
+            continue
+        # a declaration: preceded by a return type token
+        if re.search(r"(float|void|int|char\s*\*|const char\s*\*)\s*\**\s*$", prev):
+            names.add(name)
+    return sorted(names)
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_functions(os.path.join(ROOT, "include", "zq_smoke3d.h"))
+    for ref in ("ZQ_Cuda_Prepare_Advection", "Advect_Velocity", "Advect_Scalar", "AddForce3D", "Attenuation3D",
+                "SolveOpenPoissonRedBlackwithOccupy3D_MAC", "SolveClosedPoissonRedBlackwithOccupy3D_MAC"):
+        assert ref in names
+
+
+def test_library_exports_every_declared_symbol():
+    import zq_smokesimulation_amd._lib as L
+    lib = C.CDLL(L.LIB_PATH)
+    names = declared_functions(os.path.join(ROOT, "include", "zq_smoke3d.h"))
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_config_struct_size_matches_library():
+    """zq3d_config_default runs on the host only; it must agree with the ctypes layout."""
+    import zq_smokesimulation_amd._lib as L
+    cfg = L.Config()
+    assert L.lib().zq3d_config_default(C.byref(cfg), 64, 32, 16) == 0
+    assert cfg.struct_size == C.sizeof(L.Config)
+    assert (cfg.width, cfg.height, cfg.depth) == (64, 32, 16)
+    assert cfg.steps == 20 and cfg.Tamb == 10.0 and cfg.sor.open_mask == 63
+    assert abs(cfg.voxel_size - 1.0 / 64) < 1e-9
+
+
+def test_bad_arguments_fail_without_gpu():
+    """Argument validation happens before any device call."""
+    import zq_smokesimulation_amd._lib as L
+    cfg = L.Config()
+    L.lib().zq3d_config_default(C.byref(cfg), 1, 8, 8)
+    h = C.c_void_p()
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    assert "dimensions" in L.last_error()
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.adv_vel_type = L.ADV_VEL_INREG_OUTREG_BFECC
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.sor.omega = 2.5
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    assert L.lib().Attenuation3D(None, None, None, None, None, None, 1.0, 1.0, 1.0, 8, 8, 8) < 0
+
+
+def test_oracle_loads():
+    import zq_oracle
+    assert zq_oracle.lib() is not None

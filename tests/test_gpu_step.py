@@ -1,0 +1,111 @@
+"""GPU parity of whole steps of the device-resident simulation against the oracle's step.
+
+Bit-exact where no reduction is involved (open domains); with a closed domain the mean
+divergence is an FP64 sum whose order differs on the device, so fields are compared to a stated
+tolerance.  The mixed-precision runs also check the post-projection divergence bound.
+"""
+import numpy as np
+import pytest
+
+import zq_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_params(W, dt=0.02, **kw):
+    p = dict(h=1.0 / W, dt=dt, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=1.0, aT=1.0, aR=1.0,
+             adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1, omega=1.8, sor_iters=200, mixed=0,
+             max_outer=0, tol=1e-6)
+    p.update(kw)
+    return p
+
+
+def make_sim(W, H, D, p, sphere_sources):
+    import zq_smokesimulation_amd as z
+    sim = z.SmokeSim3D(W, H, D, voxel_size=p["h"], step_time=p["dt"], steps=p["steps"], Tamb=p["Tamb"],
+                       buoyancy=p["beta"], gravity=p["alpha"], confine=p["eps"], vel_atten=p["aV"],
+                       temp_atten=p["aT"], dens_atten=p["aR"], adv_vel_type=p["adv_vel_type"],
+                       adv_sca_type=p["adv_sca_type"], sor_open_mask=p["open_mask"],
+                       sor_closed_mean=p["closed_mean"], sor_omega=p["omega"], sor_iters=p["sor_iters"],
+                       sor_mixed_precision=p["mixed"], sor_max_outer=p["max_outer"], sor_tol=p["tol"],
+                       warm_start=1)
+    sim.set_sphere_sources(sphere_sources)
+    return sim
+
+
+def initial_state(W, H, D, shell):
+    occ, u, v, w = O.init_fine_grid(W, H, D, shell, 0)
+    T = O.noise(W, H, D, 7, 0.01)
+    R = np.zeros_like(T)
+    return occ, u, v, w, T, R
+
+
+def run_both(W, H, D, p, nsteps, shell, sphere_sources):
+    occ, u, v, w, T, R = initial_state(W, H, D, shell)
+    sim = make_sim(W, H, D, p, sphere_sources)
+    sim.set_state(u=u, v=v, w=w, T=T, rho=R, occ=occ)
+    p64 = np.zeros((D + 2, H + 2, W + 2)) if p["mixed"] else None
+    stats_o, stats_g = [], []
+    for s in range(nsteps):
+        print("step", s, flush=True)
+        stats_o.append(O.step(p, u, v, w, T, R, occ, p64=p64, spheres=sphere_sources))
+        stats_g.append(sim.step())
+    g = sim.get_state()
+    sim.close()
+    return (u, v, w, T, R), g, stats_o, stats_g
+
+
+SRC = [[0.5, 0.2, 0.5, 0.12, 1.0, 1.0]]
+
+
+def test_step_open_bit_exact():
+    """Open domain (p = 0 on all faces), omega 1.8: no reductions, bit-exact for 6 steps."""
+    W, H, D = 24, 28, 20
+    p = oracle_params(W, open_mask=O.ALL_FACES, closed_mean=0, sor_iters=60)
+    ref, g, _, _ = run_both(W, H, D, p, 6, 0, SRC)
+    for name, a in zip(("u", "v", "w", "T", "rho"), ref):
+        b = g[name]
+        bad = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32))
+        assert bad.size == 0, "%s: %d cells differ (max |d| %g)" % (name, bad.size, np.abs(a - b).max())
+
+
+def test_step_closed_config0_like():
+    """Closed shell, alpha/beta buoyancy, confinement, 200 SOR iterations (config 0 at 32^3)."""
+    W = H = D = 32
+    p = oracle_params(W)
+    ref, g, _, stats_g = run_both(W, H, D, p, 5, O.ALL_FACES, [[0.5, 0.1, 0.5, 0.08, 1.0, 1.0]])
+    for name, a in zip(("u", "v", "w", "T", "rho"), ref):
+        b = g[name]
+        scale = max(float(np.abs(a).max()), 1e-6)
+        err = float(np.abs(a - b).max())
+        assert err <= 1e-5 * scale, "%s: max err %g (scale %g)" % (name, err, scale)
+    assert np.isfinite(stats_g[-1]["max_div"])
+
+
+def test_step_mixed_precision_moving_sphere():
+    """Open top, walls elsewhere, moving solid sphere, mixed-precision SOR to max|div| <= 1e-6."""
+    W = H = D = 32
+    shell = O.ALL_FACES & ~O.OPEN_YHI
+    p = oracle_params(W, open_mask=O.OPEN_YHI, closed_mean=0, mixed=1, sor_iters=30, max_outer=200, tol=5e-7)
+    occ, u, v, w, T, R = initial_state(W, H, D, shell)
+    sim = make_sim(W, H, D, p, SRC)
+    sim.set_state(u=u, v=v, w=w, T=T, rho=R, occ=occ)
+    p64 = np.zeros((D + 2, H + 2, W + 2))
+    vel = (0.3, 0.0, 0.1)
+    for s in range(4):
+        print("mp step", s, flush=True)
+        c = (0.3 + 0.02 * s, 0.5, 0.4 + 0.01 * s)
+        O.stamp_sphere(occ, u, v, w, c[0], c[1], c[2], 0.1, vel, shell)
+        sim.stamp_sphere_obstacle(c[0], c[1], c[2], 0.1, vel, shell)
+        st_o = O.step(p, u, v, w, T, R, occ, p64=p64, spheres=SRC)
+        st_g = sim.step()
+        assert st_g["max_residual"] <= 5e-7
+        assert st_g["max_div"] <= 1e-6, st_g
+        assert st_g["outer"] == int(st_o[0]) // 30
+    g = sim.get_state()
+    assert np.array_equal(g["occ"], occ)
+    for name, a in zip(("u", "v", "w", "T", "rho"), (u, v, w, T, R)):
+        b = g[name]
+        scale = max(float(np.abs(a).max()), 1e-6)
+        err = float(np.abs(a - b).max())
+        assert err <= 1e-5 * scale, "%s: max err %g (scale %g)" % (name, err, scale)

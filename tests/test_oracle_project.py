@@ -1,0 +1,159 @@
+"""Pins the oracle's pressure projection.  The reference's solver (ZQlibCuda) is absent, so the
+iteration-level SOR is this project's definition; what the mathematics fixes is the converged
+result: the orthogonal projection of the free faces onto fields with zero divergence on every
+counted cell.  That projection is built here with dense linear algebra from the definition of
+the divergence alone (no stencil code) and compared with the oracle's CG and SOR."""
+import numpy as np
+import pytest
+
+import zq_oracle as O
+from util import faces
+
+
+def free_faces(occ, open_mask):
+    """Faces the projection may change: both cells fluid, or an open domain face of a fluid cell."""
+    D, H, W = occ.shape
+    fl = occ == 0
+    fu = np.zeros((D, H, W + 1), bool)
+    fu[:, :, 1:W] = fl[:, :, 1:] & fl[:, :, :-1]
+    fu[:, :, 0] = fl[:, :, 0] & bool(open_mask & O.OPEN_XLO)
+    fu[:, :, W] = fl[:, :, -1] & bool(open_mask & O.OPEN_XHI)
+    fv = np.zeros((D, H + 1, W), bool)
+    fv[:, 1:H] = fl[:, 1:] & fl[:, :-1]
+    fv[:, 0] = fl[:, 0] & bool(open_mask & O.OPEN_YLO)
+    fv[:, H] = fl[:, -1] & bool(open_mask & O.OPEN_YHI)
+    fw = np.zeros((D + 1, H, W), bool)
+    fw[1:D] = fl[1:] & fl[:-1]
+    fw[0] = fl[0] & bool(open_mask & O.OPEN_ZLO)
+    fw[D] = fl[-1] & bool(open_mask & O.OPEN_ZHI)
+    return fu, fv, fw
+
+
+def divergence_matrix(occ):
+    """Sparse-as-dense operator: cells x (all u, v, w faces); div = out - in along each axis."""
+    D, H, W = occ.shape
+    nu, nv, nw = D * H * (W + 1), D * (H + 1) * W, (D + 1) * H * W
+    M = np.zeros((D * H * W, nu + nv + nw))
+    for k in range(D):
+        for j in range(H):
+            for i in range(W):
+                c = (k * H + j) * W + i
+                M[c, (k * H + j) * (W + 1) + i + 1] += 1
+                M[c, (k * H + j) * (W + 1) + i] -= 1
+                M[c, nu + (k * (H + 1) + j + 1) * W + i] += 1
+                M[c, nu + (k * (H + 1) + j) * W + i] -= 1
+                M[c, nu + nv + ((k + 1) * H + j) * W + i] += 1
+                M[c, nu + nv + (k * H + j) * W + i] -= 1
+    return M
+
+
+def dense_projection(u, v, w, occ, open_mask):
+    free = np.concatenate([f.ravel() for f in free_faces(occ, open_mask)])
+    x = np.concatenate([u.ravel(), v.ravel(), w.ravel()]).astype(np.float64)
+    M = divergence_matrix(occ)
+    counted = O.sor_count(occ, open_mask).ravel() > 0
+    Dm = M[counted]
+    Df = Dm[:, free]
+    lam = np.linalg.lstsq(Df @ Df.T, Dm @ x, rcond=None)[0]
+    y = x.copy()
+    y[free] -= Df.T @ lam
+    nu, nv = u.size, v.size
+    return y[:nu].reshape(u.shape), y[nu:nu + nv].reshape(v.shape), y[nu + nv:].reshape(w.shape)
+
+
+def _case(rng, shape, frac, shell):
+    W, H, D = shape
+    u, v, w = faces(rng, W, H, D)
+    occ = (rng.random((D, H, W)) < frac).astype(np.uint8)
+    if shell:
+        occ[0] = occ[-1] = 1
+        occ[:, 0] = occ[:, -1] = 1
+        occ[:, :, 0] = occ[:, :, -1] = 1
+    return u, v, w, occ
+
+
+def _zero_fixed_faces(u, v, w, occ, open_mask):
+    """closed domains: zero the faces the projection cannot move so sum(div) = 0 (compatibility)"""
+    for a, f in zip((u, v, w), free_faces(occ, open_mask)):
+        a[~f] = 0
+
+
+@pytest.mark.parametrize("open_mask,shell,frac", [(63, False, 0.15), (O.OPEN_YHI, False, 0.1),
+                                                  (0, True, 0.1), (O.OPEN_XLO | O.OPEN_ZHI, False, 0.2)])
+def test_cg_equals_dense_orthogonal_projection(rng, open_mask, shell, frac):
+    u, v, w, occ = _case(rng, (5, 4, 6), frac, shell)
+    if open_mask == 0:
+        _zero_fixed_faces(u, v, w, occ, open_mask)
+    ref = dense_projection(u, v, w, occ, open_mask)
+    it, _ = O.project_cg64(u, v, w, occ, open_mask, subtract_mean=(open_mask == 0), tol=1e-13)
+    for a, b in zip((u, v, w), ref):
+        np.testing.assert_allclose(a, b, rtol=0, atol=2e-6)
+
+
+def test_sor_converges_to_cg(rng):
+    u, v, w, occ = _case(rng, (8, 7, 9), 0.1, False)
+    for omega in (1.0, 1.8):
+        a = [x.copy() for x in (u, v, w)]
+        b = [x.copy() for x in (u, v, w)]
+        O.project(*a, occ, O.OPEN_YHI | O.OPEN_XLO, 0.0, omega, 3000)
+        O.project_cg64(*b, occ, O.OPEN_YHI | O.OPEN_XLO, tol=1e-13)
+        for x, y in zip(a, b):
+            np.testing.assert_allclose(x, y, rtol=0, atol=5e-5)
+
+
+def test_projection_idempotent_and_symmetric(rng):
+    W, H, D = 6, 5, 7
+    occ = (rng.random((D, H, W)) < 0.15).astype(np.uint8)
+    mask = O.OPEN_YHI | O.OPEN_ZLO
+    free = free_faces(occ, mask)
+
+    def P(x):
+        y = [a.copy() for a in x]
+        O.project_cg64(*y, occ, mask, tol=1e-13)
+        return y
+
+    def rand():
+        x = list(faces(rng, W, H, D))
+        for a, f in zip(x, free):
+            a[~f] = 0
+        return x
+
+    a, b = rand(), rand()
+    pa, pb = P(a), P(b)
+    ppa = P(pa)
+    for x, y in zip(pa, ppa):
+        np.testing.assert_allclose(x, y, rtol=0, atol=1e-5)
+    dot = lambda x, y: sum(float((p.astype(np.float64) * q).sum()) for p, q in zip(x, y))
+    assert abs(dot(pa, b) - dot(a, pb)) <= 1e-5 * max(1.0, abs(dot(pa, b)))
+
+
+def test_neighbour_counts_by_hand():
+    occ = np.zeros((3, 3, 3), np.uint8)
+    n_open = O.sor_count(occ, O.ALL_FACES)
+    n_closed = O.sor_count(occ, 0)
+    assert n_open[1, 1, 1] == 6 and n_open[0, 0, 0] == 6 and n_closed[0, 0, 0] == 3
+    assert n_closed[1, 1, 1] == 6 and n_closed[0, 1, 1] == 5
+    occ[1, 1, 2] = 1
+    n = O.sor_count(occ, 0)
+    assert n[1, 1, 2] == 0 and n[1, 1, 1] == 5
+    assert O.sor_count(occ, O.OPEN_YHI)[1, 2, 1] == 5 + 1 - 0  # top cell: 4 fluid + open top
+
+
+def test_mixed_precision_reaches_tolerance(rng):
+    u, v, w, occ = _case(rng, (12, 10, 14), 0.1, True)
+    outer, st, p64 = O.project_mp(u, v, w, occ, 0, True, 1.8, 25, 500, 1e-9)
+    assert outer > 1
+    assert st[1] <= 1e-9
+    vmax = max(np.abs(a).max() for a in (u, v, w))
+    assert st[2] <= 1e-9 + 3 * float(np.spacing(np.float32(vmax)))
+
+
+def test_closed_mean_compatibility(rng):
+    """With the mean subtracted, the projected field's divergence equals the mean on every cell."""
+    u, v, w, occ = _case(rng, (8, 8, 8), 0.05, True)
+    m = O.mean_div32_f64(u, v, w, occ)
+    a = [x.copy() for x in (u, v, w)]
+    O.project_cg64(*a, occ, 0, subtract_mean=True, tol=1e-13)
+    div = O.divergence(*a, occ)
+    fl = occ == 0
+    np.testing.assert_allclose(div[fl], m, rtol=0, atol=2e-5)

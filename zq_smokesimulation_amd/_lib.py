@@ -1,0 +1,147 @@
+"""ctypes binding of libzq_smoke3d.so (C ABI declared in include/zq_smoke3d.h).
+
+The library is the only compute path: if it is missing or was built for another
+machine, importing this module raises.  Nothing here falls back to a CPU
+implementation.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libzq_smoke3d.so")
+
+# enum AdvectVelocityType / AdvectScalarType (ZQ_CUDA_Advection3D.h:7-21)
+ADV_VEL_INREG_OUTREG, ADV_VEL_INREG_OUTREG_BFECC, ADV_VEL_INMAC_OUTMAC, \
+    ADV_VEL_INMAC_OUTMAC_BFECC, ADV_VEL_INREG_OUTMAC, ADV_VEL_INREG_OUTMAC_BFECC = range(6)
+ADV_SCA_MAC, ADV_SCA_MAC_BFECC, ADV_SCA_REG, ADV_SCA_REG_BFECC = range(4)
+ADD_FORCE_ENTIRE, ADD_FORCE_SLICE = range(2)
+
+OPEN_XLO, OPEN_XHI, OPEN_YLO, OPEN_YHI, OPEN_ZLO, OPEN_ZHI, OPEN_ALL = 1, 2, 4, 8, 16, 32, 63
+
+FIELD_U, FIELD_V, FIELD_W, FIELD_T, FIELD_RHO, FIELD_OCC, FIELD_P64, FIELD_VEL4 = range(8)
+OK, ERR_ARG, ERR_HIP, ERR_STATE = range(4)
+
+
+class SorOpts(C.Structure):
+    _fields_ = [
+        ("open_mask", C.c_int),
+        ("closed_mean", C.c_int),
+        ("omega", C.c_float),
+        ("iters", C.c_int),
+        ("mixed_precision", C.c_int),
+        ("max_outer", C.c_int),
+        ("tol", C.c_double),
+    ]
+
+
+class ProjStats(C.Structure):
+    _fields_ = [
+        ("outer", C.c_int),
+        ("sweeps", C.c_int),
+        ("max_residual", C.c_double),
+        ("max_div", C.c_double),
+        ("mean", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_int),
+        ("width", C.c_int), ("height", C.c_int), ("depth", C.c_int),
+        ("voxel_size", C.c_float),
+        ("step_time", C.c_float),
+        ("steps", C.c_int),
+        ("Tamb", C.c_float),
+        ("buoyancy", C.c_float),
+        ("gravity", C.c_float),
+        ("confine", C.c_float),
+        ("vel_atten", C.c_float), ("temp_atten", C.c_float), ("dens_atten", C.c_float),
+        ("adv_vel_type", C.c_int),
+        ("adv_sca_type", C.c_int),
+        ("sor", SorOpts),
+        ("warm_start", C.c_int),
+    ]
+
+
+class ZQ3DError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libzq_smoke3d.so not found at %s: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+            % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    f, i, u, vp, b = C.c_float, C.c_int, C.c_uint, C.c_void_p, C.c_bool
+    L.ZQ_Cuda_Prepare_Advection.argtypes = [u, u, u, f, u, f]
+    L.ZQ_Cuda_Prepare_Advection.restype = None
+    L.Advect_Velocity.argtypes = [vp, vp, vp, vp, b, i]
+    L.Advect_Velocity.restype = f
+    L.Advect_Scalar.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i]
+    L.Advect_Scalar.restype = f
+    L.AddForce3D.argtypes = [vp, vp, vp, vp, vp, f, f, f, f, f, i, i, i, i]
+    L.AddForce3D.restype = f
+    L.Attenuation3D.argtypes = [vp, vp, vp, vp, vp, vp, f, f, f, i, i, i]
+    L.Attenuation3D.restype = f
+    L.SolveOpenPoissonRedBlackwithOccupy3D_MAC.argtypes = [vp, vp, vp, vp, i, i, i, i]
+    L.SolveOpenPoissonRedBlackwithOccupy3D_MAC.restype = f
+    L.SolveClosedPoissonRedBlackwithOccupy3D_MAC.argtypes = [vp, vp, vp, vp, f, i, i, i, i]
+    L.SolveClosedPoissonRedBlackwithOccupy3D_MAC.restype = f
+    L.ZQ3D_SolvePoissonRedBlack_ex.argtypes = [vp, vp, vp, vp, i, i, i, C.POINTER(SorOpts), C.POINTER(ProjStats)]
+    L.ZQ3D_SolvePoissonRedBlack_ex.restype = f
+    L.ZQ3D_MACtoRegular4.argtypes = [vp, vp, vp, vp, i, i, i]
+    L.ZQ3D_MACtoRegular4.restype = f
+    L.zq3d_config_default.argtypes = [C.POINTER(Config), i, i, i]
+    L.zq3d_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.zq3d_destroy.argtypes = [vp]
+    L.zq3d_destroy.restype = None
+    L.zq3d_set_stream.argtypes = [vp, vp]
+    L.zq3d_synchronize.argtypes = [vp]
+    L.zq3d_upload.argtypes = [vp, i, vp, C.c_size_t]
+    L.zq3d_download.argtypes = [vp, i, vp, C.c_size_t]
+    L.zq3d_device_ptr.argtypes = [vp, i, C.POINTER(vp), C.POINTER(C.c_size_t)]
+    L.zq3d_set_box_sources.argtypes = [vp, i, vp]
+    L.zq3d_set_sphere_sources.argtypes = [vp, i, vp]
+    L.zq3d_stamp_sphere_obstacle.argtypes = [vp, f, f, f, f, f, f, f, i]
+    L.zq3d_add_force_attenuate.argtypes = [vp, vp]
+    L.zq3d_advect_velocity.argtypes = [vp]
+    L.zq3d_project.argtypes = [vp, C.POINTER(ProjStats)]
+    L.zq3d_advect_scalar.argtypes = [vp]
+    L.zq3d_reinject.argtypes = [vp]
+    L.zq3d_step.argtypes = [vp, vp, C.POINTER(ProjStats)]
+    L.zq3d_mac_to_regular.argtypes = [vp]
+    L.zq3d_last_error.argtypes = []
+    L.zq3d_last_error.restype = C.c_char_p
+    L.zq3d_release_workspace.argtypes = []
+    L.zq3d_release_workspace.restype = None
+    return L
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def last_error():
+    return lib().zq3d_last_error().decode()
+
+
+def check(rc, what=""):
+    """Raise ZQ3DError for a nonzero status (Part 3) or a negative time (Part 1/2)."""
+    if isinstance(rc, float):
+        if rc < 0:
+            raise ZQ3DError("%s failed: %s" % (what, last_error()))
+        return rc
+    if rc != OK:
+        raise ZQ3DError("%s failed (code %d): %s" % (what, rc, last_error()))
+    return rc

@@ -1,0 +1,217 @@
+// kernels_advect.hip -- semi-Lagrangian advection of velocity, temperature and density (gfx950).
+//
+// Reference: ZQ_CUDA_Advection3D.cu binds every field to a CUDA texture (a cudaArray copy per call)
+// and samples with the hardware filter.  Here fields stay in place in HBM and the kernels do the
+// trilinear filter in FP32 (zq3d_device.h).  The cell-centred velocity is a float4 grid so each
+// of the 8 taps of a sample is one 16-byte load; the backtrace's taps are neighbours of each
+// other and of the neighbouring lanes', so they are served by L1/L2, and HBM sees roughly one
+// read of the float4 grid per kernel.
+#include "zq3d_device.h"
+#include "zq3d_kernels.h"
+
+namespace zq3d {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
+
+// MAC_to_Regular4_Kernel (ZQ_CUDA_MACtoRegular.cu:9-30)
+__global__ __launch_bounds__(BX* BY) void k_mac_to_cell4(const float* __restrict__ u, const float* __restrict__ v,
+                                                         const float* __restrict__ w, float4* __restrict__ vel4,
+                                                         Dims d) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H) return;
+    float4 c;
+    c.x = 0.5f * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
+    c.y = 0.5f * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
+    c.z = 0.5f * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
+    c.w = 0.0f;
+    vel4[cidx(d, i, j, k)] = c;
+}
+
+// Advect_Velocity_inRegular_outRegular_Kernel (ZQ_CUDA_Advection3D.cu:76-113): backtrace from the
+// cell centre through the cell grid and sample the same grid at the end point.  Output SoA
+// (adv[0..N) x, [N..2N) y, [2N..3N) z) so each face pass below reads one component plane.
+__global__ __launch_bounds__(BX* BY) void k_advect_cells(const float4* __restrict__ vel4,
+                                                         const uint8_t* __restrict__ occ, float* __restrict__ adv,
+                                                         Dims d, float h, int steps, float dtt) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H) return;
+    const RegularVel V{vel4, d};
+    const float3 last = backtrace(V, occ, d, h, steps, dtt, i + 0.5f, j + 0.5f, k + 0.5f);
+    const float3 s = V(last.x, last.y, last.z);
+    const int o = cidx(d, i, j, k);
+    const int N = d.W * d.H * d.D;
+    adv[o] = s.x;
+    adv[N + o] = s.y;
+    adv[2 * N + o] = s.z;
+}
+
+// Regular4_to_MAC_{u,v,w}_Kernel (ZQ_CUDA_MACtoRegular.cu:32-99) fused with
+// Apply_Advect_Velocity_Result_{Open,Closed}_* (ZQ_CUDA_Advection3D.cu:471-650): the face value is
+// the average of the two advected cells (a domain face copies its inside cell) and is stored only
+// where the face's cells are fluid.
+template <int AXIS>
+__global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const float* __restrict__ adv,
+                                                                 const uint8_t* __restrict__ occ,
+                                                                 float* __restrict__ face, Dims d, int is_open) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
+    if (i >= nx || j >= ny) return;
+    const int pos = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+    const int lim = AXIS == 0 ? d.W : (AXIS == 1 ? d.H : d.D);
+    const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
+    const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
+                        AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
+    const float* a = adv + (size_t)AXIS * d.W * d.H * d.D;
+    bool ok;
+    float val;
+    if (pos == 0 || pos == lim) {
+        ok = is_open && !occ[ci];
+        val = a[ci];
+    } else {
+        ok = !occ[ci] && !occ[ci - step];
+        val = 0.5f * (a[ci] + a[ci - step]);
+    }
+    if (ok) {
+        const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
+        face[fi] = val;
+    }
+}
+
+// Advect_Velocity_inRegular_outMAC_* (:115-230) and inMAC_outMAC_* (:232-377): backtrace from the
+// face centre; the face's own component sampled at the end point goes to a scratch face grid.
+template <int AXIS, class Vel>
+__global__ __launch_bounds__(BX* BY) void k_advect_faces(Vel V, const uint8_t* __restrict__ occ,
+                                                         float* __restrict__ out, Dims d, float h, int steps,
+                                                         float dtt) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
+    if (i >= nx || j >= ny) return;
+    const float px = AXIS == 0 ? (float)i : i + 0.5f;
+    const float py = AXIS == 1 ? (float)j : j + 0.5f;
+    const float pz = AXIS == 2 ? (float)k : k + 0.5f;
+    const float3 last = backtrace(V, occ, d, h, steps, dtt, px, py, pz);
+    const float3 s = V(last.x, last.y, last.z);
+    const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
+    out[fi] = AXIS == 0 ? s.x : (AXIS == 1 ? s.y : s.z);
+}
+
+// Masked copy scratch -> face (Apply_Advect_Velocity_Result_*, :471-650)
+template <int AXIS>
+__global__ __launch_bounds__(BX* BY) void k_apply_faces(const float* __restrict__ src,
+                                                        const uint8_t* __restrict__ occ, float* __restrict__ face,
+                                                        Dims d, int is_open) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
+    if (i >= nx || j >= ny) return;
+    const int pos = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+    const int lim = AXIS == 0 ? d.W : (AXIS == 1 ? d.H : d.D);
+    const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
+    const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
+                        AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
+    const bool ok = (pos == 0 || pos == lim) ? (is_open && !occ[ci]) : (!occ[ci] && !occ[ci - step]);
+    if (ok) {
+        const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
+        face[fi] = src[fi];
+    }
+}
+
+// Advect_Scalar_{Regular,MAC}_Velocity_Kernel (:379-469): backtrace from the cell centre, sample T
+// and rho (linear, clamp) at the end point.  Every cell is written, solid ones included.
+template <class Vel>
+__global__ __launch_bounds__(BX* BY) void k_advect_scalar(Vel V, const uint8_t* __restrict__ occ,
+                                                          const float* __restrict__ Tin,
+                                                          const float* __restrict__ Rin, float* __restrict__ Tout,
+                                                          float* __restrict__ Rout, Dims d, float h, int steps,
+                                                          float dtt) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H) return;
+    const float3 last = backtrace(V, occ, d, h, steps, dtt, i + 0.5f, j + 0.5f, k + 0.5f);
+    const float x = last.x - 0.5f, y = last.y - 0.5f, z = last.z - 0.5f;
+    const int o = cidx(d, i, j, k);
+    Tout[o] = sample3(Tin, d.W, d.H, d.D, x, y, z);
+    Rout[o] = sample3(Rin, d.W, d.H, d.D, x, y, z);
+}
+
+}  // namespace
+
+void launch_mac_to_cell4(const Dims& d, const float* u, const float* v, const float* w, float4* vel4,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_mac_to_cell4, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, vel4, d);
+}
+
+void launch_advect_velocity(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, int is_open, int mode,
+                            const AdvParams& ap, float4* vel4, float* scratch, hipStream_t s) {
+    const dim3 blk(BX, BY);
+    const size_t Nu = (size_t)(d.W + 1) * d.H * d.D, Nv = (size_t)d.W * (d.H + 1) * d.D;
+    float* ou = scratch;
+    float* ov = scratch + Nu;
+    float* ow = scratch + Nu + Nv;
+    if (mode == 0) {
+        launch_mac_to_cell4(d, u, v, w, vel4, s);
+        float* adv = scratch;  // 3N floats <= Nu+Nv+Nw
+        hipLaunchKernelGGL(k_advect_cells, grid_for(d.W, d.H, d.D), blk, 0, s, vel4, occ, adv, d, ap.h, ap.steps,
+                           ap.dtt);
+        hipLaunchKernelGGL(k_cells_to_faces_apply<0>, grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
+                           is_open);
+        hipLaunchKernelGGL(k_cells_to_faces_apply<1>, grid_for(d.W, d.H + 1, d.D), blk, 0, s, adv, occ, v, d,
+                           is_open);
+        hipLaunchKernelGGL(k_cells_to_faces_apply<2>, grid_for(d.W, d.H, d.D + 1), blk, 0, s, adv, occ, w, d,
+                           is_open);
+        return;
+    }
+    if (mode == 4) {
+        launch_mac_to_cell4(d, u, v, w, vel4, s);
+        const RegularVel V{vel4, d};
+        hipLaunchKernelGGL((k_advect_faces<0, RegularVel>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou, d,
+                           ap.h, ap.steps, ap.dtt);
+        hipLaunchKernelGGL((k_advect_faces<1, RegularVel>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov, d,
+                           ap.h, ap.steps, ap.dtt);
+        hipLaunchKernelGGL((k_advect_faces<2, RegularVel>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow, d,
+                           ap.h, ap.steps, ap.dtt);
+    } else {  // mode 2: inMAC_outMAC
+        const MacVel V{u, v, w, d};
+        hipLaunchKernelGGL((k_advect_faces<0, MacVel>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou, d, ap.h,
+                           ap.steps, ap.dtt);
+        hipLaunchKernelGGL((k_advect_faces<1, MacVel>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov, d, ap.h,
+                           ap.steps, ap.dtt);
+        hipLaunchKernelGGL((k_advect_faces<2, MacVel>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow, d, ap.h,
+                           ap.steps, ap.dtt);
+    }
+    hipLaunchKernelGGL(k_apply_faces<0>, grid_for(d.W + 1, d.H, d.D), blk, 0, s, ou, occ, u, d, is_open);
+    hipLaunchKernelGGL(k_apply_faces<1>, grid_for(d.W, d.H + 1, d.D), blk, 0, s, ov, occ, v, d, is_open);
+    hipLaunchKernelGGL(k_apply_faces<2>, grid_for(d.W, d.H, d.D + 1), blk, 0, s, ow, occ, w, d, is_open);
+}
+
+void launch_advect_scalar(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
+                          const float* Tin, const float* Rin, float* Tout, float* Rout, int mac, const AdvParams& ap,
+                          float4* vel4, hipStream_t s) {
+    const dim3 blk(BX, BY);
+    if (mac) {
+        const MacVel V{u, v, w, d};
+        hipLaunchKernelGGL(k_advect_scalar<MacVel>, grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout, Rout,
+                           d, ap.h, ap.steps, ap.dtt);
+    } else {
+        launch_mac_to_cell4(d, u, v, w, vel4, s);
+        const RegularVel V{vel4, d};
+        hipLaunchKernelGGL(k_advect_scalar<RegularVel>, grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout,
+                           Rout, d, ap.h, ap.steps, ap.dtt);
+    }
+}
+
+}  // namespace zq3d

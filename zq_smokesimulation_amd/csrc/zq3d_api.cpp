@@ -1,0 +1,845 @@
+// zq3d_api.cpp -- C ABI of the gfx950 smoke step (include/zq_smoke3d.h).
+//
+// Part 1/2 keep the reference's calling convention (host pointers, synchronous, elapsed ms) but
+// reuse a grow-only device workspace instead of the reference's cudaMalloc/cudaFree per call
+// (e.g. ZQ_CUDA_AddForce3D.cu:616-651).  Part 3 keeps all fields resident in HBM.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "zq3d_device.h"
+#include "zq3d_kernels.h"
+#include "zq_smoke3d.h"
+
+using namespace zq3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* where, const char* what) {
+    g_err = std::string(where) + ": " + what;
+}
+
+#define ZQ_HIP(call, where)                                                   \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            set_err(where, hipGetErrorString(e_));                            \
+            return ZQ3D_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+constexpr long long kMaxCells = 1LL << 30;
+
+int check_dims(int W, int H, int D, const char* where) {
+    if (W < 2 || H < 2 || D < 2) {
+        set_err(where, "grid dimensions must be >= 2");
+        return ZQ3D_ERR_ARG;
+    }
+    const long long pad = (long long)(W + 2) * (H + 2) * (D + 2);
+    if (pad >= kMaxCells || (long long)(W + 1) * (H + 1) * (D + 1) >= kMaxCells) {
+        set_err(where, "grid too large for 32-bit indexing");
+        return ZQ3D_ERR_ARG;
+    }
+    return ZQ3D_OK;
+}
+
+struct Sizes {
+    size_t N, Nu, Nv, Nw, Np;
+    explicit Sizes(const Dims& d)
+        : N((size_t)d.W * d.H * d.D),
+          Nu((size_t)(d.W + 1) * d.H * d.D),
+          Nv((size_t)d.W * (d.H + 1) * d.D),
+          Nw((size_t)d.W * d.H * (d.D + 1)),
+          Np((size_t)(d.W + 2) * (d.H + 2) * (d.D + 2)) {}
+};
+
+// Small per-call scalars in one device block.
+struct Scalars {
+    double sum_cnt[2];
+    double mean64;
+    double zero64;
+    unsigned long long rmax;
+    unsigned long long dmax;
+    float mean32;
+    float zero32;
+};
+
+// ---------------- grow-only workspace for the host-pointer entry points ----------------
+enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_B64, S_P64, S_SCAL, S_COUNT };
+
+struct Workspace {
+    void* ptr[S_COUNT] = {};
+    size_t cap[S_COUNT] = {};
+    std::mutex mu;
+    hipError_t ensure(int slot, size_t bytes) {
+        if (cap[slot] >= bytes) return hipSuccess;
+        if (ptr[slot]) (void)hipFree(ptr[slot]);
+        ptr[slot] = nullptr;
+        cap[slot] = 0;
+        hipError_t e = hipMalloc(&ptr[slot], bytes);
+        if (e == hipSuccess) cap[slot] = bytes;
+        return e;
+    }
+    void release() {
+        for (int i = 0; i < S_COUNT; i++) {
+            if (ptr[i]) (void)hipFree(ptr[i]);
+            ptr[i] = nullptr;
+            cap[i] = 0;
+        }
+    }
+};
+
+Workspace& ws() {
+    static Workspace w;
+    return w;
+}
+
+// ZQ_Cuda_Prepare_Advection state (the reference keeps it in module globals + __constant__,
+// ZQ_CUDA_Advection3D.cu:22-34,1652-1668)
+struct AdvState {
+    unsigned W = 0, H = 0, D = 0, steps = 0;
+    float h = 0.0f, dtt = 0.0f;
+};
+AdvState g_adv;
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer() {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, 0);
+    }
+    float stop() {
+        (void)hipEventRecord(b, 0);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
+    ~Timer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+// Collects the first HIP error of a sequence of calls.
+struct Errs {
+    hipError_t e = hipSuccess;
+    void operator()(hipError_t r) {
+        if (e == hipSuccess) e = r;
+    }
+};
+
+// Plain FP32 projection on device pointers: p = 0, setup, SOR, correct.  mean_mode: 0 = value
+// `mean` given, 1 = FP64 device mean over fluid cells.
+int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                  float mean, int mean_mode, uint8_t* nc, float* b, float* p, Scalars* sc, zq3d_proj_stats* st,
+                  hipStream_t s) {
+    const Sizes z(d);
+    launch_fill_f32(p, z.Np, 0.0f, s);
+    launch_fill_f64(sc->sum_cnt, 2, 0.0, s);
+    launch_proj_setup(d, u, v, w, occ, o.open_mask, nc, b, nullptr, p, nullptr, sc->sum_cnt, s);
+    if (mean_mode) {
+        launch_mean_finalize(sc->sum_cnt, &sc->mean32, &sc->mean64, 1, s);
+    } else {
+        launch_fill_f32(&sc->mean32, 1, mean, s);
+        launch_fill_f64(&sc->mean64, 1, (double)mean, s);
+    }
+    launch_sor(d, nc, b, &sc->mean32, p, o.omega, o.iters, s);
+    launch_correct(d, occ, o.open_mask, p, u, v, w, s);
+    if (st) {
+        launch_fill_u64(&sc->dmax, 1, 0ull, s);
+        launch_max_div64(d, u, v, w, nc, &sc->mean64, &sc->dmax, s);
+        unsigned long long bits = 0;
+        double mean64 = 0.0;
+        ZQ_HIP(hipMemcpyAsync(&bits, &sc->dmax, sizeof(bits), hipMemcpyDeviceToHost, s), "project");
+        ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project");
+        ZQ_HIP(hipStreamSynchronize(s), "project");
+        double md;
+        std::memcpy(&md, &bits, sizeof(md));
+        st->outer = 0;
+        st->sweeps = o.iters;
+        st->max_residual = -1.0;
+        st->max_div = md;
+        st->mean = mean64;
+    }
+    return ZQ3D_OK;
+}
+
+// Mixed-precision projection: FP64 b, p and residual; FP32 inner SOR on the correction e.
+// p64 holds the warm start (zeroed by the caller for a cold start).
+int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                  uint8_t* nc, double* b64, double* p64, float* r32, float* e, Scalars* sc, zq3d_proj_stats* st,
+                  hipStream_t s) {
+    const Sizes z(d);
+    launch_fill_f64(sc->sum_cnt, 2, 0.0, s);
+    launch_fill_f32(&sc->zero32, 1, 0.0f, s);
+    launch_proj_setup(d, u, v, w, occ, o.open_mask, nc, nullptr, b64, nullptr, p64, sc->sum_cnt, s);
+    launch_mean_finalize(sc->sum_cnt, nullptr, &sc->mean64, o.closed_mean, s);
+    launch_fill_f32(e, z.Np, 0.0f, s);
+    int outer = 0;
+    double rmax = 0.0;
+    for (;;) {
+        launch_fill_u64(&sc->rmax, 1, 0ull, s);
+        launch_residual64(d, nc, b64, &sc->mean64, p64, r32, &sc->rmax, s);
+        unsigned long long bits = 0;
+        ZQ_HIP(hipMemcpyAsync(&bits, &sc->rmax, sizeof(bits), hipMemcpyDeviceToHost, s), "project_mixed");
+        ZQ_HIP(hipStreamSynchronize(s), "project_mixed");
+        std::memcpy(&rmax, &bits, sizeof(rmax));
+        if (!(rmax > o.tol) || outer >= o.max_outer) break;
+        launch_sor(d, nc, r32, &sc->zero32, e, o.omega, o.iters, s);
+        launch_update_p64(d, nc, e, p64, s);
+        outer++;
+    }
+    launch_correct64(d, occ, o.open_mask, p64, u, v, w, s);
+    launch_fill_u64(&sc->dmax, 1, 0ull, s);
+    launch_max_div64(d, u, v, w, nc, &sc->mean64, &sc->dmax, s);
+    unsigned long long bits = 0;
+    double mean64 = 0.0;
+    ZQ_HIP(hipMemcpyAsync(&bits, &sc->dmax, sizeof(bits), hipMemcpyDeviceToHost, s), "project_mixed");
+    ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project_mixed");
+    ZQ_HIP(hipStreamSynchronize(s), "project_mixed");
+    if (st) {
+        double md;
+        std::memcpy(&md, &bits, sizeof(md));
+        st->outer = outer;
+        st->sweeps = outer * o.iters;
+        st->max_residual = rmax;
+        st->max_div = md;
+        st->mean = mean64;
+    }
+    return ZQ3D_OK;
+}
+
+int check_sor_opts(const zq3d_sor_opts& o, const char* where) {
+    if (o.iters < 0 || o.max_outer < 0 || !(o.omega > 0.0f && o.omega < 2.0f) || (o.open_mask & ~63)) {
+        set_err(where, "invalid SOR options (iters >= 0, 0 < omega < 2, open_mask in [0,63])");
+        return ZQ3D_ERR_ARG;
+    }
+    return ZQ3D_OK;
+}
+
+}  // namespace
+
+// =============================== Part 1 ===============================
+
+extern "C" void ZQ_Cuda_Prepare_Advection(const unsigned int width, const unsigned int height,
+                                          const unsigned int depth, const float voxelSize, const unsigned int steps,
+                                          const float deltatt) {
+    g_adv.W = width;
+    g_adv.H = height;
+    g_adv.D = depth;
+    g_adv.h = voxelSize;
+    g_adv.steps = steps;
+    g_adv.dtt = deltatt;
+}
+
+namespace {
+int adv_ready(const char* where) {
+    if (int rc = check_dims((int)g_adv.W, (int)g_adv.H, (int)g_adv.D, where)) return rc;
+    if (g_adv.steps < 1) {
+        set_err(where, "steps must be >= 1 (call ZQ_Cuda_Prepare_Advection first)");
+        return ZQ3D_ERR_ARG;
+    }
+    return ZQ3D_OK;
+}
+}  // namespace
+
+extern "C" float Advect_Velocity(float* u, float* v, float* w, const bool* occupy, bool is_open,
+                                 enum ZQ_CUDA_Advection3D::AdvectVelocityType type) {
+    const char* where = "Advect_Velocity";
+    if (adv_ready(where)) return -1.0f;
+    if (!u || !v || !w || !occupy) { set_err(where, "null pointer"); return -1.0f; }
+    const int mode = (int)type;
+    if (mode != 0 && mode != 2 && mode != 4) {
+        set_err(where, "BFECC advection modes are not built");
+        return -1.0f;
+    }
+    const Dims d{(int)g_adv.W, (int)g_adv.H, (int)g_adv.D};
+    const Sizes z(d);
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
+        W.ensure(S_VEL4, z.N * 16) || W.ensure(S_SCR, (z.Nu + z.Nv + z.Nw) * 4)) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    float* du = (float*)W.ptr[S_U];
+    float* dv = (float*)W.ptr[S_V];
+    float* dw = (float*)W.ptr[S_W];
+    uint8_t* docc = (uint8_t*)W.ptr[S_OCC];
+    E(hipMemcpy(du, u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(dv, v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(dw, w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(docc, occupy, z.N, hipMemcpyHostToDevice));
+    const AdvParams ap{g_adv.h, g_adv.dtt, (int)g_adv.steps};
+    launch_advect_velocity(d, du, dv, dw, docc, is_open ? 1 : 0, mode, ap, (float4*)W.ptr[S_VEL4],
+                           (float*)W.ptr[S_SCR], 0);
+    E(hipMemcpy(u, du, z.Nu * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(v, dv, z.Nv * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(w, dw, z.Nw * 4, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const hipError_t e = E.e;
+    const float ms = t.stop();
+    if (e != hipSuccess) { set_err(where, hipGetErrorString(e)); return -1.0f; }
+    return ms;
+}
+
+extern "C" float Advect_Scalar(const float* u, const float* v, const float* w, const bool* occupy,
+                               const float* input_temperature, const float* input_density, float* output_temperature,
+                               float* output_density, enum ZQ_CUDA_Advection3D::AdvectScalarType type) {
+    const char* where = "Advect_Scalar";
+    if (adv_ready(where)) return -1.0f;
+    if (!u || !v || !w || !occupy || !input_temperature || !input_density || !output_temperature || !output_density) {
+        set_err(where, "null pointer");
+        return -1.0f;
+    }
+    const int mode = (int)type;
+    if (mode < 0 || mode > 3) { set_err(where, "unknown scalar advection mode"); return -1.0f; }
+    const Dims d{(int)g_adv.W, (int)g_adv.H, (int)g_adv.D};
+    const Sizes z(d);
+    if (mode == 1 || mode == 3) {
+        // The reference's switch has no BFECC case and returns its zero-initialised outputs.
+        Timer t;
+        std::memset(output_temperature, 0, z.N * 4);
+        std::memset(output_density, 0, z.N * 4);
+        return t.stop();
+    }
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
+        W.ensure(S_T, z.N * 4) || W.ensure(S_R, z.N * 4) || W.ensure(S_T2, z.N * 4) || W.ensure(S_R2, z.N * 4) ||
+        W.ensure(S_VEL4, z.N * 16)) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    E(hipMemcpy(W.ptr[S_U], u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_V], v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_W], w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_OCC], occupy, z.N, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_T], input_temperature, z.N * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_R], input_density, z.N * 4, hipMemcpyHostToDevice));
+    const AdvParams ap{g_adv.h, g_adv.dtt, (int)g_adv.steps};
+    launch_advect_scalar(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (uint8_t*)W.ptr[S_OCC],
+                         (float*)W.ptr[S_T], (float*)W.ptr[S_R], (float*)W.ptr[S_T2], (float*)W.ptr[S_R2],
+                         mode == 0 ? 1 : 0, ap, (float4*)W.ptr[S_VEL4], 0);
+    E(hipMemcpy(output_temperature, W.ptr[S_T2], z.N * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(output_density, W.ptr[S_R2], z.N * 4, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const hipError_t e = E.e;
+    const float ms = t.stop();
+    if (e != hipSuccess) { set_err(where, hipGetErrorString(e)); return -1.0f; }
+    return ms;
+}
+
+extern "C" float AddForce3D(float* mac_u, float* mac_v, float* mac_w, const bool* occupy, const float* temperature,
+                            const float deltah, const float deltat, const float buoyCoeff, const float confineCoeff,
+                            const float Tamb, const int width, const int height, const int depth,
+                            enum ZQ_CUDA_AddForce3D::AddForceType type) {
+    const char* where = "AddForce3D";
+    (void)type;
+    if (check_dims(width, height, depth, where)) return -1.0f;
+    if (!mac_u || !mac_v || !mac_w || !occupy || !temperature) { set_err(where, "null pointer"); return -1.0f; }
+    const Dims d{width, height, depth};
+    const Sizes z(d);
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
+        W.ensure(S_T, z.N * 4) || W.ensure(S_VEL4, z.N * 16) || W.ensure(S_SCR, (z.Nu + z.Nv + z.Nw) * 4)) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    E(hipMemcpy(W.ptr[S_U], mac_u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_V], mac_v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_W], mac_w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_OCC], occupy, z.N, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_T], temperature, z.N * 4, hipMemcpyHostToDevice));
+    const ForceParams fp{deltah, deltat, buoyCoeff, 0.0f, confineCoeff, Tamb};
+    launch_force_atten(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T], nullptr,
+                       (uint8_t*)W.ptr[S_OCC], nullptr, fp, 0, 1.0f, 1.0f, 1.0f, (float4*)W.ptr[S_VEL4],
+                       (float*)W.ptr[S_SCR], 0);
+    E(hipMemcpy(mac_u, W.ptr[S_U], z.Nu * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_v, W.ptr[S_V], z.Nv * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_w, W.ptr[S_W], z.Nw * 4, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const hipError_t e = E.e;
+    const float ms = t.stop();
+    if (e != hipSuccess) { set_err(where, hipGetErrorString(e)); return -1.0f; }
+    return ms;
+}
+
+extern "C" float Attenuation3D(float* mac_u, float* mac_v, float* mac_w, float* temperature, float* density,
+                               const bool* occupy, const float velAtten, const float tempAtten,
+                               const float densityAtten, const int width, const int height, const int depth) {
+    const char* where = "Attenuation3D";
+    if (check_dims(width, height, depth, where)) return -1.0f;
+    if (!mac_u || !mac_v || !mac_w || !occupy || !temperature || !density) {
+        set_err(where, "null pointer");
+        return -1.0f;
+    }
+    const Dims d{width, height, depth};
+    const Sizes z(d);
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
+        W.ensure(S_T, z.N * 4) || W.ensure(S_R, z.N * 4)) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    E(hipMemcpy(W.ptr[S_U], mac_u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_V], mac_v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_W], mac_w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_OCC], occupy, z.N, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_T], temperature, z.N * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_R], density, z.N * 4, hipMemcpyHostToDevice));
+    launch_attenuation(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T],
+                       (float*)W.ptr[S_R], (uint8_t*)W.ptr[S_OCC], velAtten, tempAtten, densityAtten, 0);
+    E(hipMemcpy(mac_u, W.ptr[S_U], z.Nu * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_v, W.ptr[S_V], z.Nv * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_w, W.ptr[S_W], z.Nw * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(temperature, W.ptr[S_T], z.N * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(density, W.ptr[S_R], z.N * 4, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const hipError_t e = E.e;
+    const float ms = t.stop();
+    if (e != hipSuccess) { set_err(where, hipGetErrorString(e)); return -1.0f; }
+    return ms;
+}
+
+// =============================== Part 2 ===============================
+
+namespace {
+
+float solve_host(float* u, float* v, float* w, const bool* occupy, int W_, int H_, int D_, const zq3d_sor_opts& o,
+                 float given_mean, zq3d_proj_stats* st, const char* where) {
+    if (check_dims(W_, H_, D_, where)) return -1.0f;
+    if (check_sor_opts(o, where)) return -1.0f;
+    if (!u || !v || !w || !occupy) { set_err(where, "null pointer"); return -1.0f; }
+    const Dims d{W_, H_, D_};
+    const Sizes z(d);
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
+        W.ensure(S_NC, z.N) || W.ensure(S_B, z.N * 4) || W.ensure(S_P, z.Np * 4) || W.ensure(S_SCAL, sizeof(Scalars)) ||
+        (o.mixed_precision && (W.ensure(S_B64, z.N * 8) || W.ensure(S_P64, z.Np * 8)))) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    float* du = (float*)W.ptr[S_U];
+    float* dv = (float*)W.ptr[S_V];
+    float* dw = (float*)W.ptr[S_W];
+    uint8_t* docc = (uint8_t*)W.ptr[S_OCC];
+    E(hipMemcpy(du, u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(dv, v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(dw, w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(docc, occupy, z.N, hipMemcpyHostToDevice));
+    Scalars* sc = (Scalars*)W.ptr[S_SCAL];
+    int rc;
+    if (o.mixed_precision) {
+        launch_fill_f64((double*)W.ptr[S_P64], z.Np, 0.0, 0);
+        rc = project_mixed(d, du, dv, dw, docc, o, (uint8_t*)W.ptr[S_NC], (double*)W.ptr[S_B64],
+                           (double*)W.ptr[S_P64], (float*)W.ptr[S_B], (float*)W.ptr[S_P], sc, st, 0);
+    } else {
+        rc = project_plain(d, du, dv, dw, docc, o, given_mean, o.closed_mean ? 1 : 0, (uint8_t*)W.ptr[S_NC],
+                           (float*)W.ptr[S_B], (float*)W.ptr[S_P], sc, st, 0);
+    }
+    if (rc) return -1.0f;
+    E(hipMemcpy(u, du, z.Nu * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(v, dv, z.Nv * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(w, dw, z.Nw * 4, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const hipError_t e = E.e;
+    const float ms = t.stop();
+    if (e != hipSuccess) { set_err(where, hipGetErrorString(e)); return -1.0f; }
+    return ms;
+}
+
+}  // namespace
+
+extern "C" float SolveOpenPoissonRedBlackwithOccupy3D_MAC(float* mac_u, float* mac_v, float* mac_w,
+                                                          const bool* occupy, const int width, const int height,
+                                                          const int depth, const int maxIter) {
+    zq3d_sor_opts o{};
+    o.open_mask = ZQ3D_OPEN_ALL;
+    o.closed_mean = 0;
+    o.omega = 1.0f;
+    o.iters = maxIter;
+    return solve_host(mac_u, mac_v, mac_w, occupy, width, height, depth, o, 0.0f, nullptr,
+                      "SolveOpenPoissonRedBlackwithOccupy3D_MAC");
+}
+
+extern "C" float SolveClosedPoissonRedBlackwithOccupy3D_MAC(float* mac_u, float* mac_v, float* mac_w,
+                                                            const bool* occupy, const float div_per_volume,
+                                                            const int width, const int height, const int depth,
+                                                            const int maxIter) {
+    zq3d_sor_opts o{};
+    o.open_mask = 0;
+    o.closed_mean = 0;  // the caller's div_per_volume is used as given
+    o.omega = 1.0f;
+    o.iters = maxIter;
+    return solve_host(mac_u, mac_v, mac_w, occupy, width, height, depth, o, div_per_volume, nullptr,
+                      "SolveClosedPoissonRedBlackwithOccupy3D_MAC");
+}
+
+extern "C" float ZQ3D_SolvePoissonRedBlack_ex(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
+                                              const int width, const int height, const int depth,
+                                              const zq3d_sor_opts* opts, zq3d_proj_stats* stats) {
+    if (!opts) { set_err("ZQ3D_SolvePoissonRedBlack_ex", "null options"); return -1.0f; }
+    return solve_host(mac_u, mac_v, mac_w, occupy, width, height, depth, *opts, 0.0f, stats,
+                      "ZQ3D_SolvePoissonRedBlack_ex");
+}
+
+extern "C" float ZQ3D_MACtoRegular4(const float* mac_u, const float* mac_v, const float* mac_w, float* vel4,
+                                    const int width, const int height, const int depth) {
+    const char* where = "ZQ3D_MACtoRegular4";
+    if (check_dims(width, height, depth, where)) return -1.0f;
+    if (!mac_u || !mac_v || !mac_w || !vel4) { set_err(where, "null pointer"); return -1.0f; }
+    const Dims d{width, height, depth};
+    const Sizes z(d);
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_VEL4, z.N * 16)) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    E(hipMemcpy(W.ptr[S_U], mac_u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_V], mac_v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_W], mac_w, z.Nw * 4, hipMemcpyHostToDevice));
+    launch_mac_to_cell4(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float4*)W.ptr[S_VEL4], 0);
+    E(hipMemcpy(vel4, W.ptr[S_VEL4], z.N * 16, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const hipError_t e = E.e;
+    const float ms = t.stop();
+    if (e != hipSuccess) { set_err(where, hipGetErrorString(e)); return -1.0f; }
+    return ms;
+}
+
+// =============================== Part 3 ===============================
+
+struct zq3d_sim {
+    zq3d_config cfg;
+    Dims d;
+    hipStream_t stream = nullptr;
+    float *u = nullptr, *v = nullptr, *w = nullptr;
+    float *T = nullptr, *R = nullptr, *T2 = nullptr, *R2 = nullptr;
+    uint8_t* occ = nullptr;
+    float4* vel4 = nullptr;       // also the vorticity scratch of the force stage
+    float* scratch = nullptr;     // Nu+Nv+Nw floats: face scratch, cell force, advected cells
+    uint8_t* nc = nullptr;
+    float* b = nullptr;           // FP32 divergence / FP32 residual of the mixed solve
+    float* p = nullptr;           // padded FP32 pressure / correction e of the mixed solve
+    double* b64 = nullptr;
+    double* p64 = nullptr;
+    Scalars* sc = nullptr;
+    std::vector<float> boxes, spheres;
+    bool p64_valid = false;
+};
+
+namespace {
+
+template <class T>
+hipError_t dalloc(T** p, size_t bytes) {
+    return hipMalloc((void**)p, bytes);
+}
+
+int field_info(zq3d_sim* s, int field, void** ptr, size_t* bytes) {
+    const Sizes z(s->d);
+    switch (field) {
+    case ZQ3D_FIELD_U: *ptr = s->u; *bytes = z.Nu * 4; return 0;
+    case ZQ3D_FIELD_V: *ptr = s->v; *bytes = z.Nv * 4; return 0;
+    case ZQ3D_FIELD_W: *ptr = s->w; *bytes = z.Nw * 4; return 0;
+    case ZQ3D_FIELD_T: *ptr = s->T; *bytes = z.N * 4; return 0;
+    case ZQ3D_FIELD_RHO: *ptr = s->R; *bytes = z.N * 4; return 0;
+    case ZQ3D_FIELD_OCC: *ptr = s->occ; *bytes = z.N; return 0;
+    case ZQ3D_FIELD_P64: *ptr = s->p64; *bytes = z.Np * 8; return 0;
+    case ZQ3D_FIELD_VEL4: *ptr = s->vel4; *bytes = z.N * 16; return 0;
+    default: return -1;
+    }
+}
+
+int check_config(const zq3d_config& c, const char* where) {
+    if (c.struct_size != (int)sizeof(zq3d_config)) {
+        set_err(where, "zq3d_config.struct_size mismatch (header/library version)");
+        return ZQ3D_ERR_ARG;
+    }
+    if (int rc = check_dims(c.width, c.height, c.depth, where)) return rc;
+    if (c.steps < 1) { set_err(where, "steps must be >= 1"); return ZQ3D_ERR_ARG; }
+    if (!(c.voxel_size > 0.0f) || !(c.step_time >= 0.0f)) {
+        set_err(where, "voxel_size must be > 0 and step_time >= 0");
+        return ZQ3D_ERR_ARG;
+    }
+    if (c.adv_vel_type != 0 && c.adv_vel_type != 2 && c.adv_vel_type != 4) {
+        set_err(where, "adv_vel_type must be ADV_VEL_INREG_OUTREG, ADV_VEL_INMAC_OUTMAC or ADV_VEL_INREG_OUTMAC");
+        return ZQ3D_ERR_ARG;
+    }
+    if (c.adv_sca_type < 0 || c.adv_sca_type > 3) { set_err(where, "unknown adv_sca_type"); return ZQ3D_ERR_ARG; }
+    return check_sor_opts(c.sor, where);
+}
+
+}  // namespace
+
+extern "C" int zq3d_config_default(zq3d_config* c, int width, int height, int depth) {
+    if (!c) { set_err("zq3d_config_default", "null config"); return ZQ3D_ERR_ARG; }
+    std::memset(c, 0, sizeof(*c));
+    c->struct_size = (int)sizeof(zq3d_config);
+    c->width = width;
+    c->height = height;
+    c->depth = depth;
+    c->voxel_size = (float)(1.0 / width);
+    c->step_time = 0.01f;
+    c->steps = 20;
+    c->Tamb = 10.0f;
+    c->buoyancy = 1.0f;
+    c->gravity = 0.0f;
+    c->confine = 0.0f;
+    c->vel_atten = c->temp_atten = c->dens_atten = 1.0f;
+    c->adv_vel_type = 0;  // ADV_VEL_INREG_OUTREG
+    c->adv_sca_type = 2;  // ADV_SCA_REG
+    c->sor.open_mask = ZQ3D_OPEN_ALL;
+    c->sor.closed_mean = 0;
+    c->sor.omega = 1.0f;
+    c->sor.iters = 100;
+    c->sor.mixed_precision = 0;
+    c->sor.max_outer = 0;
+    c->sor.tol = 1e-6;
+    c->warm_start = 1;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
+    const char* where = "zq3d_create";
+    if (!cfg || !out) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    *out = nullptr;
+    if (int rc = check_config(*cfg, where)) return rc;
+    zq3d_sim* s = new zq3d_sim();
+    s->cfg = *cfg;
+    s->d = Dims{cfg->width, cfg->height, cfg->depth};
+    const Sizes z(s->d);
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    A(dalloc(&s->u, z.Nu * 4));
+    A(dalloc(&s->v, z.Nv * 4));
+    A(dalloc(&s->w, z.Nw * 4));
+    A(dalloc(&s->T, z.N * 4));
+    A(dalloc(&s->R, z.N * 4));
+    A(dalloc(&s->T2, z.N * 4));
+    A(dalloc(&s->R2, z.N * 4));
+    A(dalloc(&s->occ, z.N));
+    A(dalloc(&s->vel4, z.N * 16));
+    A(dalloc(&s->scratch, (z.Nu + z.Nv + z.Nw) * 4));
+    A(dalloc(&s->nc, z.N));
+    A(dalloc(&s->b, z.N * 4));
+    A(dalloc(&s->p, z.Np * 4));
+    A(dalloc(&s->sc, sizeof(Scalars)));
+    if (cfg->sor.mixed_precision) {
+        A(dalloc(&s->b64, z.N * 8));
+        A(dalloc(&s->p64, z.Np * 8));
+    }
+    if (e != hipSuccess) {
+        set_err(where, hipGetErrorString(e));
+        zq3d_destroy(s);
+        return ZQ3D_ERR_HIP;
+    }
+    hipStream_t st = nullptr;
+    launch_fill_f32(s->u, z.Nu, 0.0f, st);
+    launch_fill_f32(s->v, z.Nv, 0.0f, st);
+    launch_fill_f32(s->w, z.Nw, 0.0f, st);
+    launch_fill_f32(s->T, z.N, 0.0f, st);
+    launch_fill_f32(s->R, z.N, 0.0f, st);
+    launch_fill_u8(s->occ, z.N, 0, st);
+    launch_fill_f32(s->p, z.Np, 0.0f, st);
+    if (s->p64) launch_fill_f64(s->p64, z.Np, 0.0, st);
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        set_err(where, hipGetErrorString(e));
+        zq3d_destroy(s);
+        return ZQ3D_ERR_HIP;
+    }
+    *out = s;
+    return ZQ3D_OK;
+}
+
+extern "C" void zq3d_destroy(zq3d_sim* s) {
+    if (!s) return;
+    void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->nc, s->b, s->p,
+                    s->b64, s->p64, s->sc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+extern "C" int zq3d_set_stream(zq3d_sim* s, void* stream) {
+    if (!s) { set_err("zq3d_set_stream", "null sim"); return ZQ3D_ERR_ARG; }
+    s->stream = (hipStream_t)stream;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_synchronize(zq3d_sim* s) {
+    if (!s) { set_err("zq3d_synchronize", "null sim"); return ZQ3D_ERR_ARG; }
+    ZQ_HIP(hipStreamSynchronize(s->stream), "zq3d_synchronize");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_upload(zq3d_sim* s, int field, const void* host, size_t bytes) {
+    const char* where = "zq3d_upload";
+    if (!s || !host) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    void* p;
+    size_t n;
+    if (field_info(s, field, &p, &n) || !p) { set_err(where, "unknown or unallocated field"); return ZQ3D_ERR_ARG; }
+    if (bytes != n) { set_err(where, "size mismatch"); return ZQ3D_ERR_ARG; }
+    ZQ_HIP(hipMemcpyAsync(p, host, n, hipMemcpyHostToDevice, s->stream), where);
+    ZQ_HIP(hipStreamSynchronize(s->stream), where);
+    if (field == ZQ3D_FIELD_P64) s->p64_valid = true;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_download(zq3d_sim* s, int field, void* host, size_t bytes) {
+    const char* where = "zq3d_download";
+    if (!s || !host) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    void* p;
+    size_t n;
+    if (field_info(s, field, &p, &n) || !p) { set_err(where, "unknown or unallocated field"); return ZQ3D_ERR_ARG; }
+    if (bytes != n) { set_err(where, "size mismatch"); return ZQ3D_ERR_ARG; }
+    ZQ_HIP(hipMemcpyAsync(host, p, n, hipMemcpyDeviceToHost, s->stream), where);
+    ZQ_HIP(hipStreamSynchronize(s->stream), where);
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_device_ptr(zq3d_sim* s, int field, void** dev, size_t* bytes) {
+    const char* where = "zq3d_device_ptr";
+    if (!s || !dev) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    size_t n;
+    if (field_info(s, field, dev, &n) || !*dev) { set_err(where, "unknown or unallocated field"); return ZQ3D_ERR_ARG; }
+    if (bytes) *bytes = n;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_set_box_sources(zq3d_sim* s, int count, const float* boxes8) {
+    if (!s || count < 0 || (count && !boxes8)) { set_err("zq3d_set_box_sources", "bad argument"); return ZQ3D_ERR_ARG; }
+    s->boxes.assign(boxes8, boxes8 + 8 * count);
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_set_sphere_sources(zq3d_sim* s, int count, const float* sph6) {
+    if (!s || count < 0 || (count && !sph6)) { set_err("zq3d_set_sphere_sources", "bad argument"); return ZQ3D_ERR_ARG; }
+    s->spheres.assign(sph6, sph6 + 6 * count);
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_stamp_sphere_obstacle(zq3d_sim* s, float cx, float cy, float cz, float radius, float su, float sv,
+                                          float sw, int shell_mask) {
+    if (!s || (shell_mask & ~63)) { set_err("zq3d_stamp_sphere_obstacle", "bad argument"); return ZQ3D_ERR_ARG; }
+    launch_stamp_sphere(s->d, cx, cy, cz, radius, su, sv, sw, shell_mask, s->occ, s->u, s->v, s->w, s->stream);
+    ZQ_HIP(hipGetLastError(), "zq3d_stamp_sphere_obstacle");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_add_force_attenuate(zq3d_sim* s, const float* extra) {
+    if (!s) { set_err("zq3d_add_force_attenuate", "null sim"); return ZQ3D_ERR_ARG; }
+    const zq3d_config& c = s->cfg;
+    const ForceParams fp{c.voxel_size, c.step_time, c.buoyancy, c.gravity, c.confine, c.Tamb};
+    launch_force_atten(s->d, s->u, s->v, s->w, s->T, s->R, s->occ, extra, fp, 1, c.vel_atten, c.temp_atten,
+                       c.dens_atten, s->vel4, s->scratch, s->stream);
+    ZQ_HIP(hipGetLastError(), "zq3d_add_force_attenuate");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_advect_velocity(zq3d_sim* s) {
+    if (!s) { set_err("zq3d_advect_velocity", "null sim"); return ZQ3D_ERR_ARG; }
+    const zq3d_config& c = s->cfg;
+    const AdvParams ap{c.voxel_size, c.step_time / c.steps, c.steps};
+    // the reference's stage always advects with is_open = true (ZQ_SmokeSimulationUtils3D.cpp:2163)
+    launch_advect_velocity(s->d, s->u, s->v, s->w, s->occ, 1, c.adv_vel_type, ap, s->vel4, s->scratch, s->stream);
+    ZQ_HIP(hipGetLastError(), "zq3d_advect_velocity");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
+    if (!s) { set_err("zq3d_project", "null sim"); return ZQ3D_ERR_ARG; }
+    const zq3d_config& c = s->cfg;
+    int rc;
+    if (c.sor.mixed_precision) {
+        if (!c.warm_start || !s->p64_valid) launch_fill_f64(s->p64, Sizes(s->d).Np, 0.0, s->stream);
+        rc = project_mixed(s->d, s->u, s->v, s->w, s->occ, c.sor, s->nc, s->b64, s->p64, s->b, s->p, s->sc, st,
+                           s->stream);
+        s->p64_valid = (rc == ZQ3D_OK);
+    } else {
+        rc = project_plain(s->d, s->u, s->v, s->w, s->occ, c.sor, 0.0f, c.sor.closed_mean ? 1 : 0, s->nc, s->b, s->p,
+                           s->sc, st, s->stream);
+    }
+    if (rc) return rc;
+    ZQ_HIP(hipGetLastError(), "zq3d_project");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_advect_scalar(zq3d_sim* s) {
+    if (!s) { set_err("zq3d_advect_scalar", "null sim"); return ZQ3D_ERR_ARG; }
+    const zq3d_config& c = s->cfg;
+    const Sizes z(s->d);
+    if (c.adv_sca_type == 1 || c.adv_sca_type == 3) {
+        // reference behaviour: the BFECC scalar modes return zero fields (ZQ_CUDA_Advection3D.cu:1782-1794)
+        launch_fill_f32(s->T, z.N, 0.0f, s->stream);
+        launch_fill_f32(s->R, z.N, 0.0f, s->stream);
+    } else {
+        const AdvParams ap{c.voxel_size, c.step_time / c.steps, c.steps};
+        launch_advect_scalar(s->d, s->u, s->v, s->w, s->occ, s->T, s->R, s->T2, s->R2, c.adv_sca_type == 0 ? 1 : 0,
+                             ap, s->vel4, s->stream);
+        std::swap(s->T, s->T2);
+        std::swap(s->R, s->R2);
+    }
+    ZQ_HIP(hipGetLastError(), "zq3d_advect_scalar");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_reinject(zq3d_sim* s) {
+    if (!s) { set_err("zq3d_reinject", "null sim"); return ZQ3D_ERR_ARG; }
+    for (size_t i = 0; i + 8 <= s->boxes.size(); i += 8)
+        launch_reinject_box(s->d, s->boxes.data() + i, s->T, s->R, s->stream);
+    for (size_t i = 0; i + 6 <= s->spheres.size(); i += 6)
+        launch_reinject_sphere(s->d, s->spheres.data() + i, s->T, s->R, s->stream);
+    ZQ_HIP(hipGetLastError(), "zq3d_reinject");
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_step(zq3d_sim* s, const float* extra, zq3d_proj_stats* st) {
+    int rc;
+    if ((rc = zq3d_add_force_attenuate(s, extra))) return rc;
+    if ((rc = zq3d_advect_velocity(s))) return rc;
+    if ((rc = zq3d_project(s, st))) return rc;
+    if ((rc = zq3d_advect_scalar(s))) return rc;
+    return zq3d_reinject(s);
+}
+
+extern "C" int zq3d_mac_to_regular(zq3d_sim* s) {
+    if (!s) { set_err("zq3d_mac_to_regular", "null sim"); return ZQ3D_ERR_ARG; }
+    launch_mac_to_cell4(s->d, s->u, s->v, s->w, s->vel4, s->stream);
+    ZQ_HIP(hipGetLastError(), "zq3d_mac_to_regular");
+    return ZQ3D_OK;
+}
+
+extern "C" const char* zq3d_last_error(void) { return g_err.c_str(); }
+
+extern "C" void zq3d_release_workspace(void) {
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    W.release();
+}

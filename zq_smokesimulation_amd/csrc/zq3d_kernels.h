@@ -1,0 +1,74 @@
+// zq3d_kernels.h -- host-side launchers of the gfx950 kernels (device pointers, stream-ordered).
+// Every launcher checks nothing itself: callers validate shapes first (zq3d_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zq3d {
+
+struct Dims;
+
+struct ForceParams {
+    float h, dt, beta, alpha, eps, Tamb;
+};
+
+// ---- force + attenuation (ZQ_CUDA_AddForce3D.cu, ZQ_CUDA_Attenuation3D.cu) ----
+// vort: N float4 scratch, fcell: 3N float scratch.  extra (nullable): 3 floats per cell.
+// atten != 0 fuses the attenuation stage (faces *= aV on the same mask, T *= aT, rho *= aR).
+void launch_force_atten(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
+                        const float* extra, const ForceParams& fp, int atten, float aV, float aT, float aR,
+                        float4* vort, float* fcell, hipStream_t s);
+void launch_attenuation(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
+                        float aV, float aT, float aR, hipStream_t s);
+
+// ---- MAC <-> cell (ZQ_CUDA_MACtoRegular.cu) ----
+void launch_mac_to_cell4(const Dims& d, const float* u, const float* v, const float* w, float4* vel4, hipStream_t s);
+
+// ---- advection (ZQ_CUDA_Advection3D.cu) ----
+struct AdvParams {
+    float h, dtt;
+    int steps;
+};
+// mode: 0 inReg_outReg, 2 inMAC_outMAC, 4 inReg_outMAC.  scratch: vel4 (N float4), faces (Nu+Nv+Nw floats).
+void launch_advect_velocity(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, int is_open, int mode,
+                            const AdvParams& ap, float4* vel4, float* scratch_faces, hipStream_t s);
+// mac != 0: MAC velocity sampling (ADV_SCA_MAC); else cell-centred (ADV_SCA_REG).
+void launch_advect_scalar(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
+                          const float* Tin, const float* Rin, float* Tout, float* Rout, int mac,
+                          const AdvParams& ap, float4* vel4, hipStream_t s);
+
+// ---- projection ----
+// Neighbour counts, FP32 divergence b, zero p at uncounted cells; optional FP64 sum/count of the
+// divergence over fluid cells (for the closed-domain mean).
+void launch_proj_setup(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
+                       int open_mask, uint8_t* nc, float* b, double* b64, float* p_pad, double* p64_pad,
+                       double* sum_cnt, hipStream_t s);
+// mean32 = float(sum/cnt) (or 0 when want == 0); mean64 likewise in double
+void launch_mean_finalize(const double* sum_cnt, float* mean32, double* mean64, int want, hipStream_t s);
+void launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float omega,
+                int iters, hipStream_t s);
+void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
+                    float* w, hipStream_t s);
+void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, float* u, float* v,
+                      float* w, hipStream_t s);
+// r = (b64 - mean64) - (sum_nb p - n p); writes r32, max|r| (as uint64 bits, atomicMax) into *rmax
+void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
+                       const double* p64_pad, float* r32, unsigned long long* rmax, hipStream_t s);
+// p64 += e, e = 0 on counted cells
+void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s);
+// max |div64 - mean64| over counted cells into *dmax (uint64 bits of a non-negative double)
+void launch_max_div64(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* nc,
+                      const double* mean64, unsigned long long* dmax, hipStream_t s);
+
+// ---- sources, obstacles, fills ----
+void launch_reinject_box(const Dims& d, const float* box8, float* T, float* rho, hipStream_t s);
+void launch_reinject_sphere(const Dims& d, const float* sph6, float* T, float* rho, hipStream_t s);
+void launch_stamp_sphere(const Dims& d, float cx, float cy, float cz, float r, float su, float sv, float sw,
+                         int shell_mask, uint8_t* occ, float* u, float* v, float* w, hipStream_t s);
+void launch_fill_f32(float* p, size_t n, float val, hipStream_t s);
+void launch_fill_f64(double* p, size_t n, double val, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, size_t n, unsigned long long val, hipStream_t s);
+void launch_fill_u8(uint8_t* p, size_t n, uint8_t val, hipStream_t s);
+
+}  // namespace zq3d
