@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Throughput of the 3D smoke forward step on one MI355X.
+
+Workload: 256^3 MAC grid, FP32; velocity, density and temperature
+advection; buoyancy (alpha 0.05, beta 0.5) and vorticity confinement (eps 0.3); attenuation
+0.99 per step on velocity, temperature and density; mixed-precision red-black SOR projection
+(omega 1.8, FP32 sweeps, FP64 residual refinement) to max|div| <= 1e-6 with a warm start;
+MAC->regular output every 10 steps.  Closed box (one-cell solid shell), sphere source of
+radius 0.08 at (0.5, 0.1, 0.5) with density 1 and temperature 1, Tamb 0, seeded +-0.01
+temperature noise, dt 0.02, 20 advection substeps.  Inputs are resident in HBM before the
+timed region.
+
+Prints one JSON line.  value = voxels * steps / wall time (Mvoxel-steps/s).  roofline: the
+single-colour SOR sweep kernel (the dominant kernel), algorithmic bytes 13 B/voxel per red+black
+iteration (6.5 B/voxel per launch) over its event-timed mean launch time.  cpu_baseline: the
+CPU oracle on a bounded sample of the same step (see --help).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "Mvoxel-steps/s at 256³ fwd & fwd+adjoint; HBM GB/s vs peak; divergence"
+HBM_PEAK_GBS = 8000.0
+SOR_BYTES_PER_VOXEL_ITER = 13.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed steps")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed steps before the timed region")
+    ap.add_argument("--size", type=int, default=256, help="grid edge (cube)")
+    ap.add_argument("--inner", type=int, default=40, help="FP32 SOR sweeps per refinement pass")
+    ap.add_argument("--max-outer", type=int, default=1000, help="refinement passes at most per step")
+    ap.add_argument("--tol", type=float, default=1e-6, help="max|div| target")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "sor_sweep_256_traffic.txt"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel")
+    return ap.parse_args()
+
+
+def make_state(W, H, D):
+    from zq_smokesimulation_amd import noise_field
+    occ = np.zeros((D, H, W), np.uint8)
+    occ[0] = occ[-1] = 1
+    occ[:, 0] = occ[:, -1] = 1
+    occ[:, :, 0] = occ[:, :, -1] = 1
+    T = noise_field(W, H, D, 0, 0.01)
+    z = np.zeros
+    return dict(u=z((D, H, W + 1), np.float32), v=z((D, H + 1, W), np.float32), w=z((D + 1, H, W), np.float32),
+                T=T, rho=z((D, H, W), np.float32), occ=occ)
+
+
+SOURCE = [0.5, 0.1, 0.5, 0.08, 1.0, 1.0]
+
+
+def read_traffic(path):
+    """FETCH_SIZE / WRITE_SIZE per dispatch (KiB) and the copy-kernel scale, from a profile summary."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.lstrip().startswith("#"):
+            k, _, v = line.partition(":")
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "fetch_scale", "write_scale", "size")
+    if not all(k in vals for k in need):
+        return None
+    return vals
+
+
+def cpu_baseline(state, W, H, D, inner, outer_mean):
+    """Oracle step on the GPU's own state: runs with 1 and 2 refinement passes give the fixed cost
+    of a step and the cost of one pass; the per-step time uses the GPU's mean pass count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import zq_oracle as O
+    p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=0.99, aT=0.99, aR=0.99,
+             adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1, omega=1.8, sor_iters=inner, mixed=1,
+             max_outer=1, tol=0.0)
+    times = []
+    for passes in (1, 2):
+        s = {k: a.copy() for k, a in state.items()}
+        p["max_outer"] = passes
+        t0 = time.perf_counter()
+        O.step(p, s["u"], s["v"], s["w"], s["T"], s["rho"], s["occ"], p64=s["p64"], spheres=[SOURCE])
+        times.append(time.perf_counter() - t0)
+    per_pass = max(times[1] - times[0], 0.0)
+    fixed = max(times[0] - per_pass, 0.0)
+    step_s = fixed + outer_mean * per_pass
+    return {
+        "value": (W * H * D) / step_s / 1e6,
+        "unit": "Mvoxel-steps/s",
+        "cores": int(O.lib().orc_num_threads()),
+        "kind": "port",
+        "sample": ("oracle step at %dx%dx%d on the GPU's state after the timed steps, run with 1 and 2 "
+                   "refinement passes of %d SOR sweeps (%.2f s, %.2f s); step time = fixed %.2f s + %.1f passes "
+                   "(GPU mean) x %.3f s, extrapolated" % (W, H, D, inner, times[0], times[1], fixed, outer_mean,
+                                                         per_pass)),
+    }
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the smoke step on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, ROOT)
+    import torch
+    import zq_smokesimulation_amd as z
+
+    W = H = D = args.size
+    N = W * H * D
+    sim = z.SmokeSim3D(W, H, D, voxel_size=1.0 / W, step_time=0.02, steps=20, Tamb=0.0, buoyancy=0.5, gravity=0.05,
+                       confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99, sor_open_mask=0,
+                       sor_closed_mean=1, sor_omega=1.8, sor_iters=args.inner, sor_mixed_precision=1,
+                       sor_max_outer=args.max_outer, sor_tol=args.tol, warm_start=1)
+    sim.set_sphere_sources([SOURCE])
+    st = make_state(W, H, D)
+    sim.set_state(**st)
+    print("bench: %d^3, warmup %d, timed %d" % (W, args.warmup, args.steps), file=sys.stderr, flush=True)
+    step_idx = 0
+    for _ in range(args.warmup):
+        sim.step()
+        if step_idx % 10 == 0:
+            sim.mac_to_regular()
+        step_idx += 1
+    sim.synchronize()
+    torch.cuda.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    sim.set_profiling(True)
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(sim.step())
+        if step_idx % 10 == 0:
+            sim.mac_to_regular()
+        step_idx += 1
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tm = sim.timing()
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = N * args.steps / wall / 1e6
+    outer_mean = float(np.mean([s["outer"] for s in stats]))
+    sweeps_mean = float(np.mean([s["sweeps"] for s in stats]))
+    max_div = float(max(s["max_div"] for s in stats))
+
+    # every SOR iteration is one fused launch (or two single-colour launches on grids too wide for
+    # the fused kernel's LDS ring); bytes per launch follow from iterations / launches
+    iters = float(sum(s["sweeps"] for s in stats))
+    launches = max(tm["sor_launches"], 1)
+    launch_ms = tm["ms_sor"] / launches
+    bytes_per_launch = SOR_BYTES_PER_VOXEL_ITER * N * iters / launches
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    tr = read_traffic(args.traffic)
+    if tr and int(tr["size"]) == W:
+        traffic = 1024.0 * (tr["fetch_kib_per_dispatch"] * tr["fetch_scale"] +
+                            tr["write_kib_per_dispatch"] * tr["write_scale"])
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mvoxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D smoke forward step, %d^3 MAC grid, closed box, mixed-precision red-black SOR to "
+                        "max|div|<=%g (config 4)" % (W, args.tol),
+            "grid": [W, H, D],
+            "advection_substeps": 20,
+            "sor_omega": 1.8,
+            "sor_sweeps_per_pass": args.inner,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_sor_fused (one red-black SOR iteration per launch)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "mean_launch_us": round(launch_ms * 1e3, 3),
+        },
+        "projection": {
+            "sor_sweeps_per_step": sweeps_mean,
+            "refinement_passes_per_step": outer_mean,
+            "max_div": max_div,
+        },
+        "stages_ms_per_step": {k: round(tm[k] / max(tm["steps"], 1), 3) for k in
+                               ("ms_force", "ms_advect_velocity", "ms_project", "ms_sor", "ms_advect_scalar",
+                                "ms_reinject")},
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        g = sim.get_state()
+        g["p64"] = sim.download(z._lib.FIELD_P64)
+        out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean)
+    else:
+        out["cpu_baseline"] = None
+    sim.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -209,6 +209,23 @@ int zq3d_step(zq3d_sim* sim, const float* extra_force_dev, zq3d_proj_stats* stat
 /* MAC -> cell float4 into ZQ3D_FIELD_VEL4 (the reference's output conversion) */
 int zq3d_mac_to_regular(zq3d_sim* sim);
 
+/* Stage timing with HIP events on the simulation's stream (the reference prints per-stage
+ * QueryPerformanceCounter / cudaEvent times, ZQ_SmokeSimulation3D.cpp:158-226).  While enabled,
+ * each stage records an event pair and waits for it, so the numbers are exact kernel time. */
+typedef struct zq3d_timing {
+    double ms_force;            /* force + attenuation */
+    double ms_advect_velocity;
+    double ms_project;          /* whole projection, SOR included */
+    double ms_sor;              /* red-black sweep launches only */
+    double ms_advect_scalar;
+    double ms_reinject;
+    long long sor_launches;     /* single-colour sweep kernels timed in ms_sor */
+    long long steps;
+} zq3d_timing;
+/* on != 0 enables and resets the counters */
+int zq3d_set_profiling(zq3d_sim* sim, int on);
+int zq3d_get_timing(zq3d_sim* sim, zq3d_timing* out);
+
 /* message of the last failed call on this thread ("" if none) */
 const char* zq3d_last_error(void);
 /* frees the device workspace cached by the Part 1/2 host-pointer functions */

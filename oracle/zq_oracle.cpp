@@ -15,6 +15,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -912,6 +915,14 @@ void orc_stamp_sphere(int W, int H, int D, float cx, float cy, float cz, float r
                       || ((shell_mask & ORC_OPEN_ZLO) && k == 0) || ((shell_mask & ORC_OPEN_ZHI) && k == D - 1);
                 if (s) occ[d.c(i, j, k)] = 1;
             }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 void orc_noise(int W, int H, int D, uint32_t seed, float amp, float* out) {

@@ -125,6 +125,8 @@ void orc_reinject_sphere(int W, int H, int D, float cx, float cy, float cz, floa
  * then the shell of shell_mask (semantics of ZQ_SmokeSimulationUtils3D.cpp:1933-2046). */
 void orc_stamp_sphere(int W, int H, int D, float cx, float cy, float cz, float r, float su, float sv, float sw,
                       int shell_mask, uint8_t* occ, float* u, float* v, float* w);
+/* OpenMP threads the oracle's parallel loops use */
+int orc_num_threads(void);
 /* counter-based hash noise in [-amp, amp] per cell (identical in the product) */
 void orc_noise(int W, int H, int D, uint32_t seed, float amp, float* out);
 

@@ -91,6 +91,8 @@ def lib():
         L.orc_reinject_sphere.argtypes = [i, i, i, f, f, f, f, f, f, _f32p, _f32p]
         L.orc_stamp_sphere.argtypes = [i, i, i, f, f, f, f, f, f, f, i, _u8p, _f32p, _f32p, _f32p]
         L.orc_noise.argtypes = [i, i, i, u32, f, _f32p]
+        L.orc_num_threads.argtypes = []
+        L.orc_num_threads.restype = i
         L.orc_step.argtypes = [C.POINTER(StepParams), _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, vp, vp, vp]
         L.orc_step.restype = i
         _lib = L

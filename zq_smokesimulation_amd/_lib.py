@@ -67,6 +67,22 @@ class Config(C.Structure):
     ]
 
 
+class Timing(C.Structure):
+    _fields_ = [
+        ("ms_force", C.c_double),
+        ("ms_advect_velocity", C.c_double),
+        ("ms_project", C.c_double),
+        ("ms_sor", C.c_double),
+        ("ms_advect_scalar", C.c_double),
+        ("ms_reinject", C.c_double),
+        ("sor_launches", C.c_longlong),
+        ("steps", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ZQ3DError(RuntimeError):
     pass
 
@@ -118,6 +134,8 @@ def _load():
     L.zq3d_reinject.argtypes = [vp]
     L.zq3d_step.argtypes = [vp, vp, C.POINTER(ProjStats)]
     L.zq3d_mac_to_regular.argtypes = [vp]
+    L.zq3d_set_profiling.argtypes = [vp, i]
+    L.zq3d_get_timing.argtypes = [vp, C.POINTER(Timing)]
     L.zq3d_last_error.argtypes = []
     L.zq3d_last_error.restype = C.c_char_p
     L.zq3d_release_workspace.argtypes = []

@@ -9,6 +9,9 @@
 //   * colour (i+j+k)&1, colour 0 first; p = (1-omega) p + omega (sum_nb p - b)/n with the sum taken
 //     over all six neighbours of a padded pressure whose ghost layer and solid cells hold 0;
 //   * u_face -= p_r - p_l on faces between two fluid cells and on open domain faces.
+//
+// Reductions (closed-domain mean, residual and divergence maxima) are two-level and deterministic:
+// one partial per block (wave butterfly + LDS), then one block folds the partials in index order.
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
 
@@ -28,7 +31,20 @@ __device__ __forceinline__ double wave_max(double x) {
     for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
     return x;
 }
-__device__ __forceinline__ bool lane0() { return ((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0; }
+
+// Sum (or max) over a 256-thread block; the result is valid in thread 0.
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double x, double* lds4) {
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    x = MAX ? wave_max(x) : wave_sum(x);
+    if ((t & 63) == 0) lds4[t >> 6] = x;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0) r = MAX ? fmax(fmax(lds4[0], lds4[1]), fmax(lds4[2], lds4[3])) : ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+    return r;
+}
+
+__device__ __forceinline__ int block_linear() { return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x; }
 
 // 1/n for n = 1..6, each correctly rounded (the same values as 1.0f/(float)n)
 __device__ __forceinline__ float rcp_count(int n) {
@@ -37,15 +53,16 @@ __device__ __forceinline__ float rcp_count(int n) {
 }
 
 // Neighbour counts, divergence (the reference's unscaled sum, ZQ_SmokeSimulationUtils3D.cpp:2403-2405,
-// evaluated left to right), zero pressure where the count is 0, and the FP64 sum/count of the
-// divergence over fluid cells for the closed-domain mean.
+// evaluated left to right), zero pressure where the count is 0, and per-block FP64 sum/count of the
+// divergence over fluid cells (closed-domain mean).
 template <bool MP>
 __global__ __launch_bounds__(BX* BY) void k_proj_setup(const float* __restrict__ u, const float* __restrict__ v,
                                                        const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                        int open_mask, uint8_t* __restrict__ nc,
                                                        float* __restrict__ b, double* __restrict__ b64,
                                                        float* __restrict__ p, double* __restrict__ p64,
-                                                       double* __restrict__ sum_cnt, Dims d) {
+                                                       double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -85,26 +102,63 @@ __global__ __launch_bounds__(BX* BY) void k_proj_setup(const float* __restrict__
             }
         }
     }
-    if (sum_cnt) {
-        dsum = wave_sum(dsum);
-        dcnt = wave_sum(dcnt);
-        if (lane0()) {
-            atomicAdd(&sum_cnt[0], dsum);
-            atomicAdd(&sum_cnt[1], dcnt);
+    if (part) {
+        const double s = block_reduce<false>(dsum, red);
+        __syncthreads();
+        const double c = block_reduce<false>(dcnt, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            part[2 * block_linear()] = s;
+            part[2 * block_linear() + 1] = c;
         }
     }
 }
 
-__global__ void k_mean_finalize(const double* __restrict__ sum_cnt, float* __restrict__ mean32,
-                                double* __restrict__ mean64, int want) {
-    double m = 0.0;
-    if (want && sum_cnt[1] > 0.0) m = sum_cnt[0] / sum_cnt[1];
-    if (mean32) *mean32 = (float)m;
-    if (mean64) *mean64 = m;
+// Fold the per-block (sum, count) pairs in index order; mean = sum/count when want != 0.
+__global__ __launch_bounds__(1024) void k_finish_mean(const double* __restrict__ part, int nblocks,
+                                                      float* __restrict__ mean32, double* __restrict__ mean64,
+                                                      int want) {
+    __shared__ double ss[1024], sc[1024];
+    const int t = threadIdx.x;
+    double s = 0.0, c = 0.0;
+    for (int i = t; i < nblocks; i += 1024) {
+        s += part[2 * i];
+        c += part[2 * i + 1];
+    }
+    ss[t] = s;
+    sc[t] = c;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) {
+            ss[t] += ss[t + h];
+            sc[t] += sc[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double m = (want && sc[0] > 0.0) ? ss[0] / sc[0] : 0.0;
+        if (mean32) *mean32 = (float)m;
+        if (mean64) *mean64 = m;
+    }
 }
 
-// One colour of a red-black SOR iteration.  Thread (t, j, k) owns cell x = 2t + ((colour+j+k)&1);
-// the 64 lanes of a wave cover 128 consecutive cells of a row.
+// Fold per-block maxima; the result is stored as the bit pattern of a non-negative double.
+__global__ __launch_bounds__(1024) void k_finish_max(const double* __restrict__ part, int nblocks,
+                                                     unsigned long long* __restrict__ out) {
+    __shared__ double sm[1024];
+    const int t = threadIdx.x;
+    double m = 0.0;
+    for (int i = t; i < nblocks; i += 1024) m = fmax(m, part[i]);
+    sm[t] = m;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) sm[t] = fmax(sm[t], sm[t + h]);
+        __syncthreads();
+    }
+    if (t == 0) *out = (unsigned long long)__double_as_longlong(sm[0]);
+}
+
+// One colour of a red-black SOR iteration, in place.  Thread (t, j, k) owns cell
+// x = 2t + ((colour+j+k)&1).  Used when the fused kernel's plane ring does not fit in LDS.
 __global__ __launch_bounds__(BX* BY) void k_sor_half(const uint8_t* __restrict__ nc, const float* __restrict__ b,
                                                      const float* __restrict__ mean32, float* __restrict__ p, Dims d,
                                                      float omega, float omw, int colour) {
@@ -122,6 +176,177 @@ __global__ __launch_bounds__(BX* BY) void k_sor_half(const uint8_t* __restrict__
     const float bb = b[o] - *mean32;
     const float pg = (s - bb) * rcp_count(n);
     p[q] = omw * p[q] + omega * pg;
+}
+
+// One full red-black iteration p_in -> p_out in a single pass over HBM.
+//
+// A block owns TY whole rows (all of x, so no x halo) of a chunk of zc planes and walks z with a
+// ring of four planes in LDS, each plane holding rows y0-2 .. y0+TY+1 of the old pressure.  At
+// step zz it updates colour 0 of plane zz (tile rows plus one halo row on each side, recomputed
+// here so no other block's result is needed), then colour 1 of plane zz-1 (tile rows; every
+// neighbour it reads is a colour-0 value already updated in LDS), and stores plane zz-1.
+// Colour 0 reads only old colour-1 values and colour 1 only new colour-0 values, exactly as in two
+// single-colour sweeps, so results are bit-identical to the in-place half sweeps.  The chunk
+// recomputes colour 0 on the plane before and after it for the same reason.
+// Plane zz+2 is read into registers while zz is computed (one float per thread per row for each
+// 256 columns, CPT = columns per thread), hiding the HBM latency behind the updates.  The ghost
+// columns x = -1 and x = W are zero in every plane and are set once.
+// Blocks are numbered so each XCD (hardware dispatches block b to XCD b % 8) receives a contiguous
+// run of tiles, which share halo rows through that XCD's L2.
+template <int TY, int CPT>
+__global__ __launch_bounds__(256) void k_sor_fused(const uint8_t* __restrict__ nc, const float* __restrict__ b,
+                                                   const float* __restrict__ mean32, const float* __restrict__ pin,
+                                                   float* __restrict__ pout, Dims d, float omega, float omw, int zc,
+                                                   int ntiles_y) {
+    extern __shared__ float lds[];
+    constexpr int ROWS = TY + 4;
+    constexpr int NT = 256;
+    const int tid = threadIdx.x;
+    const int per = gridDim.x >> 3;  // launched with a multiple of 8 blocks
+    const int bid = blockIdx.x;
+    const int tile = (bid & 7) * per + (bid >> 3);
+    const int ty = tile % ntiles_y, tz = tile / ntiles_y;
+    const int y0 = ty * TY, z0 = tz * zc;
+    const int z1 = min(z0 + zc, d.D);
+    if (z0 >= d.D) return;
+    const int WP = d.W + 2;
+    const int PLANE = ROWS * WP;
+    const int sy = d.W + 2, sz = (d.W + 2) * (d.H + 2);
+    const float m = *mean32;
+    const int hb = NT / 2;  // threads per row when two rows are processed together
+    const int sub = tid >= hb ? 1 : 0;
+    const int lt = tid - sub * hb;
+
+    auto buf = [&](int z) { return lds + (z & 3) * PLANE; };  // z >= -2: (z & 3) is z mod 4
+
+    // ghost columns of every row of the ring
+    for (int i = tid; i < 4 * ROWS; i += NT) {
+        lds[i * WP] = 0.0f;
+        lds[i * WP + WP - 1] = 0.0f;
+    }
+
+    float pf[ROWS][CPT];
+    auto fetch = [&](int z) {  // plane z, rows y0-2.., interior columns, into registers
+        const bool zok = z >= -1 && z <= d.D;
+#pragma unroll
+        for (int r = 0; r < ROWS; r++) {
+            const int y = y0 - 2 + r;
+            const bool ok = zok && y >= -1 && y <= d.H;
+            const float* src = pin + (size_t)(z + 1) * sz + (size_t)(y + 1) * sy + 1;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = tid + c * NT;
+                pf[r][c] = (ok && x < d.W) ? src[x] : 0.0f;
+            }
+        }
+    };
+    auto commit = [&](int z) {  // registers -> LDS plane z
+        float* B = buf(z);
+#pragma unroll
+        for (int r = 0; r < ROWS; r++)
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = tid + c * NT;
+                if (x < d.W) B[r * WP + x + 1] = pf[r][c];
+            }
+    };
+
+    // Each thread updates rows first+sub, first+sub+2, ... and, in each, the cells of the colour at
+    // x = 2*(lt + c*hb) + parity.  Their counts and right-hand sides are read one step ahead into
+    // registers (cur/next pairs) so no update waits on HBM.
+    constexpr int RR = (TY + 2) / 2;  // colour-0 rows per thread (tile rows plus halo rows)
+    constexpr int RB = TY / 2;        // colour-1 rows per thread (tile rows)
+    uint8_t n0[RR][CPT], n0x[RR][CPT], n1[RB][CPT], n1x[RB][CPT];
+    float b0[RR][CPT], b0x[RR][CPT], b1[RB][CPT], b1x[RB][CPT];
+
+    auto fetch_rhs = [&](int z, int col, int first, int nrows, auto& nn, auto& bv) {
+        for (int k = 0; k < nrows; k++) {
+            const int r = first + sub + 2 * k;
+            const int y = y0 - 2 + r;
+            const int par = (col + y + z) & 1;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = 2 * (lt + c * hb) + par;
+                const bool ok = z >= 0 && z < d.D && y >= 0 && y < d.H && x < d.W;
+                const int o = (z * d.H + y) * d.W + x;
+                nn[k][c] = ok ? nc[o] : (uint8_t)0;
+                bv[k][c] = ok ? b[o] : 0.0f;
+            }
+        }
+    };
+
+    auto update = [&](int z, int col, int first, int nrows, auto& nn, auto& bv) {
+        if (z < 0 || z >= d.D) return;
+        float* Bc = buf(z);
+        const float* Bm = buf(z - 1);
+        const float* Bp = buf(z + 1);
+        for (int k = 0; k < nrows; k++) {
+            const int r = first + sub + 2 * k;
+            const int y = y0 - 2 + r;
+            if (y < 0 || y >= d.H) continue;
+            const int par = (col + y + z) & 1;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = 2 * (lt + c * hb) + par;
+                const int n = nn[k][c];
+                if (x >= d.W || !n) continue;
+                const int q = r * WP + x + 1;
+                const float s = Bc[q - 1] + Bc[q + 1] + Bc[q - WP] + Bc[q + WP] + Bm[q] + Bp[q];
+                const float bb = bv[k][c] - m;
+                const float pg = (s - bb) * rcp_count(n);
+                Bc[q] = omw * Bc[q] + omega * pg;
+            }
+        }
+    };
+
+    auto store_plane = [&](int z) {
+        const float* B = buf(z);
+        for (int r = 2; r < TY + 2; r++) {
+            const int y = y0 - 2 + r;
+            if (y >= d.H) break;
+            float* dst = pout + (size_t)(z + 1) * sz + (size_t)(y + 1) * sy + 1;
+            for (int x = tid; x < d.W; x += NT) dst[x] = B[r * WP + x + 1];
+        }
+    };
+
+    fetch(z0 - 2);
+    commit(z0 - 2);
+    fetch(z0 - 1);
+    commit(z0 - 1);
+    fetch(z0);
+    commit(z0);
+    fetch(z0 + 1);
+    fetch_rhs(z0 - 1, 0, 1, RR, n0, b0);
+    __syncthreads();
+    for (int zz = z0 - 1; zz <= z1; zz++) {
+        // LDS: planes zz-1, zz, zz+1; registers: plane zz+2, colour-0 rhs of zz, colour-1 rhs of zz-1
+        fetch_rhs(zz + 1, 0, 1, RR, n0x, b0x);
+        fetch_rhs(zz, 1, 2, RB, n1x, b1x);
+        update(zz, 0, 1, RR, n0, b0);  // colour 0, rows y0-1 .. y0+TY
+        __syncthreads();
+        if (zz - 1 >= z0) {
+            update(zz - 1, 1, 2, RB, n1, b1);  // colour 1, rows y0 .. y0+TY-1
+            __syncthreads();
+            store_plane(zz - 1);
+        }
+        commit(zz + 2);  // into the slot of plane zz-2, no longer read
+        if (zz + 1 <= z1) fetch(zz + 3);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RR; k++)
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                n0[k][c] = n0x[k][c];
+                b0[k][c] = b0x[k][c];
+            }
+#pragma unroll
+        for (int k = 0; k < RB; k++)
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                n1[k][c] = n1x[k][c];
+                b1[k][c] = b1x[k][c];
+            }
+    }
 }
 
 // Face correction u -= p_r - p_l (ZQ_SmokeSimulationUtils3D.cpp:2815-2861 open form, :2888-2922
@@ -152,11 +377,12 @@ __global__ __launch_bounds__(BX* BY) void k_correct(const uint8_t* __restrict__ 
     face[fi] = (float)((P)face[fi] - (p[qh] - p[qh - pstep]));
 }
 
-// FP64 residual r = (b64 - mean) - (sum_nb p - n p); r32 feeds the FP32 inner solve.
+// FP64 residual r = (b64 - mean) - (sum_nb p - n p); r32 feeds the FP32 inner solve; per-block max|r|.
 __global__ __launch_bounds__(BX* BY) void k_residual64(const uint8_t* __restrict__ nc, const double* __restrict__ b64,
                                                        const double* __restrict__ mean64,
                                                        const double* __restrict__ p, float* __restrict__ r32,
-                                                       unsigned long long* __restrict__ rmax, Dims d) {
+                                                       double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -176,8 +402,8 @@ __global__ __launch_bounds__(BX* BY) void k_residual64(const uint8_t* __restrict
         }
         r32[o] = r;
     }
-    m = wave_max(m);
-    if (lane0()) atomicMax(rmax, (unsigned long long)__double_as_longlong(m));
+    m = block_reduce<true>(m, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
 }
 
 __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict__ nc, float* __restrict__ e,
@@ -194,8 +420,9 @@ __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict
 
 __global__ __launch_bounds__(BX* BY) void k_max_div64(const float* __restrict__ u, const float* __restrict__ v,
                                                       const float* __restrict__ w, const uint8_t* __restrict__ nc,
-                                                      const double* __restrict__ mean64,
-                                                      unsigned long long* __restrict__ dmax, Dims d) {
+                                                      const double* __restrict__ mean64, double* __restrict__ part,
+                                                      Dims d) {
+    __shared__ double red[4];
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -206,36 +433,82 @@ __global__ __launch_bounds__(BX* BY) void k_max_div64(const float* __restrict__ 
         const double dw = (double)w[widx(d, i, j, k + 1)] - (double)w[widx(d, i, j, k)];
         m = fabs(((du + dv) + dw) - *mean64);
     }
-    m = wave_max(m);
-    if (lane0()) atomicMax(dmax, (unsigned long long)__double_as_longlong(m));
+    m = block_reduce<true>(m, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
+}
+
+int cell_blocks(const Dims& d) {
+    const dim3 g = grid_for(d.W, d.H, d.D);
+    return (int)(g.x * g.y * g.z);
+}
+
+// Fused-kernel launch shape: TY rows per block, ZC planes per chunk; 0 tiles when it cannot run.
+struct FusedShape {
+    int ty, zc, ntiles_y, nchunks, nblocks;
+    size_t lds;
+};
+
+FusedShape fused_shape(const Dims& d) {
+    FusedShape f{};
+    f.ty = 4;
+    f.zc = 16;
+    f.lds = (size_t)4 * (f.ty + 4) * (d.W + 2) * sizeof(float);
+    if (f.lds > 64 * 1024 || d.W > 512) return FusedShape{};
+    f.ntiles_y = (d.H + f.ty - 1) / f.ty;
+    f.nchunks = (d.D + f.zc - 1) / f.zc;
+    const int tiles = f.ntiles_y * f.nchunks;
+    f.nblocks = (tiles + 7) & ~7;  // multiple of 8 for the XCD numbering; spare blocks exit
+    return f;
 }
 
 }  // namespace
 
+int reduction_partials(const Dims& d) { return 2 * cell_blocks(d); }
+
 void launch_proj_setup(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
                        int open_mask, uint8_t* nc, float* b, double* b64, float* p_pad, double* p64_pad,
-                       double* sum_cnt, hipStream_t s) {
+                       double* part, hipStream_t s) {
     if (b64)
         hipLaunchKernelGGL(k_proj_setup<true>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ, open_mask,
-                           nc, b, b64, p_pad, p64_pad, sum_cnt, d);
+                           nc, b, b64, p_pad, p64_pad, part, d);
     else
         hipLaunchKernelGGL(k_proj_setup<false>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ, open_mask,
-                           nc, b, b64, p_pad, p64_pad, sum_cnt, d);
+                           nc, b, b64, p_pad, p64_pad, part, d);
 }
 
-void launch_mean_finalize(const double* sum_cnt, float* mean32, double* mean64, int want, hipStream_t s) {
-    hipLaunchKernelGGL(k_mean_finalize, dim3(1), dim3(1), 0, s, sum_cnt, mean32, mean64, want);
+void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish_mean, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), mean32, mean64, want);
 }
 
-void launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float omega,
-                int iters, hipStream_t s) {
-    const dim3 g = grid_for((d.W + 1) / 2, d.H, d.D);
+float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
+                  float omega, int iters, hipStream_t s) {
     const float omw = 1.0f - omega;
+    const FusedShape f = fused_shape(d);
+    if (f.nblocks > 0 && p_alt) {
+        float* src = p_pad;
+        float* dst = p_alt;
+        for (int it = 0; it < iters; it++) {
+            if (d.W <= 256)
+                hipLaunchKernelGGL((k_sor_fused<4, 1>), dim3(f.nblocks), dim3(256), f.lds, s, nc, b, mean32, src, dst, d,
+                                   omega, omw, f.zc, f.ntiles_y);
+            else
+                hipLaunchKernelGGL((k_sor_fused<4, 2>), dim3(f.nblocks), dim3(256), f.lds, s, nc, b, mean32, src, dst, d,
+                                   omega, omw, f.zc, f.ntiles_y);
+            float* t = src;
+            src = dst;
+            dst = t;
+        }
+        return src;
+    }
+    const dim3 g = grid_for((d.W + 1) / 2, d.H, d.D);
     for (int it = 0; it < iters; it++) {
         hipLaunchKernelGGL(k_sor_half, g, dim3(BX, BY), 0, s, nc, b, mean32, p_pad, d, omega, omw, 0);
         hipLaunchKernelGGL(k_sor_half, g, dim3(BX, BY), 0, s, nc, b, mean32, p_pad, d, omega, omw, 1);
     }
+    return p_pad;
 }
+
+int sor_launches_per_iter(const Dims& d) { return fused_shape(d).nblocks > 0 ? 1 : 2; }
 
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s) {
@@ -258,9 +531,10 @@ void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const do
 }
 
 void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
-                       const double* p64_pad, float* r32, unsigned long long* rmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_residual64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad, r32, rmax,
+                       const double* p64_pad, float* r32, double* part, unsigned long long* rmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_residual64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad, r32, part,
                        d);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), rmax);
 }
 
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s) {
@@ -268,8 +542,9 @@ void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p
 }
 
 void launch_max_div64(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* nc,
-                      const double* mean64, unsigned long long* dmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_max_div64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, nc, mean64, dmax, d);
+                      const double* mean64, double* part, unsigned long long* dmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_div64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, nc, mean64, part, d);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), dmax);
 }
 
 }  // namespace zq3d

@@ -62,7 +62,6 @@ struct Sizes {
 
 // Small per-call scalars in one device block.
 struct Scalars {
-    double sum_cnt[2];
     double mean64;
     double zero64;
     unsigned long long rmax;
@@ -72,7 +71,7 @@ struct Scalars {
 };
 
 // ---------------- grow-only workspace for the host-pointer entry points ----------------
-enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_B64, S_P64, S_SCAL, S_COUNT };
+enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_P2, S_B64, S_P64, S_PART, S_SCAL, S_COUNT };
 
 struct Workspace {
     void* ptr[S_COUNT] = {};
@@ -137,33 +136,79 @@ struct Errs {
     }
 };
 
+// Event timing of the red-black sweep loops (null when profiling is off).
+struct SorProf {
+    hipEvent_t a = nullptr, b = nullptr;
+    double ms = 0.0;
+    long long launches = 0;
+};
+
+// Device buffers of a projection.  p/p_alt: padded FP32 pressure (plain) or correction e (mixed),
+// ping-ponged by the fused sweep kernel; both keep a zero ghost layer.  part: reduction partials.
+struct ProjBufs {
+    uint8_t* nc;
+    float* b;       // FP32 divergence (plain) / FP32 residual (mixed)
+    float* p;
+    float* p_alt;
+    double* b64;
+    double* p64;
+    double* part;
+    Scalars* sc;
+};
+
+// launch_sor, timed when pf is set (the sweeps are the only work between the two events)
+hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p, float* p_alt,
+                     float omega, int iters, hipStream_t s, SorProf* pf, float** result) {
+    if (!pf) {
+        *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s);
+        return hipGetLastError();
+    }
+    Errs E;
+    E(hipEventRecord(pf->a, s));
+    *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s);
+    E(hipGetLastError());
+    E(hipEventRecord(pf->b, s));
+    E(hipEventSynchronize(pf->b));
+    float ms = 0.0f;
+    E(hipEventElapsedTime(&ms, pf->a, pf->b));
+    pf->ms += ms;
+    pf->launches += (long long)sor_launches_per_iter(d) * iters;
+    return E.e;
+}
+
+int read_double_bits(const unsigned long long* dev, double* out, hipStream_t s, const char* where) {
+    unsigned long long bits = 0;
+    ZQ_HIP(hipMemcpyAsync(&bits, dev, sizeof(bits), hipMemcpyDeviceToHost, s), where);
+    ZQ_HIP(hipStreamSynchronize(s), where);
+    std::memcpy(out, &bits, sizeof(*out));
+    return ZQ3D_OK;
+}
+
 // Plain FP32 projection on device pointers: p = 0, setup, SOR, correct.  mean_mode: 0 = value
 // `mean` given, 1 = FP64 device mean over fluid cells.
 int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
-                  float mean, int mean_mode, uint8_t* nc, float* b, float* p, Scalars* sc, zq3d_proj_stats* st,
-                  hipStream_t s) {
+                  float mean, int mean_mode, const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s,
+                  SorProf* pf = nullptr) {
     const Sizes z(d);
-    launch_fill_f32(p, z.Np, 0.0f, s);
-    launch_fill_f64(sc->sum_cnt, 2, 0.0, s);
-    launch_proj_setup(d, u, v, w, occ, o.open_mask, nc, b, nullptr, p, nullptr, sc->sum_cnt, s);
+    Scalars* sc = B.sc;
+    launch_fill_f32(B.p, z.Np, 0.0f, s);
+    launch_proj_setup(d, u, v, w, occ, o.open_mask, B.nc, B.b, nullptr, B.p, nullptr, mean_mode ? B.part : nullptr,
+                      s);
     if (mean_mode) {
-        launch_mean_finalize(sc->sum_cnt, &sc->mean32, &sc->mean64, 1, s);
+        launch_mean_finalize(d, B.part, &sc->mean32, &sc->mean64, 1, s);
     } else {
         launch_fill_f32(&sc->mean32, 1, mean, s);
         launch_fill_f64(&sc->mean64, 1, (double)mean, s);
     }
-    launch_sor(d, nc, b, &sc->mean32, p, o.omega, o.iters, s);
-    launch_correct(d, occ, o.open_mask, p, u, v, w, s);
+    float* res = nullptr;
+    ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->mean32, B.p, B.p_alt, o.omega, o.iters, s, pf, &res), "project");
+    launch_correct(d, occ, o.open_mask, res, u, v, w, s);
     if (st) {
-        launch_fill_u64(&sc->dmax, 1, 0ull, s);
-        launch_max_div64(d, u, v, w, nc, &sc->mean64, &sc->dmax, s);
-        unsigned long long bits = 0;
-        double mean64 = 0.0;
-        ZQ_HIP(hipMemcpyAsync(&bits, &sc->dmax, sizeof(bits), hipMemcpyDeviceToHost, s), "project");
+        launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+        double md = 0.0, mean64 = 0.0;
+        if (int rc = read_double_bits(&sc->dmax, &md, s, "project")) return rc;
         ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project");
         ZQ_HIP(hipStreamSynchronize(s), "project");
-        double md;
-        std::memcpy(&md, &bits, sizeof(md));
         st->outer = 0;
         st->sweeps = o.iters;
         st->max_residual = -1.0;
@@ -176,39 +221,34 @@ int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
 // Mixed-precision projection: FP64 b, p and residual; FP32 inner SOR on the correction e.
 // p64 holds the warm start (zeroed by the caller for a cold start).
 int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
-                  uint8_t* nc, double* b64, double* p64, float* r32, float* e, Scalars* sc, zq3d_proj_stats* st,
-                  hipStream_t s) {
+                  const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf = nullptr) {
     const Sizes z(d);
-    launch_fill_f64(sc->sum_cnt, 2, 0.0, s);
+    Scalars* sc = B.sc;
     launch_fill_f32(&sc->zero32, 1, 0.0f, s);
-    launch_proj_setup(d, u, v, w, occ, o.open_mask, nc, nullptr, b64, nullptr, p64, sc->sum_cnt, s);
-    launch_mean_finalize(sc->sum_cnt, nullptr, &sc->mean64, o.closed_mean, s);
+    launch_proj_setup(d, u, v, w, occ, o.open_mask, B.nc, nullptr, B.b64, nullptr, B.p64, B.part, s);
+    launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
+    float* e = B.p;
+    float* e_alt = B.p_alt;
     launch_fill_f32(e, z.Np, 0.0f, s);
     int outer = 0;
     double rmax = 0.0;
     for (;;) {
-        launch_fill_u64(&sc->rmax, 1, 0ull, s);
-        launch_residual64(d, nc, b64, &sc->mean64, p64, r32, &sc->rmax, s);
-        unsigned long long bits = 0;
-        ZQ_HIP(hipMemcpyAsync(&bits, &sc->rmax, sizeof(bits), hipMemcpyDeviceToHost, s), "project_mixed");
-        ZQ_HIP(hipStreamSynchronize(s), "project_mixed");
-        std::memcpy(&rmax, &bits, sizeof(rmax));
+        launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, &sc->rmax, s);
+        if (int rc = read_double_bits(&sc->rmax, &rmax, s, "project_mixed")) return rc;
         if (!(rmax > o.tol) || outer >= o.max_outer) break;
-        launch_sor(d, nc, r32, &sc->zero32, e, o.omega, o.iters, s);
-        launch_update_p64(d, nc, e, p64, s);
+        float* res = nullptr;
+        ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res), "project_mixed");
+        if (res != e) std::swap(e, e_alt);
+        launch_update_p64(d, B.nc, e, B.p64, s);  // also zeroes e for the next pass
         outer++;
     }
-    launch_correct64(d, occ, o.open_mask, p64, u, v, w, s);
-    launch_fill_u64(&sc->dmax, 1, 0ull, s);
-    launch_max_div64(d, u, v, w, nc, &sc->mean64, &sc->dmax, s);
-    unsigned long long bits = 0;
-    double mean64 = 0.0;
-    ZQ_HIP(hipMemcpyAsync(&bits, &sc->dmax, sizeof(bits), hipMemcpyDeviceToHost, s), "project_mixed");
+    launch_correct64(d, occ, o.open_mask, B.p64, u, v, w, s);
+    launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+    double md = 0.0, mean64 = 0.0;
+    if (int rc = read_double_bits(&sc->dmax, &md, s, "project_mixed")) return rc;
     ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project_mixed");
     ZQ_HIP(hipStreamSynchronize(s), "project_mixed");
     if (st) {
-        double md;
-        std::memcpy(&md, &bits, sizeof(md));
         st->outer = outer;
         st->sweeps = outer * o.iters;
         st->max_residual = rmax;
@@ -437,7 +477,8 @@ float solve_host(float* u, float* v, float* w, const bool* occupy, int W_, int H
     Timer t;
     Errs E;
     if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
-        W.ensure(S_NC, z.N) || W.ensure(S_B, z.N * 4) || W.ensure(S_P, z.Np * 4) || W.ensure(S_SCAL, sizeof(Scalars)) ||
+        W.ensure(S_NC, z.N) || W.ensure(S_B, z.N * 4) || W.ensure(S_P, z.Np * 4) || W.ensure(S_P2, z.Np * 4) ||
+        W.ensure(S_PART, (size_t)reduction_partials(d) * 8) || W.ensure(S_SCAL, sizeof(Scalars)) ||
         (o.mixed_precision && (W.ensure(S_B64, z.N * 8) || W.ensure(S_P64, z.Np * 8)))) {
         set_err(where, "device allocation failed");
         return -1.0f;
@@ -450,15 +491,15 @@ float solve_host(float* u, float* v, float* w, const bool* occupy, int W_, int H
     E(hipMemcpy(dv, v, z.Nv * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(dw, w, z.Nw * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(docc, occupy, z.N, hipMemcpyHostToDevice));
-    Scalars* sc = (Scalars*)W.ptr[S_SCAL];
+    const ProjBufs B{(uint8_t*)W.ptr[S_NC], (float*)W.ptr[S_B],     (float*)W.ptr[S_P],    (float*)W.ptr[S_P2],
+                     (double*)W.ptr[S_B64], (double*)W.ptr[S_P64], (double*)W.ptr[S_PART], (Scalars*)W.ptr[S_SCAL]};
+    launch_fill_f32(B.p_alt, z.Np, 0.0f, 0);  // zero ghost layer of the second sweep buffer
     int rc;
     if (o.mixed_precision) {
-        launch_fill_f64((double*)W.ptr[S_P64], z.Np, 0.0, 0);
-        rc = project_mixed(d, du, dv, dw, docc, o, (uint8_t*)W.ptr[S_NC], (double*)W.ptr[S_B64],
-                           (double*)W.ptr[S_P64], (float*)W.ptr[S_B], (float*)W.ptr[S_P], sc, st, 0);
+        launch_fill_f64(B.p64, z.Np, 0.0, 0);
+        rc = project_mixed(d, du, dv, dw, docc, o, B, st, 0);
     } else {
-        rc = project_plain(d, du, dv, dw, docc, o, given_mean, o.closed_mean ? 1 : 0, (uint8_t*)W.ptr[S_NC],
-                           (float*)W.ptr[S_B], (float*)W.ptr[S_P], sc, st, 0);
+        rc = project_plain(d, du, dv, dw, docc, o, given_mean, o.closed_mean ? 1 : 0, B, st, 0);
     }
     if (rc) return -1.0f;
     E(hipMemcpy(u, du, z.Nu * 4, hipMemcpyDeviceToHost));
@@ -547,11 +588,17 @@ struct zq3d_sim {
     uint8_t* nc = nullptr;
     float* b = nullptr;           // FP32 divergence / FP32 residual of the mixed solve
     float* p = nullptr;           // padded FP32 pressure / correction e of the mixed solve
+    float* p_alt = nullptr;       // second sweep buffer (ping-pong), zero ghost layer
+    double* part = nullptr;       // reduction partials
     double* b64 = nullptr;
     double* p64 = nullptr;
     Scalars* sc = nullptr;
     std::vector<float> boxes, spheres;
     bool p64_valid = false;
+    bool prof = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    SorProf sorprof;
+    zq3d_timing tm{};
 };
 
 namespace {
@@ -575,6 +622,26 @@ int field_info(zq3d_sim* s, int field, void** ptr, size_t* bytes) {
     default: return -1;
     }
 }
+
+ProjBufs proj_bufs(zq3d_sim* s) { return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc}; }
+
+// Times one stage of a simulation with an event pair when profiling is on.
+struct StageTimer {
+    zq3d_sim* s;
+    double* acc;
+    StageTimer(zq3d_sim* s_, double* acc_) : s(s_), acc(acc_) {
+        if (s->prof) (void)hipEventRecord(s->ev[0], s->stream);
+    }
+    int done(const char* where) {
+        if (!s->prof) return ZQ3D_OK;
+        ZQ_HIP(hipEventRecord(s->ev[1], s->stream), where);
+        ZQ_HIP(hipEventSynchronize(s->ev[1]), where);
+        float ms = 0.0f;
+        ZQ_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]), where);
+        *acc += ms;
+        return ZQ3D_OK;
+    }
+};
 
 int check_config(const zq3d_config& c, const char* where) {
     if (c.struct_size != (int)sizeof(zq3d_config)) {
@@ -649,6 +716,8 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     A(dalloc(&s->nc, z.N));
     A(dalloc(&s->b, z.N * 4));
     A(dalloc(&s->p, z.Np * 4));
+    A(dalloc(&s->p_alt, z.Np * 4));
+    A(dalloc(&s->part, (size_t)reduction_partials(s->d) * 8));
     A(dalloc(&s->sc, sizeof(Scalars)));
     if (cfg->sor.mixed_precision) {
         A(dalloc(&s->b64, z.N * 8));
@@ -667,6 +736,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     launch_fill_f32(s->R, z.N, 0.0f, st);
     launch_fill_u8(s->occ, z.N, 0, st);
     launch_fill_f32(s->p, z.Np, 0.0f, st);
+    launch_fill_f32(s->p_alt, z.Np, 0.0f, st);
     if (s->p64) launch_fill_f64(s->p64, z.Np, 0.0, st);
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -680,10 +750,12 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
 
 extern "C" void zq3d_destroy(zq3d_sim* s) {
     if (!s) return;
-    void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->nc, s->b, s->p,
+    void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->nc, s->b, s->p, s->p_alt, s->part,
                     s->b64, s->p64, s->sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : {s->ev[0], s->ev[1], s->sorprof.a, s->sorprof.b})
+        if (e) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -757,10 +829,11 @@ extern "C" int zq3d_add_force_attenuate(zq3d_sim* s, const float* extra) {
     if (!s) { set_err("zq3d_add_force_attenuate", "null sim"); return ZQ3D_ERR_ARG; }
     const zq3d_config& c = s->cfg;
     const ForceParams fp{c.voxel_size, c.step_time, c.buoyancy, c.gravity, c.confine, c.Tamb};
+    StageTimer t(s, &s->tm.ms_force);
     launch_force_atten(s->d, s->u, s->v, s->w, s->T, s->R, s->occ, extra, fp, 1, c.vel_atten, c.temp_atten,
                        c.dens_atten, s->vel4, s->scratch, s->stream);
     ZQ_HIP(hipGetLastError(), "zq3d_add_force_attenuate");
-    return ZQ3D_OK;
+    return t.done("zq3d_add_force_attenuate");
 }
 
 extern "C" int zq3d_advect_velocity(zq3d_sim* s) {
@@ -768,33 +841,36 @@ extern "C" int zq3d_advect_velocity(zq3d_sim* s) {
     const zq3d_config& c = s->cfg;
     const AdvParams ap{c.voxel_size, c.step_time / c.steps, c.steps};
     // the reference's stage always advects with is_open = true (ZQ_SmokeSimulationUtils3D.cpp:2163)
+    StageTimer t(s, &s->tm.ms_advect_velocity);
     launch_advect_velocity(s->d, s->u, s->v, s->w, s->occ, 1, c.adv_vel_type, ap, s->vel4, s->scratch, s->stream);
     ZQ_HIP(hipGetLastError(), "zq3d_advect_velocity");
-    return ZQ3D_OK;
+    return t.done("zq3d_advect_velocity");
 }
 
 extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
     if (!s) { set_err("zq3d_project", "null sim"); return ZQ3D_ERR_ARG; }
     const zq3d_config& c = s->cfg;
+    StageTimer t(s, &s->tm.ms_project);
+    SorProf* pf = s->prof ? &s->sorprof : nullptr;
     int rc;
     if (c.sor.mixed_precision) {
         if (!c.warm_start || !s->p64_valid) launch_fill_f64(s->p64, Sizes(s->d).Np, 0.0, s->stream);
-        rc = project_mixed(s->d, s->u, s->v, s->w, s->occ, c.sor, s->nc, s->b64, s->p64, s->b, s->p, s->sc, st,
-                           s->stream);
+        rc = project_mixed(s->d, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
         s->p64_valid = (rc == ZQ3D_OK);
     } else {
-        rc = project_plain(s->d, s->u, s->v, s->w, s->occ, c.sor, 0.0f, c.sor.closed_mean ? 1 : 0, s->nc, s->b, s->p,
-                           s->sc, st, s->stream);
+        rc = project_plain(s->d, s->u, s->v, s->w, s->occ, c.sor, 0.0f, c.sor.closed_mean ? 1 : 0, proj_bufs(s), st,
+                           s->stream, pf);
     }
     if (rc) return rc;
     ZQ_HIP(hipGetLastError(), "zq3d_project");
-    return ZQ3D_OK;
+    return t.done("zq3d_project");
 }
 
 extern "C" int zq3d_advect_scalar(zq3d_sim* s) {
     if (!s) { set_err("zq3d_advect_scalar", "null sim"); return ZQ3D_ERR_ARG; }
     const zq3d_config& c = s->cfg;
     const Sizes z(s->d);
+    StageTimer t(s, &s->tm.ms_advect_scalar);
     if (c.adv_sca_type == 1 || c.adv_sca_type == 3) {
         // reference behaviour: the BFECC scalar modes return zero fields (ZQ_CUDA_Advection3D.cu:1782-1794)
         launch_fill_f32(s->T, z.N, 0.0f, s->stream);
@@ -807,17 +883,18 @@ extern "C" int zq3d_advect_scalar(zq3d_sim* s) {
         std::swap(s->R, s->R2);
     }
     ZQ_HIP(hipGetLastError(), "zq3d_advect_scalar");
-    return ZQ3D_OK;
+    return t.done("zq3d_advect_scalar");
 }
 
 extern "C" int zq3d_reinject(zq3d_sim* s) {
     if (!s) { set_err("zq3d_reinject", "null sim"); return ZQ3D_ERR_ARG; }
+    StageTimer t(s, &s->tm.ms_reinject);
     for (size_t i = 0; i + 8 <= s->boxes.size(); i += 8)
         launch_reinject_box(s->d, s->boxes.data() + i, s->T, s->R, s->stream);
     for (size_t i = 0; i + 6 <= s->spheres.size(); i += 6)
         launch_reinject_sphere(s->d, s->spheres.data() + i, s->T, s->R, s->stream);
     ZQ_HIP(hipGetLastError(), "zq3d_reinject");
-    return ZQ3D_OK;
+    return t.done("zq3d_reinject");
 }
 
 extern "C" int zq3d_step(zq3d_sim* s, const float* extra, zq3d_proj_stats* st) {
@@ -826,7 +903,31 @@ extern "C" int zq3d_step(zq3d_sim* s, const float* extra, zq3d_proj_stats* st) {
     if ((rc = zq3d_advect_velocity(s))) return rc;
     if ((rc = zq3d_project(s, st))) return rc;
     if ((rc = zq3d_advect_scalar(s))) return rc;
-    return zq3d_reinject(s);
+    if ((rc = zq3d_reinject(s))) return rc;
+    s->tm.steps++;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_set_profiling(zq3d_sim* s, int on) {
+    const char* where = "zq3d_set_profiling";
+    if (!s) { set_err(where, "null sim"); return ZQ3D_ERR_ARG; }
+    if (on) {
+        for (hipEvent_t* e : {&s->ev[0], &s->ev[1], &s->sorprof.a, &s->sorprof.b})
+            if (!*e) ZQ_HIP(hipEventCreate(e), where);
+    }
+    s->prof = on != 0;
+    s->tm = zq3d_timing{};
+    s->sorprof.ms = 0.0;
+    s->sorprof.launches = 0;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_get_timing(zq3d_sim* s, zq3d_timing* out) {
+    if (!s || !out) { set_err("zq3d_get_timing", "null argument"); return ZQ3D_ERR_ARG; }
+    *out = s->tm;
+    out->ms_sor = s->sorprof.ms;
+    out->sor_launches = s->sorprof.launches;
+    return ZQ3D_OK;
 }
 
 extern "C" int zq3d_mac_to_regular(zq3d_sim* s) {

@@ -39,27 +39,33 @@ void launch_advect_scalar(const Dims& d, const float* u, const float* v, const f
                           const AdvParams& ap, float4* vel4, hipStream_t s);
 
 // ---- projection ----
-// Neighbour counts, FP32 divergence b, zero p at uncounted cells; optional FP64 sum/count of the
-// divergence over fluid cells (for the closed-domain mean).
+// Doubles needed by the reduction partials of a grid (per-block sums / maxima).
+int reduction_partials(const Dims& d);
+// Neighbour counts, FP32 divergence b (or FP64 b64), zero p at uncounted cells; when part is given,
+// per-block FP64 sum/count of the divergence over fluid cells (closed-domain mean).
 void launch_proj_setup(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
                        int open_mask, uint8_t* nc, float* b, double* b64, float* p_pad, double* p64_pad,
-                       double* sum_cnt, hipStream_t s);
-// mean32 = float(sum/cnt) (or 0 when want == 0); mean64 likewise in double
-void launch_mean_finalize(const double* sum_cnt, float* mean32, double* mean64, int want, hipStream_t s);
-void launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float omega,
-                int iters, hipStream_t s);
+                       double* part, hipStream_t s);
+// mean32/mean64 = sum/count of the setup partials (0 when want == 0)
+void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s);
+// iters red-black SOR iterations starting from p_pad.  With p_alt (same shape, ghost layer zero)
+// the fused one-pass kernel ping-pongs between the two; returns the buffer holding the result.
+float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
+                  float omega, int iters, hipStream_t s);
+// kernel launches per SOR iteration (1 fused, 2 single-colour)
+int sor_launches_per_iter(const Dims& d);
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s);
 void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, float* u, float* v,
                       float* w, hipStream_t s);
-// r = (b64 - mean64) - (sum_nb p - n p); writes r32, max|r| (as uint64 bits, atomicMax) into *rmax
+// r = (b64 - mean64) - (sum_nb p - n p); writes r32 and max|r| (bits of a non-negative double) to *rmax
 void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
-                       const double* p64_pad, float* r32, unsigned long long* rmax, hipStream_t s);
+                       const double* p64_pad, float* r32, double* part, unsigned long long* rmax, hipStream_t s);
 // p64 += e, e = 0 on counted cells
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s);
-// max |div64 - mean64| over counted cells into *dmax (uint64 bits of a non-negative double)
+// max |div64 - mean64| over counted cells into *dmax (bits of a non-negative double)
 void launch_max_div64(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* nc,
-                      const double* mean64, unsigned long long* dmax, hipStream_t s);
+                      const double* mean64, double* part, unsigned long long* dmax, hipStream_t s);
 
 // ---- sources, obstacles, fills ----
 void launch_reinject_box(const Dims& d, const float* box8, float* T, float* rho, hipStream_t s);

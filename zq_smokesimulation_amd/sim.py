@@ -155,5 +155,13 @@ class SmokeSim3D:
     def synchronize(self):
         check(_lib.lib().zq3d_synchronize(self._h), "synchronize")
 
+    def set_profiling(self, on=True):
+        check(_lib.lib().zq3d_set_profiling(self._h, int(on)), "set_profiling")
+
+    def timing(self):
+        t = _lib.Timing()
+        check(_lib.lib().zq3d_get_timing(self._h, C.byref(t)), "get_timing")
+        return t.as_dict()
+
     def set_stream(self, stream_handle):
         check(_lib.lib().zq3d_set_stream(self._h, stream_handle), "set_stream")
