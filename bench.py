@@ -11,8 +11,9 @@ temperature noise, dt 0.02, 20 advection substeps.  Inputs are resident in HBM b
 timed region.
 
 Prints one JSON line.  value = voxels * steps / wall time (Mvoxel-steps/s).  roofline: the
-single-colour SOR sweep kernel (the dominant kernel), algorithmic bytes 13 B/voxel per red+black
-iteration (6.5 B/voxel per launch) over its event-timed mean launch time.  cpu_baseline: the
+red-black SOR sweep kernel the autotuner picked (the dominant kernel), algorithmic bytes 13 B/voxel
+per red+black iteration (6.5 B/voxel per single-colour launch) over its event-timed mean launch
+time.  cpu_baseline: the
 CPU oracle on a bounded sample of the same step (see --help).
 """
 import argparse
@@ -41,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "sor_sweep_256_traffic.txt"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed region run a plain 512 MiB device copy (3 launches) whose counter "
+                         "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
     return ap.parse_args()
 
 
@@ -78,32 +82,35 @@ def read_traffic(path):
 
 
 def cpu_baseline(state, W, H, D, inner, outer_mean):
-    """Oracle step on the GPU's own state: runs with 1 and 2 refinement passes give the fixed cost
-    of a step and the cost of one pass; the per-step time uses the GPU's mean pass count."""
+    """The CPU oracle on a bounded sample of one step, on the GPU's own state:
+    (1) a whole step whose projection stops after its first FP64 residual (no refinement pass),
+    (2) `inner` red-black SOR iterations of the oracle on the same grid (one refinement pass's sweeps);
+    step time = (1) + (GPU's mean passes per step) x (2).  The passes' residual/update work is left out
+    of (2), which favours the CPU slightly."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import zq_oracle as O
     p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=0.99, aT=0.99, aR=0.99,
              adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1, omega=1.8, sor_iters=inner, mixed=1,
-             max_outer=1, tol=0.0)
-    times = []
-    for passes in (1, 2):
-        s = {k: a.copy() for k, a in state.items()}
-        p["max_outer"] = passes
-        t0 = time.perf_counter()
-        O.step(p, s["u"], s["v"], s["w"], s["T"], s["rho"], s["occ"], p64=s["p64"], spheres=[SOURCE])
-        times.append(time.perf_counter() - t0)
-    per_pass = max(times[1] - times[0], 0.0)
-    fixed = max(times[0] - per_pass, 0.0)
-    step_s = fixed + outer_mean * per_pass
+             max_outer=0, tol=0.0)
+    s = {k: a.copy() for k, a in state.items()}
+    t0 = time.perf_counter()
+    O.step(p, s["u"], s["v"], s["w"], s["T"], s["rho"], s["occ"], p64=s["p64"], spheres=[SOURCE])
+    t_fixed = time.perf_counter() - t0
+    nc = O.sor_count(state["occ"], 0)
+    rhs = O.divergence(state["u"], state["v"], state["w"], state["occ"])
+    e = np.zeros((D + 2, H + 2, W + 2), np.float32)
+    t0 = time.perf_counter()
+    O.sor(nc, rhs, e, 1.8, inner)
+    t_pass = time.perf_counter() - t0
+    step_s = t_fixed + outer_mean * t_pass
     return {
         "value": (W * H * D) / step_s / 1e6,
         "unit": "Mvoxel-steps/s",
         "cores": int(O.lib().orc_num_threads()),
         "kind": "port",
-        "sample": ("oracle step at %dx%dx%d on the GPU's state after the timed steps, run with 1 and 2 "
-                   "refinement passes of %d SOR sweeps (%.2f s, %.2f s); step time = fixed %.2f s + %.1f passes "
-                   "(GPU mean) x %.3f s, extrapolated" % (W, H, D, inner, times[0], times[1], fixed, outer_mean,
-                                                         per_pass)),
+        "sample": ("oracle at %dx%dx%d on the GPU's state after the timed steps: one step with no refinement pass "
+                   "(%.2f s) + %d SOR iterations (%.3f s); step time = %.2f + %.1f passes (GPU mean per step) x %.3f = "
+                   "%.2f s" % (W, H, D, t_fixed, inner, t_pass, t_fixed, outer_mean, t_pass, step_s)),
     }
 
 
@@ -136,7 +143,9 @@ def main():
         step_idx += 1
     sim.synchronize()
     torch.cuda.synchronize()
-    print("bench: warmup done", file=sys.stderr, flush=True)
+    tune = sim.autotune_sor(20)
+    best = tune["candidates"][tune["best"]] if tune["best"] >= 0 else None
+    print("bench: warmup done; sweep tile autotune %s" % json.dumps(tune), file=sys.stderr, flush=True)
     sim.set_profiling(True)
     stats = []
     torch.cuda.synchronize()
@@ -150,6 +159,14 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     tm = sim.timing()
+    sor_kernel = sim.sor_kernel()
+    if args.calibrate_copy:
+        a = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        bcopy = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=bcopy)
+        torch.cuda.synchronize()
+        del a, bcopy
     wall = t1 - t0
     ms_per_step = 1e3 * wall / args.steps
     value = N * args.steps / wall / 1e6
@@ -189,11 +206,12 @@ def main():
             "advection_substeps": 20,
             "sor_omega": 1.8,
             "sor_sweeps_per_pass": args.inner,
+            "sor_tile_autotuned": best,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_sor_fused (one red-black SOR iteration per launch)",
+            "kernel": sor_kernel,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -226,6 +226,25 @@ typedef struct zq3d_timing {
 int zq3d_set_profiling(zq3d_sim* sim, int on);
 int zq3d_get_timing(zq3d_sim* sim, zq3d_timing* out);
 
+/* Tile-shape autotuner of the one-pass red-black sweep kernel (no reference counterpart): each
+ * candidate (ty rows per workgroup, zc planes per chunk) valid for the grid is timed for `iters`
+ * iterations with HIP events on the simulation's stream and occupancy; the fastest is kept for
+ * this simulation.  Clobbers the projection scratch (not the fields). */
+#define ZQ3D_TUNE_MAX 16
+typedef struct zq3d_tune_report {
+    int count;                       /* candidates timed */
+    int best;                        /* index of the chosen one (-1: the grid is too wide for the fused kernel) */
+    int ty[ZQ3D_TUNE_MAX];
+    int zc[ZQ3D_TUNE_MAX];
+    float us_per_iter[ZQ3D_TUNE_MAX];
+} zq3d_tune_report;
+int zq3d_autotune_sor(zq3d_sim* sim, int iters, zq3d_tune_report* report);
+/* explicit shape; (0, 0) restores the default, (-1, 0) selects single-colour sweeps on
+ * colour-split storage */
+int zq3d_set_sor_shape(zq3d_sim* sim, int ty, int zc);
+/* name of the sweep kernel the current shape runs ("k_sor_split", "k_sor_fused" or "k_sor_half") */
+const char* zq3d_sor_kernel(zq3d_sim* sim);
+
 /* message of the last failed call on this thread ("" if none) */
 const char* zq3d_last_error(void);
 /* frees the device workspace cached by the Part 1/2 host-pointer functions */

@@ -109,3 +109,27 @@ def test_step_mixed_precision_moving_sphere():
         scale = max(float(np.abs(a).max()), 1e-6)
         err = float(np.abs(a - b).max())
         assert err <= 1e-5 * scale, "%s: max err %g (scale %g)" % (name, err, scale)
+
+
+def test_every_sweep_tile_shape_is_bit_identical():
+    """The autotuner may pick any fused-sweep tile shape: all give the oracle's iteration exactly."""
+    import zq_smokesimulation_amd as z
+    W, H, D = 36, 30, 40
+    rng = np.random.default_rng(3)
+    from util import faces, occupancy
+    u, v, w = faces(rng, W, H, D)
+    occ = occupancy(rng, W, H, D, 0.1, shell=False)
+    ref = [a.copy() for a in (u, v, w)]
+    O.project(*ref, occ, O.OPEN_YHI, 0.0, 1.8, 17)
+    sim = z.SmokeSim3D(W, H, D, sor_open_mask=O.OPEN_YHI, sor_closed_mean=0, sor_omega=1.8, sor_iters=17)
+    rep = sim.autotune_sor(3)
+    assert rep["best"] >= 0 and len(rep["candidates"]) >= 6
+    for c in rep["candidates"]:
+        print("shape", c, flush=True)
+        sim.set_sor_shape(c["ty"], c["zc"])
+        sim.set_state(u=u, v=v, w=w, occ=occ)
+        sim.project()
+        g = sim.get_state()
+        for name, a in zip("uvw", ref):
+            assert np.array_equal(g[name].view(np.uint32), a.view(np.uint32)), (c, name)
+    sim.close()

@@ -83,6 +83,25 @@ class Timing(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+TUNE_MAX = 16
+
+
+class TuneReport(C.Structure):
+    _fields_ = [
+        ("count", C.c_int),
+        ("best", C.c_int),
+        ("ty", C.c_int * TUNE_MAX),
+        ("zc", C.c_int * TUNE_MAX),
+        ("us_per_iter", C.c_float * TUNE_MAX),
+    ]
+
+    def as_dict(self):
+        n = self.count
+        return {"best": self.best,
+                "candidates": [{"ty": self.ty[i], "zc": self.zc[i], "us_per_iter": round(self.us_per_iter[i], 3)}
+                               for i in range(n)]}
+
+
 class ZQ3DError(RuntimeError):
     pass
 
@@ -136,6 +155,10 @@ def _load():
     L.zq3d_mac_to_regular.argtypes = [vp]
     L.zq3d_set_profiling.argtypes = [vp, i]
     L.zq3d_get_timing.argtypes = [vp, C.POINTER(Timing)]
+    L.zq3d_autotune_sor.argtypes = [vp, i, C.POINTER(TuneReport)]
+    L.zq3d_set_sor_shape.argtypes = [vp, i, i]
+    L.zq3d_sor_kernel.argtypes = [vp]
+    L.zq3d_sor_kernel.restype = C.c_char_p
     L.zq3d_last_error.argtypes = []
     L.zq3d_last_error.restype = C.c_char_p
     L.zq3d_release_workspace.argtypes = []

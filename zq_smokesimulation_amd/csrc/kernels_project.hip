@@ -349,6 +349,80 @@ __global__ __launch_bounds__(256) void k_sor_fused(const uint8_t* __restrict__ n
     }
 }
 
+// ---- colour-split layout ----
+// Colour c = (x+y+z)&1 cells of row (y,z) sit at x = 2*xh + par with par = (c+y+z)&1, stored at
+// xh (Wh = (W+1)/2 slots per row; a slot past the row end is not a cell and holds 0).  Every
+// neighbour of a colour-c cell has the other colour: x-1 and x+1 are slots xh-1+par and xh+par of
+// the same row, y+-1 and z+-1 are slot xh of those rows.  Pressure arrays are padded by one slot
+// (zero) on every side.
+
+__device__ __forceinline__ int split_q(int Wh, int H, int xh, int y, int z) {
+    return ((z + 1) * (H + 2) + (y + 1)) * (Wh + 2) + (xh + 1);
+}
+
+// interleaved (nc, b, padded p) -> split (n_c, b_c, p_c), both colours; blockIdx.z = 2*z + colour
+__global__ __launch_bounds__(BX* BY) void k_split_in(const uint8_t* __restrict__ nc, const float* __restrict__ b,
+                                                     const float* __restrict__ p, uint8_t* __restrict__ n0,
+                                                     uint8_t* __restrict__ n1, float* __restrict__ b0,
+                                                     float* __restrict__ b1, float* __restrict__ p0,
+                                                     float* __restrict__ p1, Dims d, int Wh) {
+    const int xh = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int z = blockIdx.z >> 1, c = blockIdx.z & 1;
+    if (xh >= Wh || y >= d.H) return;
+    const int x = 2 * xh + ((c + y + z) & 1);
+    const int oh = (z * d.H + y) * Wh + xh;
+    const int qh = split_q(Wh, d.H, xh, y, z);
+    const bool ok = x < d.W;
+    const uint8_t n = ok ? nc[cidx(d, x, y, z)] : (uint8_t)0;
+    const float bv = ok ? b[cidx(d, x, y, z)] : 0.0f;
+    const float pv = ok ? p[pidx(d, x, y, z)] : 0.0f;
+    (c ? n1 : n0)[oh] = n;
+    (c ? b1 : b0)[oh] = bv;
+    (c ? p1 : p0)[qh] = pv;
+}
+
+// split padded p_c -> interleaved padded p (interior cells)
+__global__ __launch_bounds__(BX* BY) void k_split_out(const float* __restrict__ p0, const float* __restrict__ p1,
+                                                      float* __restrict__ p, Dims d, int Wh) {
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const int c = (x + y + z) & 1;
+    p[pidx(d, x, y, z)] = (c ? p1 : p0)[split_q(Wh, d.H, x >> 1, y, z)];
+}
+
+// One colour of an SOR iteration on split storage: pc (this colour) updated in place from po.
+// 1-D grid; each XCD (block b runs on XCD b % 8) receives a contiguous z-slab of tiles so the
+// plane above and below a tile are usually in the same L2.
+__global__ __launch_bounds__(BX* BY) void k_sor_split(const uint8_t* __restrict__ nn, const float* __restrict__ bb,
+                                                      const float* __restrict__ mean32, float* __restrict__ pc,
+                                                      const float* __restrict__ po, Dims d, int Wh, int gx, int gy,
+                                                      float omega, float omw, int colour) {
+    const int per = gridDim.x >> 3;
+    const int bid = blockIdx.x;
+    const int tile = (bid & 7) * per + (bid >> 3);
+    const int bx = tile % gx;
+    const int rest = tile / gx;
+    const int by = rest % gy;
+    const int z = rest / gy;
+    const int xh = bx * BX + (threadIdx.x & (BX - 1));
+    const int y = by * BY + (threadIdx.x >> 6);
+    if (z >= d.D || xh >= Wh || y >= d.H) return;
+    const int par = (colour + y + z) & 1;
+    if (2 * xh + par >= d.W) return;
+    const int oh = (z * d.H + y) * Wh + xh;
+    const int n = nn[oh];
+    if (!n) return;
+    const int SP = Wh + 2, SZ = SP * (d.H + 2);
+    const int q = split_q(Wh, d.H, xh, y, z);
+    const float s = po[q - 1 + par] + po[q + par] + po[q - SP] + po[q + SP] + po[q - SZ] + po[q + SZ];
+    const float bv = bb[oh] - *mean32;
+    const float pg = (s - bv) * rcp_count(n);
+    pc[q] = omw * pc[q] + omega * pg;
+}
+
 // Face correction u -= p_r - p_l (ZQ_SmokeSimulationUtils3D.cpp:2815-2861 open form, :2888-2922
 // closed form).  P = float (plain) or double (mixed precision; the face is rounded once).
 template <int AXIS, class P>
@@ -442,16 +516,19 @@ int cell_blocks(const Dims& d) {
     return (int)(g.x * g.y * g.z);
 }
 
-// Fused-kernel launch shape: TY rows per block, ZC planes per chunk; 0 tiles when it cannot run.
+// Fused-kernel launch shape: TY rows per block, ZC planes per chunk; nblocks 0 when it cannot run.
 struct FusedShape {
     int ty, zc, ntiles_y, nchunks, nblocks;
     size_t lds;
 };
 
-FusedShape fused_shape(const Dims& d) {
+bool shape_supported(int ty) { return ty == 2 || ty == 4 || ty == 8; }
+
+FusedShape fused_shape(const Dims& d, SorShape want) {
     FusedShape f{};
-    f.ty = 4;
-    f.zc = 16;
+    f.ty = want.ty ? want.ty : 4;
+    f.zc = want.zc ? want.zc : 16;
+    if (!shape_supported(f.ty) || f.zc < 1) return FusedShape{};
     f.lds = (size_t)4 * (f.ty + 4) * (d.W + 2) * sizeof(float);
     if (f.lds > 64 * 1024 || d.W > 512) return FusedShape{};
     f.ntiles_y = (d.H + f.ty - 1) / f.ty;
@@ -459,6 +536,17 @@ FusedShape fused_shape(const Dims& d) {
     const int tiles = f.ntiles_y * f.nchunks;
     f.nblocks = (tiles + 7) & ~7;  // multiple of 8 for the XCD numbering; spare blocks exit
     return f;
+}
+
+template <int TY>
+void launch_fused(const FusedShape& f, const Dims& d, const uint8_t* nc, const float* b, const float* mean32,
+                  const float* src, float* dst, float omega, float omw, hipStream_t s) {
+    if (d.W <= 256)
+        hipLaunchKernelGGL((k_sor_fused<TY, 1>), dim3(f.nblocks), dim3(256), f.lds, s, nc, b, mean32, src, dst, d, omega,
+                           omw, f.zc, f.ntiles_y);
+    else
+        hipLaunchKernelGGL((k_sor_fused<TY, 2>), dim3(f.nblocks), dim3(256), f.lds, s, nc, b, mean32, src, dst, d, omega,
+                           omw, f.zc, f.ntiles_y);
 }
 
 }  // namespace
@@ -480,24 +568,42 @@ void launch_mean_finalize(const Dims& d, const double* part, float* mean32, doub
     hipLaunchKernelGGL(k_finish_mean, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), mean32, mean64, want);
 }
 
+size_t split_padded_size(const Dims& d) { return (size_t)((d.W + 1) / 2 + 2) * (d.H + 2) * (d.D + 2); }
+size_t split_cell_size(const Dims& d) { return (size_t)((d.W + 1) / 2) * d.H * d.D; }
+
 float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
-                  float omega, int iters, hipStream_t s) {
+                  float omega, int iters, hipStream_t s, SorShape shape, const SplitBufs* sp, hipEvent_t ev0,
+                  hipEvent_t ev1) {
     const float omw = 1.0f - omega;
-    const FusedShape f = fused_shape(d);
+    if (shape.ty == SOR_SPLIT && sp) {
+        const int Wh = (d.W + 1) / 2;
+        hipLaunchKernelGGL(k_split_in, dim3((Wh + BX - 1) / BX, (d.H + BY - 1) / BY, 2 * d.D), dim3(BX, BY), 0, s, nc,
+                           b, p_pad, sp->n[0], sp->n[1], sp->b[0], sp->b[1], sp->p[0], sp->p[1], d, Wh);
+        const int gx = (Wh + BX - 1) / BX, gy = (d.H + BY - 1) / BY;
+        const int nb = (gx * gy * d.D + 7) & ~7;
+        if (ev0) (void)hipEventRecord(ev0, s);
+        for (int it = 0; it < iters; it++)
+            for (int c = 0; c < 2; c++)
+                hipLaunchKernelGGL(k_sor_split, dim3(nb), dim3(BX * BY), 0, s, sp->n[c], sp->b[c], mean32, sp->p[c],
+                                   sp->p[1 - c], d, Wh, gx, gy, omega, omw, c);
+        if (ev1) (void)hipEventRecord(ev1, s);
+        hipLaunchKernelGGL(k_split_out, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, sp->p[0], sp->p[1], p_pad, d, Wh);
+        return p_pad;
+    }
+    if (ev0) (void)hipEventRecord(ev0, s);
+    const FusedShape f = fused_shape(d, shape);
     if (f.nblocks > 0 && p_alt) {
         float* src = p_pad;
         float* dst = p_alt;
         for (int it = 0; it < iters; it++) {
-            if (d.W <= 256)
-                hipLaunchKernelGGL((k_sor_fused<4, 1>), dim3(f.nblocks), dim3(256), f.lds, s, nc, b, mean32, src, dst, d,
-                                   omega, omw, f.zc, f.ntiles_y);
-            else
-                hipLaunchKernelGGL((k_sor_fused<4, 2>), dim3(f.nblocks), dim3(256), f.lds, s, nc, b, mean32, src, dst, d,
-                                   omega, omw, f.zc, f.ntiles_y);
+            if (f.ty == 2) launch_fused<2>(f, d, nc, b, mean32, src, dst, omega, omw, s);
+            else if (f.ty == 8) launch_fused<8>(f, d, nc, b, mean32, src, dst, omega, omw, s);
+            else launch_fused<4>(f, d, nc, b, mean32, src, dst, omega, omw, s);
             float* t = src;
             src = dst;
             dst = t;
         }
+        if (ev1) (void)hipEventRecord(ev1, s);
         return src;
     }
     const dim3 g = grid_for((d.W + 1) / 2, d.H, d.D);
@@ -505,10 +611,32 @@ float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float*
         hipLaunchKernelGGL(k_sor_half, g, dim3(BX, BY), 0, s, nc, b, mean32, p_pad, d, omega, omw, 0);
         hipLaunchKernelGGL(k_sor_half, g, dim3(BX, BY), 0, s, nc, b, mean32, p_pad, d, omega, omw, 1);
     }
+    if (ev1) (void)hipEventRecord(ev1, s);
     return p_pad;
 }
 
-int sor_launches_per_iter(const Dims& d) { return fused_shape(d).nblocks > 0 ? 1 : 2; }
+const char* sor_kernel_name(const Dims& d, SorShape shape) {
+    if (shape.ty == SOR_SPLIT) return "k_sor_split";
+    return fused_shape(d, shape).nblocks > 0 ? "k_sor_fused" : "k_sor_half";
+}
+
+int sor_launches_per_iter(const Dims& d, SorShape shape) {
+    if (shape.ty == SOR_SPLIT) return 2;
+    return fused_shape(d, shape).nblocks > 0 ? 1 : 2;
+}
+
+int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
+    static const int tys[] = {2, 4, 8};
+    static const int zcs[] = {8, 16, 32, 64};
+    int n = 0;
+    for (int ty : tys)
+        for (int zc : zcs) {
+            const SorShape sh{ty, zc};
+            if (fused_shape(d, sh).nblocks > 0 && n < max) out[n++] = sh;
+        }
+    if (n < max) out[n++] = SorShape{SOR_SPLIT, 0};
+    return n;
+}
 
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s) {

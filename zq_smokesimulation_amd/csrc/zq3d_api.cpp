@@ -154,25 +154,27 @@ struct ProjBufs {
     double* p64;
     double* part;
     Scalars* sc;
+    SorShape shape;  // sweep algorithm / tile shape ({0, 0}: default)
+    SplitBufs split; // colour-split scratch (used when shape.ty == SOR_SPLIT)
 };
 
 // launch_sor, timed when pf is set (the sweeps are the only work between the two events)
 hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p, float* p_alt,
-                     float omega, int iters, hipStream_t s, SorProf* pf, float** result) {
+                     float omega, int iters, hipStream_t s, SorProf* pf, float** result, SorShape shape,
+                     const SplitBufs* split) {
     if (!pf) {
-        *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s);
+        *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s, shape, split);
         return hipGetLastError();
     }
     Errs E;
-    E(hipEventRecord(pf->a, s));
-    *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s);
+    // the events bracket the sweep launches only (not the split-layout conversions)
+    *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s, shape, split, pf->a, pf->b);
     E(hipGetLastError());
-    E(hipEventRecord(pf->b, s));
     E(hipEventSynchronize(pf->b));
     float ms = 0.0f;
     E(hipEventElapsedTime(&ms, pf->a, pf->b));
     pf->ms += ms;
-    pf->launches += (long long)sor_launches_per_iter(d) * iters;
+    pf->launches += (long long)sor_launches_per_iter(d, shape) * iters;
     return E.e;
 }
 
@@ -201,7 +203,8 @@ int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
         launch_fill_f64(&sc->mean64, 1, (double)mean, s);
     }
     float* res = nullptr;
-    ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->mean32, B.p, B.p_alt, o.omega, o.iters, s, pf, &res), "project");
+    ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->mean32, B.p, B.p_alt, o.omega, o.iters, s, pf, &res, B.shape, &B.split),
+           "project");
     launch_correct(d, occ, o.open_mask, res, u, v, w, s);
     if (st) {
         launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
@@ -237,7 +240,8 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
         if (int rc = read_double_bits(&sc->rmax, &rmax, s, "project_mixed")) return rc;
         if (!(rmax > o.tol) || outer >= o.max_outer) break;
         float* res = nullptr;
-        ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res), "project_mixed");
+        ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res, B.shape, &B.split),
+               "project_mixed");
         if (res != e) std::swap(e, e_alt);
         launch_update_p64(d, B.nc, e, B.p64, s);  // also zeroes e for the next pass
         outer++;
@@ -491,8 +495,9 @@ float solve_host(float* u, float* v, float* w, const bool* occupy, int W_, int H
     E(hipMemcpy(dv, v, z.Nv * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(dw, w, z.Nw * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(docc, occupy, z.N, hipMemcpyHostToDevice));
-    const ProjBufs B{(uint8_t*)W.ptr[S_NC], (float*)W.ptr[S_B],     (float*)W.ptr[S_P],    (float*)W.ptr[S_P2],
-                     (double*)W.ptr[S_B64], (double*)W.ptr[S_P64], (double*)W.ptr[S_PART], (Scalars*)W.ptr[S_SCAL]};
+    const ProjBufs B{(uint8_t*)W.ptr[S_NC],  (float*)W.ptr[S_B],      (float*)W.ptr[S_P],
+                     (float*)W.ptr[S_P2],    (double*)W.ptr[S_B64],   (double*)W.ptr[S_P64],
+                     (double*)W.ptr[S_PART], (Scalars*)W.ptr[S_SCAL], SorShape{0, 0}, SplitBufs{}};
     launch_fill_f32(B.p_alt, z.Np, 0.0f, 0);  // zero ghost layer of the second sweep buffer
     int rc;
     if (o.mixed_precision) {
@@ -590,6 +595,9 @@ struct zq3d_sim {
     float* p = nullptr;           // padded FP32 pressure / correction e of the mixed solve
     float* p_alt = nullptr;       // second sweep buffer (ping-pong), zero ghost layer
     double* part = nullptr;       // reduction partials
+    float* split_p = nullptr;     // colour-split pressures (2 x padded), right-hand sides, counts
+    float* split_b = nullptr;
+    uint8_t* split_n = nullptr;
     double* b64 = nullptr;
     double* p64 = nullptr;
     Scalars* sc = nullptr;
@@ -599,6 +607,7 @@ struct zq3d_sim {
     hipEvent_t ev[2] = {nullptr, nullptr};
     SorProf sorprof;
     zq3d_timing tm{};
+    SorShape shape{0, 0};  // fused sweep tile shape chosen by zq3d_autotune_sor / zq3d_set_sor_shape
 };
 
 namespace {
@@ -623,7 +632,11 @@ int field_info(zq3d_sim* s, int field, void** ptr, size_t* bytes) {
     }
 }
 
-ProjBufs proj_bufs(zq3d_sim* s) { return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc}; }
+ProjBufs proj_bufs(zq3d_sim* s) {
+    const size_t sp = split_padded_size(s->d), sc = split_cell_size(s->d);
+    const SplitBufs split{{s->split_p, s->split_p + sp}, {s->split_b, s->split_b + sc}, {s->split_n, s->split_n + sc}};
+    return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc, s->shape, split};
+}
 
 // Times one stage of a simulation with an event pair when profiling is on.
 struct StageTimer {
@@ -718,6 +731,9 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     A(dalloc(&s->p, z.Np * 4));
     A(dalloc(&s->p_alt, z.Np * 4));
     A(dalloc(&s->part, (size_t)reduction_partials(s->d) * 8));
+    A(dalloc(&s->split_p, 2 * split_padded_size(s->d) * 4));
+    A(dalloc(&s->split_b, 2 * split_cell_size(s->d) * 4));
+    A(dalloc(&s->split_n, 2 * split_cell_size(s->d)));
     A(dalloc(&s->sc, sizeof(Scalars)));
     if (cfg->sor.mixed_precision) {
         A(dalloc(&s->b64, z.N * 8));
@@ -737,6 +753,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     launch_fill_u8(s->occ, z.N, 0, st);
     launch_fill_f32(s->p, z.Np, 0.0f, st);
     launch_fill_f32(s->p_alt, z.Np, 0.0f, st);
+    launch_fill_f32(s->split_p, 2 * split_padded_size(s->d), 0.0f, st);
     if (s->p64) launch_fill_f64(s->p64, z.Np, 0.0, st);
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -751,6 +768,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
 extern "C" void zq3d_destroy(zq3d_sim* s) {
     if (!s) return;
     void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->nc, s->b, s->p, s->p_alt, s->part,
+                    s->split_p, s->split_b, s->split_n,
                     s->b64, s->p64, s->sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -920,6 +938,78 @@ extern "C" int zq3d_set_profiling(zq3d_sim* s, int on) {
     s->sorprof.ms = 0.0;
     s->sorprof.launches = 0;
     return ZQ3D_OK;
+}
+
+extern "C" int zq3d_set_sor_shape(zq3d_sim* s, int ty, int zc) {
+    if (!s || ty < SOR_SPLIT || zc < 0) { set_err("zq3d_set_sor_shape", "bad argument"); return ZQ3D_ERR_ARG; }
+    SorShape cand[32];
+    const int n = sor_shape_candidates(s->d, cand, 32);
+    bool ok = (ty == 0 && zc == 0);
+    for (int i = 0; i < n && !ok; i++) ok = cand[i].ty == ty && cand[i].zc == zc;
+    if (!ok) { set_err("zq3d_set_sor_shape", "shape not supported for this grid"); return ZQ3D_ERR_ARG; }
+    s->shape = SorShape{ty, zc};
+    return ZQ3D_OK;
+}
+
+// Times every fused-sweep tile shape valid for the grid on the simulation's own occupancy (the
+// sweep's cost depends on it only through skipped solid cells) and keeps the fastest.
+extern "C" int zq3d_autotune_sor(zq3d_sim* s, int iters, zq3d_tune_report* rep) {
+    const char* where = "zq3d_autotune_sor";
+    if (!s || iters < 1) { set_err(where, "bad argument"); return ZQ3D_ERR_ARG; }
+    const Sizes z(s->d);
+    SorShape cand[ZQ3D_TUNE_MAX];
+    const int n = sor_shape_candidates(s->d, cand, ZQ3D_TUNE_MAX);
+    if (rep) std::memset(rep, 0, sizeof(*rep));
+    if (n == 0) {
+        s->shape = SorShape{0, 0};
+        if (rep) rep->best = -1;
+        return ZQ3D_OK;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    ZQ_HIP(hipEventCreate(&a), where);
+    ZQ_HIP(hipEventCreate(&b), where);
+    launch_fill_f32(&s->sc->mean32, 1, 0.0f, s->stream);
+    launch_proj_setup(s->d, s->u, s->v, s->w, s->occ, s->cfg.sor.open_mask, s->nc, s->b, nullptr, s->p, nullptr,
+                      nullptr, s->stream);
+    int best = -1;
+    float best_us = 0.0f;
+    Errs E;
+    const ProjBufs B = proj_bufs(s);
+    for (int i = 0; i < n; i++) {
+        launch_fill_f32(s->p, z.Np, 0.0f, s->stream);
+        launch_sor(s->d, s->nc, s->b, &s->sc->mean32, s->p, s->p_alt, 1.8f, 2, s->stream, cand[i], &B.split);
+        E(hipEventRecord(a, s->stream));
+        launch_sor(s->d, s->nc, s->b, &s->sc->mean32, s->p, s->p_alt, 1.8f, iters, s->stream, cand[i], &B.split);
+        E(hipEventRecord(b, s->stream));
+        E(hipEventSynchronize(b));
+        float ms = 0.0f;
+        E(hipEventElapsedTime(&ms, a, b));
+        const float us = 1e3f * ms / iters;
+        if (rep) {
+            rep->ty[i] = cand[i].ty;
+            rep->zc[i] = cand[i].zc;
+            rep->us_per_iter[i] = us;
+        }
+        if (best < 0 || us < best_us) {
+            best = i;
+            best_us = us;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    launch_fill_f32(s->p, z.Np, 0.0f, s->stream);
+    if (E.e != hipSuccess) { set_err(where, hipGetErrorString(E.e)); return ZQ3D_ERR_HIP; }
+    s->shape = cand[best];
+    if (rep) {
+        rep->count = n;
+        rep->best = best;
+    }
+    return ZQ3D_OK;
+}
+
+extern "C" const char* zq3d_sor_kernel(zq3d_sim* s) {
+    if (!s) return "";
+    return sor_kernel_name(s->d, s->shape);
 }
 
 extern "C" int zq3d_get_timing(zq3d_sim* s, zq3d_timing* out) {

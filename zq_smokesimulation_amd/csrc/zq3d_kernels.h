@@ -48,12 +48,34 @@ void launch_proj_setup(const Dims& d, const float* u, const float* v, const floa
                        double* part, hipStream_t s);
 // mean32/mean64 = sum/count of the setup partials (0 when want == 0)
 void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s);
+// Sweep algorithm / tile shape: ty rows per block (2, 4 or 8) and zc planes per chunk of the fused
+// one-pass kernel; ty == SOR_SPLIT selects single-colour sweeps on colour-split storage; {0, 0}
+// selects the built-in default.
+constexpr int SOR_SPLIT = -1;
+struct SorShape {
+    int ty, zc;
+};
+// Colour-split scratch: per colour a padded pressure (split_padded_size floats), right-hand side
+// and counts (split_cell_size each).  Padding slots must hold 0.
+struct SplitBufs {
+    float* p[2];
+    float* b[2];
+    uint8_t* n[2];
+};
+size_t split_padded_size(const Dims& d);
+size_t split_cell_size(const Dims& d);
 // iters red-black SOR iterations starting from p_pad.  With p_alt (same shape, ghost layer zero)
-// the fused one-pass kernel ping-pongs between the two; returns the buffer holding the result.
+// the fused one-pass kernel ping-pongs between the two; the split algorithm converts to split
+// storage, sweeps, and converts back into p_pad.  Returns the buffer holding the result.
 float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
-                  float omega, int iters, hipStream_t s);
+                  float omega, int iters, hipStream_t s, SorShape shape = SorShape{0, 0},
+                  const SplitBufs* split = nullptr, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// name of the sweep kernel a shape runs
+const char* sor_kernel_name(const Dims& d, SorShape shape);
 // kernel launches per SOR iteration (1 fused, 2 single-colour)
-int sor_launches_per_iter(const Dims& d);
+int sor_launches_per_iter(const Dims& d, SorShape shape = SorShape{0, 0});
+// fused-kernel shapes valid for this grid (LDS ring within 64 KiB); returns the count
+int sor_shape_candidates(const Dims& d, SorShape* out, int max);
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s);
 void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, float* u, float* v,

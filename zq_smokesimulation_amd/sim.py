@@ -163,5 +163,17 @@ class SmokeSim3D:
         check(_lib.lib().zq3d_get_timing(self._h, C.byref(t)), "get_timing")
         return t.as_dict()
 
+    def autotune_sor(self, iters=20):
+        """Time every fused-sweep tile shape on this grid and keep the fastest; returns the report."""
+        rep = _lib.TuneReport()
+        check(_lib.lib().zq3d_autotune_sor(self._h, int(iters), C.byref(rep)), "autotune_sor")
+        return rep.as_dict()
+
+    def sor_kernel(self):
+        return _lib.lib().zq3d_sor_kernel(self._h).decode()
+
+    def set_sor_shape(self, ty, zc):
+        check(_lib.lib().zq3d_set_sor_shape(self._h, int(ty), int(zc)), "set_sor_shape")
+
     def set_stream(self, stream_handle):
         check(_lib.lib().zq3d_set_stream(self._h, stream_handle), "set_stream")
