@@ -133,6 +133,15 @@ float ZQ3D_SolvePoissonRedBlack_ex(float* mac_u, float* mac_v, float* mac_w, con
                                    const int height, const int depth, const zq3d_sor_opts* opts,
                                    zq3d_proj_stats* stats);
 
+/* No reference counterpart (extends AddForce3D): also applies the density term
+ * f_y -= gravityCoeff*rho (the reference's gravityCoeff, declared unused there) and adds an
+ * optional cell-centred force field (3 floats per cell) before the face scatter.  density and
+ * extra_force may be NULL (density is required when gravityCoeff != 0). */
+float ZQ3D_AddForce3D_ex(float* mac_u, float* mac_v, float* mac_w, const bool* occupy, const float* temperature,
+                         const float* density, const float* extra_force, const float deltah, const float deltat,
+                         const float buoyCoeff, const float gravityCoeff, const float confineCoeff, const float Tamb,
+                         const int width, const int height, const int depth);
+
 /* replaces the reference's MAC->cell conversion (ZQ_CUDA_MACtoRegular.cu:9-30 on the device, CPU
  * twin MACtoRegular4 at ZQ_SmokeSimulationUtils3D.cpp:355-374): vel4 gets (u,v,w,0) per cell. */
 float ZQ3D_MACtoRegular4(const float* mac_u, const float* mac_v, const float* mac_w, float* vel4, const int width,
@@ -244,6 +253,44 @@ int zq3d_autotune_sor(zq3d_sim* sim, int iters, zq3d_tune_report* report);
 int zq3d_set_sor_shape(zq3d_sim* sim, int ty, int zc);
 /* name of the sweep kernel the current shape runs ("k_sor_split", "k_sor_fused" or "k_sor_half") */
 const char* zq3d_sor_kernel(zq3d_sim* sim);
+
+/* ====================== Part 4: host-side scene set-up (no GPU) ====================== */
+
+/* A reference config file parsed by ZQ_SmokeSimulationPara3D::LoadFromFile
+ * (ZQ_SmokeSimulationUtils3D.cpp:78-352, same keys and quirks, plus the extension keys
+ * buoyancyCoeff gravityCoeff sphereSource omega mixedPrecision sorTol maxOuter openFaces wallFaces
+ * tempNoise deviceResident warmStart). */
+#define ZQ3D_MAX_SOURCES 16
+typedef struct zq3d_scene {
+    int struct_size;              /* sizeof(zq3d_scene) */
+    zq3d_config config;           /* grid, physics and projection as a device-resident config */
+    int frame_count;
+    int solver;                   /* GlobalSolverType3D value */
+    int wall_faces;               /* domain faces given a one-cell solid shell */
+    int internal_object;          /* 0 none, 1 sphere, 2 box, 3 oval, 4 special1 */
+    int use_moving_object;
+    int export_type;              /* 0 .di3, 1 .zqwv, 2 .zqci */
+    int device_resident;
+    int sub_size, max_iter, inner_iter, steps;
+    float vel_atten_coeff, temp_atten_coeff, dens_atten_coeff;
+    int adv_vel_type_para;        /* ZQ_SmokeSimulationPara3D::AdvectVelocityType value */
+    int adv_sca_type_para;        /* ZQ_SmokeSimulationPara3D::AdvectScalarType value */
+    int add_force_type;           /* ZQ_SmokeSimulationPara3D::AddForceType value */
+    int n_box_sources;
+    float box_sources[ZQ3D_MAX_SOURCES][8];
+    int n_sphere_sources;
+    float sphere_sources[ZQ3D_MAX_SOURCES][6];
+    float temp_noise;
+    unsigned noise_seed;
+    char moving_object_file[200];
+} zq3d_scene;
+/* 0 on success; nonzero when the file is missing or LoadFromFile rejects it */
+int zq3d_scene_load(const char* path, zq3d_scene* out);
+/* ZQ_InitFineGrid3D for the config at path: occupancy (W*H*D bytes) of the initial grid */
+int zq3d_scene_init_occupancy(const char* path, uint8_t* occ, size_t bytes);
+/* ZQ_ApplyMovingObject3D applied after `frames` object updates of the config's moving-object file,
+ * into caller arrays (occ W*H*D bytes, faces in the reference layout, updated in place) */
+int zq3d_scene_apply_moving_objects(const char* path, int frames, uint8_t* occ, float* u, float* v, float* w);
 
 /* message of the last failed call on this thread ("" if none) */
 const char* zq3d_last_error(void);

@@ -169,7 +169,7 @@ def _load():
 def lib():
     global _lib
     if _lib is None:
-        _lib = _load()
+        _lib = _bind_scene(_load())
     return _lib
 
 
@@ -186,3 +186,43 @@ def check(rc, what=""):
     if rc != OK:
         raise ZQ3DError("%s failed (code %d): %s" % (what, rc, last_error()))
     return rc
+
+
+MAX_SOURCES = 16
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_int),
+        ("config", Config),
+        ("frame_count", C.c_int),
+        ("solver", C.c_int),
+        ("wall_faces", C.c_int),
+        ("internal_object", C.c_int),
+        ("use_moving_object", C.c_int),
+        ("export_type", C.c_int),
+        ("device_resident", C.c_int),
+        ("sub_size", C.c_int), ("max_iter", C.c_int), ("inner_iter", C.c_int), ("steps", C.c_int),
+        ("vel_atten_coeff", C.c_float), ("temp_atten_coeff", C.c_float), ("dens_atten_coeff", C.c_float),
+        ("adv_vel_type_para", C.c_int),
+        ("adv_sca_type_para", C.c_int),
+        ("add_force_type", C.c_int),
+        ("n_box_sources", C.c_int),
+        ("box_sources", (C.c_float * 8) * MAX_SOURCES),
+        ("n_sphere_sources", C.c_int),
+        ("sphere_sources", (C.c_float * 6) * MAX_SOURCES),
+        ("temp_noise", C.c_float),
+        ("noise_seed", C.c_uint),
+        ("moving_object_file", C.c_char * 200),
+    ]
+
+
+def _bind_scene(L):
+    vp = C.c_void_p
+    L.zq3d_scene_load.argtypes = [C.c_char_p, C.POINTER(Scene)]
+    L.zq3d_scene_init_occupancy.argtypes = [C.c_char_p, vp, C.c_size_t]
+    L.zq3d_scene_apply_moving_objects.argtypes = [C.c_char_p, C.c_int, vp, vp, vp, vp]
+    L.ZQ3D_AddForce3D_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
+    L.ZQ3D_AddForce3D_ex.restype = C.c_float
+    return L

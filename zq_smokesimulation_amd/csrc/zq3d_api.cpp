@@ -425,6 +425,50 @@ extern "C" float AddForce3D(float* mac_u, float* mac_v, float* mac_w, const bool
     return ms;
 }
 
+extern "C" float ZQ3D_AddForce3D_ex(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
+                                    const float* temperature, const float* density, const float* extra_force,
+                                    const float deltah, const float deltat, const float buoyCoeff,
+                                    const float gravityCoeff, const float confineCoeff, const float Tamb,
+                                    const int width, const int height, const int depth) {
+    const char* where = "ZQ3D_AddForce3D_ex";
+    if (check_dims(width, height, depth, where)) return -1.0f;
+    if (!mac_u || !mac_v || !mac_w || !occupy || !temperature || (gravityCoeff != 0.0f && !density)) {
+        set_err(where, "null pointer");
+        return -1.0f;
+    }
+    const Dims d{width, height, depth};
+    const Sizes z(d);
+    Workspace& W = ws();
+    std::lock_guard<std::mutex> lk(W.mu);
+    Timer t;
+    Errs E;
+    if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
+        W.ensure(S_T, z.N * 4) || W.ensure(S_R, z.N * 4) || W.ensure(S_T2, z.N * 12) || W.ensure(S_VEL4, z.N * 16) ||
+        W.ensure(S_SCR, (z.Nu + z.Nv + z.Nw) * 4)) {
+        set_err(where, "device allocation failed");
+        return -1.0f;
+    }
+    E(hipMemcpy(W.ptr[S_U], mac_u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_V], mac_v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_W], mac_w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_OCC], occupy, z.N, hipMemcpyHostToDevice));
+    E(hipMemcpy(W.ptr[S_T], temperature, z.N * 4, hipMemcpyHostToDevice));
+    if (density) E(hipMemcpy(W.ptr[S_R], density, z.N * 4, hipMemcpyHostToDevice));
+    if (extra_force) E(hipMemcpy(W.ptr[S_T2], extra_force, z.N * 12, hipMemcpyHostToDevice));
+    const ForceParams fp{deltah, deltat, buoyCoeff, gravityCoeff, confineCoeff, Tamb};
+    launch_force_atten(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T],
+                       density ? (float*)W.ptr[S_R] : nullptr, (uint8_t*)W.ptr[S_OCC],
+                       extra_force ? (float*)W.ptr[S_T2] : nullptr, fp, 0, 1.0f, 1.0f, 1.0f, (float4*)W.ptr[S_VEL4],
+                       (float*)W.ptr[S_SCR], 0);
+    E(hipMemcpy(mac_u, W.ptr[S_U], z.Nu * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_v, W.ptr[S_V], z.Nv * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_w, W.ptr[S_W], z.Nw * 4, hipMemcpyDeviceToHost));
+    E(hipGetLastError());
+    const float ms = t.stop();
+    if (E.e != hipSuccess) { set_err(where, hipGetErrorString(E.e)); return -1.0f; }
+    return ms;
+}
+
 extern "C" float Attenuation3D(float* mac_u, float* mac_v, float* mac_w, float* temperature, float* density,
                                const bool* occupy, const float velAtten, const float tempAtten,
                                const float densityAtten, const int width, const int height, const int depth) {
