@@ -18,34 +18,36 @@ constexpr int BX = 64, BY = 4;
 inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
 
 // MAC_to_Regular4_Kernel (ZQ_CUDA_MACtoRegular.cu:9-30)
-__global__ __launch_bounds__(BX* BY) void k_mac_to_cell4(const float* __restrict__ u, const float* __restrict__ v,
-                                                         const float* __restrict__ w, float4* __restrict__ vel4,
+template <class R>
+__global__ __launch_bounds__(BX* BY) void k_mac_to_cell4(const R* __restrict__ u, const R* __restrict__ v,
+                                                         const R* __restrict__ w, vec4_t<R>* __restrict__ vel4,
                                                          Dims d) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    float4 c;
-    c.x = 0.5f * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
-    c.y = 0.5f * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
-    c.z = 0.5f * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
-    c.w = 0.0f;
+    vec4_t<R> c;
+    c.x = R(0.5) * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
+    c.y = R(0.5) * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
+    c.z = R(0.5) * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
+    c.w = R(0);
     vel4[cidx(d, i, j, k)] = c;
 }
 
 // Advect_Velocity_inRegular_outRegular_Kernel (ZQ_CUDA_Advection3D.cu:76-113): backtrace from the
 // cell centre through the cell grid and sample the same grid at the end point.  Output SoA
 // (adv[0..N) x, [N..2N) y, [2N..3N) z) so each face pass below reads one component plane.
-__global__ __launch_bounds__(BX* BY) void k_advect_cells(const float4* __restrict__ vel4,
-                                                         const uint8_t* __restrict__ occ, float* __restrict__ adv,
-                                                         Dims d, float h, int steps, float dtt) {
+template <class R>
+__global__ __launch_bounds__(BX* BY) void k_advect_cells(const vec4_t<R>* __restrict__ vel4,
+                                                         const uint8_t* __restrict__ occ, R* __restrict__ adv,
+                                                         Dims d, R h, int steps, R dtt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    const RegularVel V{vel4, d};
-    const float3 last = backtrace(V, occ, d, h, steps, dtt, i + 0.5f, j + 0.5f, k + 0.5f);
-    const float3 s = V(last.x, last.y, last.z);
+    const RegularVel<R> V{vel4, d};
+    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
+    const R3<R> s = V(last.x, last.y, last.z);
     const int o = cidx(d, i, j, k);
     const int N = d.W * d.H * d.D;
     adv[o] = s.x;
@@ -57,10 +59,10 @@ __global__ __launch_bounds__(BX* BY) void k_advect_cells(const float4* __restric
 // Apply_Advect_Velocity_Result_{Open,Closed}_* (ZQ_CUDA_Advection3D.cu:471-650): the face value is
 // the average of the two advected cells (a domain face copies its inside cell) and is stored only
 // where the face's cells are fluid.
-template <int AXIS>
-__global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const float* __restrict__ adv,
+template <class R, int AXIS>
+__global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const R* __restrict__ adv,
                                                                  const uint8_t* __restrict__ occ,
-                                                                 float* __restrict__ face, Dims d, int is_open) {
+                                                                 R* __restrict__ face, Dims d, int is_open) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -71,15 +73,15 @@ __global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const float* __
     const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
     const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
                         AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
-    const float* a = adv + (size_t)AXIS * d.W * d.H * d.D;
+    const R* a = adv + (size_t)AXIS * d.W * d.H * d.D;
     bool ok;
-    float val;
+    R val;
     if (pos == 0 || pos == lim) {
         ok = is_open && !occ[ci];
         val = a[ci];
     } else {
         ok = !occ[ci] && !occ[ci - step];
-        val = 0.5f * (a[ci] + a[ci - step]);
+        val = R(0.5) * (a[ci] + a[ci - step]);
     }
     if (ok) {
         const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
@@ -89,29 +91,27 @@ __global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const float* __
 
 // Advect_Velocity_inRegular_outMAC_* (:115-230) and inMAC_outMAC_* (:232-377): backtrace from the
 // face centre; the face's own component sampled at the end point goes to a scratch face grid.
-template <int AXIS, class Vel>
-__global__ __launch_bounds__(BX* BY) void k_advect_faces(Vel V, const uint8_t* __restrict__ occ,
-                                                         float* __restrict__ out, Dims d, float h, int steps,
-                                                         float dtt) {
+template <class R, int AXIS, class Vel>
+__global__ __launch_bounds__(BX* BY) void k_advect_faces(Vel V, const uint8_t* __restrict__ occ, R* __restrict__ out,
+                                                         Dims d, R h, int steps, R dtt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
     if (i >= nx || j >= ny) return;
-    const float px = AXIS == 0 ? (float)i : i + 0.5f;
-    const float py = AXIS == 1 ? (float)j : j + 0.5f;
-    const float pz = AXIS == 2 ? (float)k : k + 0.5f;
-    const float3 last = backtrace(V, occ, d, h, steps, dtt, px, py, pz);
-    const float3 s = V(last.x, last.y, last.z);
+    const R px = AXIS == 0 ? (R)i : i + R(0.5);
+    const R py = AXIS == 1 ? (R)j : j + R(0.5);
+    const R pz = AXIS == 2 ? (R)k : k + R(0.5);
+    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, px, py, pz);
+    const R3<R> s = V(last.x, last.y, last.z);
     const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
     out[fi] = AXIS == 0 ? s.x : (AXIS == 1 ? s.y : s.z);
 }
 
 // Masked copy scratch -> face (Apply_Advect_Velocity_Result_*, :471-650)
-template <int AXIS>
-__global__ __launch_bounds__(BX* BY) void k_apply_faces(const float* __restrict__ src,
-                                                        const uint8_t* __restrict__ occ, float* __restrict__ face,
-                                                        Dims d, int is_open) {
+template <class R, int AXIS>
+__global__ __launch_bounds__(BX* BY) void k_apply_faces(const R* __restrict__ src, const uint8_t* __restrict__ occ,
+                                                        R* __restrict__ face, Dims d, int is_open) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -131,87 +131,99 @@ __global__ __launch_bounds__(BX* BY) void k_apply_faces(const float* __restrict_
 
 // Advect_Scalar_{Regular,MAC}_Velocity_Kernel (:379-469): backtrace from the cell centre, sample T
 // and rho (linear, clamp) at the end point.  Every cell is written, solid ones included.
-template <class Vel>
+template <class R, class Vel>
 __global__ __launch_bounds__(BX* BY) void k_advect_scalar(Vel V, const uint8_t* __restrict__ occ,
-                                                          const float* __restrict__ Tin,
-                                                          const float* __restrict__ Rin, float* __restrict__ Tout,
-                                                          float* __restrict__ Rout, Dims d, float h, int steps,
-                                                          float dtt) {
+                                                          const R* __restrict__ Tin, const R* __restrict__ Rin,
+                                                          R* __restrict__ Tout, R* __restrict__ Rout, Dims d, R h,
+                                                          int steps, R dtt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    const float3 last = backtrace(V, occ, d, h, steps, dtt, i + 0.5f, j + 0.5f, k + 0.5f);
-    const float x = last.x - 0.5f, y = last.y - 0.5f, z = last.z - 0.5f;
+    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
+    const R x = last.x - R(0.5), y = last.y - R(0.5), z = last.z - R(0.5);
     const int o = cidx(d, i, j, k);
-    Tout[o] = sample3(Tin, d.W, d.H, d.D, x, y, z);
-    Rout[o] = sample3(Rin, d.W, d.H, d.D, x, y, z);
+    Tout[o] = sample3<R>(Tin, d.W, d.H, d.D, x, y, z);
+    Rout[o] = sample3<R>(Rin, d.W, d.H, d.D, x, y, z);
 }
 
 }  // namespace
 
-void launch_mac_to_cell4(const Dims& d, const float* u, const float* v, const float* w, float4* vel4,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_mac_to_cell4, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, vel4, d);
+template <class R>
+void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vel4, hipStream_t s) {
+    hipLaunchKernelGGL(k_mac_to_cell4<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, vel4, d);
 }
 
-void launch_advect_velocity(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, int is_open, int mode,
-                            const AdvParams& ap, float4* vel4, float* scratch, hipStream_t s) {
+template <class R>
+void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, int is_open, int mode,
+                            const AdvParams& ap, vec4_t<R>* vel4, R* scratch, hipStream_t s) {
     const dim3 blk(BX, BY);
     const size_t Nu = (size_t)(d.W + 1) * d.H * d.D, Nv = (size_t)d.W * (d.H + 1) * d.D;
-    float* ou = scratch;
-    float* ov = scratch + Nu;
-    float* ow = scratch + Nu + Nv;
+    R* ou = scratch;
+    R* ov = scratch + Nu;
+    R* ow = scratch + Nu + Nv;
+    const R h = (R)ap.h, dtt = (R)ap.dtt;
     if (mode == 0) {
-        launch_mac_to_cell4(d, u, v, w, vel4, s);
-        float* adv = scratch;  // 3N floats <= Nu+Nv+Nw
-        hipLaunchKernelGGL(k_advect_cells, grid_for(d.W, d.H, d.D), blk, 0, s, vel4, occ, adv, d, ap.h, ap.steps,
-                           ap.dtt);
-        hipLaunchKernelGGL(k_cells_to_faces_apply<0>, grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        R* adv = scratch;  // 3N values <= Nu+Nv+Nw
+        hipLaunchKernelGGL(k_advect_cells<R>, grid_for(d.W, d.H, d.D), blk, 0, s, vel4, occ, adv, d, h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
                            is_open);
-        hipLaunchKernelGGL(k_cells_to_faces_apply<1>, grid_for(d.W, d.H + 1, d.D), blk, 0, s, adv, occ, v, d,
+        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 1>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, adv, occ, v, d,
                            is_open);
-        hipLaunchKernelGGL(k_cells_to_faces_apply<2>, grid_for(d.W, d.H, d.D + 1), blk, 0, s, adv, occ, w, d,
+        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 2>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, adv, occ, w, d,
                            is_open);
         return;
     }
     if (mode == 4) {
-        launch_mac_to_cell4(d, u, v, w, vel4, s);
-        const RegularVel V{vel4, d};
-        hipLaunchKernelGGL((k_advect_faces<0, RegularVel>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou, d,
-                           ap.h, ap.steps, ap.dtt);
-        hipLaunchKernelGGL((k_advect_faces<1, RegularVel>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov, d,
-                           ap.h, ap.steps, ap.dtt);
-        hipLaunchKernelGGL((k_advect_faces<2, RegularVel>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow, d,
-                           ap.h, ap.steps, ap.dtt);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        const RegularVel<R> V{vel4, d};
+        hipLaunchKernelGGL((k_advect_faces<R, 0, RegularVel<R>>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou,
+                           d, h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_advect_faces<R, 1, RegularVel<R>>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov,
+                           d, h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_advect_faces<R, 2, RegularVel<R>>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow,
+                           d, h, ap.steps, dtt);
     } else {  // mode 2: inMAC_outMAC
-        const MacVel V{u, v, w, d};
-        hipLaunchKernelGGL((k_advect_faces<0, MacVel>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou, d, ap.h,
-                           ap.steps, ap.dtt);
-        hipLaunchKernelGGL((k_advect_faces<1, MacVel>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov, d, ap.h,
-                           ap.steps, ap.dtt);
-        hipLaunchKernelGGL((k_advect_faces<2, MacVel>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow, d, ap.h,
-                           ap.steps, ap.dtt);
+        const MacVel<R> V{u, v, w, d};
+        hipLaunchKernelGGL((k_advect_faces<R, 0, MacVel<R>>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou, d,
+                           h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_advect_faces<R, 1, MacVel<R>>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov, d,
+                           h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_advect_faces<R, 2, MacVel<R>>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow, d,
+                           h, ap.steps, dtt);
     }
-    hipLaunchKernelGGL(k_apply_faces<0>, grid_for(d.W + 1, d.H, d.D), blk, 0, s, ou, occ, u, d, is_open);
-    hipLaunchKernelGGL(k_apply_faces<1>, grid_for(d.W, d.H + 1, d.D), blk, 0, s, ov, occ, v, d, is_open);
-    hipLaunchKernelGGL(k_apply_faces<2>, grid_for(d.W, d.H, d.D + 1), blk, 0, s, ow, occ, w, d, is_open);
+    hipLaunchKernelGGL((k_apply_faces<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, ou, occ, u, d, is_open);
+    hipLaunchKernelGGL((k_apply_faces<R, 1>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, ov, occ, v, d, is_open);
+    hipLaunchKernelGGL((k_apply_faces<R, 2>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, ow, occ, w, d, is_open);
 }
 
-void launch_advect_scalar(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
-                          const float* Tin, const float* Rin, float* Tout, float* Rout, int mac, const AdvParams& ap,
-                          float4* vel4, hipStream_t s) {
+template <class R>
+void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, const uint8_t* occ, const R* Tin,
+                          const R* Rin, R* Tout, R* Rout, int mac, const AdvParams& ap, vec4_t<R>* vel4,
+                          hipStream_t s) {
     const dim3 blk(BX, BY);
+    const R h = (R)ap.h, dtt = (R)ap.dtt;
     if (mac) {
-        const MacVel V{u, v, w, d};
-        hipLaunchKernelGGL(k_advect_scalar<MacVel>, grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout, Rout,
-                           d, ap.h, ap.steps, ap.dtt);
+        const MacVel<R> V{u, v, w, d};
+        hipLaunchKernelGGL((k_advect_scalar<R, MacVel<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout,
+                           Rout, d, h, ap.steps, dtt);
     } else {
-        launch_mac_to_cell4(d, u, v, w, vel4, s);
-        const RegularVel V{vel4, d};
-        hipLaunchKernelGGL(k_advect_scalar<RegularVel>, grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout,
-                           Rout, d, ap.h, ap.steps, ap.dtt);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        const RegularVel<R> V{vel4, d};
+        hipLaunchKernelGGL((k_advect_scalar<R, RegularVel<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin,
+                           Tout, Rout, d, h, ap.steps, dtt);
     }
 }
+
+#define ZQ3D_ADVECT_INSTANTIATE(R)                                                                                     \
+    template void launch_mac_to_cell4<R>(const Dims&, const R*, const R*, const R*, vec4_t<R>*, hipStream_t);          \
+    template void launch_advect_velocity<R>(const Dims&, R*, R*, R*, const uint8_t*, int, int, const AdvParams&,       \
+                                            vec4_t<R>*, R*, hipStream_t);                                              \
+    template void launch_advect_scalar<R>(const Dims&, const R*, const R*, const R*, const uint8_t*, const R*,         \
+                                          const R*, R*, R*, int, const AdvParams&, vec4_t<R>*, hipStream_t);
+ZQ3D_ADVECT_INSTANTIATE(float)
+ZQ3D_ADVECT_INSTANTIATE(double)
+#undef ZQ3D_ADVECT_INSTANTIATE
 
 }  // namespace zq3d

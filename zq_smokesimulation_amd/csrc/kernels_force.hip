@@ -16,45 +16,53 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-__device__ __forceinline__ float cell_u(const float* __restrict__ u, const Dims& d, int i, int j, int k) {
-    return 0.5f * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
+template <class R>
+__device__ __forceinline__ R cell_u(const R* __restrict__ u, const Dims& d, int i, int j, int k) {
+    return R(0.5) * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
 }
-__device__ __forceinline__ float cell_v(const float* __restrict__ v, const Dims& d, int i, int j, int k) {
-    return 0.5f * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
+template <class R>
+__device__ __forceinline__ R cell_v(const R* __restrict__ v, const Dims& d, int i, int j, int k) {
+    return R(0.5) * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
 }
-__device__ __forceinline__ float cell_w(const float* __restrict__ w, const Dims& d, int i, int j, int k) {
-    return 0.5f * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
+template <class R>
+__device__ __forceinline__ R cell_w(const R* __restrict__ w, const Dims& d, int i, int j, int k) {
+    return R(0.5) * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
 }
 
 // Compute_Vorticity_Vector_Scale_Kernel (ZQ_CUDA_AddForce3D.cu:11-34): central differences of the
 // cell velocity over 2h on interior cells, zero on the border (the reference memsets the arrays).
-__global__ __launch_bounds__(BX* BY) void k_vorticity(const float* __restrict__ u, const float* __restrict__ v,
-                                                      const float* __restrict__ w, float4* __restrict__ vort,
-                                                      Dims d, float two_h) {
+template <class R>
+__global__ __launch_bounds__(BX* BY) void k_vorticity(const R* __restrict__ u, const R* __restrict__ v,
+                                                      const R* __restrict__ w, vec4_t<R>* __restrict__ vort, Dims d,
+                                                      R two_h) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    vec4_t<R> out;
+    out.x = out.y = out.z = out.w = R(0);
     if (i >= 1 && i < d.W - 1 && j >= 1 && j < d.H - 1 && k >= 1 && k < d.D - 1) {
-        const float wx = (cell_w(w, d, i, j + 1, k) - cell_w(w, d, i, j - 1, k) - cell_v(v, d, i, j, k + 1) +
-                          cell_v(v, d, i, j, k - 1)) / two_h;
-        const float wy = (cell_u(u, d, i, j, k + 1) - cell_u(u, d, i, j, k - 1) - cell_w(w, d, i + 1, j, k) +
-                          cell_w(w, d, i - 1, j, k)) / two_h;
-        const float wz = (cell_v(v, d, i + 1, j, k) - cell_v(v, d, i - 1, j, k) - cell_u(u, d, i, j + 1, k) +
-                          cell_u(u, d, i, j - 1, k)) / two_h;
-        out = make_float4(wx, wy, wz, sqrtf(wx * wx + wy * wy + wz * wz));
+        const R wx = (cell_w(w, d, i, j + 1, k) - cell_w(w, d, i, j - 1, k) - cell_v(v, d, i, j, k + 1) +
+                      cell_v(v, d, i, j, k - 1)) / two_h;
+        const R wy = (cell_u(u, d, i, j, k + 1) - cell_u(u, d, i, j, k - 1) - cell_w(w, d, i + 1, j, k) +
+                      cell_w(w, d, i - 1, j, k)) / two_h;
+        const R wz = (cell_v(v, d, i + 1, j, k) - cell_v(v, d, i - 1, j, k) - cell_u(u, d, i, j + 1, k) +
+                      cell_u(u, d, i, j - 1, k)) / two_h;
+        out.x = wx;
+        out.y = wy;
+        out.z = wz;
+        out.w = zsqrt(wx * wx + wy * wy + wz * wz);
     }
     vort[cidx(d, i, j, k)] = out;
 }
 
 // Compute_Gradient_of_VorticityScale_Kernel (:64-111) + Compute_Force_Kernel (:161-184).
 // Output is SoA: fcell[0..N) = fx, [N..2N) = fy, [2N..3N) = fz.
-template <bool ATTEN>
-__global__ __launch_bounds__(BX* BY) void k_force_cells(const float4* __restrict__ vort, float* __restrict__ T,
-                                                        float* __restrict__ rho, const float* __restrict__ extra,
-                                                        float* __restrict__ fcell, Dims d, float ch, float beta,
-                                                        float alpha, float Tamb, float aT, float aR) {
+template <class R, bool ATTEN>
+__global__ __launch_bounds__(BX* BY) void k_force_cells(const vec4_t<R>* __restrict__ vort, R* __restrict__ T,
+                                                        R* __restrict__ rho, const R* __restrict__ extra,
+                                                        R* __restrict__ fcell, Dims d, R ch, R beta, R alpha, R Tamb,
+                                                        R aT, R aR) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -62,31 +70,31 @@ __global__ __launch_bounds__(BX* BY) void k_force_cells(const float4* __restrict
     const int o = cidx(d, i, j, k);
     const int HW = d.H * d.W;
     const int N = HW * d.D;
-    const float s0 = vort[o].w;
-    float gx, gy, gz;
+    const R s0 = vort[o].w;
+    R gx, gy, gz;
     if (i == 0) gx = vort[o + 1].w - s0;
     else if (i == d.W - 1) gx = s0 - vort[o - 1].w;
-    else gx = 0.5f * (vort[o + 1].w - vort[o - 1].w);
+    else gx = R(0.5) * (vort[o + 1].w - vort[o - 1].w);
     if (j == 0) gy = vort[o + d.W].w - s0;
     else if (j == d.H - 1) gy = s0 - vort[o - d.W].w;
-    else gy = 0.5f * (vort[o + d.W].w - vort[o - d.W].w);
+    else gy = R(0.5) * (vort[o + d.W].w - vort[o - d.W].w);
     if (k == 0) gz = vort[o + HW].w - s0;
     else if (k == d.D - 1) gz = s0 - vort[o - HW].w;
-    else gz = 0.5f * (vort[o + HW].w - vort[o - HW].w);
-    const float len = sqrtf(gx * gx + gy * gy + gz * gz);
-    if (len != 0.0f) {
+    else gz = R(0.5) * (vort[o + HW].w - vort[o - HW].w);
+    const R len = zsqrt(gx * gx + gy * gy + gz * gz);
+    if (len != R(0)) {
         gx = gx / len;
         gy = gy / len;
         gz = gz / len;
     }
-    const float4 wv = vort[o];
-    float fx = ch * (gy * wv.z - gz * wv.y);
-    float fy = ch * (gz * wv.x - gx * wv.z);
-    float fz = ch * (gx * wv.y - gy * wv.x);
-    const float t = T[o];
-    const float r = (ATTEN || alpha != 0.0f) ? rho[o] : 0.0f;  // AddForce3D passes no density
+    const vec4_t<R> wv = vort[o];
+    R fx = ch * (gy * wv.z - gz * wv.y);
+    R fy = ch * (gz * wv.x - gx * wv.z);
+    R fz = ch * (gx * wv.y - gy * wv.x);
+    const R t = T[o];
+    const R r = (ATTEN || alpha != R(0)) ? rho[o] : R(0);  // AddForce3D passes no density
     fy += beta * (t - Tamb);
-    if (alpha != 0.0f) fy -= alpha * r;
+    if (alpha != R(0)) fy -= alpha * r;
     if (extra) {
         fx += extra[o * 3 + 0];
         fy += extra[o * 3 + 1];
@@ -106,10 +114,9 @@ __global__ __launch_bounds__(BX* BY) void k_force_cells(const float4* __restrict
 // so the face becomes (face + dt*avg f) * aV in one read-modify-write.  AXIS 0/1/2 = u/v/w.
 // The reference's AddForce_w_Kernel also runs its interior formula at z = 0 reading cell z-1
 // (:380-384, out of bounds); this follows the slice variant (:400-419).
-template <int AXIS, bool ATTEN>
-__global__ __launch_bounds__(BX* BY) void k_force_faces(float* __restrict__ face, const uint8_t* __restrict__ occ,
-                                                        const float* __restrict__ fc, Dims d, float dt, float dth,
-                                                        float aV) {
+template <class R, int AXIS, bool ATTEN>
+__global__ __launch_bounds__(BX* BY) void k_force_faces(R* __restrict__ face, const uint8_t* __restrict__ occ,
+                                                        const R* __restrict__ fc, Dims d, R dt, R dth, R aV) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -121,9 +128,9 @@ __global__ __launch_bounds__(BX* BY) void k_force_faces(float* __restrict__ face
     const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
                         AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
     const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
-    const float* f = fc + (size_t)AXIS * d.W * d.H * d.D;
+    const R* f = fc + (size_t)AXIS * d.W * d.H * d.D;
     bool ok;
-    float add;
+    R add;
     if (pos == 0) {
         ok = !occ[ci];
         add = dt * f[ci];
@@ -135,7 +142,7 @@ __global__ __launch_bounds__(BX* BY) void k_force_faces(float* __restrict__ face
         add = dth * (f[ci] + f[ci - step]);
     }
     if (ok) {
-        float x = face[fi] + add;
+        R x = face[fi] + add;
         if (ATTEN) x = x * aV;
         face[fi] = x;
     }
@@ -171,36 +178,57 @@ __global__ __launch_bounds__(256) void k_atten_cells(float* __restrict__ T, floa
 
 inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
 
+template <class R, bool ATTEN>
+void force_faces(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, const R* fcell, R dt, R dth, R aV,
+                 hipStream_t s) {
+    const dim3 blk(BX, BY, 1);
+    hipLaunchKernelGGL((k_force_faces<R, 0, ATTEN>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, u, occ, fcell, d, dt, dth,
+                       aV);
+    hipLaunchKernelGGL((k_force_faces<R, 1, ATTEN>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, v, occ, fcell, d, dt, dth,
+                       aV);
+    hipLaunchKernelGGL((k_force_faces<R, 2, ATTEN>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, w, occ, fcell, d, dt, dth,
+                       aV);
+}
+
 }  // namespace
 
-void launch_force_atten(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
-                        const float* extra, const ForceParams& fp, int atten, float aV, float aT, float aR,
-                        float4* vort, float* fcell, hipStream_t s) {
+template <class R>
+void launch_vorticity(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vort, float h, hipStream_t s) {
+    const R two_h = R(2) * (R)h;
+    hipLaunchKernelGGL(k_vorticity<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY, 1), 0, s, u, v, w, vort, d, two_h);
+}
+
+template <class R>
+void launch_force_atten(const Dims& d, R* u, R* v, R* w, R* T, R* rho, const uint8_t* occ, const R* extra,
+                        const ForceParams& fp, int atten, float aV, float aT, float aR, vec4_t<R>* vort, R* fcell,
+                        hipStream_t s) {
     const dim3 blk(BX, BY, 1);
-    const float two_h = 2 * fp.h;
-    hipLaunchKernelGGL(k_vorticity, grid_for(d.W, d.H, d.D), blk, 0, s, u, v, w, vort, d, two_h);
-    const float ch = fp.eps * fp.h;
-    const float dth = fp.dt * 0.5f;
+    launch_vorticity<R>(d, u, v, w, vort, fp.h, s);
+    // the products are formed in float, as the reference's host code passes them
+    const R ch = (R)(fp.eps * fp.h);
+    const R dth = (R)(fp.dt * 0.5f);
+    const R dt = (R)fp.dt;
     if (atten) {
-        hipLaunchKernelGGL(k_force_cells<true>, grid_for(d.W, d.H, d.D), blk, 0, s, vort, T, rho, extra, fcell, d,
-                           ch, fp.beta, fp.alpha, fp.Tamb, aT, aR);
-        hipLaunchKernelGGL((k_force_faces<0, true>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, u, occ, fcell, d, fp.dt,
-                           dth, aV);
-        hipLaunchKernelGGL((k_force_faces<1, true>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, v, occ, fcell, d, fp.dt,
-                           dth, aV);
-        hipLaunchKernelGGL((k_force_faces<2, true>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, w, occ, fcell, d, fp.dt,
-                           dth, aV);
+        hipLaunchKernelGGL((k_force_cells<R, true>), grid_for(d.W, d.H, d.D), blk, 0, s, vort, T, rho, extra, fcell, d,
+                           ch, (R)fp.beta, (R)fp.alpha, (R)fp.Tamb, (R)aT, (R)aR);
+        force_faces<R, true>(d, u, v, w, occ, fcell, dt, dth, (R)aV, s);
     } else {
-        hipLaunchKernelGGL(k_force_cells<false>, grid_for(d.W, d.H, d.D), blk, 0, s, vort, T, rho, extra, fcell, d,
-                           ch, fp.beta, fp.alpha, fp.Tamb, 1.0f, 1.0f);
-        hipLaunchKernelGGL((k_force_faces<0, false>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, u, occ, fcell, d,
-                           fp.dt, dth, 1.0f);
-        hipLaunchKernelGGL((k_force_faces<1, false>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, v, occ, fcell, d,
-                           fp.dt, dth, 1.0f);
-        hipLaunchKernelGGL((k_force_faces<2, false>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, w, occ, fcell, d,
-                           fp.dt, dth, 1.0f);
+        hipLaunchKernelGGL((k_force_cells<R, false>), grid_for(d.W, d.H, d.D), blk, 0, s, vort, T, rho, extra, fcell,
+                           d, ch, (R)fp.beta, (R)fp.alpha, (R)fp.Tamb, R(1), R(1));
+        force_faces<R, false>(d, u, v, w, occ, fcell, dt, dth, R(1), s);
     }
 }
+
+template void launch_vorticity<float>(const Dims&, const float*, const float*, const float*, float4*, float,
+                                      hipStream_t);
+template void launch_vorticity<double>(const Dims&, const double*, const double*, const double*, double4*, float,
+                                       hipStream_t);
+template void launch_force_atten<float>(const Dims&, float*, float*, float*, float*, float*, const uint8_t*,
+                                        const float*, const ForceParams&, int, float, float, float, float4*, float*,
+                                        hipStream_t);
+template void launch_force_atten<double>(const Dims&, double*, double*, double*, double*, double*, const uint8_t*,
+                                         const double*, const ForceParams&, int, float, float, float, double4*,
+                                         double*, hipStream_t);
 
 void launch_attenuation(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
                         float aV, float aT, float aR, hipStream_t s) {

@@ -13,24 +13,26 @@ inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (
 // ZQ_ReinjectTemperatureDensity3D (ZQ_SmokeSimulationUtils3D.cpp:3842-3882), box sources,
 // randDensity off: cells whose centre is within the half extents get rho_src and T_src.
 // box = cx, cy, cz, hx, hy, hz (normalized by W/H/D), rho_src, T_src.
-__global__ __launch_bounds__(BX* BY) void k_reinject_box(float* __restrict__ T, float* __restrict__ rho, Dims d,
-                                                         float cx, float cy, float cz, float hx, float hy, float hz,
-                                                         float rd, float rt) {
+// The adjoint runs the same kernel with zero values: a reinjected cell forgets its past.
+template <class R>
+__global__ __launch_bounds__(BX* BY) void k_reinject_box(R* __restrict__ T, R* __restrict__ rho, Dims d, float cx,
+                                                         float cy, float cz, float hx, float hy, float hz, float rd,
+                                                         float rt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
     if (fabsf(i + 0.5f - cx) <= hx && fabsf(j + 0.5f - cy) <= hy && fabsf(k + 0.5f - cz) <= hz) {
         const int o = cidx(d, i, j, k);
-        rho[o] = rd * 1.0f;
-        T[o] = rt;
+        rho[o] = (R)(rd * 1.0f);
+        T[o] = (R)rt;
     }
 }
 
 // Sphere source (not in the reference): centre normalized by W/H/D, radius in units of W.
-__global__ __launch_bounds__(BX* BY) void k_reinject_sphere(float* __restrict__ T, float* __restrict__ rho, Dims d,
-                                                            float x0, float y0, float z0, float r2, float rd,
-                                                            float rt) {
+template <class R>
+__global__ __launch_bounds__(BX* BY) void k_reinject_sphere(R* __restrict__ T, R* __restrict__ rho, Dims d, float x0,
+                                                            float y0, float z0, float r2, float rd, float rt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -38,8 +40,8 @@ __global__ __launch_bounds__(BX* BY) void k_reinject_sphere(float* __restrict__ 
     const float dx = i + 0.5f - x0, dy = j + 0.5f - y0, dz = k + 0.5f - z0;
     if (dx * dx + dy * dy + dz * dz <= r2) {
         const int o = cidx(d, i, j, k);
-        rho[o] = rd;
-        T[o] = rt;
+        rho[o] = (R)rd;
+        T[o] = (R)rt;
     }
 }
 
@@ -87,18 +89,25 @@ void fill(T* p, size_t n, T val, hipStream_t s) {
 
 }  // namespace
 
-void launch_reinject_box(const Dims& d, const float* b, float* T, float* rho, hipStream_t s) {
+template <class R>
+void launch_reinject_box(const Dims& d, const float* b, R* T, R* rho, hipStream_t s) {
     const float cx = b[0] * d.W, cy = b[1] * d.H, cz = b[2] * d.D;
     const float hx = b[3] * d.W, hy = b[4] * d.H, hz = b[5] * d.D;
-    hipLaunchKernelGGL(k_reinject_box, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, T, rho, d, cx, cy, cz, hx, hy, hz,
-                       b[6], b[7]);
+    hipLaunchKernelGGL(k_reinject_box<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, T, rho, d, cx, cy, cz, hx, hy,
+                       hz, b[6], b[7]);
 }
 
-void launch_reinject_sphere(const Dims& d, const float* sp, float* T, float* rho, hipStream_t s) {
+template <class R>
+void launch_reinject_sphere(const Dims& d, const float* sp, R* T, R* rho, hipStream_t s) {
     const float x0 = sp[0] * d.W, y0 = sp[1] * d.H, z0 = sp[2] * d.D, rr = sp[3] * d.W;
-    hipLaunchKernelGGL(k_reinject_sphere, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, T, rho, d, x0, y0, z0,
+    hipLaunchKernelGGL(k_reinject_sphere<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, T, rho, d, x0, y0, z0,
                        rr * rr, sp[4], sp[5]);
 }
+
+template void launch_reinject_box<float>(const Dims&, const float*, float*, float*, hipStream_t);
+template void launch_reinject_box<double>(const Dims&, const float*, double*, double*, hipStream_t);
+template void launch_reinject_sphere<float>(const Dims&, const float*, float*, float*, hipStream_t);
+template void launch_reinject_sphere<double>(const Dims&, const float*, double*, double*, hipStream_t);
 
 void launch_stamp_sphere(const Dims& d, float cx, float cy, float cz, float r, float su, float sv, float sw,
                          int shell_mask, uint8_t* occ, float* u, float* v, float* w, hipStream_t s) {

@@ -425,9 +425,9 @@ __global__ __launch_bounds__(BX* BY) void k_sor_split(const uint8_t* __restrict_
 
 // Face correction u -= p_r - p_l (ZQ_SmokeSimulationUtils3D.cpp:2815-2861 open form, :2888-2922
 // closed form).  P = float (plain) or double (mixed precision; the face is rounded once).
-template <int AXIS, class P>
+template <int AXIS, class P, class F>
 __global__ __launch_bounds__(BX* BY) void k_correct(const uint8_t* __restrict__ occ, int open_mask,
-                                                    const P* __restrict__ p, float* __restrict__ face, Dims d) {
+                                                    const P* __restrict__ p, F* __restrict__ face, Dims d) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -448,7 +448,127 @@ __global__ __launch_bounds__(BX* BY) void k_correct(const uint8_t* __restrict__ 
     // padded index of the cell on the high side of the face (ghost when pos == lim)
     const int qh = pidx(d, i, j, k);
     const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
-    face[fi] = (float)((P)face[fi] - (p[qh] - p[qh - pstep]));
+    face[fi] = (F)((P)face[fi] - (p[qh] - p[qh - pstep]));
+}
+
+// Transposed correction: every face, solid and ghost pressures being 0 (the adjoint of the
+// projection with respect to the faces the forward correction leaves alone is not zero: those
+// faces enter the divergence).
+template <int AXIS, class F>
+__global__ __launch_bounds__(BX* BY) void k_correct_all(const double* __restrict__ p, F* __restrict__ face, Dims d) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
+    if (i >= nx || j >= ny) return;
+    const int pstep = AXIS == 0 ? 1 : (AXIS == 1 ? d.W + 2 : (d.W + 2) * (d.H + 2));
+    const int qh = pidx(d, i, j, k);
+    const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
+    face[fi] = (F)((double)face[fi] - (p[qh] - p[qh - pstep]));
+}
+
+// Mixed-precision set-up on faces of type F: neighbour counts, FP64 divergence (FREE: only the
+// faces between two fluid cells and open domain faces, i.e. those whose neighbour is counted),
+// p64 = 0 on uncounted cells, per-block sum/count of the divergence over fluid cells.
+template <class F, bool FREE>
+__global__ __launch_bounds__(BX* BY) void k_proj_setup_mp(const F* __restrict__ u, const F* __restrict__ v,
+                                                          const F* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                          int open_mask, uint8_t* __restrict__ nc,
+                                                          double* __restrict__ b64, double* __restrict__ p64,
+                                                          double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double dsum = 0.0, dcnt = 0.0;
+    if (i < d.W && j < d.H) {
+        const int o = cidx(d, i, j, k);
+        const int HW = d.W * d.H;
+        const bool fluid = !occ[o];
+        bool c[6] = {false, false, false, false, false, false};
+        if (fluid) {
+            c[0] = (i > 0) ? !occ[o - 1] : ((open_mask & 1) != 0);
+            c[1] = (i < d.W - 1) ? !occ[o + 1] : ((open_mask & 2) != 0);
+            c[2] = (j > 0) ? !occ[o - d.W] : ((open_mask & 4) != 0);
+            c[3] = (j < d.H - 1) ? !occ[o + d.W] : ((open_mask & 8) != 0);
+            c[4] = (k > 0) ? !occ[o - HW] : ((open_mask & 16) != 0);
+            c[5] = (k < d.D - 1) ? !occ[o + HW] : ((open_mask & 32) != 0);
+        }
+        const int n = (int)c[0] + (int)c[1] + (int)c[2] + (int)c[3] + (int)c[4] + (int)c[5];
+        nc[o] = (uint8_t)n;
+        double u0 = (double)u[uidx(d, i, j, k)], u1 = (double)u[uidx(d, i + 1, j, k)];
+        double v0 = (double)v[vidx(d, i, j, k)], v1 = (double)v[vidx(d, i, j + 1, k)];
+        double w0 = (double)w[widx(d, i, j, k)], w1 = (double)w[widx(d, i, j, k + 1)];
+        if (FREE) {
+            u0 = c[0] ? u0 : 0.0;
+            u1 = c[1] ? u1 : 0.0;
+            v0 = c[2] ? v0 : 0.0;
+            v1 = c[3] ? v1 : 0.0;
+            w0 = c[4] ? w0 : 0.0;
+            w1 = c[5] ? w1 : 0.0;
+        }
+        const double dd = ((u1 - u0) + (v1 - v0)) + (w1 - w0);
+        b64[o] = n ? dd : 0.0;
+        if (!n) p64[pidx(d, i, j, k)] = 0.0;
+        if (fluid) {
+            dsum = dd;
+            dcnt = 1.0;
+        }
+    }
+    const double sm = block_reduce<false>(dsum, red);
+    __syncthreads();
+    const double ct = block_reduce<false>(dcnt, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[2 * block_linear()] = sm;
+        part[2 * block_linear() + 1] = ct;
+    }
+}
+
+// per-block max |b64| over counted cells
+__global__ __launch_bounds__(BX* BY) void k_max_abs64(const uint8_t* __restrict__ nc, const double* __restrict__ b64,
+                                                      double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double m = 0.0;
+    if (i < d.W && j < d.H) {
+        const int o = cidx(d, i, j, k);
+        if (nc[o]) m = fabs(b64[o]);
+    }
+    m = block_reduce<true>(m, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
+}
+
+// per-block sum/count of p64 over fluid cells
+__global__ __launch_bounds__(BX* BY) void k_sum_p64(const uint8_t* __restrict__ occ, const double* __restrict__ p,
+                                                    double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double sm = 0.0, ct = 0.0;
+    if (i < d.W && j < d.H && !occ[cidx(d, i, j, k)]) {
+        sm = p[pidx(d, i, j, k)];
+        ct = 1.0;
+    }
+    sm = block_reduce<false>(sm, red);
+    __syncthreads();
+    ct = block_reduce<false>(ct, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[2 * block_linear()] = sm;
+        part[2 * block_linear() + 1] = ct;
+    }
+}
+
+__global__ __launch_bounds__(BX* BY) void k_sub_p64(const uint8_t* __restrict__ occ, double* __restrict__ p,
+                                                    const double* __restrict__ mean64, Dims d) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H || occ[cidx(d, i, j, k)]) return;
+    const int q = pidx(d, i, j, k);
+    p[q] = p[q] - *mean64;
 }
 
 // FP64 residual r = (b64 - mean) - (sum_nb p - n p); r32 feeds the FP32 inner solve; per-block max|r|.
@@ -492,8 +612,9 @@ __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict
     e[q] = 0.0f;
 }
 
-__global__ __launch_bounds__(BX* BY) void k_max_div64(const float* __restrict__ u, const float* __restrict__ v,
-                                                      const float* __restrict__ w, const uint8_t* __restrict__ nc,
+template <class F>
+__global__ __launch_bounds__(BX* BY) void k_max_div64(const F* __restrict__ u, const F* __restrict__ v,
+                                                      const F* __restrict__ w, const uint8_t* __restrict__ nc,
                                                       const double* __restrict__ mean64, double* __restrict__ part,
                                                       Dims d) {
     __shared__ double red[4];
@@ -640,22 +761,60 @@ int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
 
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s) {
-    hipLaunchKernelGGL((k_correct<0, float>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask, p_pad,
-                       u, d);
-    hipLaunchKernelGGL((k_correct<1, float>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, occ, open_mask, p_pad,
-                       v, d);
-    hipLaunchKernelGGL((k_correct<2, float>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, occ, open_mask, p_pad,
-                       w, d);
+    hipLaunchKernelGGL((k_correct<0, float, float>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
+                       p_pad, u, d);
+    hipLaunchKernelGGL((k_correct<1, float, float>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, occ, open_mask,
+                       p_pad, v, d);
+    hipLaunchKernelGGL((k_correct<2, float, float>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, occ, open_mask,
+                       p_pad, w, d);
 }
 
-void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, float* u, float* v,
-                      float* w, hipStream_t s) {
-    hipLaunchKernelGGL((k_correct<0, double>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
+template <class F>
+void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, F* u, F* v, F* w,
+                      hipStream_t s) {
+    hipLaunchKernelGGL((k_correct<0, double, F>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
                        p64_pad, u, d);
-    hipLaunchKernelGGL((k_correct<1, double>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, occ, open_mask,
+    hipLaunchKernelGGL((k_correct<1, double, F>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, occ, open_mask,
                        p64_pad, v, d);
-    hipLaunchKernelGGL((k_correct<2, double>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, occ, open_mask,
+    hipLaunchKernelGGL((k_correct<2, double, F>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, occ, open_mask,
                        p64_pad, w, d);
+}
+
+template <class F>
+void launch_correct_all64(const Dims& d, const double* p64_pad, F* u, F* v, F* w, hipStream_t s) {
+    hipLaunchKernelGGL((k_correct_all<0, F>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, p64_pad, u, d);
+    hipLaunchKernelGGL((k_correct_all<1, F>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, p64_pad, v, d);
+    hipLaunchKernelGGL((k_correct_all<2, F>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, p64_pad, w, d);
+}
+
+void launch_proj_setup64(const Dims& d, const double* u, const double* v, const double* w, const uint8_t* occ,
+                         int open_mask, int free_only, uint8_t* nc, double* b64, double* p64_pad, double* part,
+                         hipStream_t s) {
+    if (free_only)
+        hipLaunchKernelGGL((k_proj_setup_mp<double, true>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ,
+                           open_mask, nc, b64, p64_pad, part, d);
+    else
+        hipLaunchKernelGGL((k_proj_setup_mp<double, false>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w,
+                           occ, open_mask, nc, b64, p64_pad, part, d);
+}
+
+void launch_proj_setup_free(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
+                            int open_mask, uint8_t* nc, double* b64, double* p64_pad, double* part, hipStream_t s) {
+    hipLaunchKernelGGL((k_proj_setup_mp<float, true>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ,
+                       open_mask, nc, b64, p64_pad, part, d);
+}
+
+void launch_max_abs64(const Dims& d, const uint8_t* nc, const double* b64, double* part, unsigned long long* out,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_max_abs64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, part, d);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), out);
+}
+
+void launch_remove_mean_p64(const Dims& d, const uint8_t* occ, double* p64_pad, double* part, double* mean64,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_p64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, occ, p64_pad, part, d);
+    hipLaunchKernelGGL(k_finish_mean, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), nullptr, mean64, 1);
+    hipLaunchKernelGGL(k_sub_p64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, occ, p64_pad, mean64, d);
 }
 
 void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
@@ -669,10 +828,20 @@ void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p
     hipLaunchKernelGGL(k_update_p64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, e_pad, p64_pad, d);
 }
 
-void launch_max_div64(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* nc,
-                      const double* mean64, double* part, unsigned long long* dmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_max_div64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, nc, mean64, part, d);
+template <class F>
+void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* nc, const double* mean64,
+                      double* part, unsigned long long* dmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_div64<F>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, nc, mean64, part, d);
     hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), dmax);
 }
+
+#define ZQ3D_PROJ_INSTANTIATE(F)                                                                                       \
+    template void launch_correct64<F>(const Dims&, const uint8_t*, int, const double*, F*, F*, F*, hipStream_t);       \
+    template void launch_correct_all64<F>(const Dims&, const double*, F*, F*, F*, hipStream_t);                        \
+    template void launch_max_div64<F>(const Dims&, const F*, const F*, const F*, const uint8_t*, const double*,        \
+                                      double*, unsigned long long*, hipStream_t);
+ZQ3D_PROJ_INSTANTIATE(float)
+ZQ3D_PROJ_INSTANTIATE(double)
+#undef ZQ3D_PROJ_INSTANTIATE
 
 }  // namespace zq3d

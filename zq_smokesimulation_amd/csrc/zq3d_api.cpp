@@ -412,9 +412,9 @@ extern "C" float AddForce3D(float* mac_u, float* mac_v, float* mac_w, const bool
     E(hipMemcpy(W.ptr[S_OCC], occupy, z.N, hipMemcpyHostToDevice));
     E(hipMemcpy(W.ptr[S_T], temperature, z.N * 4, hipMemcpyHostToDevice));
     const ForceParams fp{deltah, deltat, buoyCoeff, 0.0f, confineCoeff, Tamb};
-    launch_force_atten(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T], nullptr,
-                       (uint8_t*)W.ptr[S_OCC], nullptr, fp, 0, 1.0f, 1.0f, 1.0f, (float4*)W.ptr[S_VEL4],
-                       (float*)W.ptr[S_SCR], 0);
+    launch_force_atten<float>(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T],
+                              nullptr, (uint8_t*)W.ptr[S_OCC], nullptr, fp, 0, 1.0f, 1.0f, 1.0f,
+                              (float4*)W.ptr[S_VEL4], (float*)W.ptr[S_SCR], 0);
     E(hipMemcpy(mac_u, W.ptr[S_U], z.Nu * 4, hipMemcpyDeviceToHost));
     E(hipMemcpy(mac_v, W.ptr[S_V], z.Nv * 4, hipMemcpyDeviceToHost));
     E(hipMemcpy(mac_w, W.ptr[S_W], z.Nw * 4, hipMemcpyDeviceToHost));
@@ -456,7 +456,7 @@ extern "C" float ZQ3D_AddForce3D_ex(float* mac_u, float* mac_v, float* mac_w, co
     if (density) E(hipMemcpy(W.ptr[S_R], density, z.N * 4, hipMemcpyHostToDevice));
     if (extra_force) E(hipMemcpy(W.ptr[S_T2], extra_force, z.N * 12, hipMemcpyHostToDevice));
     const ForceParams fp{deltah, deltat, buoyCoeff, gravityCoeff, confineCoeff, Tamb};
-    launch_force_atten(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T],
+    launch_force_atten<float>(d, (float*)W.ptr[S_U], (float*)W.ptr[S_V], (float*)W.ptr[S_W], (float*)W.ptr[S_T],
                        density ? (float*)W.ptr[S_R] : nullptr, (uint8_t*)W.ptr[S_OCC],
                        extra_force ? (float*)W.ptr[S_T2] : nullptr, fp, 0, 1.0f, 1.0f, 1.0f, (float4*)W.ptr[S_VEL4],
                        (float*)W.ptr[S_SCR], 0);

@@ -5,38 +5,48 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace zq3d {
+#include "zq3d_device.h"
 
-struct Dims;
+namespace zq3d {
 
 struct ForceParams {
     float h, dt, beta, alpha, eps, Tamb;
 };
 
+// Launchers templated on R are instantiated for float (the product path) and double (the FP64
+// check mode of the adjoint); parameters stay float, as the reference passes them.
+
 // ---- force + attenuation (ZQ_CUDA_AddForce3D.cu, ZQ_CUDA_Attenuation3D.cu) ----
-// vort: N float4 scratch, fcell: 3N float scratch.  extra (nullable): 3 floats per cell.
+// vort: N 4-wide scratch, fcell: 3N scratch.  extra (nullable): 3 values per cell.
 // atten != 0 fuses the attenuation stage (faces *= aV on the same mask, T *= aT, rho *= aR).
-void launch_force_atten(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
-                        const float* extra, const ForceParams& fp, int atten, float aV, float aT, float aR,
-                        float4* vort, float* fcell, hipStream_t s);
+template <class R>
+void launch_force_atten(const Dims& d, R* u, R* v, R* w, R* T, R* rho, const uint8_t* occ, const R* extra,
+                        const ForceParams& fp, int atten, float aV, float aT, float aR, vec4_t<R>* vort, R* fcell,
+                        hipStream_t s);
+// (wx, wy, wz, |w|) per cell, zero on the border cells
+template <class R>
+void launch_vorticity(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vort, float h, hipStream_t s);
 void launch_attenuation(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
                         float aV, float aT, float aR, hipStream_t s);
 
 // ---- MAC <-> cell (ZQ_CUDA_MACtoRegular.cu) ----
-void launch_mac_to_cell4(const Dims& d, const float* u, const float* v, const float* w, float4* vel4, hipStream_t s);
+template <class R>
+void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vel4, hipStream_t s);
 
 // ---- advection (ZQ_CUDA_Advection3D.cu) ----
 struct AdvParams {
     float h, dtt;
     int steps;
 };
-// mode: 0 inReg_outReg, 2 inMAC_outMAC, 4 inReg_outMAC.  scratch: vel4 (N float4), faces (Nu+Nv+Nw floats).
-void launch_advect_velocity(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, int is_open, int mode,
-                            const AdvParams& ap, float4* vel4, float* scratch_faces, hipStream_t s);
+// mode: 0 inReg_outReg, 2 inMAC_outMAC, 4 inReg_outMAC.  scratch: vel4 (N), faces (Nu+Nv+Nw values).
+template <class R>
+void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, int is_open, int mode,
+                            const AdvParams& ap, vec4_t<R>* vel4, R* scratch_faces, hipStream_t s);
 // mac != 0: MAC velocity sampling (ADV_SCA_MAC); else cell-centred (ADV_SCA_REG).
-void launch_advect_scalar(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
-                          const float* Tin, const float* Rin, float* Tout, float* Rout, int mac,
-                          const AdvParams& ap, float4* vel4, hipStream_t s);
+template <class R>
+void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, const uint8_t* occ, const R* Tin,
+                          const R* Rin, R* Tout, R* Rout, int mac, const AdvParams& ap, vec4_t<R>* vel4,
+                          hipStream_t s);
 
 // ---- projection ----
 // Doubles needed by the reduction partials of a grid (per-block sums / maxima).
@@ -46,6 +56,14 @@ int reduction_partials(const Dims& d);
 void launch_proj_setup(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
                        int open_mask, uint8_t* nc, float* b, double* b64, float* p_pad, double* p64_pad,
                        double* part, hipStream_t s);
+// FP64 variant on FP64 faces (always b64/p64); free_only != 0 sums only the faces the correction
+// updates (between two fluid cells, and open domain faces): the right-hand side of the adjoint.
+void launch_proj_setup64(const Dims& d, const double* u, const double* v, const double* w, const uint8_t* occ,
+                         int open_mask, int free_only, uint8_t* nc, double* b64, double* p64_pad, double* part,
+                         hipStream_t s);
+// FP32-face variant of the adjoint right-hand side (free faces only) into b64
+void launch_proj_setup_free(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* occ,
+                            int open_mask, uint8_t* nc, double* b64, double* p64_pad, double* part, hipStream_t s);
 // mean32/mean64 = sum/count of the setup partials (0 when want == 0)
 void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s);
 // Sweep algorithm / tile shape: ty rows per block (2, 4 or 8) and zc planes per chunk of the fused
@@ -78,20 +96,35 @@ int sor_launches_per_iter(const Dims& d, SorShape shape = SorShape{0, 0});
 int sor_shape_candidates(const Dims& d, SorShape* out, int max);
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s);
-void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, float* u, float* v,
-                      float* w, hipStream_t s);
+template <class F>
+void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, F* u, F* v, F* w,
+                      hipStream_t s);
+// u -= p_r - p_l on every face (the adjoint of the projection: the faces the forward correction
+// leaves alone carry the pressure's sensitivity too)
+template <class F>
+void launch_correct_all64(const Dims& d, const double* p64_pad, F* u, F* v, F* w, hipStream_t s);
 // r = (b64 - mean64) - (sum_nb p - n p); writes r32 and max|r| (bits of a non-negative double) to *rmax
 void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
                        const double* p64_pad, float* r32, double* part, unsigned long long* rmax, hipStream_t s);
 // p64 += e, e = 0 on counted cells
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s);
 // max |div64 - mean64| over counted cells into *dmax (bits of a non-negative double)
-void launch_max_div64(const Dims& d, const float* u, const float* v, const float* w, const uint8_t* nc,
-                      const double* mean64, double* part, unsigned long long* dmax, hipStream_t s);
+template <class F>
+void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* nc, const double* mean64,
+                      double* part, unsigned long long* dmax, hipStream_t s);
+// max |b64| over counted cells into *out (bits of a non-negative double)
+void launch_max_abs64(const Dims& d, const uint8_t* nc, const double* b64, double* part, unsigned long long* out,
+                      hipStream_t s);
+// mean of p64 over fluid cells, subtracted from those cells (fixes the free constant of a
+// closed-domain pressure); part holds the partials, mean64 receives the mean
+void launch_remove_mean_p64(const Dims& d, const uint8_t* occ, double* p64_pad, double* part, double* mean64,
+                            hipStream_t s);
 
 // ---- sources, obstacles, fills ----
-void launch_reinject_box(const Dims& d, const float* box8, float* T, float* rho, hipStream_t s);
-void launch_reinject_sphere(const Dims& d, const float* sph6, float* T, float* rho, hipStream_t s);
+template <class R>
+void launch_reinject_box(const Dims& d, const float* box8, R* T, R* rho, hipStream_t s);
+template <class R>
+void launch_reinject_sphere(const Dims& d, const float* sph6, R* T, R* rho, hipStream_t s);
 void launch_stamp_sphere(const Dims& d, float cx, float cy, float cz, float r, float su, float sv, float sw,
                          int shell_mask, uint8_t* occ, float* u, float* v, float* w, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float val, hipStream_t s);
