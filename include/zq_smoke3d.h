@@ -292,6 +292,46 @@ int zq3d_scene_init_occupancy(const char* path, uint8_t* occ, size_t bytes);
  * into caller arrays (occ W*H*D bytes, faces in the reference layout, updated in place) */
 int zq3d_scene_apply_moving_objects(const char* path, int frames, uint8_t* occ, float* u, float* v, float* w);
 
+/* ====================== Part 5: adjoint rollout (gradient-based control) ====================== */
+
+/* No reference counterpart.  A rollout of nsteps steps of the device-resident simulation (force +
+ * attenuation with a per-step cell-centred control force added to the cell force, velocity
+ * advection, mixed-precision projection, scalar advection, reinjection) and its hand-written
+ * adjoint.  The forward keeps a checkpoint (state and the projection's warm start) every
+ * checkpoint_every steps; the backward recomputes each segment from its checkpoint, storing the
+ * segment's intermediate states, and runs the adjoint steps in reverse.
+ * precision 4: every field FP32 (the product kernels); precision 8: every field FP64, the FD check
+ * mode (the inner SOR sweeps stay FP32, the FP64 refinement converges to cfg->sor.tol).
+ * Requires cfg->sor.mixed_precision, cfg->steps <= 32 and a non-BFECC advection mode.
+ * Data pointers are device pointers of the chosen precision (occupancy: bytes), stream-ordered on
+ * the rollout's stream.  The adjoint projection solves to adjoint_rtol * max|rhs| (default 1e-6),
+ * warm-started from the previous adjoint step. */
+typedef struct zq3d_adj zq3d_adj;
+typedef struct zq3d_adj_stats {
+    long long forward_sweeps;    /* SOR sweeps of the last call's forward projections (recomputation included) */
+    long long adjoint_sweeps;    /* SOR sweeps of the adjoint projections */
+    int recomputed_steps;        /* forward steps rerun by the last backward */
+    double max_div;              /* largest post-projection max|div| of the last forward */
+    double ms;                   /* wall time of the call on the stream (HIP events) */
+} zq3d_adj_stats;
+int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps, int checkpoint_every, zq3d_adj** out);
+void zq3d_adj_destroy(zq3d_adj* a);
+/* hipStream_t (NULL = the default stream) */
+int zq3d_adj_set_stream(zq3d_adj* a, void* stream);
+int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8, int n_sphere, const float* spheres6);
+int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol);
+/* initial state: ZQ3D_FIELD_U/V/W/T/RHO (device, chosen precision) or ZQ3D_FIELD_OCC (device bytes) */
+int zq3d_adj_set_initial(zq3d_adj* a, int field, const void* dev);
+/* runs the rollout from the initial state.  controls: NULL or nsteps x (W*H*D) x 3 (per cell x, y, z);
+ * rho_out: NULL or W*H*D, the density after the last step */
+int zq3d_adj_forward(zq3d_adj* a, const void* controls, zq3d_adj_stats* st, void* rho_out);
+/* given dL/d(density after the last step), writes dL/d(controls) (nsteps x W*H*D x 3); controls must
+ * be those of the preceding forward */
+int zq3d_adj_backward(zq3d_adj* a, const void* controls, const void* rho_bar, void* controls_bar,
+                      zq3d_adj_stats* st);
+/* after a forward: the final state field; after a backward: with grad != 0, dL/d(initial field) */
+int zq3d_adj_get(zq3d_adj* a, int field, int grad, void* dev);
+
 /* message of the last failed call on this thread ("" if none) */
 const char* zq3d_last_error(void);
 /* frees the device workspace cached by the Part 1/2 host-pointer functions */

@@ -169,7 +169,7 @@ def _load():
 def lib():
     global _lib
     if _lib is None:
-        _lib = _bind_scene(_load())
+        _lib = _bind_adjoint(_bind_scene(_load()))
     return _lib
 
 
@@ -225,4 +225,32 @@ def _bind_scene(L):
     L.ZQ3D_AddForce3D_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
     L.ZQ3D_AddForce3D_ex.restype = C.c_float
+    return L
+
+
+class AdjStats(C.Structure):
+    _fields_ = [
+        ("forward_sweeps", C.c_longlong),
+        ("adjoint_sweeps", C.c_longlong),
+        ("recomputed_steps", C.c_int),
+        ("max_div", C.c_double),
+        ("ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _bind_adjoint(L):
+    vp = C.c_void_p
+    L.zq3d_adj_create.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.zq3d_adj_destroy.argtypes = [vp]
+    L.zq3d_adj_destroy.restype = None
+    L.zq3d_adj_set_stream.argtypes = [vp, vp]
+    L.zq3d_adj_set_sources.argtypes = [vp, C.c_int, vp, C.c_int, vp]
+    L.zq3d_adj_set_adjoint_tol.argtypes = [vp, C.c_double]
+    L.zq3d_adj_set_initial.argtypes = [vp, C.c_int, vp]
+    L.zq3d_adj_forward.argtypes = [vp, vp, C.POINTER(AdjStats), vp]
+    L.zq3d_adj_backward.argtypes = [vp, vp, vp, vp, C.POINTER(AdjStats)]
+    L.zq3d_adj_get.argtypes = [vp, C.c_int, C.c_int, vp]
     return L

@@ -13,27 +13,21 @@
 #include <vector>
 
 #include "zq3d_device.h"
+#include "zq3d_internal.h"
 #include "zq3d_kernels.h"
 #include "zq_smoke3d.h"
 
 using namespace zq3d;
+using namespace zq3d::detail;
 
-namespace {
+namespace zq3d {
+namespace detail {
 
 thread_local std::string g_err;
 
 void set_err(const char* where, const char* what) {
     g_err = std::string(where) + ": " + what;
 }
-
-#define ZQ_HIP(call, where)                                                   \
-    do {                                                                      \
-        hipError_t e_ = (call);                                               \
-        if (e_ != hipSuccess) {                                               \
-            set_err(where, hipGetErrorString(e_));                            \
-            return ZQ3D_ERR_HIP;                                              \
-        }                                                                     \
-    } while (0)
 
 constexpr long long kMaxCells = 1LL << 30;
 
@@ -50,25 +44,10 @@ int check_dims(int W, int H, int D, const char* where) {
     return ZQ3D_OK;
 }
 
-struct Sizes {
-    size_t N, Nu, Nv, Nw, Np;
-    explicit Sizes(const Dims& d)
-        : N((size_t)d.W * d.H * d.D),
-          Nu((size_t)(d.W + 1) * d.H * d.D),
-          Nv((size_t)d.W * (d.H + 1) * d.D),
-          Nw((size_t)d.W * d.H * (d.D + 1)),
-          Np((size_t)(d.W + 2) * (d.H + 2) * (d.D + 2)) {}
-};
+}  // namespace detail
+}  // namespace zq3d
 
-// Small per-call scalars in one device block.
-struct Scalars {
-    double mean64;
-    double zero64;
-    unsigned long long rmax;
-    unsigned long long dmax;
-    float mean32;
-    float zero32;
-};
+namespace {
 
 // ---------------- grow-only workspace for the host-pointer entry points ----------------
 enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_P2, S_B64, S_P64, S_PART, S_SCAL, S_COUNT };
@@ -128,35 +107,10 @@ struct Timer {
     }
 };
 
-// Collects the first HIP error of a sequence of calls.
-struct Errs {
-    hipError_t e = hipSuccess;
-    void operator()(hipError_t r) {
-        if (e == hipSuccess) e = r;
-    }
-};
+}  // namespace
 
-// Event timing of the red-black sweep loops (null when profiling is off).
-struct SorProf {
-    hipEvent_t a = nullptr, b = nullptr;
-    double ms = 0.0;
-    long long launches = 0;
-};
-
-// Device buffers of a projection.  p/p_alt: padded FP32 pressure (plain) or correction e (mixed),
-// ping-ponged by the fused sweep kernel; both keep a zero ghost layer.  part: reduction partials.
-struct ProjBufs {
-    uint8_t* nc;
-    float* b;       // FP32 divergence (plain) / FP32 residual (mixed)
-    float* p;
-    float* p_alt;
-    double* b64;
-    double* p64;
-    double* part;
-    Scalars* sc;
-    SorShape shape;  // sweep algorithm / tile shape ({0, 0}: default)
-    SplitBufs split; // colour-split scratch (used when shape.ty == SOR_SPLIT)
-};
+namespace zq3d {
+namespace detail {
 
 // launch_sor, timed when pf is set (the sweeps are the only work between the two events)
 hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p, float* p_alt,
@@ -190,7 +144,7 @@ int read_double_bits(const unsigned long long* dev, double* out, hipStream_t s, 
 // `mean` given, 1 = FP64 device mean over fluid cells.
 int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                   float mean, int mean_mode, const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s,
-                  SorProf* pf = nullptr) {
+                  SorProf* pf) {
     const Sizes z(d);
     Scalars* sc = B.sc;
     launch_fill_f32(B.p, z.Np, 0.0f, s);
@@ -224,7 +178,7 @@ int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
 // Mixed-precision projection: FP64 b, p and residual; FP32 inner SOR on the correction e.
 // p64 holds the warm start (zeroed by the caller for a cold start).
 int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
-                  const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf = nullptr) {
+                  const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf) {
     const Sizes z(d);
     Scalars* sc = B.sc;
     launch_fill_f32(&sc->zero32, 1, 0.0f, s);
@@ -270,7 +224,8 @@ int check_sor_opts(const zq3d_sor_opts& o, const char* where) {
     return ZQ3D_OK;
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace zq3d
 
 // =============================== Part 1 ===============================
 

@@ -1,0 +1,97 @@
+// zq3d_internal.h -- pieces of the C-ABI implementation shared by zq3d_api.cpp (Parts 1-3) and
+// zq3d_adjoint.cpp (Part 5): error reporting, sizes, projection drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "zq3d_kernels.h"
+#include "zq_smoke3d.h"
+
+namespace zq3d {
+namespace detail {
+
+extern thread_local std::string g_err;
+void set_err(const char* where, const char* what);
+int check_dims(int W, int H, int D, const char* where);
+int check_sor_opts(const zq3d_sor_opts& o, const char* where);
+
+#define ZQ_HIP(call, where)                                                   \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            zq3d::detail::set_err(where, hipGetErrorString(e_));              \
+            return ZQ3D_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+struct Sizes {
+    size_t N, Nu, Nv, Nw, Np;
+    explicit Sizes(const Dims& d)
+        : N((size_t)d.W * d.H * d.D),
+          Nu((size_t)(d.W + 1) * d.H * d.D),
+          Nv((size_t)d.W * (d.H + 1) * d.D),
+          Nw((size_t)d.W * d.H * (d.D + 1)),
+          Np((size_t)(d.W + 2) * (d.H + 2) * (d.D + 2)) {}
+};
+
+// Small per-call scalars in one device block.
+struct Scalars {
+    double mean64;
+    double zero64;
+    double aux64;   // the removed pressure mean of an adjoint projection
+    unsigned long long rmax;
+    unsigned long long dmax;
+    float mean32;
+    float zero32;
+};
+
+// Collects the first HIP error of a sequence of calls.
+struct Errs {
+    hipError_t e = hipSuccess;
+    void operator()(hipError_t r) {
+        if (e == hipSuccess) e = r;
+    }
+};
+
+// Event timing of the red-black sweep loops (null when profiling is off).
+struct SorProf {
+    hipEvent_t a = nullptr, b = nullptr;
+    double ms = 0.0;
+    long long launches = 0;
+};
+
+// Device buffers of a projection.  p/p_alt: padded FP32 pressure (plain) or correction e (mixed),
+// ping-ponged by the fused sweep kernel; both keep a zero ghost layer.  part: reduction partials.
+struct ProjBufs {
+    uint8_t* nc;
+    float* b;       // FP32 divergence (plain) / FP32 residual (mixed)
+    float* p;
+    float* p_alt;
+    double* b64;
+    double* p64;
+    double* part;
+    Scalars* sc;
+    SorShape shape;  // sweep algorithm / tile shape ({0, 0}: default)
+    SplitBufs split; // colour-split scratch (used when shape.ty == SOR_SPLIT)
+};
+
+// launch_sor, timed when pf is set (the sweeps are the only work between the two events)
+hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p, float* p_alt,
+                     float omega, int iters, hipStream_t s, SorProf* pf, float** result, SorShape shape,
+                     const SplitBufs* split);
+int read_double_bits(const unsigned long long* dev, double* out, hipStream_t s, const char* where);
+
+// Plain FP32 projection on device pointers: p = 0, setup, SOR, correct.  mean_mode: 0 = value
+// `mean` given, 1 = FP64 device mean over fluid cells.
+int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                  float mean, int mean_mode, const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s,
+                  SorProf* pf = nullptr);
+// Mixed-precision projection: FP64 b, p and residual; FP32 inner SOR on the correction e.
+// p64 holds the warm start (zeroed by the caller for a cold start).
+int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                  const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf = nullptr);
+
+}  // namespace detail
+}  // namespace zq3d
