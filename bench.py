@@ -10,6 +10,12 @@ radius 0.08 at (0.5, 0.1, 0.5) with density 1 and temperature 1, Tamb 0, seeded 
 temperature noise, dt 0.02, 20 advection substeps.  Inputs are resident in HBM before the
 timed region.
 
+fwd_adjoint: a 64-step window of the same workload, started from the state the timed steps
+reached, run forward with a checkpoint every 8 steps (sqrt of the window) and then through the
+hand-written adjoint (each earlier segment recomputed from its checkpoint), for the loss
+1/2 |rho_64 - target|^2 with a target of 8 seeded Gaussian blobs and per-step control forces
+(zero); value = voxels * window / (forward + backward wall time).
+
 Prints one JSON line.  value = voxels * steps / wall time (Mvoxel-steps/s).  roofline: the
 red-black SOR sweep kernel the autotuner picked (the dominant kernel), algorithmic bytes 13 B/voxel
 per red+black iteration (6.5 B/voxel per single-colour launch) over its event-timed mean launch
@@ -40,6 +46,7 @@ def parse():
     ap.add_argument("--max-outer", type=int, default=1000, help="refinement passes at most per step")
     ap.add_argument("--tol", type=float, default=1e-6, help="max|div| target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "sor_sweep_256_traffic.txt"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -111,6 +118,71 @@ def cpu_baseline(state, W, H, D, inner, outer_mean):
         "sample": ("oracle at %dx%dx%d on the GPU's state after the timed steps: one step with no refinement pass "
                    "(%.2f s) + %d SOR iterations (%.3f s); step time = %.2f + %.1f passes (GPU mean per step) x %.3f = "
                    "%.2f s" % (W, H, D, t_fixed, inner, t_pass, t_fixed, outer_mean, t_pass, step_s)),
+    }
+
+
+def gaussian_blobs(W, H, D, count, seed):
+    """Sum of `count` seeded Gaussian blobs: centres in [0.2, 0.8]^3, sigma in [0.05, 0.15],
+    amplitude in [0.5, 1] (domain units: x / W etc.)."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid((np.arange(D) + 0.5) / D, (np.arange(H) + 0.5) / H, (np.arange(W) + 0.5) / W,
+                          indexing="ij")
+    out = np.zeros((D, H, W), np.float32)
+    for _ in range(count):
+        c = rng.uniform(0.2, 0.8, size=3)
+        sig = rng.uniform(0.05, 0.15)
+        amp = rng.uniform(0.5, 1.0)
+        r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+        out += (amp * np.exp(-r2 / (2 * sig * sig))).astype(np.float32)
+    return out
+
+
+def adjoint_window(state, W, H, D, args, fwd_ms_per_step):
+    """Forward + adjoint of a window of args.adj_window steps from `state` (FP32, sqrt checkpoints)."""
+    import math
+
+    import torch
+    from zq_smokesimulation_amd.adjoint import SmokeRollout
+    n = args.adj_window
+    K = max(1, int(round(math.sqrt(n))))
+    eng = SmokeRollout(W, H, D, n, K, dtype=torch.float32, voxel_size=1.0 / W, step_time=0.02, steps=20, Tamb=0.0,
+                       buoyancy=0.5, gravity=0.05, confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99,
+                       sor_open_mask=0, sor_closed_mean=1, sor_omega=1.8, sor_iters=args.inner,
+                       sor_max_outer=args.max_outer, sor_tol=args.tol, warm_start=1)
+    eng.set_initial(**{k: torch.from_numpy(state[k]) for k in ("u", "v", "w", "T", "rho", "occ", "p64")})
+    eng.set_sources(spheres=[SOURCE])
+    target = torch.from_numpy(gaussian_blobs(W, H, D, 8, 0)).cuda()
+    controls = torch.zeros(eng.controls_shape(), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho = eng.forward(controls)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g = eng.backward(controls, rho - target)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fb, bb = eng.last_forward, eng.last_backward
+    gmax = float(g.abs().max())
+    finite = bool(torch.isfinite(g).all())
+    eng.close()
+    total = t2 - t0
+    return {
+        "value": round(W * H * D * n / total / 1e6, 3),
+        "unit": "Mvoxel-steps/s",
+        "window": n,
+        "checkpoint_every": K,
+        "ms_forward": round(1e3 * (t1 - t0), 1),
+        "ms_backward": round(1e3 * (t2 - t1), 1),
+        "ms_per_step": round(1e3 * total / n, 3),
+        "ratio_to_window_forward": round(total / (t1 - t0), 3),
+        "ratio_to_timed_forward": round(1e3 * total / n / fwd_ms_per_step, 3),
+        "recomputed_steps": bb["recomputed_steps"],
+        "forward_sweeps_per_step": round(fb["forward_sweeps"] / n, 1),
+        "adjoint_sweeps_per_step": round(bb["adjoint_sweeps"] / n, 1),
+        "max_div": fb["max_div"],
+        "grad_max_abs": gmax,
+        "grad_finite": finite,
+        "loss": "1/2 |rho_%d - target|^2, target = 8 seeded Gaussian blobs, controls = per-step cell forces" % n,
     }
 
 
@@ -229,14 +301,17 @@ def main():
                                ("ms_force", "ms_advect_velocity", "ms_project", "ms_sor", "ms_advect_scalar",
                                 "ms_reinject")},
     }
+    g = sim.get_state()
+    g["p64"] = sim.download(z._lib.FIELD_P64)
+    sim.close()
+    if args.adj_window > 0:
+        print("bench: forward+adjoint window of %d steps" % args.adj_window, file=sys.stderr, flush=True)
+        out["fwd_adjoint"] = adjoint_window(g, W, H, D, args, ms_per_step)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        g = sim.get_state()
-        g["p64"] = sim.download(z._lib.FIELD_P64)
         out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean)
     else:
         out["cpu_baseline"] = None
-    sim.close()
     print(json.dumps(out))
 
 

@@ -304,8 +304,10 @@ int zq3d_scene_apply_moving_objects(const char* path, int frames, uint8_t* occ, 
  * mode (the inner SOR sweeps stay FP32, the FP64 refinement converges to cfg->sor.tol).
  * Requires cfg->sor.mixed_precision, cfg->steps <= 32 and a non-BFECC advection mode.
  * Data pointers are device pointers of the chosen precision (occupancy: bytes), stream-ordered on
- * the rollout's stream.  The adjoint projection solves to adjoint_rtol * max|rhs| (default 1e-6),
- * warm-started from the previous adjoint step. */
+ * the rollout's stream.  The adjoint projection of step s is warm-started from the previous adjoint
+ * step and solved to rtol * max|rhs|, where rtol is set by zq3d_adj_set_adjoint_tol or, when that is
+ * 0 (the default), equals the forward projection's own tol / max|rhs| at step s (the same relative
+ * accuracy as the forward). */
 typedef struct zq3d_adj zq3d_adj;
 typedef struct zq3d_adj_stats {
     long long forward_sweeps;    /* SOR sweeps of the last call's forward projections (recomputation included) */
@@ -320,7 +322,8 @@ void zq3d_adj_destroy(zq3d_adj* a);
 int zq3d_adj_set_stream(zq3d_adj* a, void* stream);
 int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8, int n_sphere, const float* spheres6);
 int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol);
-/* initial state: ZQ3D_FIELD_U/V/W/T/RHO (device, chosen precision) or ZQ3D_FIELD_OCC (device bytes) */
+/* initial state: ZQ3D_FIELD_U/V/W/T/RHO (device, chosen precision), ZQ3D_FIELD_OCC (device bytes) or
+ * ZQ3D_FIELD_P64 (device doubles, padded: the forward projection's warm start; default 0) */
 int zq3d_adj_set_initial(zq3d_adj* a, int field, const void* dev);
 /* runs the rollout from the initial state.  controls: NULL or nsteps x (W*H*D) x 3 (per cell x, y, z);
  * rho_out: NULL or W*H*D, the density after the last step */

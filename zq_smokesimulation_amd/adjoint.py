@@ -17,7 +17,7 @@ from ._lib import check
 from .sim import default_config
 
 _FIELDS = {"u": _lib.FIELD_U, "v": _lib.FIELD_V, "w": _lib.FIELD_W, "T": _lib.FIELD_T, "rho": _lib.FIELD_RHO,
-           "occ": _lib.FIELD_OCC}
+           "occ": _lib.FIELD_OCC, "p64": _lib.FIELD_P64}
 
 
 def _ptr(t):
@@ -71,7 +71,7 @@ class SmokeRollout:
     def shape(self, name):
         W, H, D = self.W, self.H, self.D
         return {"u": (D, H, W + 1), "v": (D, H + 1, W), "w": (D + 1, H, W), "T": (D, H, W), "rho": (D, H, W),
-                "occ": (D, H, W)}[name]
+                "occ": (D, H, W), "p64": (D + 2, H + 2, W + 2)}[name]
 
     def _stream(self):
         s = torch.cuda.current_stream(self.device).cuda_stream
@@ -85,10 +85,11 @@ class SmokeRollout:
               "zq3d_adj_set_sources")
 
     def set_initial(self, **fields):
-        """u, v, w, T, rho (value dtype) and occ (uint8), any subset; unset fields stay zero."""
+        """u, v, w, T, rho (value dtype), occ (uint8) and p64 (float64, the padded warm start of the
+        forward projection), any subset; unset fields stay zero."""
         self._stream()
         for name, t in fields.items():
-            want = torch.uint8 if name == "occ" else self.dtype
+            want = torch.uint8 if name == "occ" else (torch.float64 if name == "p64" else self.dtype)
             t = torch.as_tensor(t).to(device=self.device, dtype=want).contiguous()
             if tuple(t.shape) != self.shape(name):
                 raise ValueError("%s: shape %s, expected %s" % (name, tuple(t.shape), self.shape(name)))
@@ -96,6 +97,7 @@ class SmokeRollout:
         torch.cuda.current_stream(self.device).synchronize()
 
     def set_adjoint_tol(self, rtol):
+        """Adjoint projection tolerance relative to max|rhs|; 0 matches each step's forward projection."""
         check(_lib.lib().zq3d_adj_set_adjoint_tol(self._h, float(rtol)), "zq3d_adj_set_adjoint_tol")
 
     # ---- rollout -------------------------------------------------------------------

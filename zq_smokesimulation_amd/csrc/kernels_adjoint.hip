@@ -27,6 +27,18 @@ inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (
 __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 
+// LDS atomics (ds_add_f32 / ds_add_f64).  The window pointer travels through a struct, where the
+// compiler no longer sees that it addresses LDS and would emit flat atomics; the explicit address
+// space keeps them on the LDS path.
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(3))) double lds_double;
+__device__ __forceinline__ void lds_add(float* p, float v) {
+    __hip_atomic_fetch_add((lds_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(double* p, double v) {
+    __hip_atomic_fetch_add((lds_double*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Texels and weights of one trilinear sample (the forward's axis_setup).
 template <class R>
 struct Tri {
@@ -169,6 +181,260 @@ __device__ __forceinline__ void reverse_path(const VA& A, const R3<R>* q, int n,
         const R3<R> jt = A.vjp(p.x, p.y, p.z, g);
         g = R3<R>{g.x - c * jt.x, g.y - c * jt.y, g.z - c * jt.z};
     }
+}
+
+// ---- tile-local accumulation of the cell-grid scatters ----
+// The backtrace paths of a tile of neighbouring cells stay close together, so the taps they
+// scatter into fill a small box.  A block first traces its paths and reduces the box of every tap
+// it will touch; when the box fits CAP cells the scatters go to an LDS window (ds atomics) that is
+// flushed to HBM once, one global atomic per nonzero entry, instead of one per tap per path step.
+// Larger boxes (very fast flow) use global atomics directly.  NA arrays of W*H*D cells.
+constexpr int TX = 8, TY = 8, TZ = 4;
+
+template <class R, int NA, int CAP>
+struct TileWin {
+    R* s;  // LDS, NA * CAP
+    R* const* g;
+    Dims d;
+    int ox, oy, oz, wx, wy;
+    bool lds;
+    __device__ __forceinline__ void add(int a, int i, int j, int k, R v) {
+        if (lds) lds_add(&s[a * CAP + ((k - oz) * wy + (j - oy)) * wx + (i - ox)], v);
+        else atomic_add(g[a] + cidx(d, i, j, k), v);
+    }
+    __device__ __forceinline__ void scatter(int a, const Tri<R>& t, R v) {
+        if (v == R(0)) return;
+        const R bx = R(1) - t.ax, by = R(1) - t.ay, bz = R(1) - t.az;
+        add(a, t.i0, t.j0, t.k0, v * bx * by * bz);
+        add(a, t.i1, t.j0, t.k0, v * t.ax * by * bz);
+        add(a, t.i0, t.j1, t.k0, v * bx * t.ay * bz);
+        add(a, t.i1, t.j1, t.k0, v * t.ax * t.ay * bz);
+        add(a, t.i0, t.j0, t.k1, v * bx * by * t.az);
+        add(a, t.i1, t.j0, t.k1, v * t.ax * by * t.az);
+        add(a, t.i0, t.j1, t.k1, v * bx * t.ay * t.az);
+        add(a, t.i1, t.j1, t.k1, v * t.ax * t.ay * t.az);
+    }
+};
+
+__device__ __forceinline__ void box_grow(int* b, int i0, int i1, int j0, int j1, int k0, int k1) {
+    b[0] = min(b[0], i0);
+    b[1] = min(b[1], j0);
+    b[2] = min(b[2], k0);
+    b[3] = max(b[3], i1);
+    b[4] = max(b[4], j1);
+    b[5] = max(b[5], k1);
+}
+
+template <class R>
+__device__ __forceinline__ Tri<R> cell_tri(const Dims& d, const R3<R>& p) {
+    return tri<R>(d.W, d.H, d.D, p.x - R(0.5), p.y - R(0.5), p.z - R(0.5));
+}
+
+// Reduces the per-thread boxes, decides LDS vs global, zeroes the window; all threads call it.
+template <class R, int NA, int CAP>
+__device__ __forceinline__ bool tile_setup(TileWin<R, NA, CAP>& w, const int* mine, int* sbox, int tid) {
+    if (tid == 0) {
+        sbox[0] = sbox[1] = sbox[2] = 0x7fffffff;
+        sbox[3] = sbox[4] = sbox[5] = -1;
+    }
+    __syncthreads();
+    if (mine[3] >= 0) {
+        atomicMin(&sbox[0], mine[0]);
+        atomicMin(&sbox[1], mine[1]);
+        atomicMin(&sbox[2], mine[2]);
+        atomicMax(&sbox[3], mine[3]);
+        atomicMax(&sbox[4], mine[4]);
+        atomicMax(&sbox[5], mine[5]);
+    }
+    __syncthreads();
+    if (sbox[3] < 0) return false;  // nothing to scatter in this tile
+    w.ox = sbox[0];
+    w.oy = sbox[1];
+    w.oz = sbox[2];
+    w.wx = sbox[3] - sbox[0] + 1;
+    w.wy = sbox[4] - sbox[1] + 1;
+    const int vol = w.wx * w.wy * (sbox[5] - sbox[2] + 1);
+    w.lds = vol <= CAP;
+    if (w.lds)
+        for (int x = tid; x < NA * CAP; x += TX * TY * TZ) w.s[x] = R(0);
+    __syncthreads();
+    return true;
+}
+
+template <class R, int NA, int CAP>
+__device__ __forceinline__ void tile_flush(TileWin<R, NA, CAP>& w, const int* sbox, int tid) {
+    __syncthreads();
+    if (!w.lds) return;
+    const int wz = sbox[5] - sbox[2] + 1;
+    const int vol = w.wx * w.wy * wz;
+    for (int x = tid; x < vol; x += TX * TY * TZ) {
+        const int i = w.ox + x % w.wx;
+        const int j = w.oy + (x / w.wx) % w.wy;
+        const int k = w.oz + x / (w.wx * w.wy);
+        for (int a = 0; a < NA; a++) {
+            const R v = w.s[a * CAP + x];
+            if (v != R(0)) atomic_add(w.g[a] + cidx(w.d, i, j, k), v);
+        }
+    }
+}
+
+// Carry the end point's adjoint back along the path into the velocity adjoint (window arrays 0..2).
+// Consecutive path points usually share a trilinear cell: the 8 taps are loaded once per cell
+// (Cell4Taps) and the path's weighted contributions to them are summed in registers, reaching the
+// window only when the path moves to another cell.
+template <class R, int NA, int CAP>
+__device__ __forceinline__ void reverse_path_tile(const vec4_t<R>* __restrict__ vel4, const Dims& d,
+                                                  TileWin<R, NA, CAP>& w, const R3<R>* q, int n, R c, R3<R> g) {
+    Cell4Taps<R> cc;
+    R acc[8][3];
+    auto flush = [&]() {
+        if (cc.i0 < 0) return;
+        const int ti[8] = {cc.i0, cc.i1, cc.i0, cc.i1, cc.i0, cc.i1, cc.i0, cc.i1};
+        const int tj[8] = {cc.j0, cc.j0, cc.j1, cc.j1, cc.j0, cc.j0, cc.j1, cc.j1};
+        const int tk[8] = {cc.k0, cc.k0, cc.k0, cc.k0, cc.k1, cc.k1, cc.k1, cc.k1};
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (acc[t][a] != R(0)) w.add(a, ti[t], tj[t], tk[t], acc[t][a]);
+    };
+    for (int m = n - 2; m >= 0; m--) {
+        const R3<R> p = q[m];
+        const Tri<R> t = cell_tri<R>(d, p);
+        if (!cc.same(t.i0, t.i1, t.j0, t.j1, t.k0, t.k1)) {
+            flush();
+            cc.fetch(vel4, d.W, d.H, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+#pragma unroll
+            for (int x = 0; x < 8; x++) acc[x][0] = acc[x][1] = acc[x][2] = R(0);
+        }
+        // put -c g at p: weights of the 8 taps (order t000 t100 t010 t110 t001 t101 t011 t111)
+        const R bx = R(1) - t.ax, by = R(1) - t.ay, bz = R(1) - t.az;
+        const R wt[8] = {bx * by * bz,   t.ax * by * bz,   bx * t.ay * bz,   t.ax * t.ay * bz,
+                         bx * by * t.az, t.ax * by * t.az, bx * t.ay * t.az, t.ax * t.ay * t.az};
+        const R gx = -c * g.x, gy = -c * g.y, gz = -c * g.z;
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            acc[x][0] += gx * wt[x];
+            acc[x][1] += gy * wt[x];
+            acc[x][2] += gz * wt[x];
+        }
+        // J^T g from the cached taps
+        auto dot = [&](const vec4_t<R>& v) { return g.x * v.x + g.y * v.y + g.z * v.z; };
+        const R3<R> jt = grad_taps<R>(dot(cc.t000), dot(cc.t100), dot(cc.t010), dot(cc.t110), dot(cc.t001),
+                                      dot(cc.t101), dot(cc.t011), dot(cc.t111), t);
+        g = R3<R>{g.x - c * jt.x, g.y - c * jt.y, g.z - c * jt.z};
+    }
+    flush();
+}
+
+// inRegular_outRegular velocity advection adjoint, tiled (see k_adj_advect_cells).
+template <class R, int CAP>
+__global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_cells_tile(RegularVel<R> V, R* __restrict__ vb,
+                                                                      const uint8_t* __restrict__ occ,
+                                                                      const R* __restrict__ advb, R h, int steps,
+                                                                      R dtt) {
+    __shared__ R swin[3 * CAP];
+    __shared__ int sbox[6];
+    const Dims d = V.d;
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TX + (tid % TX);
+    const int j = blockIdx.y * TY + (tid / TX) % TY;
+    const int k = blockIdx.z * TZ + tid / (TX * TY);
+    const size_t N = (size_t)d.W * d.H * d.D;
+    R* const gptr[3] = {vb, vb + N, vb + 2 * N};
+    TileWin<R, 3, CAP> w{swin, gptr, d, 0, 0, 0, 0, 0, false};
+    R3<R> q[ZQ3D_ADJ_MAX_STEPS];
+    int n = 0;
+    R3<R> ab{R(0), R(0), R(0)}, last{R(0), R(0), R(0)};
+    int mine[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
+    bool act = i < d.W && j < d.H && k < d.D;
+    if (act) {
+        const int o = cidx(d, i, j, k);
+        ab = R3<R>{advb[o], advb[N + o], advb[2 * N + o]};
+        act = !(ab.x == R(0) && ab.y == R(0) && ab.z == R(0));
+    }
+    if (act) {
+        const R sx = i + R(0.5), sy = j + R(0.5), sz = k + R(0.5);
+        const RegularVelCached<R> Vc{V.vel4, d, {}};
+        n = trace_path<R>(Vc, occ, d, h, steps, dtt, sx, sy, sz, q);
+        last = n ? q[n - 1] : R3<R>{sx, sy, sz};
+        Tri<R> t = cell_tri<R>(d, last);
+        box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+        for (int m = 0; m < n - 1; m++) {
+            t = cell_tri<R>(d, q[m]);
+            box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+        }
+    }
+    if (!tile_setup<R, 3, CAP>(w, mine, sbox, tid)) return;
+    if (act) {
+        const Tri<R> t = cell_tri<R>(d, last);
+        w.scatter(0, t, ab.x);
+        w.scatter(1, t, ab.y);
+        w.scatter(2, t, ab.z);
+        if (n >= 2) {
+            const RegularVelAdj<R> A{V.vel4, nullptr, d};
+            reverse_path_tile<R, 3, CAP>(V.vel4, d, w, q, n, dtt / h, A.vjp(last.x, last.y, last.z, ab));
+        }
+    }
+    tile_flush<R, 3, CAP>(w, sbox, tid);
+}
+
+// Scalar advection adjoint through the cell velocity, tiled: window arrays 0..2 velocity adjoint,
+// 3 and 4 the adjoints of the advected T and rho.
+template <class R, int CAP>
+__global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_scalar_tile(RegularVel<R> V, R* __restrict__ vb,
+                                                                       const uint8_t* __restrict__ occ,
+                                                                       const R* __restrict__ Tin,
+                                                                       const R* __restrict__ Rin,
+                                                                       const R* __restrict__ Tb,
+                                                                       const R* __restrict__ Rb, R* __restrict__ Tb_in,
+                                                                       R* __restrict__ Rb_in, R h, int steps, R dtt) {
+    __shared__ R swin[5 * CAP];
+    __shared__ int sbox[6];
+    const Dims d = V.d;
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TX + (tid % TX);
+    const int j = blockIdx.y * TY + (tid / TX) % TY;
+    const int k = blockIdx.z * TZ + tid / (TX * TY);
+    const size_t N = (size_t)d.W * d.H * d.D;
+    R* const gptr[5] = {vb, vb + N, vb + 2 * N, Tb_in, Rb_in};
+    TileWin<R, 5, CAP> w{swin, gptr, d, 0, 0, 0, 0, 0, false};
+    R3<R> q[ZQ3D_ADJ_MAX_STEPS];
+    int n = 0;
+    R tb = R(0), rb = R(0);
+    R3<R> last{R(0), R(0), R(0)};
+    int mine[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
+    bool act = i < d.W && j < d.H && k < d.D;
+    if (act) {
+        const int o = cidx(d, i, j, k);
+        tb = Tb[o];
+        rb = Rb[o];
+        act = !(tb == R(0) && rb == R(0));
+    }
+    if (act) {
+        const R sx = i + R(0.5), sy = j + R(0.5), sz = k + R(0.5);
+        const RegularVelCached<R> Vc{V.vel4, d, {}};
+        n = trace_path<R>(Vc, occ, d, h, steps, dtt, sx, sy, sz, q);
+        last = n ? q[n - 1] : R3<R>{sx, sy, sz};
+        Tri<R> t = cell_tri<R>(d, last);
+        box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+        for (int m = 0; m < n - 1; m++) {
+            t = cell_tri<R>(d, q[m]);
+            box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+        }
+    }
+    if (!tile_setup<R, 5, CAP>(w, mine, sbox, tid)) return;
+    if (act) {
+        const Tri<R> t = cell_tri<R>(d, last);
+        w.scatter(3, t, tb);
+        w.scatter(4, t, rb);
+        if (n >= 2) {
+            const R3<R> gt = grad3<R>(Tin, d.W, d.H, t), gr = grad3<R>(Rin, d.W, d.H, t);
+            const R3<R> g{tb * gt.x + rb * gr.x, tb * gt.y + rb * gr.y, tb * gt.z + rb * gr.z};
+            reverse_path_tile<R, 5, CAP>(V.vel4, d, w, q, n, dtt / h, g);
+        }
+    }
+    tile_flush<R, 5, CAP>(w, sbox, tid);
 }
 
 // ---- scalar advection (Advect_Scalar_*, ZQ_CUDA_Advection3D.cu:379-469) ----
@@ -495,6 +761,14 @@ __global__ __launch_bounds__(BX* BY) void k_adj_force_faces(R* __restrict__ face
     face[fi] = x;
 }
 
+inline dim3 tile_grid(const Dims& d) { return dim3((d.W + TX - 1) / TX, (d.H + TY - 1) / TY, (d.D + TZ - 1) / TZ); }
+
+// LDS window capacity in cells: 40 KB for the five-array window
+template <class R>
+constexpr int tile_cap() {
+    return sizeof(R) == 4 ? 2048 : 1024;
+}
+
 }  // namespace
 
 template <class R>
@@ -527,9 +801,8 @@ void launch_adj_advect_velocity(const Dims& d, const R* u, const R* v, const R* 
                            is_open);
         launch_fill_adj<R>(cellb, 3 * N, s);
         const RegularVel<R> V{vel4, d};
-        const RegularVelAdj<R> A{vel4, cellb, d};
-        hipLaunchKernelGGL(k_adj_advect_cells<R>, grid_for(d.W, d.H, d.D), blk, 0, s, V, A, occ, advb, d, h, ap.steps,
-                           dtt);
+        hipLaunchKernelGGL((k_adj_advect_cells_tile<R, tile_cap<R>()>), tile_grid(d), dim3(TX * TY * TZ), 0, s, V,
+                           cellb, occ, advb, h, ap.steps, dtt);
         hipLaunchKernelGGL((k_adj_vel_to_faces<R, 0, true>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, ub, (R*)nullptr,
                            cellb, occ, d, is_open);
         hipLaunchKernelGGL((k_adj_vel_to_faces<R, 1, true>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, vb, (R*)nullptr,
@@ -592,9 +865,8 @@ void launch_adj_advect_scalar(const Dims& d, const R* u, const R* v, const R* w,
     }
     launch_fill_adj<R>(cellb, 3 * N, s);
     const RegularVel<R> V{vel4, d};
-    const RegularVelAdj<R> A{vel4, cellb, d};
-    hipLaunchKernelGGL((k_adj_advect_scalar<R, RegularVel<R>, RegularVelAdj<R>>), grid_for(d.W, d.H, d.D), blk, 0, s,
-                       V, A, occ, Tin, Rin, Tb, Rb, Tb_in, Rb_in, d, h, ap.steps, dtt);
+    hipLaunchKernelGGL((k_adj_advect_scalar_tile<R, tile_cap<R>()>), tile_grid(d), dim3(TX * TY * TZ), 0, s, V, cellb,
+                       occ, Tin, Rin, Tb, Rb, Tb_in, Rb_in, h, ap.steps, dtt);
     launch_add_cells_to_faces<R>(d, cellb, ub, vb, wb, s);
 }
 

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(BX* BY) void k_advect_cells(const vec4_t<R>* __rest
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    const RegularVel<R> V{vel4, d};
+    const RegularVelCached<R> V{vel4, d, {}};
     const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
     const R3<R> s = V(last.x, last.y, last.z);
     const int o = cidx(d, i, j, k);
@@ -177,13 +177,13 @@ void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ,
     }
     if (mode == 4) {
         launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
-        const RegularVel<R> V{vel4, d};
-        hipLaunchKernelGGL((k_advect_faces<R, 0, RegularVel<R>>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou,
-                           d, h, ap.steps, dtt);
-        hipLaunchKernelGGL((k_advect_faces<R, 1, RegularVel<R>>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ, ov,
-                           d, h, ap.steps, dtt);
-        hipLaunchKernelGGL((k_advect_faces<R, 2, RegularVel<R>>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ, ow,
-                           d, h, ap.steps, dtt);
+        const RegularVelCached<R> V{vel4, d, {}};
+        hipLaunchKernelGGL((k_advect_faces<R, 0, RegularVelCached<R>>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ,
+                           ou, d, h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_advect_faces<R, 1, RegularVelCached<R>>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, occ,
+                           ov, d, h, ap.steps, dtt);
+        hipLaunchKernelGGL((k_advect_faces<R, 2, RegularVelCached<R>>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, occ,
+                           ow, d, h, ap.steps, dtt);
     } else {  // mode 2: inMAC_outMAC
         const MacVel<R> V{u, v, w, d};
         hipLaunchKernelGGL((k_advect_faces<R, 0, MacVel<R>>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ, ou, d,
@@ -210,9 +210,9 @@ void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, con
                            Rout, d, h, ap.steps, dtt);
     } else {
         launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
-        const RegularVel<R> V{vel4, d};
-        hipLaunchKernelGGL((k_advect_scalar<R, RegularVel<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin,
-                           Tout, Rout, d, h, ap.steps, dtt);
+        const RegularVelCached<R> V{vel4, d, {}};
+        hipLaunchKernelGGL((k_advect_scalar<R, RegularVelCached<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin,
+                           Rin, Tout, Rout, d, h, ap.steps, dtt);
     }
 }
 

@@ -26,11 +26,12 @@ using namespace zq3d::detail;
 
 namespace {
 
-// Mixed-precision projection on faces of type F.  adjoint: right-hand side from the free faces,
-// tolerance rtol * max|rhs|, closed-domain constant removed, correction on every face.
+// Mixed-precision projection on faces of type F.  Forward: tolerance o.tol on max|r|.  adjoint:
+// right-hand side from the free faces, tolerance rtol * max|rhs|, closed-domain constant removed,
+// correction on every face.  *bmax (nullable) receives max|rhs| over counted cells.
 template <class F>
 int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_sor_opts& o, const ProjBufs& B,
-               bool adjoint, double rtol, zq3d_proj_stats* st, hipStream_t s) {
+               bool adjoint, double rtol, zq3d_proj_stats* st, hipStream_t s, double* bmax_out) {
     const char* where = adjoint ? "adjoint projection" : "projection";
     const Sizes z(d);
     Scalars* sc = B.sc;
@@ -48,11 +49,12 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
     }
     launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
     double tol = o.tol;
-    if (adjoint) {
+    if (adjoint || bmax_out) {
         double bmax = 0.0;
         launch_max_abs64(d, B.nc, B.b64, B.part, &sc->dmax, s);
         if (int rc = read_double_bits(&sc->dmax, &bmax, s, where)) return rc;
-        tol = rtol * bmax;
+        if (adjoint) tol = rtol * bmax;
+        if (bmax_out) *bmax_out = bmax;
     }
     float* e = B.p;
     float* e_alt = B.p_alt;
@@ -99,11 +101,13 @@ struct zq3d_adj {
     int nsteps = 0, K = 1, ncp = 0;
     hipStream_t stream = nullptr;
     std::vector<float> boxes, spheres, boxes0, spheres0;  // *0: source values zeroed (adjoint)
-    double rtol = 1e-6;
+    double rtol = 0.0;          // adjoint projection tolerance relative to max|rhs|; 0: match the forward
+    std::vector<double> fwd_rel;  // per step: forward tolerance / max|rhs| of its projection
     std::vector<void*> allocs;
     // state: u, v, w, T, rho, T2, rho2 (value type), vel4 (4-wide), scratch faces (Nu+Nv+Nw)
     void *u, *v, *w, *T, *R, *T2, *R2, *vel4, *scratch;
     void *u0, *v0, *w0, *T0, *R0;  // initial state
+    double* p640;                  // initial warm start of the forward projection
     uint8_t* occ;
     // projection
     uint8_t* nc;
@@ -221,7 +225,10 @@ int forward_step(zq3d_adj* a, int s, const R* controls, Slot<R>* sl, zq3d_adj_st
     launch_advect_velocity<R>(d, u, v, w, a->occ, 1, c.adv_vel_type, ap, (vec4_t<R>*)a->vel4, (R*)a->scratch,
                               a->stream);
     zq3d_proj_stats ps{};
-    if (int rc = project_mp<R>(d, u, v, w, a->occ, c.sor, proj_bufs(a, a->p64), false, 0.0, &ps, a->stream)) return rc;
+    double bmax = 0.0;
+    if (int rc = project_mp<R>(d, u, v, w, a->occ, c.sor, proj_bufs(a, a->p64), false, 0.0, &ps, a->stream, &bmax))
+        return rc;
+    a->fwd_rel[s] = bmax > 0.0 ? c.sor.tol / bmax : 1.0;
     if (sl) ZQ_HIP(copy_faces<R>(a, sl->up, sl->vp, sl->wp, u, v, w), where);
     launch_advect_scalar<R>(d, u, v, w, a->occ, T, Rr, (R*)a->T2, (R*)a->R2, c.adv_sca_type == 0 ? 1 : 0, ap,
                             (vec4_t<R>*)a->vel4, a->stream);
@@ -262,7 +269,10 @@ int adjoint_step(zq3d_adj* a, int s, const Slot<R>& sl, R* controls_bar, zq3d_ad
     std::swap(a->Rb, a->Rb2);
     // projection
     zq3d_proj_stats ps{};
-    if (int rc = project_mp<R>(d, ub, vb, wb, a->occ, c.sor, proj_bufs(a, a->p64_adj), true, a->rtol, &ps, a->stream))
+    // the same relative accuracy as the forward projection of this step, unless set explicitly
+    const double rtol = a->rtol > 0.0 ? a->rtol : std::min(1.0, a->fwd_rel[s]);
+    if (int rc = project_mp<R>(d, ub, vb, wb, a->occ, c.sor, proj_bufs(a, a->p64_adj), true, rtol, &ps, a->stream,
+                               nullptr))
         return rc;
     // velocity advection of u_f
     if (c.adv_vel_type != 2) launch_mac_to_cell4<R>(d, sl.uf, sl.vf, sl.wf, vel4, a->stream);
@@ -286,7 +296,7 @@ int run_forward(zq3d_adj* a, const R* controls, R* rho_out, zq3d_adj_stats* st) 
     ZQ_HIP(copy_faces<R>(a, (R*)a->u, (R*)a->v, (R*)a->w, (const R*)a->u0, (const R*)a->v0, (const R*)a->w0), where);
     ZQ_HIP(hipMemcpyAsync(a->T, a->T0, z.N * sizeof(R), hipMemcpyDeviceToDevice, a->stream), where);
     ZQ_HIP(hipMemcpyAsync(a->R, a->R0, z.N * sizeof(R), hipMemcpyDeviceToDevice, a->stream), where);
-    launch_fill_f64(a->p64, z.Np, 0.0, a->stream);
+    ZQ_HIP(hipMemcpyAsync(a->p64, a->p640, z.Np * 8, hipMemcpyDeviceToDevice, a->stream), where);
     const int last_seg = a->ncp - 1;
     for (int s = 0; s < a->nsteps; s++) {
         if (s % a->K == 0) ZQ_HIP(checkpoint<R>(a, s / a->K, true), where);
@@ -379,6 +389,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     a->prec = precision;
     a->nsteps = nsteps;
     a->K = std::min(checkpoint_every, nsteps);
+    a->fwd_rel.assign(nsteps, 1.0);
     a->ncp = (nsteps + a->K - 1) / a->K;
     const Sizes z(a->d);
     const size_t e = (size_t)precision;
@@ -395,6 +406,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     A((void**)&a->split_b, 2 * split_cell_size(a->d) * 4);
     A((void**)&a->split_n, 2 * split_cell_size(a->d));
     A((void**)&a->b64, z.N * 8); A((void**)&a->p64, z.Np * 8); A((void**)&a->p64_adj, z.Np * 8);
+    A((void**)&a->p640, z.Np * 8);
     A((void**)&a->part, (size_t)reduction_partials(a->d) * 8);
     A((void**)&a->sc, sizeof(Scalars));
     a->cp_bytes = (faces + 2 * z.N) * e + z.Np * 8;
@@ -423,6 +435,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     (void)hipMemsetAsync(a->split_p, 0, 2 * split_padded_size(a->d) * 4, s);
     (void)hipMemsetAsync(a->p64, 0, z.Np * 8, s);
     (void)hipMemsetAsync(a->p64_adj, 0, z.Np * 8, s);
+    (void)hipMemsetAsync(a->p640, 0, z.Np * 8, s);
     if (hipStreamSynchronize(s) != hipSuccess) {
         set_err(where, "initialisation failed");
         zq3d_adj_destroy(a);
@@ -459,7 +472,7 @@ extern "C" int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8,
 }
 
 extern "C" int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol) {
-    if (!a || !(rtol > 0.0)) { set_err("zq3d_adj_set_adjoint_tol", "bad argument"); return ZQ3D_ERR_ARG; }
+    if (!a || !(rtol >= 0.0)) { set_err("zq3d_adj_set_adjoint_tol", "bad argument"); return ZQ3D_ERR_ARG; }
     a->rtol = rtol;
     return ZQ3D_OK;
 }
@@ -476,6 +489,7 @@ int field_ptr(zq3d_adj* a, int field, bool initial, bool grad, void** p, size_t*
     case ZQ3D_FIELD_T: *p = grad ? a->Tb : (initial ? a->T0 : a->T); *bytes = z.N * e; return 0;
     case ZQ3D_FIELD_RHO: *p = grad ? a->Rb : (initial ? a->R0 : a->R); *bytes = z.N * e; return 0;
     case ZQ3D_FIELD_OCC: if (grad) return 1; *p = a->occ; *bytes = z.N; return 0;
+    case ZQ3D_FIELD_P64: if (grad || !initial) return 1; *p = a->p640; *bytes = z.Np * 8; return 0;
     default: return 1;
     }
 }

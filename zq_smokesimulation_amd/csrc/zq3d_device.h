@@ -140,6 +140,60 @@ struct RegularVel {
     }
 };
 
+// The 8 taps of one trilinear cell of a 4-wide grid, kept between samples: consecutive backtrace
+// positions move a small fraction of a cell, so most samples of a path reuse the taps of the
+// previous one and load nothing.  The arithmetic is sample4's, so results are identical.
+template <class R>
+struct Cell4Taps {
+    int i0 = -1, i1 = -1, j0 = -1, j1 = -1, k0 = -1, k1 = -1;
+    vec4_t<R> t000, t100, t010, t110, t001, t101, t011, t111;
+    __device__ __forceinline__ bool same(int a0, int a1, int b0, int b1, int c0, int c1) const {
+        return a0 == i0 && a1 == i1 && b0 == j0 && b1 == j1 && c0 == k0 && c1 == k1;
+    }
+    __device__ __forceinline__ void fetch(const vec4_t<R>* __restrict__ g, int nx, int ny, int a0, int a1, int b0,
+                                          int b1, int c0, int c1) {
+        if (same(a0, a1, b0, b1, c0, c1)) return;
+        i0 = a0, i1 = a1, j0 = b0, j1 = b1, k0 = c0, k1 = c1;
+        const int r00 = (c0 * ny + b0) * nx, r10 = (c0 * ny + b1) * nx;
+        const int r01 = (c1 * ny + b0) * nx, r11 = (c1 * ny + b1) * nx;
+        t000 = g[r00 + a0], t100 = g[r00 + a1], t010 = g[r10 + a0], t110 = g[r10 + a1];
+        t001 = g[r01 + a0], t101 = g[r01 + a1], t011 = g[r11 + a0], t111 = g[r11 + a1];
+    }
+};
+
+// RegularVel with a per-thread tap cache (one instance per thread; not shared).
+template <class R>
+struct RegularVelCached {
+    const vec4_t<R>* vel4;
+    Dims d;
+    mutable Cell4Taps<R> c;
+    __device__ __forceinline__ R3<R> operator()(R px, R py, R pz) const {
+        int i0, i1, j0, j1, k0, k1;
+        R ax, ay, az;
+        axis_setup(px - R(0.5), d.W, i0, i1, ax);
+        axis_setup(py - R(0.5), d.H, j0, j1, ay);
+        axis_setup(pz - R(0.5), d.D, k0, k1, az);
+        c.fetch(vel4, d.W, d.H, i0, i1, j0, j1, k0, k1);
+        R3<R> out;
+        {
+            const R c00 = lerp(c.t000.x, c.t100.x, ax), c10 = lerp(c.t010.x, c.t110.x, ax);
+            const R c01 = lerp(c.t001.x, c.t101.x, ax), c11 = lerp(c.t011.x, c.t111.x, ax);
+            out.x = lerp(lerp(c00, c10, ay), lerp(c01, c11, ay), az);
+        }
+        {
+            const R c00 = lerp(c.t000.y, c.t100.y, ax), c10 = lerp(c.t010.y, c.t110.y, ax);
+            const R c01 = lerp(c.t001.y, c.t101.y, ax), c11 = lerp(c.t011.y, c.t111.y, ax);
+            out.y = lerp(lerp(c00, c10, ay), lerp(c01, c11, ay), az);
+        }
+        {
+            const R c00 = lerp(c.t000.z, c.t100.z, ax), c10 = lerp(c.t010.z, c.t110.z, ax);
+            const R c01 = lerp(c.t001.z, c.t101.z, ax), c11 = lerp(c.t011.z, c.t111.z, ax);
+            out.z = lerp(lerp(c00, c10, ay), lerp(c01, c11, ay), az);
+        }
+        return out;
+    }
+};
+
 template <class R>
 struct MacVel {
     const R* u;
