@@ -121,28 +121,13 @@ def cpu_baseline(state, W, H, D, inner, outer_mean):
     }
 
 
-def gaussian_blobs(W, H, D, count, seed):
-    """Sum of `count` seeded Gaussian blobs: centres in [0.2, 0.8]^3, sigma in [0.05, 0.15],
-    amplitude in [0.5, 1] (domain units: x / W etc.)."""
-    rng = np.random.default_rng(seed)
-    z, y, x = np.meshgrid((np.arange(D) + 0.5) / D, (np.arange(H) + 0.5) / H, (np.arange(W) + 0.5) / W,
-                          indexing="ij")
-    out = np.zeros((D, H, W), np.float32)
-    for _ in range(count):
-        c = rng.uniform(0.2, 0.8, size=3)
-        sig = rng.uniform(0.05, 0.15)
-        amp = rng.uniform(0.5, 1.0)
-        r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
-        out += (amp * np.exp(-r2 / (2 * sig * sig))).astype(np.float32)
-    return out
-
-
 def adjoint_window(state, W, H, D, args, fwd_ms_per_step):
     """Forward + adjoint of a window of args.adj_window steps from `state` (FP32, sqrt checkpoints)."""
     import math
 
     import torch
     from zq_smokesimulation_amd.adjoint import SmokeRollout
+    from zq_smokesimulation_amd.gradcheck import gaussian_blobs
     n = args.adj_window
     K = max(1, int(round(math.sqrt(n))))
     eng = SmokeRollout(W, H, D, n, K, dtype=torch.float32, voxel_size=1.0 / W, step_time=0.02, steps=20, Tamb=0.0,

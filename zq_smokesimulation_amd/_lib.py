@@ -114,6 +114,13 @@ def _load():
         raise ImportError(
             "libzq_smoke3d.so not found at %s: build it first (python -c 'import __graft_entry__ as g; g.build()')"
             % LIB_PATH)
+    # One HIP runtime per process: torch ships its own libamdhip64 (same SONAME, libamdhip64.so.7).
+    # Loaded first, it also serves this library, so torch tensors and streams and these kernels
+    # share one runtime; loaded second, it would bring up a second HIP/HSA runtime beside ours.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     f, i, u, vp, b = C.c_float, C.c_int, C.c_uint, C.c_void_p, C.c_bool
     L.ZQ_Cuda_Prepare_Advection.argtypes = [u, u, u, f, u, f]
