@@ -335,6 +335,46 @@ int zq3d_adj_backward(zq3d_adj* a, const void* controls, const void* rho_bar, vo
 /* after a forward: the final state field; after a backward: with grad != 0, dL/d(initial field) */
 int zq3d_adj_get(zq3d_adj* a, int field, int grad, void* dev);
 
+/* ====================== Part 6: ensembles of uneven grids ====================== */
+
+/* No reference counterpart.  count independent simulations, each with its own zq3d_config (grid
+ * size and physics; ADV_VEL_INREG_OUTREG, ADV_SCA_REG and the plain FP32 projection with its own
+ * sweep count), stepped together: every stage is one persistent launch over all members' tiles,
+ * and the step (about 20 launches plus two per SOR iteration) is captured once in a hipGraph and
+ * replayed.  Each member evolves bit for bit as a zq3d_sim of the same config.  Runs on its own
+ * HIP stream (default priority). */
+typedef struct zq3d_ens zq3d_ens;
+int zq3d_ens_create(int count, const zq3d_config* configs, zq3d_ens** out);
+void zq3d_ens_destroy(zq3d_ens* e);
+/* at most 4 box and 4 sphere sources per member (formats of zq3d_set_box/sphere_sources) */
+int zq3d_ens_set_sources(zq3d_ens* e, int member, int n_box, const float* boxes8, int n_sphere,
+                         const float* spheres6);
+/* ZQ3D_FIELD_U/V/W/T/RHO/OCC of one member, host memory, sizes as for zq3d_upload */
+int zq3d_ens_upload(zq3d_ens* e, int member, int field, const void* host, size_t bytes);
+int zq3d_ens_download(zq3d_ens* e, int member, int field, void* host, size_t bytes);
+/* `steps` steps of every member; use_graph != 0 replays the captured step (captured on first use
+ * and again after an autotune) */
+int zq3d_ens_step(zq3d_ens* e, int steps, int use_graph);
+int zq3d_ens_synchronize(zq3d_ens* e);
+/* total cells of all members */
+long long zq3d_ens_cells(zq3d_ens* e);
+/* the ensemble's hipStream_t */
+int zq3d_ens_stream(zq3d_ens* e, void** stream);
+/* Launch-shape autotuner: times the sweep kernel for each tile shape (tbx x tby cells) and
+ * persistent launch size (blocks; 0 = one workgroup per tile) and the cell kernels for each launch
+ * size, on the members' own grids and occupancy, and keeps the fastest.  Fields are unchanged. */
+#define ZQ3D_ENS_TUNE_MAX 24
+typedef struct zq3d_ens_tune {
+    int count;
+    int tbx[ZQ3D_ENS_TUNE_MAX], tby[ZQ3D_ENS_TUNE_MAX], blocks[ZQ3D_ENS_TUNE_MAX];
+    float us_per_iter[ZQ3D_ENS_TUNE_MAX];
+    int best_tbx, best_tby, best_sweep_blocks;
+    float best_us_per_iter;
+    int best_blocks;       /* launch size of the other stages */
+    float probe_ms;        /* cell advection probe at that size */
+} zq3d_ens_tune;
+int zq3d_ens_autotune(zq3d_ens* e, int iters, zq3d_ens_tune* report);
+
 /* message of the last failed call on this thread ("" if none) */
 const char* zq3d_last_error(void);
 /* frees the device workspace cached by the Part 1/2 host-pointer functions */

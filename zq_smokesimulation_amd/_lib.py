@@ -176,7 +176,7 @@ def _load():
 def lib():
     global _lib
     if _lib is None:
-        _lib = _bind_adjoint(_bind_scene(_load()))
+        _lib = _bind_ensemble(_bind_adjoint(_bind_scene(_load())))
     return _lib
 
 
@@ -260,4 +260,44 @@ def _bind_adjoint(L):
     L.zq3d_adj_forward.argtypes = [vp, vp, C.POINTER(AdjStats), vp]
     L.zq3d_adj_backward.argtypes = [vp, vp, vp, vp, C.POINTER(AdjStats)]
     L.zq3d_adj_get.argtypes = [vp, C.c_int, C.c_int, vp]
+    return L
+
+
+ENS_TUNE_MAX = 24
+
+
+class EnsTune(C.Structure):
+    _fields_ = [
+        ("count", C.c_int),
+        ("tbx", C.c_int * ENS_TUNE_MAX), ("tby", C.c_int * ENS_TUNE_MAX), ("blocks", C.c_int * ENS_TUNE_MAX),
+        ("us_per_iter", C.c_float * ENS_TUNE_MAX),
+        ("best_tbx", C.c_int), ("best_tby", C.c_int), ("best_sweep_blocks", C.c_int),
+        ("best_us_per_iter", C.c_float),
+        ("best_blocks", C.c_int),
+        ("probe_ms", C.c_float),
+    ]
+
+    def as_dict(self):
+        n = self.count
+        return {"candidates": [{"tbx": self.tbx[i], "tby": self.tby[i], "blocks": self.blocks[i],
+                                "us_per_iter": round(self.us_per_iter[i], 3)} for i in range(n)],
+                "best": {"tbx": self.best_tbx, "tby": self.best_tby, "sweep_blocks": self.best_sweep_blocks,
+                         "us_per_iter": round(self.best_us_per_iter, 3), "blocks": self.best_blocks,
+                         "probe_ms": round(self.probe_ms, 4)}}
+
+
+def _bind_ensemble(L):
+    vp = C.c_void_p
+    L.zq3d_ens_create.argtypes = [C.c_int, C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.zq3d_ens_destroy.argtypes = [vp]
+    L.zq3d_ens_destroy.restype = None
+    L.zq3d_ens_set_sources.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, vp]
+    L.zq3d_ens_upload.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    L.zq3d_ens_download.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    L.zq3d_ens_step.argtypes = [vp, C.c_int, C.c_int]
+    L.zq3d_ens_synchronize.argtypes = [vp]
+    L.zq3d_ens_cells.argtypes = [vp]
+    L.zq3d_ens_cells.restype = C.c_longlong
+    L.zq3d_ens_stream.argtypes = [vp, C.POINTER(C.c_void_p)]
+    L.zq3d_ens_autotune.argtypes = [vp, C.c_int, C.POINTER(EnsTune)]
     return L

@@ -8,6 +8,7 @@
 // read of the float4 grid per kernel.
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
+#include "zq3d_stencils.h"
 
 namespace zq3d {
 
@@ -17,76 +18,28 @@ constexpr int BX = 64, BY = 4;
 
 inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
 
-// MAC_to_Regular4_Kernel (ZQ_CUDA_MACtoRegular.cu:9-30)
+// The cell / face bodies live in zq3d_stencils.h (shared with the ensemble kernels).
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_mac_to_cell4(const R* __restrict__ u, const R* __restrict__ v,
                                                          const R* __restrict__ w, vec4_t<R>* __restrict__ vel4,
                                                          Dims d) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    vec4_t<R> c;
-    c.x = R(0.5) * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
-    c.y = R(0.5) * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
-    c.z = R(0.5) * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
-    c.w = R(0);
-    vel4[cidx(d, i, j, k)] = c;
+    st_mac_to_cell4<R>(u, v, w, vel4, d, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
-// Advect_Velocity_inRegular_outRegular_Kernel (ZQ_CUDA_Advection3D.cu:76-113): backtrace from the
-// cell centre through the cell grid and sample the same grid at the end point.  Output SoA
-// (adv[0..N) x, [N..2N) y, [2N..3N) z) so each face pass below reads one component plane.
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_advect_cells(const vec4_t<R>* __restrict__ vel4,
                                                          const uint8_t* __restrict__ occ, R* __restrict__ adv,
                                                          Dims d, R h, int steps, R dtt) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    const RegularVelCached<R> V{vel4, d, {}};
-    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
-    const R3<R> s = V(last.x, last.y, last.z);
-    const int o = cidx(d, i, j, k);
-    const int N = d.W * d.H * d.D;
-    adv[o] = s.x;
-    adv[N + o] = s.y;
-    adv[2 * N + o] = s.z;
+    st_advect_cell<R>(vel4, occ, adv, d, h, steps, dtt, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
+                      blockIdx.z);
 }
 
-// Regular4_to_MAC_{u,v,w}_Kernel (ZQ_CUDA_MACtoRegular.cu:32-99) fused with
-// Apply_Advect_Velocity_Result_{Open,Closed}_* (ZQ_CUDA_Advection3D.cu:471-650): the face value is
-// the average of the two advected cells (a domain face copies its inside cell) and is stored only
-// where the face's cells are fluid.
 template <class R, int AXIS>
 __global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const R* __restrict__ adv,
                                                                  const uint8_t* __restrict__ occ,
                                                                  R* __restrict__ face, Dims d, int is_open) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
-    if (i >= nx || j >= ny) return;
-    const int pos = AXIS == 0 ? i : (AXIS == 1 ? j : k);
-    const int lim = AXIS == 0 ? d.W : (AXIS == 1 ? d.H : d.D);
-    const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
-    const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
-                        AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
-    const R* a = adv + (size_t)AXIS * d.W * d.H * d.D;
-    bool ok;
-    R val;
-    if (pos == 0 || pos == lim) {
-        ok = is_open && !occ[ci];
-        val = a[ci];
-    } else {
-        ok = !occ[ci] && !occ[ci - step];
-        val = R(0.5) * (a[ci] + a[ci - step]);
-    }
-    if (ok) {
-        const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
-        face[fi] = val;
-    }
+    st_cells_to_face<R, AXIS>(adv, occ, face, d, is_open, blockIdx.x * BX + threadIdx.x,
+                              blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
 // Advect_Velocity_inRegular_outMAC_* (:115-230) and inMAC_outMAC_* (:232-377): backtrace from the
@@ -129,22 +82,13 @@ __global__ __launch_bounds__(BX* BY) void k_apply_faces(const R* __restrict__ sr
     }
 }
 
-// Advect_Scalar_{Regular,MAC}_Velocity_Kernel (:379-469): backtrace from the cell centre, sample T
-// and rho (linear, clamp) at the end point.  Every cell is written, solid ones included.
 template <class R, class Vel>
 __global__ __launch_bounds__(BX* BY) void k_advect_scalar(Vel V, const uint8_t* __restrict__ occ,
                                                           const R* __restrict__ Tin, const R* __restrict__ Rin,
                                                           R* __restrict__ Tout, R* __restrict__ Rout, Dims d, R h,
                                                           int steps, R dtt) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
-    const R x = last.x - R(0.5), y = last.y - R(0.5), z = last.z - R(0.5);
-    const int o = cidx(d, i, j, k);
-    Tout[o] = sample3<R>(Tin, d.W, d.H, d.D, x, y, z);
-    Rout[o] = sample3<R>(Rin, d.W, d.H, d.D, x, y, z);
+    st_advect_scalar<R, Vel>(V, occ, Tin, Rin, Tout, Rout, d, h, steps, dtt, blockIdx.x * BX + threadIdx.x,
+                             blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
 }  // namespace

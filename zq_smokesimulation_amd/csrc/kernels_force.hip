@@ -9,6 +9,7 @@
 // Threads: 64 x 4 per block, x across the wave's lanes (coalesced rows), one cell per thread.
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
+#include "zq3d_stencils.h"
 
 namespace zq3d {
 
@@ -16,136 +17,29 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-template <class R>
-__device__ __forceinline__ R cell_u(const R* __restrict__ u, const Dims& d, int i, int j, int k) {
-    return R(0.5) * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
-}
-template <class R>
-__device__ __forceinline__ R cell_v(const R* __restrict__ v, const Dims& d, int i, int j, int k) {
-    return R(0.5) * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
-}
-template <class R>
-__device__ __forceinline__ R cell_w(const R* __restrict__ w, const Dims& d, int i, int j, int k) {
-    return R(0.5) * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
-}
-
-// Compute_Vorticity_Vector_Scale_Kernel (ZQ_CUDA_AddForce3D.cu:11-34): central differences of the
-// cell velocity over 2h on interior cells, zero on the border (the reference memsets the arrays).
+// The bodies live in zq3d_stencils.h (shared with the ensemble kernels).
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_vorticity(const R* __restrict__ u, const R* __restrict__ v,
                                                       const R* __restrict__ w, vec4_t<R>* __restrict__ vort, Dims d,
                                                       R two_h) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    vec4_t<R> out;
-    out.x = out.y = out.z = out.w = R(0);
-    if (i >= 1 && i < d.W - 1 && j >= 1 && j < d.H - 1 && k >= 1 && k < d.D - 1) {
-        const R wx = (cell_w(w, d, i, j + 1, k) - cell_w(w, d, i, j - 1, k) - cell_v(v, d, i, j, k + 1) +
-                      cell_v(v, d, i, j, k - 1)) / two_h;
-        const R wy = (cell_u(u, d, i, j, k + 1) - cell_u(u, d, i, j, k - 1) - cell_w(w, d, i + 1, j, k) +
-                      cell_w(w, d, i - 1, j, k)) / two_h;
-        const R wz = (cell_v(v, d, i + 1, j, k) - cell_v(v, d, i - 1, j, k) - cell_u(u, d, i, j + 1, k) +
-                      cell_u(u, d, i, j - 1, k)) / two_h;
-        out.x = wx;
-        out.y = wy;
-        out.z = wz;
-        out.w = zsqrt(wx * wx + wy * wy + wz * wz);
-    }
-    vort[cidx(d, i, j, k)] = out;
+    st_vorticity<R>(u, v, w, vort, d, two_h, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
+                    blockIdx.z);
 }
 
-// Compute_Gradient_of_VorticityScale_Kernel (:64-111) + Compute_Force_Kernel (:161-184).
-// Output is SoA: fcell[0..N) = fx, [N..2N) = fy, [2N..3N) = fz.
 template <class R, bool ATTEN>
 __global__ __launch_bounds__(BX* BY) void k_force_cells(const vec4_t<R>* __restrict__ vort, R* __restrict__ T,
                                                         R* __restrict__ rho, const R* __restrict__ extra,
                                                         R* __restrict__ fcell, Dims d, R ch, R beta, R alpha, R Tamb,
                                                         R aT, R aR) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    const int o = cidx(d, i, j, k);
-    const int HW = d.H * d.W;
-    const int N = HW * d.D;
-    const R s0 = vort[o].w;
-    R gx, gy, gz;
-    if (i == 0) gx = vort[o + 1].w - s0;
-    else if (i == d.W - 1) gx = s0 - vort[o - 1].w;
-    else gx = R(0.5) * (vort[o + 1].w - vort[o - 1].w);
-    if (j == 0) gy = vort[o + d.W].w - s0;
-    else if (j == d.H - 1) gy = s0 - vort[o - d.W].w;
-    else gy = R(0.5) * (vort[o + d.W].w - vort[o - d.W].w);
-    if (k == 0) gz = vort[o + HW].w - s0;
-    else if (k == d.D - 1) gz = s0 - vort[o - HW].w;
-    else gz = R(0.5) * (vort[o + HW].w - vort[o - HW].w);
-    const R len = zsqrt(gx * gx + gy * gy + gz * gz);
-    if (len != R(0)) {
-        gx = gx / len;
-        gy = gy / len;
-        gz = gz / len;
-    }
-    const vec4_t<R> wv = vort[o];
-    R fx = ch * (gy * wv.z - gz * wv.y);
-    R fy = ch * (gz * wv.x - gx * wv.z);
-    R fz = ch * (gx * wv.y - gy * wv.x);
-    const R t = T[o];
-    const R r = (ATTEN || alpha != R(0)) ? rho[o] : R(0);  // AddForce3D passes no density
-    fy += beta * (t - Tamb);
-    if (alpha != R(0)) fy -= alpha * r;
-    if (extra) {
-        fx += extra[o * 3 + 0];
-        fy += extra[o * 3 + 1];
-        fz += extra[o * 3 + 2];
-    }
-    fcell[o] = fx;
-    fcell[N + o] = fy;
-    fcell[2 * N + o] = fz;
-    if (ATTEN) {
-        T[o] = t * aT;
-        rho[o] = r * aR;
-    }
+    st_force_cell<R, ATTEN>(vort, T, rho, extra, fcell, d, ch, beta, alpha, Tamb, aT, aR,
+                            blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
-// AddForce_{u,v,w}_Kernel (:211-420) fused with Atten_{u,v,w}_Kernel (ZQ_CUDA_Attenuation3D.cu:8-96):
-// both stages use the same face mask (both cells fluid; a domain face needs its inside cell fluid),
-// so the face becomes (face + dt*avg f) * aV in one read-modify-write.  AXIS 0/1/2 = u/v/w.
-// The reference's AddForce_w_Kernel also runs its interior formula at z = 0 reading cell z-1
-// (:380-384, out of bounds); this follows the slice variant (:400-419).
 template <class R, int AXIS, bool ATTEN>
 __global__ __launch_bounds__(BX* BY) void k_force_faces(R* __restrict__ face, const uint8_t* __restrict__ occ,
                                                         const R* __restrict__ fc, Dims d, R dt, R dth, R aV) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
-    if (i >= nx || j >= ny) return;
-    const int pos = AXIS == 0 ? i : (AXIS == 1 ? j : k);
-    const int lim = AXIS == 0 ? d.W : (AXIS == 1 ? d.H : d.D);
-    const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
-    const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
-                        AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
-    const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
-    const R* f = fc + (size_t)AXIS * d.W * d.H * d.D;
-    bool ok;
-    R add;
-    if (pos == 0) {
-        ok = !occ[ci];
-        add = dt * f[ci];
-    } else if (pos == lim) {
-        ok = !occ[ci];  // ci already points at the last cell
-        add = dt * f[ci];
-    } else {
-        ok = !occ[ci - step] && !occ[ci];
-        add = dth * (f[ci] + f[ci - step]);
-    }
-    if (ok) {
-        R x = face[fi] + add;
-        if (ATTEN) x = x * aV;
-        face[fi] = x;
-    }
+    st_force_face<R, AXIS, ATTEN>(face, occ, fc, d, dt, dth, aV, blockIdx.x * BX + threadIdx.x,
+                                  blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
 // Atten_{u,v,w}_Kernel without a force (:8-96)

@@ -1,6 +1,7 @@
 // kernels_misc.hip -- smoke sources, the analytic moving-sphere obstacle and fill kernels (gfx950).
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
+#include "zq3d_stencils.h"
 
 namespace zq3d {
 
@@ -10,39 +11,21 @@ constexpr int BX = 64, BY = 4;
 
 inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
 
-// ZQ_ReinjectTemperatureDensity3D (ZQ_SmokeSimulationUtils3D.cpp:3842-3882), box sources,
-// randDensity off: cells whose centre is within the half extents get rho_src and T_src.
-// box = cx, cy, cz, hx, hy, hz (normalized by W/H/D), rho_src, T_src.
-// The adjoint runs the same kernel with zero values: a reinjected cell forgets its past.
+// Reinjection bodies: zq3d_stencils.h.  The adjoint runs the same kernels with zero values: a
+// reinjected cell forgets its past.
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_reinject_box(R* __restrict__ T, R* __restrict__ rho, Dims d, float cx,
                                                          float cy, float cz, float hx, float hy, float hz, float rd,
                                                          float rt) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    if (fabsf(i + 0.5f - cx) <= hx && fabsf(j + 0.5f - cy) <= hy && fabsf(k + 0.5f - cz) <= hz) {
-        const int o = cidx(d, i, j, k);
-        rho[o] = (R)(rd * 1.0f);
-        T[o] = (R)rt;
-    }
+    st_reinject_box<R>(T, rho, d, cx, cy, cz, hx, hy, hz, rd, rt, blockIdx.x * BX + threadIdx.x,
+                       blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
-// Sphere source (not in the reference): centre normalized by W/H/D, radius in units of W.
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_reinject_sphere(R* __restrict__ T, R* __restrict__ rho, Dims d, float x0,
                                                             float y0, float z0, float r2, float rd, float rt) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= d.W || j >= d.H) return;
-    const float dx = i + 0.5f - x0, dy = j + 0.5f - y0, dz = k + 0.5f - z0;
-    if (dx * dx + dy * dy + dz * dz <= r2) {
-        const int o = cidx(d, i, j, k);
-        rho[o] = (R)rd;
-        T[o] = (R)rt;
-    }
+    st_reinject_sphere<R>(T, rho, d, x0, y0, z0, r2, rd, rt, blockIdx.x * BX + threadIdx.x,
+                          blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
 // Occupancy of an analytic solid sphere plus the wall shell, with the six faces of every solid

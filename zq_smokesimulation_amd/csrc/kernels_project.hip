@@ -14,6 +14,7 @@
 // one partial per block (wave butterfly + LDS), then one block folds the partials in index order.
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
+#include "zq3d_stencils.h"
 
 namespace zq3d {
 
@@ -23,34 +24,7 @@ constexpr int BX = 64, BY = 4;
 
 inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
 
-__device__ __forceinline__ double wave_sum(double x) {
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-__device__ __forceinline__ double wave_max(double x) {
-    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
-    return x;
-}
-
-// Sum (or max) over a 256-thread block; the result is valid in thread 0.
-template <bool MAX>
-__device__ __forceinline__ double block_reduce(double x, double* lds4) {
-    const int t = threadIdx.y * blockDim.x + threadIdx.x;
-    x = MAX ? wave_max(x) : wave_sum(x);
-    if ((t & 63) == 0) lds4[t >> 6] = x;
-    __syncthreads();
-    double r = 0.0;
-    if (t == 0) r = MAX ? fmax(fmax(lds4[0], lds4[1]), fmax(lds4[2], lds4[3])) : ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
-    return r;
-}
-
 __device__ __forceinline__ int block_linear() { return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x; }
-
-// 1/n for n = 1..6, each correctly rounded (the same values as 1.0f/(float)n)
-__device__ __forceinline__ float rcp_count(int n) {
-    return n == 1 ? 1.0f : n == 2 ? 0.5f : n == 3 ? (1.0f / 3.0f) : n == 4 ? 0.25f : n == 5 ? (1.0f / 5.0f)
-                                                                                            : (1.0f / 6.0f);
-}
 
 // Neighbour counts, divergence (the reference's unscaled sum, ZQ_SmokeSimulationUtils3D.cpp:2403-2405,
 // evaluated left to right), zero pressure where the count is 0, and per-block FP64 sum/count of the
@@ -67,7 +41,9 @@ __global__ __launch_bounds__(BX* BY) void k_proj_setup(const float* __restrict__
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     double dsum = 0.0, dcnt = 0.0;
-    if (i < d.W && j < d.H) {
+    if (!MP) {
+        st_proj_setup_plain(u, v, w, occ, open_mask, nc, b, p, d, i, j, k, dsum, dcnt);
+    } else if (i < d.W && j < d.H) {
         const int o = cidx(d, i, j, k);
         const int HW = d.W * d.H;
         int n = 0;
@@ -84,20 +60,12 @@ __global__ __launch_bounds__(BX* BY) void k_proj_setup(const float* __restrict__
         const float u0 = u[uidx(d, i, j, k)], u1 = u[uidx(d, i + 1, j, k)];
         const float v0 = v[vidx(d, i, j, k)], v1 = v[vidx(d, i, j + 1, k)];
         const float w0 = w[widx(d, i, j, k)], w1 = w[widx(d, i, j, k + 1)];
-        if (MP) {
+        {
             const double dd = (((double)u1 - (double)u0) + ((double)v1 - (double)v0)) + ((double)w1 - (double)w0);
             b64[o] = n ? dd : 0.0;
             if (!n) p64[pidx(d, i, j, k)] = 0.0;
             if (fluid) {
                 dsum = dd;
-                dcnt = 1.0;
-            }
-        } else {
-            const float dv = u1 - u0 + v1 - v0 + w1 - w0;
-            b[o] = fluid ? dv : 0.0f;
-            if (!n) p[pidx(d, i, j, k)] = 0.0f;
-            if (fluid) {
-                dsum = (double)dv;
                 dcnt = 1.0;
             }
         }
@@ -118,27 +86,7 @@ __global__ __launch_bounds__(1024) void k_finish_mean(const double* __restrict__
                                                       float* __restrict__ mean32, double* __restrict__ mean64,
                                                       int want) {
     __shared__ double ss[1024], sc[1024];
-    const int t = threadIdx.x;
-    double s = 0.0, c = 0.0;
-    for (int i = t; i < nblocks; i += 1024) {
-        s += part[2 * i];
-        c += part[2 * i + 1];
-    }
-    ss[t] = s;
-    sc[t] = c;
-    __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if (t < h) {
-            ss[t] += ss[t + h];
-            sc[t] += sc[t + h];
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        const double m = (want && sc[0] > 0.0) ? ss[0] / sc[0] : 0.0;
-        if (mean32) *mean32 = (float)m;
-        if (mean64) *mean64 = m;
-    }
+    st_fold_mean(part, nblocks, mean32, mean64, want, ss, sc);
 }
 
 // Fold per-block maxima; the result is stored as the bit pattern of a non-negative double.
@@ -162,20 +110,8 @@ __global__ __launch_bounds__(1024) void k_finish_max(const double* __restrict__ 
 __global__ __launch_bounds__(BX* BY) void k_sor_half(const uint8_t* __restrict__ nc, const float* __restrict__ b,
                                                      const float* __restrict__ mean32, float* __restrict__ p, Dims d,
                                                      float omega, float omw, int colour) {
-    const int t = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    const int i = 2 * t + ((colour + j + k) & 1);
-    if (i >= d.W || j >= d.H) return;
-    const int o = cidx(d, i, j, k);
-    const int n = nc[o];
-    if (!n) return;
-    const int sy = d.W + 2, sz = (d.W + 2) * (d.H + 2);
-    const int q = pidx(d, i, j, k);
-    const float s = p[q - 1] + p[q + 1] + p[q - sy] + p[q + sy] + p[q - sz] + p[q + sz];
-    const float bb = b[o] - *mean32;
-    const float pg = (s - bb) * rcp_count(n);
-    p[q] = omw * p[q] + omega * pg;
+    st_sor_half(nc, b, mean32, p, d, omega, omw, colour, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
+                blockIdx.z);
 }
 
 // One full red-black iteration p_in -> p_out in a single pass over HBM.
@@ -428,27 +364,8 @@ __global__ __launch_bounds__(BX* BY) void k_sor_split(const uint8_t* __restrict_
 template <int AXIS, class P, class F>
 __global__ __launch_bounds__(BX* BY) void k_correct(const uint8_t* __restrict__ occ, int open_mask,
                                                     const P* __restrict__ p, F* __restrict__ face, Dims d) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
-    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
-    if (i >= nx || j >= ny) return;
-    const int pos = AXIS == 0 ? i : (AXIS == 1 ? j : k);
-    const int lim = AXIS == 0 ? d.W : (AXIS == 1 ? d.H : d.D);
-    const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
-    const int pstep = AXIS == 0 ? 1 : (AXIS == 1 ? d.W + 2 : (d.W + 2) * (d.H + 2));
-    const int lo_bit = AXIS == 0 ? 1 : (AXIS == 1 ? 4 : 16);
-    const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
-                        AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
-    bool ok;
-    if (pos == 0) ok = (open_mask & lo_bit) && !occ[ci];
-    else if (pos == lim) ok = (open_mask & (lo_bit << 1)) && !occ[ci];
-    else ok = !occ[ci - step] && !occ[ci];
-    if (!ok) return;
-    // padded index of the cell on the high side of the face (ghost when pos == lim)
-    const int qh = pidx(d, i, j, k);
-    const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
-    face[fi] = (F)((P)face[fi] - (p[qh] - p[qh - pstep]));
+    st_correct<AXIS, P, F>(occ, open_mask, p, face, d, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
+                           blockIdx.z);
 }
 
 // Transposed correction: every face, solid and ghost pressures being 0 (the adjoint of the
