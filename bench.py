@@ -16,6 +16,11 @@ hand-written adjoint (each earlier segment recomputed from its checkpoint), for 
 1/2 |rho_64 - target|^2 with a target of 8 seeded Gaussian blobs and per-step control forces
 (zero); value = voxels * window / (forward + backward wall time).
 
+ensemble: 64 independent grids with per-axis sizes drawn from {32..96} (seeded, anisotropic),
+alpha in [0.02, 0.1] and beta in [0.2, 1.0] per member, closed boxes with the same sphere
+source, plain red-black SOR (omega 1.8, 200 iterations), 100 steps replayed from one captured
+hipGraph after the launch-shape autotuner; value = all members' voxels * steps / wall time.
+
 Prints one JSON line.  value = voxels * steps / wall time (Mvoxel-steps/s).  roofline: the
 red-black SOR sweep kernel the autotuner picked (the dominant kernel), algorithmic bytes 13 B/voxel
 per red+black iteration (6.5 B/voxel per single-colour launch) over its event-timed mean launch
@@ -47,6 +52,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6, help="max|div| target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
+    ap.add_argument("--ens-members", type=int, default=64, help="ensemble members (0: skip the ensemble leg)")
+    ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "sor_sweep_256_traffic.txt"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -118,6 +125,58 @@ def cpu_baseline(state, W, H, D, inner, outer_mean):
         "sample": ("oracle at %dx%dx%d on the GPU's state after the timed steps: one step with no refinement pass "
                    "(%.2f s) + %d SOR iterations (%.3f s); step time = %.2f + %.1f passes (GPU mean per step) x %.3f = "
                    "%.2f s" % (W, H, D, t_fixed, inner, t_pass, t_fixed, outer_mean, t_pass, step_s)),
+    }
+
+
+def ensemble_leg(args):
+    """The batched-ensemble workload (see the module docstring)."""
+    import zq_smokesimulation_amd as z
+    from zq_smokesimulation_amd.ensemble import SmokeEnsemble, member_config
+    rng = np.random.default_rng(0)
+    cfgs, states = [], []
+    for m in range(args.ens_members):
+        W, H, D = (int(x) for x in rng.integers(32, 97, size=3))
+        alpha, beta = float(rng.uniform(0.02, 0.1)), float(rng.uniform(0.2, 1.0))
+        cfgs.append(member_config(W, H, D, voxel_size=1.0 / W, step_time=0.02, steps=20, Tamb=0.0, buoyancy=beta,
+                                  gravity=alpha, confine=0.3, sor_open_mask=0, sor_closed_mean=1, sor_omega=1.8,
+                                  sor_iters=200, sor_mixed_precision=0))
+        st = make_state(W, H, D)
+        st["T"] = z.noise_field(W, H, D, m, 0.01)
+        states.append(st)
+    ens = SmokeEnsemble(cfgs)
+    for m, st in enumerate(states):
+        ens.set_state(m, **st)
+        ens.set_sources(m, spheres=[SOURCE])
+    cells = ens.cells()
+    ens.step(5, graph=False)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(5, graph=False)
+    ens.synchronize()
+    eager = (time.perf_counter() - t0) / 5
+    tune = ens.autotune(10)
+    ens.step(1, graph=True)  # capture
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.ens_steps, graph=True)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    rho0 = ens.download(0, z._lib.FIELD_RHO)
+    ens.close()
+    sizes = [(c.width, c.height, c.depth) for c in cfgs]
+    return {
+        "value": round(cells * args.ens_steps / wall / 1e6, 3),
+        "unit": "Mvoxel-steps/s",
+        "members": len(cfgs),
+        "cells": cells,
+        "steps": args.ens_steps,
+        "ms_per_step": round(1e3 * wall / args.ens_steps, 3),
+        "ms_per_step_eager": round(1e3 * eager, 3),
+        "sor_iterations": 200,
+        "smallest": min(sizes, key=lambda t: t[0] * t[1] * t[2]),
+        "largest": max(sizes, key=lambda t: t[0] * t[1] * t[2]),
+        "autotune": tune["best"],
+        "density_finite": bool(np.isfinite(rho0).all()),
     }
 
 
@@ -292,6 +351,9 @@ def main():
     if args.adj_window > 0:
         print("bench: forward+adjoint window of %d steps" % args.adj_window, file=sys.stderr, flush=True)
         out["fwd_adjoint"] = adjoint_window(g, W, H, D, args, ms_per_step)
+    if args.ens_members > 0:
+        print("bench: ensemble of %d grids" % args.ens_members, file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_leg(args)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean)
