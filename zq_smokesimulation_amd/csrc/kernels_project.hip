@@ -329,6 +329,17 @@ __global__ __launch_bounds__(BX* BY) void k_split_out(const float* __restrict__ 
     p[pidx(d, x, y, z)] = (c ? p1 : p0)[split_q(Wh, d.H, x >> 1, y, z)];
 }
 
+// interleaved counts -> split n_c, both colours (slots past a row's end hold 0); blockIdx.z = 2*z + colour
+__global__ __launch_bounds__(BX* BY) void k_split_counts(const uint8_t* __restrict__ nc, uint8_t* __restrict__ n0,
+                                                         uint8_t* __restrict__ n1, Dims d, int Wh) {
+    const int xh = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int z = blockIdx.z >> 1, c = blockIdx.z & 1;
+    if (xh >= Wh || y >= d.H) return;
+    const int x = 2 * xh + ((c + y + z) & 1);
+    (c ? n1 : n0)[(z * d.H + y) * Wh + xh] = x < d.W ? nc[cidx(d, x, y, z)] : (uint8_t)0;
+}
+
 // One colour of an SOR iteration on split storage: pc (this colour) updated in place from po.
 // 1-D grid; each XCD (block b runs on XCD b % 8) receives a contiguous z-slab of tiles so the
 // plane above and below a tile are usually in the same L2.
@@ -489,10 +500,14 @@ __global__ __launch_bounds__(BX* BY) void k_sub_p64(const uint8_t* __restrict__ 
 }
 
 // FP64 residual r = (b64 - mean) - (sum_nb p - n p); r32 feeds the FP32 inner solve; per-block max|r|.
+// SPLIT: r32 goes to the colour-split right-hand sides r0/r1 (slot x >> 1 of row (y, z) of colour
+// (x+y+z)&1) instead of the interleaved array r0; the arithmetic and the partials are the same.
+template <bool SPLIT>
 __global__ __launch_bounds__(BX* BY) void k_residual64(const uint8_t* __restrict__ nc, const double* __restrict__ b64,
                                                        const double* __restrict__ mean64,
-                                                       const double* __restrict__ p, float* __restrict__ r32,
-                                                       double* __restrict__ part, Dims d) {
+                                                       const double* __restrict__ p, float* __restrict__ r0,
+                                                       float* __restrict__ r1, double* __restrict__ part, Dims d,
+                                                       int Wh) {
     __shared__ double red[4];
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
@@ -511,22 +526,27 @@ __global__ __launch_bounds__(BX* BY) void k_residual64(const uint8_t* __restrict
             r = (float)rr;
             m = fabs(rr);
         }
-        r32[o] = r;
+        if (SPLIT) (((i + j + k) & 1) ? r1 : r0)[(k * d.H + j) * Wh + (i >> 1)] = r;
+        else r0[o] = r;
     }
     m = block_reduce<true>(m, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
 }
 
-__global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict__ nc, float* __restrict__ e,
-                                                       double* __restrict__ p, Dims d) {
+// p64 += e, e = 0 on counted cells; SPLIT: e lives in the padded colour-split arrays e0/e1.
+template <bool SPLIT>
+__global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict__ nc, float* __restrict__ e0,
+                                                       float* __restrict__ e1, double* __restrict__ p, Dims d,
+                                                       int Wh) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
     if (!nc[cidx(d, i, j, k)]) return;
     const int q = pidx(d, i, j, k);
-    p[q] = p[q] + (double)e[q];
-    e[q] = 0.0f;
+    float* ep = SPLIT ? &(((i + j + k) & 1) ? e1 : e0)[split_q(Wh, d.H, i >> 1, j, k)] : &e0[q];
+    p[q] = p[q] + (double)*ep;
+    *ep = 0.0f;
 }
 
 template <class F>
@@ -609,6 +629,26 @@ void launch_mean_finalize(const Dims& d, const double* part, float* mean32, doub
 size_t split_padded_size(const Dims& d) { return (size_t)((d.W + 1) / 2 + 2) * (d.H + 2) * (d.D + 2); }
 size_t split_cell_size(const Dims& d) { return (size_t)((d.W + 1) / 2) * d.H * d.D; }
 
+void launch_sor_split_sweeps(const Dims& d, const SplitBufs& sp, const float* mean32, float omega, int iters,
+                             hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    const float omw = 1.0f - omega;
+    const int Wh = (d.W + 1) / 2;
+    const int gx = (Wh + BX - 1) / BX, gy = (d.H + BY - 1) / BY;
+    const int nb = (gx * gy * d.D + 7) & ~7;
+    if (ev0) (void)hipEventRecord(ev0, s);
+    for (int it = 0; it < iters; it++)
+        for (int c = 0; c < 2; c++)
+            hipLaunchKernelGGL(k_sor_split, dim3(nb), dim3(BX * BY), 0, s, sp.n[c], sp.b[c], mean32, sp.p[c],
+                               sp.p[1 - c], d, Wh, gx, gy, omega, omw, c);
+    if (ev1) (void)hipEventRecord(ev1, s);
+}
+
+void launch_split_counts(const Dims& d, const uint8_t* nc, const SplitBufs& sp, hipStream_t s) {
+    const int Wh = (d.W + 1) / 2;
+    hipLaunchKernelGGL(k_split_counts, dim3((Wh + BX - 1) / BX, (d.H + BY - 1) / BY, 2 * d.D), dim3(BX, BY), 0, s,
+                       nc, sp.n[0], sp.n[1], d, Wh);
+}
+
 float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
                   float omega, int iters, hipStream_t s, SorShape shape, const SplitBufs* sp, hipEvent_t ev0,
                   hipEvent_t ev1) {
@@ -617,14 +657,7 @@ float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float*
         const int Wh = (d.W + 1) / 2;
         hipLaunchKernelGGL(k_split_in, dim3((Wh + BX - 1) / BX, (d.H + BY - 1) / BY, 2 * d.D), dim3(BX, BY), 0, s, nc,
                            b, p_pad, sp->n[0], sp->n[1], sp->b[0], sp->b[1], sp->p[0], sp->p[1], d, Wh);
-        const int gx = (Wh + BX - 1) / BX, gy = (d.H + BY - 1) / BY;
-        const int nb = (gx * gy * d.D + 7) & ~7;
-        if (ev0) (void)hipEventRecord(ev0, s);
-        for (int it = 0; it < iters; it++)
-            for (int c = 0; c < 2; c++)
-                hipLaunchKernelGGL(k_sor_split, dim3(nb), dim3(BX * BY), 0, s, sp->n[c], sp->b[c], mean32, sp->p[c],
-                                   sp->p[1 - c], d, Wh, gx, gy, omega, omw, c);
-        if (ev1) (void)hipEventRecord(ev1, s);
+        launch_sor_split_sweeps(d, *sp, mean32, omega, iters, s, ev0, ev1);
         hipLaunchKernelGGL(k_split_out, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, sp->p[0], sp->p[1], p_pad, d, Wh);
         return p_pad;
     }
@@ -736,13 +769,27 @@ void launch_remove_mean_p64(const Dims& d, const uint8_t* occ, double* p64_pad, 
 
 void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
                        const double* p64_pad, float* r32, double* part, unsigned long long* rmax, hipStream_t s) {
-    hipLaunchKernelGGL(k_residual64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad, r32, part,
-                       d);
+    hipLaunchKernelGGL(k_residual64<false>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad,
+                       r32, nullptr, part, d, 0);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), rmax);
+}
+
+void launch_residual64_split(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
+                             const double* p64_pad, const SplitBufs& sp, double* part, unsigned long long* rmax,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_residual64<true>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad,
+                       sp.b[0], sp.b[1], part, d, (d.W + 1) / 2);
     hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), rmax);
 }
 
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s) {
-    hipLaunchKernelGGL(k_update_p64, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, e_pad, p64_pad, d);
+    hipLaunchKernelGGL(k_update_p64<false>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, e_pad, nullptr, p64_pad,
+                       d, 0);
+}
+
+void launch_update_p64_split(const Dims& d, const uint8_t* nc, const SplitBufs& sp, double* p64_pad, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_p64<true>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, sp.p[0], sp.p[1],
+                       p64_pad, d, (d.W + 1) / 2);
 }
 
 template <class F>

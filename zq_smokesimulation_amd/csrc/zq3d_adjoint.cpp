@@ -56,22 +56,9 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
         if (adjoint) tol = rtol * bmax;
         if (bmax_out) *bmax_out = bmax;
     }
-    float* e = B.p;
-    float* e_alt = B.p_alt;
-    launch_fill_f32(e, z.Np, 0.0f, s);
     int outer = 0;
     double rmax = 0.0;
-    for (;;) {
-        launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, &sc->rmax, s);
-        if (int rc = read_double_bits(&sc->rmax, &rmax, s, where)) return rc;
-        if (!(rmax > tol) || outer >= o.max_outer) break;
-        float* res = nullptr;
-        ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, nullptr, &res, B.shape, &B.split),
-               where);
-        if (res != e) std::swap(e, e_alt);
-        launch_update_p64(d, B.nc, e, B.p64, s);
-        outer++;
-    }
+    if (int rc = refine_mixed(d, o, tol, B, s, nullptr, where, &outer, &rmax)) return rc;
     double md = -1.0;
     if (adjoint) {
         if (o.closed_mean) launch_remove_mean_p64(d, occ, B.p64, B.part, &sc->aux64, s);

@@ -112,6 +112,19 @@ struct Timer {
 namespace zq3d {
 namespace detail {
 
+namespace {
+hipError_t account_sweeps(const Dims& d, SorShape shape, int iters, SorProf* pf) {
+    Errs E;
+    E(hipGetLastError());
+    E(hipEventSynchronize(pf->b));
+    float ms = 0.0f;
+    E(hipEventElapsedTime(&ms, pf->a, pf->b));
+    pf->ms += ms;
+    pf->launches += (long long)sor_launches_per_iter(d, shape) * iters;
+    return E.e;
+}
+}  // namespace
+
 // launch_sor, timed when pf is set (the sweeps are the only work between the two events)
 hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p, float* p_alt,
                      float omega, int iters, hipStream_t s, SorProf* pf, float** result, SorShape shape,
@@ -120,16 +133,51 @@ hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const flo
         *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s, shape, split);
         return hipGetLastError();
     }
-    Errs E;
     // the events bracket the sweep launches only (not the split-layout conversions)
     *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s, shape, split, pf->a, pf->b);
-    E(hipGetLastError());
-    E(hipEventSynchronize(pf->b));
-    float ms = 0.0f;
-    E(hipEventElapsedTime(&ms, pf->a, pf->b));
-    pf->ms += ms;
-    pf->launches += (long long)sor_launches_per_iter(d, shape) * iters;
-    return E.e;
+    return account_sweeps(d, shape, iters, pf);
+}
+
+int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
+                 const char* where, int* outer_out, double* rmax_out) {
+    const Sizes z(d);
+    Scalars* sc = B.sc;
+    const bool split = B.shape.ty == SOR_SPLIT;
+    float* e = B.p;
+    float* e_alt = B.p_alt;
+    if (split) {
+        launch_split_counts(d, B.nc, B.split, s);
+        launch_fill_f32(B.split.p[0], split_padded_size(d), 0.0f, s);
+        launch_fill_f32(B.split.p[1], split_padded_size(d), 0.0f, s);
+    } else {
+        launch_fill_f32(e, z.Np, 0.0f, s);
+    }
+    int outer = 0;
+    double rmax = 0.0;
+    for (;;) {
+        if (split)
+            launch_residual64_split(d, B.nc, B.b64, &sc->mean64, B.p64, B.split, B.part, &sc->rmax, s);
+        else
+            launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, &sc->rmax, s);
+        if (int rc = read_double_bits(&sc->rmax, &rmax, s, where)) return rc;
+        if (!(rmax > tol) || outer >= o.max_outer) break;
+        if (split) {
+            launch_sor_split_sweeps(d, B.split, &sc->zero32, o.omega, o.iters, s, pf ? pf->a : nullptr,
+                                    pf ? pf->b : nullptr);
+            ZQ_HIP(pf ? account_sweeps(d, B.shape, o.iters, pf) : hipGetLastError(), where);
+            launch_update_p64_split(d, B.nc, B.split, B.p64, s);  // also zeroes e for the next pass
+        } else {
+            float* res = nullptr;
+            ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res, B.shape, &B.split),
+                   where);
+            if (res != e) std::swap(e, e_alt);
+            launch_update_p64(d, B.nc, e, B.p64, s);  // also zeroes e for the next pass
+        }
+        outer++;
+    }
+    *outer_out = outer;
+    *rmax_out = rmax;
+    return ZQ3D_OK;
 }
 
 int read_double_bits(const unsigned long long* dev, double* out, hipStream_t s, const char* where) {
@@ -184,22 +232,9 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
     launch_fill_f32(&sc->zero32, 1, 0.0f, s);
     launch_proj_setup(d, u, v, w, occ, o.open_mask, B.nc, nullptr, B.b64, nullptr, B.p64, B.part, s);
     launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
-    float* e = B.p;
-    float* e_alt = B.p_alt;
-    launch_fill_f32(e, z.Np, 0.0f, s);
     int outer = 0;
     double rmax = 0.0;
-    for (;;) {
-        launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, &sc->rmax, s);
-        if (int rc = read_double_bits(&sc->rmax, &rmax, s, "project_mixed")) return rc;
-        if (!(rmax > o.tol) || outer >= o.max_outer) break;
-        float* res = nullptr;
-        ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res, B.shape, &B.split),
-               "project_mixed");
-        if (res != e) std::swap(e, e_alt);
-        launch_update_p64(d, B.nc, e, B.p64, s);  // also zeroes e for the next pass
-        outer++;
-    }
+    if (int rc = refine_mixed(d, o, o.tol, B, s, pf, "project_mixed", &outer, &rmax)) return rc;
     launch_correct64(d, occ, o.open_mask, B.p64, u, v, w, s);
     launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
     double md = 0.0, mean64 = 0.0;

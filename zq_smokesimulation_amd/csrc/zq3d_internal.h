@@ -82,6 +82,11 @@ hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const flo
                      float omega, int iters, hipStream_t s, SorProf* pf, float** result, SorShape shape,
                      const SplitBufs* split);
 int read_double_bits(const unsigned long long* dev, double* out, hipStream_t s, const char* where);
+// Mixed-precision refinement of B.p64: FP64 residual, FP32 SOR on the correction e (zero start),
+// p64 += e, until max|r| <= tol or o.max_outer passes.  With the colour-split sweep algorithm the
+// residual and e stay in split storage across passes (counts split once per call).
+int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
+                 const char* where, int* outer, double* rmax);
 
 // Plain FP32 projection on device pointers: p = 0, setup, SOR, correct.  mean_mode: 0 = value
 // `mean` given, 1 = FP64 device mean over fluid cells.

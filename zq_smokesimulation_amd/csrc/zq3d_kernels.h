@@ -88,6 +88,12 @@ size_t split_cell_size(const Dims& d);
 float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
                   float omega, int iters, hipStream_t s, SorShape shape = SorShape{0, 0},
                   const SplitBufs* split = nullptr, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// iters single-colour sweep pairs on the split arrays as they stand (no layout conversion); events
+// (nullable) bracket the sweeps
+void launch_sor_split_sweeps(const Dims& d, const SplitBufs& sp, const float* mean32, float omega, int iters,
+                             hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// split counts n_0 / n_1 from the interleaved counts
+void launch_split_counts(const Dims& d, const uint8_t* nc, const SplitBufs& sp, hipStream_t s);
 // name of the sweep kernel a shape runs
 const char* sor_kernel_name(const Dims& d, SorShape shape);
 // kernel launches per SOR iteration (1 fused, 2 single-colour)
@@ -106,8 +112,14 @@ void launch_correct_all64(const Dims& d, const double* p64_pad, F* u, F* v, F* w
 // r = (b64 - mean64) - (sum_nb p - n p); writes r32 and max|r| (bits of a non-negative double) to *rmax
 void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
                        const double* p64_pad, float* r32, double* part, unsigned long long* rmax, hipStream_t s);
+// launch_residual64 writing r32 into the colour-split right-hand sides sp.b[0] / sp.b[1]
+void launch_residual64_split(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
+                             const double* p64_pad, const SplitBufs& sp, double* part, unsigned long long* rmax,
+                             hipStream_t s);
 // p64 += e, e = 0 on counted cells
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s);
+// the same with e in the colour-split arrays sp.p[0] / sp.p[1]
+void launch_update_p64_split(const Dims& d, const uint8_t* nc, const SplitBufs& sp, double* p64_pad, hipStream_t s);
 // max |div64 - mean64| over counted cells into *dmax (bits of a non-negative double)
 template <class F>
 void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* nc, const double* mean64,
