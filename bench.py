@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
     ap.add_argument("--ens-members", type=int, default=64, help="ensemble members (0: skip the ensemble leg)")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "sor_sweep_256_traffic.txt"),
-                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel")
+    ap.add_argument("--traffic", default=None,
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel (default: "
+                         "profiles/<kernel>_<W>_traffic.txt for the kernel the autotuner picked)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region run a plain 512 MiB device copy (3 launches) whose counter "
                          "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
@@ -85,11 +86,14 @@ def read_traffic(path):
     for line in open(path):
         if ":" in line and not line.lstrip().startswith("#"):
             k, _, v = line.partition(":")
+            words = v.split()
+            if not words:
+                continue
             try:
-                vals[k.strip()] = float(v.split()[0])
-            except (ValueError, IndexError):
-                pass
-    need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "fetch_scale", "write_scale", "size")
+                vals[k.strip()] = float(words[0])
+            except ValueError:
+                vals[k.strip()] = words[0]
+    need = ("kernel", "fetch_kib_per_dispatch", "write_kib_per_dispatch", "fetch_scale", "write_scale", "size")
     if not all(k in vals for k in need):
         return None
     return vals
@@ -298,8 +302,8 @@ def main():
     bytes_per_launch = SOR_BYTES_PER_VOXEL_ITER * N * iters / launches
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     traffic = None
-    tr = read_traffic(args.traffic)
-    if tr and int(tr["size"]) == W:
+    tr = read_traffic(args.traffic or os.path.join(ROOT, "profiles", "%s_%d_traffic.txt" % (sor_kernel, W)))
+    if tr and int(tr["size"]) == W and tr["kernel"] == sor_kernel:
         traffic = 1024.0 * (tr["fetch_kib_per_dispatch"] * tr["fetch_scale"] +
                             tr["write_kib_per_dispatch"] * tr["write_scale"])
     out = {

@@ -235,11 +235,14 @@ typedef struct zq3d_timing {
 int zq3d_set_profiling(zq3d_sim* sim, int on);
 int zq3d_get_timing(zq3d_sim* sim, zq3d_timing* out);
 
-/* Tile-shape autotuner of the one-pass red-black sweep kernel (no reference counterpart): each
- * candidate (ty rows per workgroup, zc planes per chunk) valid for the grid is timed for `iters`
- * iterations with HIP events on the simulation's stream and occupancy; the fastest is kept for
- * this simulation.  Clobbers the projection scratch (not the fields). */
-#define ZQ3D_TUNE_MAX 16
+/* Sweep-algorithm and tile-shape autotuner (no reference counterpart): each candidate valid for
+ * the grid is timed for `iters` iterations with HIP events on the simulation's stream and
+ * occupancy; the fastest is kept for this simulation.  Candidates (ty, zc): ty > 0 is the one-pass
+ * kernel on interleaved storage (ty rows per workgroup, zc planes per chunk); ty == -1 is two
+ * single-colour sweeps per iteration on colour-split storage; ty == -r (r = 4, 8, 16) is one launch
+ * per iteration on colour-split storage (r rows by zc planes per workgroup).  All produce the same
+ * bits.  Clobbers the projection scratch (not the fields). */
+#define ZQ3D_TUNE_MAX 32
 typedef struct zq3d_tune_report {
     int count;                       /* candidates timed */
     int best;                        /* index of the chosen one (-1: the grid is too wide for the fused kernel) */
@@ -248,10 +251,10 @@ typedef struct zq3d_tune_report {
     float us_per_iter[ZQ3D_TUNE_MAX];
 } zq3d_tune_report;
 int zq3d_autotune_sor(zq3d_sim* sim, int iters, zq3d_tune_report* report);
-/* explicit shape; (0, 0) restores the default, (-1, 0) selects single-colour sweeps on
- * colour-split storage */
+/* explicit shape, one of the autotuner's candidates; (0, 0) restores the default */
 int zq3d_set_sor_shape(zq3d_sim* sim, int ty, int zc);
-/* name of the sweep kernel the current shape runs ("k_sor_split", "k_sor_fused" or "k_sor_half") */
+/* name of the sweep kernel the current shape runs ("k_sor_split_rb", "k_sor_split", "k_sor_fused"
+ * or "k_sor_half") */
 const char* zq3d_sor_kernel(zq3d_sim* sim);
 
 /* ====================== Part 4: host-side scene set-up (no GPU) ====================== */

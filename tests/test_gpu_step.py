@@ -133,3 +133,59 @@ def test_every_sweep_tile_shape_is_bit_identical():
         for name, a in zip("uvw", ref):
             assert np.array_equal(g[name].view(np.uint32), a.view(np.uint32)), (c, name)
     sim.close()
+
+
+@pytest.mark.parametrize("ty", [-4, -8, -16])
+def test_one_launch_iteration_multi_tile(ty):
+    """The one-launch red-black iteration on colour-split storage works on tiles of 64 slots (128
+    cells) in x; at W = 150 a row spans three tiles, the last partial, and the chunks of zc planes
+    leave a partial last chunk.  Every (rows, planes) shape reproduces the oracle's SOR bit for bit,
+    for an odd iteration count (result in the second buffer pair) and an even one."""
+    import zq_smokesimulation_amd as z
+    from util import faces, occupancy
+    W, H, D = 150, 22, 19
+    rng = np.random.default_rng(5)
+    u, v, w = faces(rng, W, H, D)
+    occ = occupancy(rng, W, H, D, 0.1, shell=False)
+    for iters in (17, 12):
+        ref = [a.copy() for a in (u, v, w)]
+        O.project(*ref, occ, O.OPEN_YHI | O.OPEN_XLO, 0.0, 1.8, iters)
+        sim = z.SmokeSim3D(W, H, D, sor_open_mask=O.OPEN_YHI | O.OPEN_XLO, sor_closed_mean=0, sor_omega=1.8,
+                           sor_iters=iters)
+        for zc in (4, 8, 16, 32):
+            print("shape", ty, zc, "iters", iters, flush=True)
+            sim.set_sor_shape(ty, zc)
+            assert sim.sor_kernel() == "k_sor_split_rb"
+            sim.set_state(u=u, v=v, w=w, occ=occ)
+            sim.project()
+            g = sim.get_state()
+            for name, a in zip("uvw", ref):
+                assert np.array_equal(g[name].view(np.uint32), a.view(np.uint32)), (ty, zc, iters, name)
+        sim.close()
+
+
+def test_one_launch_iteration_mixed_precision_same_bits():
+    """Mixed-precision steps (FP64 residual, FP32 correction sweeps, several refinement passes with
+    the correction kept in split storage) give the same bits with one-launch iterations as with
+    single-colour sweeps, and reach the divergence bound."""
+    W, H, D = 96, 64, 48
+    p = oracle_params(W, mixed=1, sor_iters=15, max_outer=400, tol=1e-6, open_mask=O.OPEN_YHI, closed_mean=0)
+    shell = O.ALL_FACES & ~O.OPEN_YHI
+    out = []
+    for shape in ((-1, 0), (-8, 8)):
+        occ, u, v, w, T, R = initial_state(W, H, D, shell)
+        sim = make_sim(W, H, D, p, SRC)
+        sim.set_sor_shape(*shape)
+        sim.set_state(u=u, v=v, w=w, T=T, rho=R, occ=occ)
+        sts = []
+        for s in range(3):
+            print("shape", shape, "step", s, flush=True)
+            st = sim.step()
+            assert st["max_div"] <= 1e-6, st
+            sts.append(st["outer"])
+        out.append((sim.get_state(), sts))
+        sim.close()
+    (g0, s0), (g1, s1) = out
+    assert s0 == s1 and s0[0] > 1
+    for name in ("u", "v", "w", "T", "rho"):
+        assert np.array_equal(g0[name].view(np.uint32), g1[name].view(np.uint32)), name

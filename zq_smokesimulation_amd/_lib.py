@@ -83,7 +83,7 @@ class Timing(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-TUNE_MAX = 16
+TUNE_MAX = 32
 
 
 class TuneReport(C.Structure):

@@ -129,7 +129,10 @@ int adj_alloc(zq3d_adj* a, void** p, size_t bytes) {
 
 ProjBufs proj_bufs(zq3d_adj* a, double* p64) {
     const size_t sp = split_padded_size(a->d), sc = split_cell_size(a->d);
-    const SplitBufs split{{a->split_p, a->split_p + sp}, {a->split_b, a->split_b + sc}, {a->split_n, a->split_n + sc}};
+    const SplitBufs split{{a->split_p, a->split_p + sp},
+                          {a->split_b, a->split_b + sc},
+                          {a->split_n, a->split_n + sc},
+                          {a->split_p + 2 * sp, a->split_p + 3 * sp}};
     return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, SorShape{SOR_SPLIT, 0}, split};
 }
 
@@ -389,7 +392,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     A(&a->u0, z.Nu * e); A(&a->v0, z.Nv * e); A(&a->w0, z.Nw * e); A(&a->T0, z.N * e); A(&a->R0, z.N * e);
     A((void**)&a->occ, z.N); A((void**)&a->nc, z.N);
     A((void**)&a->b, z.N * 4); A((void**)&a->p, z.Np * 4); A((void**)&a->p_alt, z.Np * 4);
-    A((void**)&a->split_p, 2 * split_padded_size(a->d) * 4);
+    A((void**)&a->split_p, 4 * split_padded_size(a->d) * 4);
     A((void**)&a->split_b, 2 * split_cell_size(a->d) * 4);
     A((void**)&a->split_n, 2 * split_cell_size(a->d));
     A((void**)&a->b64, z.N * 8); A((void**)&a->p64, z.Np * 8); A((void**)&a->p64_adj, z.Np * 8);
@@ -419,7 +422,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     (void)hipMemsetAsync(a->occ, 0, z.N, s);
     (void)hipMemsetAsync(a->p, 0, z.Np * 4, s);
     (void)hipMemsetAsync(a->p_alt, 0, z.Np * 4, s);
-    (void)hipMemsetAsync(a->split_p, 0, 2 * split_padded_size(a->d) * 4, s);
+    (void)hipMemsetAsync(a->split_p, 0, 4 * split_padded_size(a->d) * 4, s);
     (void)hipMemsetAsync(a->p64, 0, z.Np * 8, s);
     (void)hipMemsetAsync(a->p64_adj, 0, z.Np * 8, s);
     (void)hipMemsetAsync(a->p640, 0, z.Np * 8, s);

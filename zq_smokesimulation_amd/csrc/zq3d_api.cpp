@@ -142,7 +142,8 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
                  const char* where, int* outer_out, double* rmax_out) {
     const Sizes z(d);
     Scalars* sc = B.sc;
-    const bool split = B.shape.ty == SOR_SPLIT;
+    const bool split = is_split(B.shape);
+    int pair = 0;  // split pressure pair holding the correction
     float* e = B.p;
     float* e_alt = B.p_alt;
     if (split) {
@@ -162,10 +163,10 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
         if (int rc = read_double_bits(&sc->rmax, &rmax, s, where)) return rc;
         if (!(rmax > tol) || outer >= o.max_outer) break;
         if (split) {
-            launch_sor_split_sweeps(d, B.split, &sc->zero32, o.omega, o.iters, s, pf ? pf->a : nullptr,
-                                    pf ? pf->b : nullptr);
+            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, o.iters, B.shape, s,
+                                       pf ? pf->a : nullptr, pf ? pf->b : nullptr);
             ZQ_HIP(pf ? account_sweeps(d, B.shape, o.iters, pf) : hipGetLastError(), where);
-            launch_update_p64_split(d, B.nc, B.split, B.p64, s);  // also zeroes e for the next pass
+            launch_update_p64_split(d, B.nc, pair ? B.split.alt : B.split.p, B.p64, s);  // also zeroes e
         } else {
             float* res = nullptr;
             ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res, B.shape, &B.split),
@@ -629,7 +630,7 @@ struct zq3d_sim {
     float* p = nullptr;           // padded FP32 pressure / correction e of the mixed solve
     float* p_alt = nullptr;       // second sweep buffer (ping-pong), zero ghost layer
     double* part = nullptr;       // reduction partials
-    float* split_p = nullptr;     // colour-split pressures (2 x padded), right-hand sides, counts
+    float* split_p = nullptr;     // colour-split pressures (2 pairs x 2 colours, padded), rhs, counts
     float* split_b = nullptr;
     uint8_t* split_n = nullptr;
     double* b64 = nullptr;
@@ -668,7 +669,10 @@ int field_info(zq3d_sim* s, int field, void** ptr, size_t* bytes) {
 
 ProjBufs proj_bufs(zq3d_sim* s) {
     const size_t sp = split_padded_size(s->d), sc = split_cell_size(s->d);
-    const SplitBufs split{{s->split_p, s->split_p + sp}, {s->split_b, s->split_b + sc}, {s->split_n, s->split_n + sc}};
+    const SplitBufs split{{s->split_p, s->split_p + sp},
+                          {s->split_b, s->split_b + sc},
+                          {s->split_n, s->split_n + sc},
+                          {s->split_p + 2 * sp, s->split_p + 3 * sp}};
     return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc, s->shape, split};
 }
 
@@ -765,7 +769,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     A(dalloc(&s->p, z.Np * 4));
     A(dalloc(&s->p_alt, z.Np * 4));
     A(dalloc(&s->part, (size_t)reduction_partials(s->d) * 8));
-    A(dalloc(&s->split_p, 2 * split_padded_size(s->d) * 4));
+    A(dalloc(&s->split_p, 4 * split_padded_size(s->d) * 4));
     A(dalloc(&s->split_b, 2 * split_cell_size(s->d) * 4));
     A(dalloc(&s->split_n, 2 * split_cell_size(s->d)));
     A(dalloc(&s->sc, sizeof(Scalars)));
@@ -787,7 +791,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     launch_fill_u8(s->occ, z.N, 0, st);
     launch_fill_f32(s->p, z.Np, 0.0f, st);
     launch_fill_f32(s->p_alt, z.Np, 0.0f, st);
-    launch_fill_f32(s->split_p, 2 * split_padded_size(s->d), 0.0f, st);
+    launch_fill_f32(s->split_p, 4 * split_padded_size(s->d), 0.0f, st);
     if (s->p64) launch_fill_f64(s->p64, z.Np, 0.0, st);
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -975,7 +979,7 @@ extern "C" int zq3d_set_profiling(zq3d_sim* s, int on) {
 }
 
 extern "C" int zq3d_set_sor_shape(zq3d_sim* s, int ty, int zc) {
-    if (!s || ty < SOR_SPLIT || zc < 0) { set_err("zq3d_set_sor_shape", "bad argument"); return ZQ3D_ERR_ARG; }
+    if (!s || zc < 0) { set_err("zq3d_set_sor_shape", "bad argument"); return ZQ3D_ERR_ARG; }
     SorShape cand[32];
     const int n = sor_shape_candidates(s->d, cand, 32);
     bool ok = (ty == 0 && zc == 0);
@@ -1012,9 +1016,10 @@ extern "C" int zq3d_autotune_sor(zq3d_sim* s, int iters, zq3d_tune_report* rep) 
     for (int i = 0; i < n; i++) {
         launch_fill_f32(s->p, z.Np, 0.0f, s->stream);
         launch_sor(s->d, s->nc, s->b, &s->sc->mean32, s->p, s->p_alt, 1.8f, 2, s->stream, cand[i], &B.split);
-        E(hipEventRecord(a, s->stream));
-        launch_sor(s->d, s->nc, s->b, &s->sc->mean32, s->p, s->p_alt, 1.8f, iters, s->stream, cand[i], &B.split);
-        E(hipEventRecord(b, s->stream));
+        // the events bracket the sweeps only (not the split-layout conversions)
+        launch_sor(s->d, s->nc, s->b, &s->sc->mean32, s->p, s->p_alt, 1.8f, iters, s->stream, cand[i], &B.split, a,
+                   b);
+        E(hipGetLastError());
         E(hipEventSynchronize(b));
         float ms = 0.0f;
         E(hipEventElapsedTime(&ms, a, b));

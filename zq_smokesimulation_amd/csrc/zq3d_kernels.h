@@ -67,18 +67,22 @@ void launch_proj_setup_free(const Dims& d, const float* u, const float* v, const
 // mean32/mean64 = sum/count of the setup partials (0 when want == 0)
 void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s);
 // Sweep algorithm / tile shape: ty rows per block (2, 4 or 8) and zc planes per chunk of the fused
-// one-pass kernel; ty == SOR_SPLIT selects single-colour sweeps on colour-split storage; {0, 0}
-// selects the built-in default.
+// one-pass kernel on interleaved storage; ty == SOR_SPLIT selects single-colour sweeps on
+// colour-split storage; ty == -r (r = 4, 8 or 16) selects one launch per red-black iteration on
+// colour-split storage with r rows by zc planes per workgroup; {0, 0} selects the built-in default.
 constexpr int SOR_SPLIT = -1;
 struct SorShape {
     int ty, zc;
 };
-// Colour-split scratch: per colour a padded pressure (split_padded_size floats), right-hand side
-// and counts (split_cell_size each).  Padding slots must hold 0.
+inline bool is_split(SorShape s) { return s.ty <= SOR_SPLIT; }
+// Colour-split scratch: per colour a padded pressure (split_padded_size floats), a second padded
+// pressure the one-launch iteration ping-pongs with, right-hand side and counts (split_cell_size
+// each).  Padding slots and the slots past a row's end must hold 0 in p and alt.
 struct SplitBufs {
     float* p[2];
     float* b[2];
     uint8_t* n[2];
+    float* alt[2];
 };
 size_t split_padded_size(const Dims& d);
 size_t split_cell_size(const Dims& d);
@@ -88,10 +92,11 @@ size_t split_cell_size(const Dims& d);
 float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p_pad, float* p_alt,
                   float omega, int iters, hipStream_t s, SorShape shape = SorShape{0, 0},
                   const SplitBufs* split = nullptr, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
-// iters single-colour sweep pairs on the split arrays as they stand (no layout conversion); events
-// (nullable) bracket the sweeps
-void launch_sor_split_sweeps(const Dims& d, const SplitBufs& sp, const float* mean32, float omega, int iters,
-                             hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// iters red-black iterations on the split arrays as they stand (no layout conversion), starting
+// from the pressure pair `from` (0: sp.p, 1: sp.alt) with the split algorithm `shape`; returns the
+// pair holding the result.  Events (nullable) bracket the sweeps.
+int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const float* mean32, float omega, int iters,
+                        SorShape shape, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 // split counts n_0 / n_1 from the interleaved counts
 void launch_split_counts(const Dims& d, const uint8_t* nc, const SplitBufs& sp, hipStream_t s);
 // name of the sweep kernel a shape runs
@@ -118,8 +123,8 @@ void launch_residual64_split(const Dims& d, const uint8_t* nc, const double* b64
                              hipStream_t s);
 // p64 += e, e = 0 on counted cells
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s);
-// the same with e in the colour-split arrays sp.p[0] / sp.p[1]
-void launch_update_p64_split(const Dims& d, const uint8_t* nc, const SplitBufs& sp, double* p64_pad, hipStream_t s);
+// the same with e in a colour-split pressure pair e[0] / e[1]
+void launch_update_p64_split(const Dims& d, const uint8_t* nc, float* const* e, double* p64_pad, hipStream_t s);
 // max |div64 - mean64| over counted cells into *dmax (bits of a non-negative double)
 template <class F>
 void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* nc, const double* mean64,
