@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
     ap.add_argument("--ens-members", type=int, default=64, help="ensemble members (0: skip the ensemble leg)")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
+    ap.add_argument("--sor-shape", default=None,
+                    help="sweep shape 'ty,zc' to use instead of autotuning it (a zq3d_autotune_sor candidate)")
     ap.add_argument("--traffic", default=None,
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel (default: "
                          "profiles/<kernel>_<W>_traffic.txt for the kernel the autotuner picked)")
@@ -184,7 +186,7 @@ def ensemble_leg(args):
     }
 
 
-def adjoint_window(state, W, H, D, args, fwd_ms_per_step):
+def adjoint_window(state, W, H, D, args, fwd_ms_per_step, shape):
     """Forward + adjoint of a window of args.adj_window steps from `state` (FP32, sqrt checkpoints)."""
     import math
 
@@ -197,6 +199,8 @@ def adjoint_window(state, W, H, D, args, fwd_ms_per_step):
                        buoyancy=0.5, gravity=0.05, confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99,
                        sor_open_mask=0, sor_closed_mean=1, sor_omega=1.8, sor_iters=args.inner,
                        sor_max_outer=args.max_outer, sor_tol=args.tol, warm_start=1)
+    if shape is not None:
+        eng.set_sor_shape(shape["ty"], shape["zc"])
     eng.set_initial(**{k: torch.from_numpy(state[k]) for k in ("u", "v", "w", "T", "rho", "occ", "p64")})
     eng.set_sources(spheres=[SOURCE])
     target = torch.from_numpy(gaussian_blobs(W, H, D, 8, 0)).cuda()
@@ -263,9 +267,15 @@ def main():
         step_idx += 1
     sim.synchronize()
     torch.cuda.synchronize()
-    tune = sim.autotune_sor(20)
-    best = tune["candidates"][tune["best"]] if tune["best"] >= 0 else None
-    print("bench: warmup done; sweep tile autotune %s" % json.dumps(tune), file=sys.stderr, flush=True)
+    if args.sor_shape:
+        ty, zc = (int(x) for x in args.sor_shape.split(","))
+        sim.set_sor_shape(ty, zc)
+        best = {"ty": ty, "zc": zc, "us_per_iter": None}
+        print("bench: warmup done; sweep shape set to %s" % args.sor_shape, file=sys.stderr, flush=True)
+    else:
+        tune = sim.autotune_sor(20)
+        best = tune["candidates"][tune["best"]] if tune["best"] >= 0 else None
+        print("bench: warmup done; sweep tile autotune %s" % json.dumps(tune), file=sys.stderr, flush=True)
     sim.set_profiling(True)
     stats = []
     torch.cuda.synchronize()
@@ -354,7 +364,7 @@ def main():
     sim.close()
     if args.adj_window > 0:
         print("bench: forward+adjoint window of %d steps" % args.adj_window, file=sys.stderr, flush=True)
-        out["fwd_adjoint"] = adjoint_window(g, W, H, D, args, ms_per_step)
+        out["fwd_adjoint"] = adjoint_window(g, W, H, D, args, ms_per_step, best)
     if args.ens_members > 0:
         print("bench: ensemble of %d grids" % args.ens_members, file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_leg(args)

@@ -325,6 +325,9 @@ void zq3d_adj_destroy(zq3d_adj* a);
 int zq3d_adj_set_stream(zq3d_adj* a, void* stream);
 int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8, int n_sphere, const float* spheres6);
 int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol);
+/* sweep algorithm / tile shape of every projection of the rollout, one of zq3d_autotune_sor's
+ * candidates for this grid (default (-1, 0)); all give the same bits */
+int zq3d_adj_set_sor_shape(zq3d_adj* a, int ty, int zc);
 /* initial state: ZQ3D_FIELD_U/V/W/T/RHO (device, chosen precision), ZQ3D_FIELD_OCC (device bytes) or
  * ZQ3D_FIELD_P64 (device doubles, padded: the forward projection's warm start; default 0) */
 int zq3d_adj_set_initial(zq3d_adj* a, int field, const void* dev);

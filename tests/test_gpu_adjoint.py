@@ -189,6 +189,33 @@ def test_checkpoint_spacing_does_not_change_gradient():
         np.testing.assert_allclose(g, out[0], rtol=1e-7, atol=1e-9 * np.abs(out[0]).max())
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_sweep_shape_does_not_change_rollout_or_gradient(dtype):
+    """Every projection of the rollout (forward, recomputation, adjoint) runs the sweep algorithm set
+    by set_sor_shape; single-colour split sweeps and one-launch iterations give the same forward
+    bits.  The gradient is compared to a tolerance: the advection adjoint accumulates with atomics,
+    so its last bits depend on the order in which the GPU retires them, whatever the shape."""
+    W, H, D, nsteps = 40, 36, 28, 4
+    out = []
+    for shape in ((-1, 0), (-16, 16), (-32, 8)):
+        eng, _ = make_engine(W, H, D, nsteps, 2, dtype, closed=True, obstacle=True)
+        eng.set_sor_shape(*shape)
+        gen = torch.Generator(device="cpu").manual_seed(9)
+        ctrl = (0.5 * torch.randn(eng.controls_shape(), generator=gen, dtype=dtype)).to(DEV)
+        target = torch.rand((D, H, W), generator=gen, dtype=dtype).to(DEV)
+        rho = eng.forward(ctrl)
+        g = eng.backward(ctrl, rho - target)
+        print("shape", shape, eng.last_forward, eng.last_backward, flush=True)
+        out.append((rho.cpu().numpy(), g.cpu().numpy()))
+        eng.close()
+    gmax = np.abs(out[0][1]).max()
+    assert gmax > 0
+    tol = 1e-5 if dtype == torch.float32 else 1e-12
+    for rho, g in out[1:]:
+        np.testing.assert_array_equal(rho, out[0][0])
+        np.testing.assert_allclose(g, out[0][1], rtol=0, atol=tol * gmax)
+
+
 def test_fp32_rollout_is_the_product_step():
     import zq_smokesimulation_amd as z
     W, H, D, nsteps = 24, 28, 20, 5

@@ -135,7 +135,7 @@ def test_every_sweep_tile_shape_is_bit_identical():
     sim.close()
 
 
-@pytest.mark.parametrize("ty", [-4, -8, -16])
+@pytest.mark.parametrize("ty", [-4, -8, -16, -32])
 def test_one_launch_iteration_multi_tile(ty):
     """The one-launch red-black iteration on colour-split storage works on tiles of 64 slots (128
     cells) in x; at W = 150 a row spans three tiles, the last partial, and the chunks of zc planes

@@ -100,6 +100,10 @@ class SmokeRollout:
         """Adjoint projection tolerance relative to max|rhs|; 0 matches each step's forward projection."""
         check(_lib.lib().zq3d_adj_set_adjoint_tol(self._h, float(rtol)), "zq3d_adj_set_adjoint_tol")
 
+    def set_sor_shape(self, ty, zc):
+        """Sweep algorithm of every projection (a SmokeSim3D.autotune_sor candidate); same bits."""
+        check(_lib.lib().zq3d_adj_set_sor_shape(self._h, int(ty), int(zc)), "zq3d_adj_set_sor_shape")
+
     # ---- rollout -------------------------------------------------------------------
     def controls_shape(self):
         return (self.nsteps, self.D, self.H, self.W, 3)

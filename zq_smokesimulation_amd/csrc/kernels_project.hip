@@ -851,8 +851,6 @@ void launch_mean_finalize(const Dims& d, const double* part, float* mean32, doub
 size_t split_padded_size(const Dims& d) { return (size_t)((d.W + 1) / 2 + 2) * (d.H + 2) * (d.D + 2); }
 size_t split_cell_size(const Dims& d) { return (size_t)((d.W + 1) / 2) * d.H * d.D; }
 
-bool split_rb_rows_ok(int ty) { return ty == 4 || ty == 8 || ty == 16; }
-
 template <int TY, int NW>
 void launch_split_rb(const Dims& d, const SplitBufs& sp, const float* const* src, float* const* dst,
                      const float* mean32, float omega, float omw, int zc, hipStream_t s) {
@@ -883,7 +881,8 @@ int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const floa
             float* const* dst = from ? sp.p : sp.alt;
             if (ty == 4) launch_split_rb<4, 4>(d, sp, src, dst, mean32, omega, omw, zc, s);
             else if (ty == 8) launch_split_rb<8, 8>(d, sp, src, dst, mean32, omega, omw, zc, s);
-            else launch_split_rb<16, 8>(d, sp, src, dst, mean32, omega, omw, zc, s);
+            else if (ty == 16) launch_split_rb<16, 8>(d, sp, src, dst, mean32, omega, omw, zc, s);
+            else launch_split_rb<32, 16>(d, sp, src, dst, mean32, omega, omw, zc, s);
             from = 1 - from;
         }
     }
@@ -957,7 +956,7 @@ int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
             if (fused_shape(d, sh).nblocks > 0 && n < max) out[n++] = sh;
         }
     if (n < max) out[n++] = SorShape{SOR_SPLIT, 0};
-    static const int rb_rows[] = {4, 8, 16};
+    static const int rb_rows[] = {4, 8, 16, 32};
     static const int rb_zc[] = {4, 8, 16, 32};
     for (int ty : rb_rows)
         for (int zc : rb_zc)

@@ -89,6 +89,7 @@ struct zq3d_adj {
     hipStream_t stream = nullptr;
     std::vector<float> boxes, spheres, boxes0, spheres0;  // *0: source values zeroed (adjoint)
     double rtol = 0.0;          // adjoint projection tolerance relative to max|rhs|; 0: match the forward
+    SorShape shape{SOR_SPLIT, 0};  // sweep algorithm of every projection (forward and adjoint)
     std::vector<double> fwd_rel;  // per step: forward tolerance / max|rhs| of its projection
     std::vector<void*> allocs;
     // state: u, v, w, T, rho, T2, rho2 (value type), vel4 (4-wide), scratch faces (Nu+Nv+Nw)
@@ -133,7 +134,7 @@ ProjBufs proj_bufs(zq3d_adj* a, double* p64) {
                           {a->split_b, a->split_b + sc},
                           {a->split_n, a->split_n + sc},
                           {a->split_p + 2 * sp, a->split_p + 3 * sp}};
-    return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, SorShape{SOR_SPLIT, 0}, split};
+    return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, a->shape, split};
 }
 
 // pointers into one tape slot
@@ -464,6 +465,18 @@ extern "C" int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8,
 extern "C" int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol) {
     if (!a || !(rtol >= 0.0)) { set_err("zq3d_adj_set_adjoint_tol", "bad argument"); return ZQ3D_ERR_ARG; }
     a->rtol = rtol;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_adj_set_sor_shape(zq3d_adj* a, int ty, int zc) {
+    const char* where = "zq3d_adj_set_sor_shape";
+    if (!a) { set_err(where, "null handle"); return ZQ3D_ERR_ARG; }
+    SorShape cand[ZQ3D_TUNE_MAX];
+    const int n = sor_shape_candidates(a->d, cand, ZQ3D_TUNE_MAX);
+    bool ok = false;
+    for (int i = 0; i < n; i++) ok = ok || (cand[i].ty == ty && cand[i].zc == zc);
+    if (!ok) { set_err(where, "shape not among the sweep candidates for this grid"); return ZQ3D_ERR_ARG; }
+    a->shape = SorShape{ty, zc};
     return ZQ3D_OK;
 }
 
