@@ -346,8 +346,8 @@ int zq3d_adj_get(zq3d_adj* a, int field, int grad, void* dev);
 /* No reference counterpart.  count independent simulations, each with its own zq3d_config (grid
  * size and physics; ADV_VEL_INREG_OUTREG, ADV_SCA_REG and the plain FP32 projection with its own
  * sweep count), stepped together: every stage is one persistent launch over all members' tiles,
- * and the step (about 20 launches plus two per SOR iteration) is captured once in a hipGraph and
- * replayed.  Each member evolves bit for bit as a zq3d_sim of the same config.  Runs on its own
+ * and the step (about 20 launches plus one or two per SOR iteration) is captured once in a hipGraph
+ * and replayed.  Each member evolves bit for bit as a zq3d_sim of the same config.  Runs on its own
  * HIP stream (default priority). */
 typedef struct zq3d_ens zq3d_ens;
 int zq3d_ens_create(int count, const zq3d_config* configs, zq3d_ens** out);
@@ -366,20 +366,25 @@ int zq3d_ens_synchronize(zq3d_ens* e);
 long long zq3d_ens_cells(zq3d_ens* e);
 /* the ensemble's hipStream_t */
 int zq3d_ens_stream(zq3d_ens* e, void** stream);
-/* Launch-shape autotuner: times the sweep kernel for each tile shape (tbx x tby cells) and
- * persistent launch size (blocks; 0 = one workgroup per tile) and the cell kernels for each launch
- * size, on the members' own grids and occupancy, and keeps the fastest.  Fields are unchanged. */
+/* Launch-shape autotuner: times the sweep kernels -- kind 0: single-colour half sweeps for each
+ * tile shape (tbx x tby cells) and persistent launch size (blocks; 0 = one workgroup per tile);
+ * kind 1: one launch per red-black iteration on colour-split storage, tiles of tbx slots x tby rows
+ * x `blocks` planes (the default before any autotune is 64 x 16 x 16) -- and the cell kernels for
+ * each launch size, on the members' own grids and occupancy, and keeps the fastest.  All give the
+ * same bits.  Fields are unchanged. */
 #define ZQ3D_ENS_TUNE_MAX 24
 typedef struct zq3d_ens_tune {
     int count;
-    int tbx[ZQ3D_ENS_TUNE_MAX], tby[ZQ3D_ENS_TUNE_MAX], blocks[ZQ3D_ENS_TUNE_MAX];
+    int kind[ZQ3D_ENS_TUNE_MAX], tbx[ZQ3D_ENS_TUNE_MAX], tby[ZQ3D_ENS_TUNE_MAX], blocks[ZQ3D_ENS_TUNE_MAX];
     float us_per_iter[ZQ3D_ENS_TUNE_MAX];
-    int best_tbx, best_tby, best_sweep_blocks;
+    int best_kind, best_tbx, best_tby, best_sweep_blocks;  /* kind 1: best_sweep_blocks = planes */
     float best_us_per_iter;
     int best_blocks;       /* launch size of the other stages */
     float probe_ms;        /* cell advection probe at that size */
 } zq3d_ens_tune;
 int zq3d_ens_autotune(zq3d_ens* e, int iters, zq3d_ens_tune* report);
+/* selects one of the autotuner's sweep candidates (kind, tbx, tby, blocks as reported) */
+int zq3d_ens_set_sweep(zq3d_ens* e, int kind, int tbx, int tby, int blocks);
 
 /* message of the last failed call on this thread ("" if none) */
 const char* zq3d_last_error(void);

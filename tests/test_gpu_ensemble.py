@@ -103,6 +103,46 @@ def test_ensemble_matches_single_grids():
     ens.close()
 
 
+def test_every_sweep_candidate_is_bit_identical():
+    """Every sweep candidate of the autotuner (half sweeps, and one-launch iterations on colour-split
+    storage with tiles of 32 or 64 slots) steps every member bit for bit as its SmokeSim3D; a member
+    150 cells wide spans several tiles per row."""
+    import zq_smokesimulation_amd as z
+    from zq_smokesimulation_amd.ensemble import SmokeEnsemble, member_config
+    members = [((150, 20, 18), True, True, 0.05, 0.5, 23), ((33, 40, 28), False, False, 0.02, 1.0, 30),
+               ((61, 37, 45), True, True, 0.03, 0.9, 24)]
+    cfgs, sims, states = [], [], []
+    for n, ((W, H, D), closed, obst, a, b, it) in enumerate(members):
+        args = member_args(closed, a, b, it, W)
+        cfgs.append(member_config(W, H, D, **args))
+        sim = z.SmokeSim3D(W, H, D, **args)
+        st = initial(W, H, D, closed, obst, 10 + n)
+        sim.set_state(**st)
+        sim.set_sphere_sources(SPHERE)
+        sims.append(sim)
+        states.append(st)
+    ens = SmokeEnsemble(cfgs)
+    for n, st in enumerate(states):
+        ens.set_state(n, **st)
+        ens.set_sources(n, spheres=SPHERE)
+    tune = ens.autotune(2)
+    kinds = {c["kind"] for c in tune["candidates"]}
+    assert kinds == {"half_sweeps", "split_one_launch"}
+    for c in tune["candidates"]:
+        if c["kind"] == "half_sweeps" and c["blocks"] not in (0, 256):
+            continue
+        print("candidate", c, flush=True)
+        ens.set_sweep(c["kind"], c["tbx"], c["tby"], c["blocks"])
+        ens.step(1, graph=True)
+        for sim in sims:
+            sim.step()
+        ens.synchronize()
+        compare(ens, sims, "candidate %s" % c)
+    for sim in sims:
+        sim.close()
+    ens.close()
+
+
 def test_ensemble_rejects_mixed_precision():
     from zq_smokesimulation_amd import ZQ3DError
     from zq_smokesimulation_amd.ensemble import SmokeEnsemble, member_config

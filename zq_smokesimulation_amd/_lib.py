@@ -270,9 +270,10 @@ ENS_TUNE_MAX = 24
 class EnsTune(C.Structure):
     _fields_ = [
         ("count", C.c_int),
+        ("kind", C.c_int * ENS_TUNE_MAX),
         ("tbx", C.c_int * ENS_TUNE_MAX), ("tby", C.c_int * ENS_TUNE_MAX), ("blocks", C.c_int * ENS_TUNE_MAX),
         ("us_per_iter", C.c_float * ENS_TUNE_MAX),
-        ("best_tbx", C.c_int), ("best_tby", C.c_int), ("best_sweep_blocks", C.c_int),
+        ("best_kind", C.c_int), ("best_tbx", C.c_int), ("best_tby", C.c_int), ("best_sweep_blocks", C.c_int),
         ("best_us_per_iter", C.c_float),
         ("best_blocks", C.c_int),
         ("probe_ms", C.c_float),
@@ -280,9 +281,12 @@ class EnsTune(C.Structure):
 
     def as_dict(self):
         n = self.count
-        return {"candidates": [{"tbx": self.tbx[i], "tby": self.tby[i], "blocks": self.blocks[i],
-                                "us_per_iter": round(self.us_per_iter[i], 3)} for i in range(n)],
-                "best": {"tbx": self.best_tbx, "tby": self.best_tby, "sweep_blocks": self.best_sweep_blocks,
+        kinds = ("half_sweeps", "split_one_launch")
+        return {"candidates": [{"kind": kinds[self.kind[i]], "tbx": self.tbx[i], "tby": self.tby[i],
+                                "blocks": self.blocks[i], "us_per_iter": round(self.us_per_iter[i], 3)}
+                               for i in range(n)],
+                "best": {"kind": kinds[self.best_kind], "tbx": self.best_tbx, "tby": self.best_tby,
+                         "sweep_blocks": self.best_sweep_blocks,
                          "us_per_iter": round(self.best_us_per_iter, 3), "blocks": self.best_blocks,
                          "probe_ms": round(self.probe_ms, 4)}}
 
@@ -301,4 +305,5 @@ def _bind_ensemble(L):
     L.zq3d_ens_cells.restype = C.c_longlong
     L.zq3d_ens_stream.argtypes = [vp, C.POINTER(C.c_void_p)]
     L.zq3d_ens_autotune.argtypes = [vp, C.c_int, C.POINTER(EnsTune)]
+    L.zq3d_ens_set_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     return L

@@ -9,6 +9,7 @@
 // of the same configuration (plain FP32 projection).  The red-black sweep takes its tile shape
 // (TBX x TBY cells) as a template parameter chosen by the ensemble autotuner.
 #include "zq3d_ensemble.h"
+#include "zq3d_sor_split.h"
 #include "zq3d_stencils.h"
 
 namespace zq3d {
@@ -153,6 +154,51 @@ __global__ __launch_bounds__(NT) void ke_sor_half(const EnsMember* __restrict__ 
     });
 }
 
+// every member's counts, right-hand side and pressure -> its colour-split arrays (pressure pair 0)
+__global__ __launch_bounds__(NT) void ke_split_in(const EnsMember* __restrict__ M, EnsTable T, int B) {
+    for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
+        const EnsMember& m = M[s];
+        if (i >= m.d.W || j >= m.d.H) return;
+        const int Wh = (m.d.W + 1) / 2, c = (i + j + k) & 1;
+        const int oh = (k * m.d.H + j) * Wh + (i >> 1);
+        m.sn[c][oh] = m.nc[cidx(m.d, i, j, k)];
+        m.sb[c][oh] = m.b[cidx(m.d, i, j, k)];
+        m.sp[c][split_q(Wh, m.d.H, i >> 1, j, k)] = m.p[pidx(m.d, i, j, k)];
+    });
+}
+
+// the pair holding every member's result (its own iteration count's parity) -> interleaved pressure
+__global__ __launch_bounds__(NT) void ke_split_out(const EnsMember* __restrict__ M, EnsTable T, int B) {
+    for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
+        const EnsMember& m = M[s];
+        if (i >= m.d.W || j >= m.d.H) return;
+        const int Wh = (m.d.W + 1) / 2, c = (i + j + k) & 1;
+        m.p[pidx(m.d, i, j, k)] = m.sp[2 * (m.iters & 1) + c][split_q(Wh, m.d.H, i >> 1, j, k)];
+    });
+}
+
+// iteration `it` of every member's SOR in one launch: one workgroup per tile of the table (members'
+// tiles numbered member after member, handed out so each XCD gets a contiguous run); iteration it
+// reads pair it & 1 and writes the other; members whose own iteration count is reached are left
+// alone
+template <int TX, int TY, int NW>
+__global__ __launch_bounds__(64 * NW) void ke_sor_split_rb(const EnsMember* __restrict__ M, EnsTable T, int B,
+                                                           int zc, int it) {
+    const int per = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= T.start[B]) return;  // spare block: uniform exit before any barrier
+    const int s = tile_member(T.start, B, tile);
+    const EnsMember& m = M[s];
+    if (it >= m.iters) return;
+    const int local = tile - T.start[s];
+    const int gx = T.gx[s], gy = T.gy[s];
+    const int bx = local % gx, by = (local / gx) % gy, z0 = (local / (gx * gy)) * zc;
+    const int src = it & 1;
+    sor_split_rb_tile<TX, TY, NW>(m.sn[0], m.sn[1], m.sb[0], m.sb[1], *m.mean32, m.sp[2 * src], m.sp[2 * src + 1],
+                                  m.sp[2 - 2 * src], m.sp[3 - 2 * src], m.d, (m.d.W + 1) / 2, bx * TX, by * TY, z0,
+                                  min(z0 + zc, m.d.D), m.omega, m.omw);
+}
+
 template <int AXIS>
 __global__ __launch_bounds__(NT) void ke_correct(const EnsMember* __restrict__ M, EnsTable T, int B) {
     for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
@@ -180,6 +226,8 @@ inline unsigned grid_of(const EnsTable& T, int blocks) {
 
 // tile layouts of the sweep kernel (TBX x TBY) the autotuner may pick
 const int kEnsSweepShapes[kEnsSweepShapeCount][2] = {{64, 4}, {32, 8}, {16, 16}};
+const int kEnsRbShapes[kEnsRbShapeCount][3] = {{64, 8, 8},  {64, 8, 16},  {64, 16, 8}, {64, 16, 16},
+                                               {64, 16, 32}, {64, 32, 8}, {64, 32, 16}};
 
 void ens_launch_step(const EnsLaunch& L, hipStream_t s) {
     const dim3 nt(NT);
@@ -201,7 +249,9 @@ void ens_launch_step(const EnsLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL(ke_cells_to_faces<2>, dim3(gw), nt, 0, s, M, L.wfaces, B);
     // plain projection
     ens_launch_setup(L, s);
+    if (L.sweep_rb) ens_launch_split_in(L, s);
     ens_launch_sweeps(L, 0, L.max_iters, s);
+    if (L.sweep_rb) hipLaunchKernelGGL(ke_split_out, dim3(gc), nt, 0, s, M, L.cells, B);
     hipLaunchKernelGGL(ke_correct<0>, dim3(gu), nt, 0, s, M, L.ufaces, B);
     hipLaunchKernelGGL(ke_correct<1>, dim3(gv), nt, 0, s, M, L.vfaces, B);
     hipLaunchKernelGGL(ke_correct<2>, dim3(gw), nt, 0, s, M, L.wfaces, B);
@@ -225,8 +275,28 @@ void ens_launch_probe(const EnsLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL(ke_advect_cells, dim3(gc), dim3(NT), 0, s, L.members, L.cells, L.count);
 }
 
+void ens_launch_split_in(const EnsLaunch& L, hipStream_t s) {
+    hipLaunchKernelGGL(ke_split_in, dim3(grid_of(L.cells, L.blocks)), dim3(NT), 0, s, L.members, L.cells, L.count);
+}
+
 void ens_launch_sweeps(const EnsLaunch& L, int it0, int it1, hipStream_t s) {
     const dim3 nt(NT);
+    if (L.sweep_rb) {
+        const int r = L.sweep_rb - 1;
+        const int ty = kEnsRbShapes[r][1], zc = kEnsRbShapes[r][2];  // tiles of 64 slots
+        const EnsTable& T = L.rb[r];
+        const unsigned nb = (unsigned)((T.total + 7) & ~7);  // a multiple of 8 for the XCD numbering
+        const EnsMember* M = L.members;
+        for (int it = it0; it < it1; it++) {
+            if (ty == 8)
+                hipLaunchKernelGGL((ke_sor_split_rb<64, 8, 8>), dim3(nb), dim3(512), 0, s, M, T, L.count, zc, it);
+            else if (ty == 16)
+                hipLaunchKernelGGL((ke_sor_split_rb<64, 16, 8>), dim3(nb), dim3(512), 0, s, M, T, L.count, zc, it);
+            else
+                hipLaunchKernelGGL((ke_sor_split_rb<64, 32, 16>), dim3(nb), dim3(1024), 0, s, M, T, L.count, zc, it);
+        }
+        return;
+    }
     const EnsTable& H = L.half[L.sweep_shape];
     const unsigned gh = grid_of(H, L.sweep_blocks);
     for (int it = it0; it < it1; it++)

@@ -26,13 +26,15 @@ struct zq3d_ens {
     hipStream_t stream = nullptr;
     EnsLaunch L{};
     // pool offsets per member (elements)
-    std::vector<size_t> off_u, off_v, off_w, off_c, off_p, off_part;
+    std::vector<size_t> off_u, off_v, off_w, off_c, off_p, off_part, off_sc, off_sp;
     float *u = nullptr, *v = nullptr, *w = nullptr, *T = nullptr, *R = nullptr, *T2 = nullptr, *R2 = nullptr;
     uint8_t *occ = nullptr, *nc = nullptr;
     float4* vel4 = nullptr;
     float *scratch = nullptr, *b = nullptr, *p = nullptr, *mean32 = nullptr;
     double* part = nullptr;
-    size_t Nu = 0, Nv = 0, Nw = 0, Nc = 0, Np = 0, Nparts = 0;
+    uint8_t* split_n = nullptr;  // colour-split counts (2 per member), right-hand sides, pressures (4)
+    float *split_b = nullptr, *split_p = nullptr;
+    size_t Nu = 0, Nv = 0, Nw = 0, Nc = 0, Np = 0, Nparts = 0, Nsc = 0, Nsp = 0;
     long long cells = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -104,6 +106,12 @@ void build_tables(zq3d_ens* e, std::vector<int>& flat, std::vector<size_t>& at) 
         const int tbx = kEnsSweepShapes[sh][0], tby = kEnsSweepShapes[sh][1];
         add([tbx, tby](const Dims& d, int& x, int& y, int& z) { x = cdiv((d.W + 1) / 2, tbx), y = cdiv(d.H, tby), z = d.D; });
     }
+    for (int r = 0; r < kEnsRbShapeCount; r++) {
+        const int tx = kEnsRbShapes[r][0], ty = kEnsRbShapes[r][1], zc = kEnsRbShapes[r][2];
+        add([tx, ty, zc](const Dims& d, int& x, int& y, int& z) {
+            x = cdiv((d.W + 1) / 2, tx), y = cdiv(d.H, ty), z = cdiv(d.D, zc);
+        });
+    }
 }
 
 EnsTable table_at(const zq3d_ens* e, const std::vector<int>& flat, size_t at) {
@@ -153,6 +161,10 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
         e->off_c.push_back(e->Nc);
         e->off_p.push_back(e->Np);
         e->off_part.push_back(e->Nparts);
+        e->off_sc.push_back(e->Nsc);
+        e->off_sp.push_back(e->Nsp);
+        e->Nsc += 2 * split_cell_size(d);
+        e->Nsp += 4 * split_padded_size(d);
         e->Nu += z.Nu;
         e->Nv += z.Nv;
         e->Nw += z.Nw;
@@ -172,6 +184,8 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
     bad |= ens_alloc(e, &e->scratch, (e->Nu + e->Nv + e->Nw) * 4) | ens_alloc(e, &e->b, e->Nc * 4) |
            ens_alloc(e, &e->p, e->Np * 4);
     bad |= ens_alloc(e, &e->part, e->Nparts * 8) | ens_alloc(e, &e->mean32, (size_t)count * 4);
+    bad |= ens_alloc(e, &e->split_n, e->Nsc) | ens_alloc(e, &e->split_b, e->Nsc * 4) |
+           ens_alloc(e, &e->split_p, e->Nsp * 4);
     bad |= ens_alloc(e, &e->dev_members, sizeof(EnsMember) * count) | ens_alloc(e, &e->dev_tables, flat.size() * 4);
     if (bad || hipStreamCreate(&e->stream) != hipSuccess) {
         set_err(where, "device allocation failed");
@@ -202,6 +216,12 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
         m.p = e->p + e->off_p[s];
         m.part = e->part + e->off_part[s];
         m.mean32 = e->mean32 + s;
+        const size_t sc = split_cell_size(d), sp = split_padded_size(d);
+        for (int c = 0; c < 2; c++) {
+            m.sn[c] = e->split_n + e->off_sc[s] + c * sc;
+            m.sb[c] = e->split_b + e->off_sc[s] + c * sc;
+        }
+        for (int q = 0; q < 4; q++) m.sp[q] = e->split_p + e->off_sp[s] + q * sp;
         // the single-grid launchers' host arithmetic (launch_force_atten, launch_advect_*, launch_sor)
         m.h = c.voxel_size;
         m.dt = c.step_time;
@@ -235,6 +255,10 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
     E(hipMemsetAsync(e->p, 0, e->Np * 4, e->stream));
     E(hipMemsetAsync(e->nc, 0, e->Nc, e->stream));
     E(hipMemsetAsync(e->b, 0, e->Nc * 4, e->stream));
+    // slots past a row's end and the padding of the split arrays stay 0 from here on
+    E(hipMemsetAsync(e->split_n, 0, e->Nsc, e->stream));
+    E(hipMemsetAsync(e->split_b, 0, e->Nsc * 4, e->stream));
+    E(hipMemsetAsync(e->split_p, 0, e->Nsp * 4, e->stream));
     if (E.e != hipSuccess) {
         set_err(where, hipGetErrorString(E.e));
         zq3d_ens_destroy(e);
@@ -252,7 +276,9 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
     L.vfaces = table_at(e, flat, at[2]);
     L.wfaces = table_at(e, flat, at[3]);
     for (int sh = 0; sh < kEnsSweepShapeCount; sh++) L.half[sh] = table_at(e, flat, at[4 + sh]);
+    for (int r = 0; r < kEnsRbShapeCount; r++) L.rb[r] = table_at(e, flat, at[4 + kEnsSweepShapeCount + r]);
     L.sweep_shape = 0;
+    L.sweep_rb = 4;  // one-launch iterations, 64 slots x 16 rows x 16 planes per workgroup
     L.blocks = 0;
     L.sweep_blocks = 0;
     L.p_pool = e->p;
@@ -363,13 +389,39 @@ extern "C" int zq3d_ens_stream(zq3d_ens* e, void** stream) {
     return ZQ3D_OK;
 }
 
+extern "C" int zq3d_ens_set_sweep(zq3d_ens* e, int kind, int tbx, int tby, int blocks) {
+    const char* where = "zq3d_ens_set_sweep";
+    if (!e) { set_err(where, "null ensemble"); return ZQ3D_ERR_ARG; }
+    EnsLaunch L = e->L;
+    bool ok = false;
+    if (kind == 0) {
+        for (int sh = 0; sh < kEnsSweepShapeCount; sh++)
+            if (kEnsSweepShapes[sh][0] == tbx && kEnsSweepShapes[sh][1] == tby && blocks >= 0) {
+                L.sweep_rb = 0;
+                L.sweep_shape = sh;
+                L.sweep_blocks = blocks;
+                ok = true;
+            }
+    } else if (kind == 1) {
+        for (int r = 0; r < kEnsRbShapeCount; r++)
+            if (kEnsRbShapes[r][0] == tbx && kEnsRbShapes[r][1] == tby && kEnsRbShapes[r][2] == blocks) {
+                L.sweep_rb = r + 1;
+                ok = true;
+            }
+    }
+    if (!ok) { set_err(where, "not a sweep candidate (see zq3d_ens_autotune)"); return ZQ3D_ERR_ARG; }
+    e->L = L;
+    e->graph_stale = true;
+    return ZQ3D_OK;
+}
+
 // Times the sweep kernel for every tile shape and persistent-launch size, and the cell kernels for
 // every launch size, on the ensemble's own grids; keeps the fastest of each.  The fields are left
 // as they were (only projection and advection scratch is written).
 extern "C" int zq3d_ens_autotune(zq3d_ens* e, int iters, zq3d_ens_tune* rep) {
     const char* where = "zq3d_ens_autotune";
     if (!e || iters < 1) { set_err(where, "bad argument"); return ZQ3D_ERR_ARG; }
-    static const int blocks[] = {0, 256, 512, 1024, 2048, 4096};
+    static const int blocks[] = {0, 256, 1024, 4096};
     const int nb = (int)(sizeof(blocks) / sizeof(blocks[0]));
     hipEvent_t a = nullptr, b = nullptr;
     ZQ_HIP(hipEventCreate(&a), where);
@@ -387,32 +439,44 @@ extern "C" int zq3d_ens_autotune(zq3d_ens* e, int iters, zq3d_ens_tune* rep) {
         E(hipEventElapsedTime(&ms, a, b));
         return ms;
     };
-    // the sweep's cost depends on the counts of the live occupancy
+    // the sweeps run on the live counts and right-hand sides (interleaved and colour-split)
     ens_launch_setup(e->L, e->stream);
-    int best_sh = 0, best_sb = 0;
+    ens_launch_split_in(e->L, e->stream);
+    int best_sh = 0, best_sb = 0, best_rb = 0;
     float best = -1.0f;
     int n = 0;
+    auto consider = [&](int tbx, int tby, int blk, float us, int sh, int sb, int rb) {
+        if (rep && n < ZQ3D_ENS_TUNE_MAX) {
+            rep->kind[n] = rb ? 1 : 0;
+            rep->tbx[n] = tbx;
+            rep->tby[n] = tby;
+            rep->blocks[n] = blk;
+            rep->us_per_iter[n] = us;
+            n++;
+        }
+        if (best < 0.0f || us < best) {
+            best = us;
+            best_sh = sh;
+            best_sb = sb;
+            best_rb = rb;
+        }
+    };
+    L.sweep_rb = 0;
     for (int sh = 0; sh < kEnsSweepShapeCount; sh++)
         for (int bi = 0; bi < nb; bi++) {
             L.sweep_shape = sh;
             L.sweep_blocks = blocks[bi];
             const float ms = time_it([&] { ens_launch_sweeps(L, 0, iters, e->stream); });
-            const float us = 1e3f * ms / iters;
-            if (rep && n < ZQ3D_ENS_TUNE_MAX) {
-                rep->tbx[n] = kEnsSweepShapes[sh][0];
-                rep->tby[n] = kEnsSweepShapes[sh][1];
-                rep->blocks[n] = blocks[bi];
-                rep->us_per_iter[n] = us;
-                n++;
-            }
-            if (best < 0.0f || us < best) {
-                best = us;
-                best_sh = sh;
-                best_sb = blocks[bi];
-            }
+            consider(kEnsSweepShapes[sh][0], kEnsSweepShapes[sh][1], blocks[bi], 1e3f * ms / iters, sh, blocks[bi], 0);
         }
+    for (int r = 0; r < kEnsRbShapeCount; r++) {
+        L.sweep_rb = r + 1;
+        const float ms = time_it([&] { ens_launch_sweeps(L, 0, iters, e->stream); });
+        consider(kEnsRbShapes[r][0], kEnsRbShapes[r][1], kEnsRbShapes[r][2], 1e3f * ms / iters, 0, 0, r + 1);
+    }
     L.sweep_shape = best_sh;
     L.sweep_blocks = best_sb;
+    L.sweep_rb = best_rb;
     int best_b = 0;
     float best_cell = -1.0f;
     for (int bi = 0; bi < nb; bi++) {
@@ -431,9 +495,10 @@ extern "C" int zq3d_ens_autotune(zq3d_ens* e, int iters, zq3d_ens_tune* rep) {
     e->graph_stale = true;
     if (rep) {
         rep->count = n;
-        rep->best_tbx = kEnsSweepShapes[best_sh][0];
-        rep->best_tby = kEnsSweepShapes[best_sh][1];
-        rep->best_sweep_blocks = best_sb;
+        rep->best_kind = best_rb ? 1 : 0;
+        rep->best_tbx = best_rb ? kEnsRbShapes[best_rb - 1][0] : kEnsSweepShapes[best_sh][0];
+        rep->best_tby = best_rb ? kEnsRbShapes[best_rb - 1][1] : kEnsSweepShapes[best_sh][1];
+        rep->best_sweep_blocks = best_rb ? kEnsRbShapes[best_rb - 1][2] : best_sb;
         rep->best_us_per_iter = best;
         rep->best_blocks = best_b;
         rep->probe_ms = best_cell;

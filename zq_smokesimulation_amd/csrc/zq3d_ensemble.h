@@ -21,6 +21,11 @@ struct EnsMember {
     float4* vel4;    // cell velocity; also the vorticity scratch of the force stage
     float* scratch;  // 3N: cell force, then advected cell velocity
     float *b, *p;    // divergence, padded pressure
+    // colour-split SOR storage: counts and right-hand side per colour, padded pressure per
+    // (pair, colour) at sp[2 * pair + colour]; the one-launch iteration ping-pongs between pairs
+    uint8_t* sn[2];
+    float* sb[2];
+    float* sp[4];
     double* part;    // (sum, count) per set-up tile
     float* mean32;
     float h, dt, dth, two_h, ch, dtt;
@@ -44,6 +49,9 @@ struct EnsTable {
 
 constexpr int kEnsSweepShapeCount = 3;
 extern const int kEnsSweepShapes[kEnsSweepShapeCount][2];  // TBX x TBY of the sweep kernel
+// slots x rows x planes per workgroup of the one-launch red-black iteration on colour-split storage
+constexpr int kEnsRbShapeCount = 7;
+extern const int kEnsRbShapes[kEnsRbShapeCount][3];
 
 struct EnsLaunch {
     const EnsMember* members;  // device array
@@ -51,6 +59,8 @@ struct EnsLaunch {
     EnsTable cells, ufaces, vfaces, wfaces;
     EnsTable half[kEnsSweepShapeCount];  // half-row grids of the sweep, one per tile shape
     int sweep_shape;                     // index into kEnsSweepShapes
+    EnsTable rb[kEnsRbShapeCount];       // tile tables of the one-launch iteration, one per shape
+    int sweep_rb;                        // 0: half sweeps of sweep_shape; r > 0: kEnsRbShapes[r - 1]
     int blocks;                          // workgroups of the persistent launches (<= 0: one per tile)
     int sweep_blocks;
     int max_iters;
@@ -65,7 +75,9 @@ void ens_launch_setup(const EnsLaunch& L, hipStream_t s);
 // mac -> cell velocity and cell advection of every member: writes only scratch, so it can be
 // repeated to time launch shapes
 void ens_launch_probe(const EnsLaunch& L, hipStream_t s);
-// red-black iterations [it0, it1) of every member's SOR
+// red-black iterations [it0, it1) of every member's SOR (on colour-split storage when sweep_rb)
 void ens_launch_sweeps(const EnsLaunch& L, int it0, int it1, hipStream_t s);
+// interleaved counts, right-hand side and pressure -> colour-split storage (pair 0)
+void ens_launch_split_in(const EnsLaunch& L, hipStream_t s);
 
 }  // namespace zq3d

@@ -1,0 +1,240 @@
+// zq3d_sor_split.h -- colour-split storage of the SOR and the one-pass red-black iteration of one
+// tile, shared by the single-grid kernel (kernels_project.hip) and the ensemble kernel
+// (kernels_ensemble.hip).
+#pragma once
+
+#include "zq3d_device.h"
+#include "zq3d_stencils.h"
+
+namespace zq3d {
+
+// ---- colour-split layout ----
+// Colour c = (x+y+z)&1 cells of row (y,z) sit at x = 2*xh + par with par = (c+y+z)&1, stored at
+// xh (Wh = (W+1)/2 slots per row; a slot past the row end is not a cell and holds 0).  Every
+// neighbour of a colour-c cell has the other colour: x-1 and x+1 are slots xh-1+par and xh+par of
+// the same row, y+-1 and z+-1 are slot xh of those rows.  Pressure arrays are padded by one slot
+// (zero) on every side: slot xh of a row sits at SPLIT_XOFF + xh of split_row(Wh) floats.  (Rows
+// padded to a multiple of 32 floats with the interior on a 128-byte line measured slower at 256^3:
+// 51 against 46 us per one-launch iteration.)
+
+constexpr int SPLIT_XOFF = 1;
+
+__host__ __device__ inline int split_row(int Wh) { return Wh + 2; }
+
+__device__ __forceinline__ int split_q(int Wh, int H, int xh, int y, int z) {
+    return ((z + 1) * (H + 2) + (y + 1)) * split_row(Wh) + SPLIT_XOFF + xh;
+}
+
+// One full red-black iteration of one tile on split storage: src pair (s0 red, s1 black) -> dst
+// pair (d0, d1), read-only sources so workgroups never see each other's updates.  The calling
+// workgroup (NW waves, every thread calling) owns a tile of TX slots x TY rows and walks
+// planes z0..z1-1.  At step t it stages the old black plane t+2 (tile + 2-cell halo) into an LDS
+// ring, updates red at plane t on the tile + 1-cell halo (the halo red values are recomputed, not
+// exchanged) into a second ring, then updates black at plane t-1 from the new red planes t-2..t.
+// The global loads of step t+1 are issued at the top of step t into registers, so they are in
+// flight while step t computes.  Per cell the arithmetic and the order of the six-term sum are
+// those of a single-colour sweep (k_sor_split): the result is bit-identical to two such sweeps.
+// m is the mean subtracted from the right-hand side.  A caller that runs several tiles in one
+// workgroup must synchronise the workgroup between them (the LDS rings are reused).
+// (TX is the widest tile; a row's last tile may be narrower.)
+constexpr int SPLIT_TX = 64;  // tile width of the single-grid kernel
+
+struct SplitCellIn {
+    float p, b;
+    int n;  // -1: not a cell (outside the grid or past the row's end)
+};
+
+// Per-thread items of one plane: every index except the plane's own stride is fixed for the whole
+// walk, so it is computed once; a colour's cells sit at even or odd x depending on the plane's
+// parity, hence two validity bits (for the last slot of a row of odd width).
+struct SplitItem {
+    int off;   // (ys+1)*split_row(Wh) + SPLIT_XOFF + xs: padded split index without the plane term
+    int oh;    // ys*Wh + xs: unpadded split index without the plane term
+    int lds;   // LDS index of the item within a ring plane (red and black items: in the red ring)
+    int kix;   // red: K-ring index of (xr, yr+1); black: K-ring index of the cell itself
+    int bits;  // 1: a cell when the colour's parity is 0, 2: when it is 1, 4: tile-owned, 8: ys odd
+};
+
+template <int TX, int TY, int NW>
+__device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1,
+                                                  const float* __restrict__ b0, const float* __restrict__ b1, float m,
+                                                  const float* __restrict__ s0, const float* __restrict__ s1,
+                                                  float* __restrict__ d0, float* __restrict__ d1, const Dims& d,
+                                                  int Wh, int x0, int y0, int z0, int z1, float omega, float omw) {
+    constexpr int KX = TX + 4, KY = TY + 4, RX = TX + 2, RY = TY + 2;  // ring plane row strides, rows
+    constexpr int KPL = KY * KX, RPL = RY * RX;                        // floats per ring plane
+    constexpr int NT = 64 * NW;
+    constexpr int NK = (KY * KX + NT - 1) / NT;  // staged items per thread
+    constexpr int NR = (RY * RX + NT - 1) / NT;  // red items per thread
+    constexpr int NB = (TY * TX + NT - 1) / NT;  // black items per thread
+    // old black planes t-1 .. t+2 (slot x0-2+c, row y0-2+r) and new red planes t-2 .. t (slot
+    // x0-1+c, row y0-1+r); 1/n for n = 1..6
+    __shared__ float Ks[5 * KPL];
+    __shared__ float Rs[4 * RPL];
+    __shared__ float rtab[8];
+    // the tile's own width (the last tile of a row may be narrower); items are numbered row by row
+    // over the tile's region so a wave reads contiguous runs of slots
+    const int tw = min(TX, Wh - x0), kw = tw + 4, rw = tw + 2;
+    const int SP = split_row(Wh), SZ = SP * (d.H + 2), HW = d.H * Wh;
+
+    // ---- items, computed once ----
+    // staged old black: KY rows x (tw + 4) columns, slot x0-2+c, row y0-2+r
+    SplitItem ki[NK];
+#pragma unroll
+    for (int i = 0; i < NK; i++) {
+        const int f = threadIdx.x + NT * i, r = f / kw, c = f % kw;
+        const int xs = x0 - 2 + c, ys = y0 - 2 + r;
+        const bool mine = r < KY;
+        const bool ok = mine && xs >= -1 && xs <= Wh && ys >= -1 && ys <= d.H;
+        ki[i] = SplitItem{(ys + 1) * SP + SPLIT_XOFF + xs, 0, mine ? r * KX + c : -1, 0, ok ? 3 : 0};
+    }
+    // red: RY rows x (tw + 2) columns, slot x0-1+xr, row y0-1+yr
+    SplitItem ri[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int f = threadIdx.x + NT * i, yr = f / rw, xr = f % rw;
+        const int xs = x0 - 1 + xr, ys = y0 - 1 + yr;
+        const bool mine = yr < RY;
+        const bool in = mine && xs >= 0 && xs < Wh && ys >= 0 && ys < d.H;
+        int bits = 0;
+        if (in && 2 * xs < d.W) bits |= 1;
+        if (in && 2 * xs + 1 < d.W) bits |= 2;
+        if (in && xr >= 1 && xr <= tw && yr >= 1 && yr <= TY) bits |= 4;
+        if (ys & 1) bits |= 8;
+        ri[i] = SplitItem{(ys + 1) * SP + SPLIT_XOFF + xs, ys * Wh + xs, mine ? yr * RX + xr : -1,
+                          mine ? (yr + 1) * KX + xr : 0, bits};
+    }
+    // black: TY rows x tw columns, slot x0+xb, row y0+yb
+    SplitItem bi[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int f = threadIdx.x + NT * i, yb = f / tw, xb = f % tw;
+        const int xs = x0 + xb, ys = y0 + yb;
+        const bool mine = yb < TY;
+        const bool in = mine && ys < d.H;
+        int bits = 0;
+        if (in && 2 * xs < d.W) bits |= 1;
+        if (in && 2 * xs + 1 < d.W) bits |= 2;
+        if (ys & 1) bits |= 8;
+        bi[i] = SplitItem{(ys + 1) * SP + SPLIT_XOFF + xs, ys * Wh + xs, mine ? yb * RX + xb : -1,
+                          mine ? (yb + 2) * KX + xb + 2 : 0, bits};
+    }
+    if (threadIdx.x < 8) rtab[threadIdx.x] = threadIdx.x >= 1 && threadIdx.x <= 6 ? rcp_count(threadIdx.x) : 0.0f;
+
+    // ---- per-plane work; loads are branch-free (a lane with nothing to fetch reads index 0, a
+    // padding slot holding 0) so no path leaves a register half-written by a load ----
+    auto load_k = [&](int z, float (&o)[NK]) {
+        const bool zin = z >= -1 && z <= d.D;
+        const int zo = (z + 1) * SZ;
+#pragma unroll
+        for (int i = 0; i < NK; i++) o[i] = s1[(zin && ki[i].bits) ? zo + ki[i].off : 0];
+    };
+    auto store_k = [&](int z, const float (&o)[NK]) {
+        float* K = Ks + ((z + 10) % 5) * KPL;
+#pragma unroll
+        for (int i = 0; i < NK; i++)
+            if (ki[i].lds >= 0) K[ki[i].lds] = o[i];
+    };
+    // a colour-c cell of row ys, plane t has parity (c + ys + t) & 1
+    auto cell = [](const SplitItem& it, int par) { return (it.bits >> par) & 1; };
+    auto load_red = [&](int t, SplitCellIn (&o)[NR]) {
+        const bool tin = t >= 0 && t < d.D;
+        const int zo = (t + 1) * SZ, zh = t * HW;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const bool ok = tin && cell(ri[i], ((ri[i].bits >> 3) ^ t) & 1);
+            const int oh = ok ? zh + ri[i].oh : 0;
+            const int q = ok ? zo + ri[i].off : 0;
+            const int n = n0[oh];
+            const float bv = b0[oh];
+            const float pv = s0[q];
+            o[i] = SplitCellIn{pv, bv, ok ? n : -1};
+        }
+    };
+    auto load_black = [&](int t, SplitCellIn (&o)[NB]) {
+        const bool tin = t >= z0 && t < z1;
+        const int zh = t * HW;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const bool ok = tin && cell(bi[i], ((bi[i].bits >> 3) ^ t ^ 1) & 1);
+            const int oh = ok ? zh + bi[i].oh : 0;
+            const int n = n1[oh];
+            const float bv = b1[oh];
+            o[i] = SplitCellIn{0.0f, bv, ok ? n : -1};
+        }
+    };
+    // The updates are evaluated for every item a lane holds and selected afterwards (n > 0:
+    // updated, n == 0: kept, n < 0: not a cell); the LDS reads stay inside the ring planes.
+    auto red = [&](const SplitItem& it, int t, const SplitCellIn& c, bool own_plane) {
+        const int par = ((it.bits >> 3) ^ t) & 1;
+        const float* K0 = Ks + ((t + 10) % 5) * KPL;
+        const float* Km = Ks + ((t + 9) % 5) * KPL;
+        const float* Kp = Ks + ((t + 11) % 5) * KPL;
+        const int kc = it.kix;  // K[yr+1][xr]
+        const float sum = K0[kc + par] + K0[kc + 1 + par] + K0[kc + 1 - KX] + K0[kc + 1 + KX] + Km[kc + 1] + Kp[kc + 1];
+        const float bv = c.b - m;
+        const float pg = (sum - bv) * rtab[c.n & 7];
+        const float upd = omw * c.p + omega * pg;
+        const float pc = c.n > 0 ? upd : c.p;
+        if (c.n >= 0 && own_plane && (it.bits & 4)) d0[(t + 1) * SZ + it.off] = pc;
+        Rs[((t + 8) & 3) * RPL + it.lds] = c.n >= 0 ? pc : 0.0f;
+    };
+    auto black = [&](const SplitItem& it, int t, const SplitCellIn& c) {
+        const int par = ((it.bits >> 3) ^ t ^ 1) & 1;
+        const float p0 = Ks[((t + 10) % 5) * KPL + it.kix];
+        const float* R0 = Rs + ((t + 8) & 3) * RPL;
+        const float* Rm = Rs + ((t + 7) & 3) * RPL;
+        const float* Rp = Rs + ((t + 9) & 3) * RPL;
+        const int rc = it.lds + RX;  // R[yb+1][xb]
+        const float sum = R0[rc + par] + R0[rc + 1 + par] + R0[rc + 1 - RX] + R0[rc + 1 + RX] + Rm[rc + 1] + Rp[rc + 1];
+        const float bv = c.b - m;
+        const float pg = (sum - bv) * rtab[c.n & 7];
+        const float upd = omw * p0 + omega * pg;
+        if (c.n >= 0) d1[(t + 1) * SZ + it.off] = c.n > 0 ? upd : p0;
+    };
+
+    float kc[NK];
+    SplitCellIn rc[NR], bc[NB];
+    load_k(z0 - 2, kc);
+    store_k(z0 - 2, kc);
+    load_k(z0 - 1, kc);
+    store_k(z0 - 1, kc);
+    load_k(z0, kc);
+    store_k(z0, kc);
+    load_k(z0 + 1, kc);
+    load_red(z0 - 1, rc);
+    load_black(z0 - 2, bc);  // not a chunk plane: no cells
+    // Land the prologue's loads here: with them still pending at the loop entry, the wait inserted
+    // before their first use in the loop would also drain the next step's prefetches every step.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int t = z0 - 1; t <= z1; t++) {
+        // loads for step t+1 (at t == z1 every lane's guards skip them); unconditional, like the
+        // hand-over below, so no path reaches the loop top with a prefetch register still pending
+        float kn[NK];
+        SplitCellIn rn[NR], bn[NB];
+        load_k(t + 3, kn);
+        load_red(t + 1, rn);
+        load_black(t, bn);
+        store_k(t + 2, kc);
+        const bool own = t >= z0 && t < z1;
+#pragma unroll
+        for (int i = 0; i < NR; i++)
+            if (ri[i].lds >= 0) red(ri[i], t, rc[i], own);
+        __syncthreads();
+        if (t - 1 >= z0) {
+#pragma unroll
+            for (int i = 0; i < NB; i++)
+                if (bi[i].lds >= 0) black(bi[i], t - 1, bc[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < NK; i++) kc[i] = kn[i];
+#pragma unroll
+        for (int i = 0; i < NR; i++) rc[i] = rn[i];
+#pragma unroll
+        for (int i = 0; i < NB; i++) bc[i] = bn[i];
+    }
+}
+
+
+}  // namespace zq3d

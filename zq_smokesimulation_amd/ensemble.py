@@ -101,3 +101,8 @@ class SmokeEnsemble:
         rep = _lib.EnsTune()
         check(_lib.lib().zq3d_ens_autotune(self._h, int(iters), C.byref(rep)), "zq3d_ens_autotune")
         return rep.as_dict()
+
+    def set_sweep(self, kind, tbx, tby, blocks):
+        """Selects an autotuner candidate: kind "half_sweeps" (0) or "split_one_launch" (1)."""
+        k = {"half_sweeps": 0, "split_one_launch": 1}.get(kind, kind)
+        check(_lib.lib().zq3d_ens_set_sweep(self._h, int(k), int(tbx), int(tby), int(blocks)), "zq3d_ens_set_sweep")
