@@ -278,16 +278,20 @@ __device__ __forceinline__ void tile_flush(TileWin<R, NA, CAP>& w, const int* sb
     }
 }
 
-// Carry the end point's adjoint back along the path into the velocity adjoint (window arrays 0..2).
-// Consecutive path points usually share a trilinear cell: the 8 taps are loaded once per cell
-// (Cell4Taps) and the path's weighted contributions to them are summed in registers, reaching the
-// window only when the path moves to another cell.
+// Carry the end point's adjoint back along the path into the velocity adjoint (window arrays 0..2),
+// one path point at a time, last to first.  Consecutive path points usually share a trilinear
+// cell: the 8 taps are loaded once per cell (Cell4Taps) and the path's weighted contributions to
+// them are summed in registers, reaching the window only when the path moves to another cell.
 template <class R, int NA, int CAP>
-__device__ __forceinline__ void reverse_path_tile(const vec4_t<R>* __restrict__ vel4, const Dims& d,
-                                                  TileWin<R, NA, CAP>& w, const R3<R>* q, int n, R c, R3<R> g) {
+struct RevTile {
+    const vec4_t<R>* vel4;
+    Dims d;
+    TileWin<R, NA, CAP>* w;
+    R c;
+    R3<R> g;
     Cell4Taps<R> cc;
     R acc[8][3];
-    auto flush = [&]() {
+    __device__ __forceinline__ void flush() {
         if (cc.i0 < 0) return;
         const int ti[8] = {cc.i0, cc.i1, cc.i0, cc.i1, cc.i0, cc.i1, cc.i0, cc.i1};
         const int tj[8] = {cc.j0, cc.j0, cc.j1, cc.j1, cc.j0, cc.j0, cc.j1, cc.j1};
@@ -296,10 +300,9 @@ __device__ __forceinline__ void reverse_path_tile(const vec4_t<R>* __restrict__ 
         for (int t = 0; t < 8; t++)
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                if (acc[t][a] != R(0)) w.add(a, ti[t], tj[t], tk[t], acc[t][a]);
-    };
-    for (int m = n - 2; m >= 0; m--) {
-        const R3<R> p = q[m];
+                if (acc[t][a] != R(0)) w->add(a, ti[t], tj[t], tk[t], acc[t][a]);
+    }
+    __device__ __forceinline__ void step(const R3<R>& p) {
         const Tri<R> t = cell_tri<R>(d, p);
         if (!cc.same(t.i0, t.i1, t.j0, t.j1, t.k0, t.k1)) {
             flush();
@@ -324,7 +327,36 @@ __device__ __forceinline__ void reverse_path_tile(const vec4_t<R>* __restrict__ 
                                       dot(cc.t101), dot(cc.t011), dot(cc.t111), t);
         g = R3<R>{g.x - c * jt.x, g.y - c * jt.y, g.z - c * jt.z};
     }
-    flush();
+};
+
+// trace_path that also grows box by the cell box of every accepted point, and sets *last to the
+// last accepted point (the start when none is accepted, whose box then goes in instead): one pass
+// over the path instead of a second one over the stored points.
+template <class R, class Vel>
+__device__ __forceinline__ int trace_box(const Vel& V, const uint8_t* __restrict__ occ, const Dims& d, R h,
+                                         int steps, R dtt, R px, R py, R pz, R3<R>* q, R3<R>* last, int* box) {
+    R3<R> vel = V(px, py, pz);
+    int n = 0;
+    int istep = 0;
+    *last = R3<R>{px, py, pz};
+    do {
+        if (!trace_open(occ, d, px, py, pz)) break;
+        const R3<R> p{px, py, pz};
+        q[n++] = p;
+        const Tri<R> t = cell_tri<R>(d, p);
+        box_grow(box, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+        *last = p;
+        px = px - (vel.x * dtt) / h;
+        py = py - (vel.y * dtt) / h;
+        pz = pz - (vel.z * dtt) / h;
+        vel = V(px, py, pz);
+        istep++;
+    } while (istep < steps);
+    if (n == 0) {
+        const Tri<R> t = cell_tri<R>(d, *last);
+        box_grow(box, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
+    }
+    return n;
 }
 
 // inRegular_outRegular velocity advection adjoint, tiled (see k_adj_advect_cells).
@@ -354,16 +386,8 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_cells_tile(RegularVel
         act = !(ab.x == R(0) && ab.y == R(0) && ab.z == R(0));
     }
     if (act) {
-        const R sx = i + R(0.5), sy = j + R(0.5), sz = k + R(0.5);
         const RegularVelCached<R> Vc{V.vel4, d, {}};
-        n = trace_path<R>(Vc, occ, d, h, steps, dtt, sx, sy, sz, q);
-        last = n ? q[n - 1] : R3<R>{sx, sy, sz};
-        Tri<R> t = cell_tri<R>(d, last);
-        box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
-        for (int m = 0; m < n - 1; m++) {
-            t = cell_tri<R>(d, q[m]);
-            box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
-        }
+        n = trace_box<R>(Vc, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5), q, &last, mine);
     }
     if (!tile_setup<R, 3, CAP>(w, mine, sbox, tid)) return;
     if (act) {
@@ -373,7 +397,9 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_cells_tile(RegularVel
         w.scatter(2, t, ab.z);
         if (n >= 2) {
             const RegularVelAdj<R> A{V.vel4, nullptr, d};
-            reverse_path_tile<R, 3, CAP>(V.vel4, d, w, q, n, dtt / h, A.vjp(last.x, last.y, last.z, ab));
+            RevTile<R, 3, CAP> rev{V.vel4, d, &w, dtt / h, A.vjp(last.x, last.y, last.z, ab), {}, {}};
+            for (int m = n - 2; m >= 0; m--) rev.step(q[m]);
+            rev.flush();
         }
     }
     tile_flush<R, 3, CAP>(w, sbox, tid);
@@ -412,16 +438,8 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_scalar_tile(RegularVe
         act = !(tb == R(0) && rb == R(0));
     }
     if (act) {
-        const R sx = i + R(0.5), sy = j + R(0.5), sz = k + R(0.5);
         const RegularVelCached<R> Vc{V.vel4, d, {}};
-        n = trace_path<R>(Vc, occ, d, h, steps, dtt, sx, sy, sz, q);
-        last = n ? q[n - 1] : R3<R>{sx, sy, sz};
-        Tri<R> t = cell_tri<R>(d, last);
-        box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
-        for (int m = 0; m < n - 1; m++) {
-            t = cell_tri<R>(d, q[m]);
-            box_grow(mine, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1);
-        }
+        n = trace_box<R>(Vc, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5), q, &last, mine);
     }
     if (!tile_setup<R, 5, CAP>(w, mine, sbox, tid)) return;
     if (act) {
@@ -431,7 +449,9 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_scalar_tile(RegularVe
         if (n >= 2) {
             const R3<R> gt = grad3<R>(Tin, d.W, d.H, t), gr = grad3<R>(Rin, d.W, d.H, t);
             const R3<R> g{tb * gt.x + rb * gr.x, tb * gt.y + rb * gr.y, tb * gt.z + rb * gr.z};
-            reverse_path_tile<R, 5, CAP>(V.vel4, d, w, q, n, dtt / h, g);
+            RevTile<R, 5, CAP> rev{V.vel4, d, &w, dtt / h, g, {}, {}};
+            for (int m = n - 2; m >= 0; m--) rev.step(q[m]);
+            rev.flush();
         }
     }
     tile_flush<R, 5, CAP>(w, sbox, tid);
