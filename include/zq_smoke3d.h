@@ -239,9 +239,11 @@ int zq3d_get_timing(zq3d_sim* sim, zq3d_timing* out);
  * the grid is timed for `iters` iterations with HIP events on the simulation's stream and
  * occupancy; the fastest is kept for this simulation.  Candidates (ty, zc): ty > 0 is the one-pass
  * kernel on interleaved storage (ty rows per workgroup, zc planes per chunk); ty == -1 is two
- * single-colour sweeps per iteration on colour-split storage; ty == -r (r = 4, 8, 16) is one launch
- * per iteration on colour-split storage (r rows by zc planes per workgroup).  All produce the same
- * bits.  Clobbers the projection scratch (not the fields). */
+ * single-colour sweeps per iteration on colour-split storage; ty == -r (r = 4, 8, 16, 32) is one
+ * launch per iteration on colour-split storage (r rows by zc = 8, 16, 32 planes per workgroup);
+ * ty == -(100 + r) (r = 16, 32) is two iterations per launch on colour-split storage.  All produce
+ * the same bits.
+ * Clobbers the projection scratch (not the fields). */
 #define ZQ3D_TUNE_MAX 32
 typedef struct zq3d_tune_report {
     int count;                       /* candidates timed */

@@ -381,6 +381,25 @@ __global__ __launch_bounds__(64 * NW) void k_sor_split_rb(const uint8_t* __restr
                               min(z0 + zc, d.D), omega, omw);
 }
 
+// Two red+black iterations per launch on split storage (sor_split_rb2_tile); grid as k_sor_split_rb.
+template <int TY, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sor_split_rb2(const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1,
+                                                           const float* __restrict__ b0, const float* __restrict__ b1,
+                                                           const float* __restrict__ mean32,
+                                                           const float* __restrict__ s0, const float* __restrict__ s1,
+                                                           float* __restrict__ d0, float* __restrict__ d1, Dims d,
+                                                           int Wh, int gx, int gy, int zc, float omega, float omw) {
+    const int per = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int bx = tile % gx;
+    const int rest = tile / gx;
+    const int by = rest % gy;
+    const int z0 = (rest / gy) * zc;
+    if (z0 >= d.D) return;  // spare block (the grid is a multiple of 8): uniform exit before any barrier
+    sor_split_rb2_tile<SPLIT_TX, TY, NW>(n0, n1, b0, b1, *mean32, s0, s1, d0, d1, d, Wh, bx * SPLIT_TX, by * TY, z0,
+                                         min(z0 + zc, d.D), omega, omw);
+}
+
 // Face correction u -= p_r - p_l (ZQ_SmokeSimulationUtils3D.cpp:2815-2861 open form, :2888-2922
 // closed form).  P = float (plain) or double (mixed precision; the face is rounded once).
 template <int AXIS, class P, class F>
@@ -640,15 +659,22 @@ void launch_mean_finalize(const Dims& d, const double* part, float* mean32, doub
 size_t split_padded_size(const Dims& d) { return (size_t)split_row((d.W + 1) / 2) * (d.H + 2) * (d.D + 2); }
 size_t split_cell_size(const Dims& d) { return (size_t)((d.W + 1) / 2) * d.H * d.D; }
 
-template <int TY, int NW>
+template <int TY, int NW, bool TWO>
 void launch_split_rb(const Dims& d, const SplitBufs& sp, const float* const* src, float* const* dst,
                      const float* mean32, float omega, float omw, int zc, hipStream_t s) {
     const int Wh = (d.W + 1) / 2;
     const int gx = (Wh + SPLIT_TX - 1) / SPLIT_TX, gy = (d.H + TY - 1) / TY, gz = (d.D + zc - 1) / zc;
     const int nb = (gx * gy * gz + 7) & ~7;
-    hipLaunchKernelGGL((k_sor_split_rb<TY, NW>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1], sp.b[0], sp.b[1], mean32,
-                       src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw);
+    if (TWO)
+        hipLaunchKernelGGL((k_sor_split_rb2<TY, NW>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1], sp.b[0],
+                           sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw);
+    else
+        hipLaunchKernelGGL((k_sor_split_rb<TY, NW>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1], sp.b[0],
+                           sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw);
 }
+
+// split shape ty = -r (one iteration per launch) or -(SOR_TWO + r) (two iterations per launch)
+inline int split_rows(SorShape sh) { return sh.ty <= -SOR_TWO ? -sh.ty - SOR_TWO : -sh.ty; }
 
 int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const float* mean32, float omega, int iters,
                         SorShape shape, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
@@ -664,14 +690,22 @@ int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const floa
                 hipLaunchKernelGGL(k_sor_split, dim3(nb), dim3(BX * BY), 0, s, sp.n[c], sp.b[c], mean32, p[c],
                                    p[1 - c], d, Wh, gx, gy, omega, omw, c);
     } else {
-        const int ty = -shape.ty, zc = shape.zc;
-        for (int it = 0; it < iters; it++) {
+        const int ty = split_rows(shape), zc = shape.zc;
+        const bool two = shape.ty <= -SOR_TWO;
+        for (int it = 0; it < iters;) {
             float* const* src = from ? sp.alt : sp.p;
             float* const* dst = from ? sp.p : sp.alt;
-            if (ty == 4) launch_split_rb<4, 4>(d, sp, src, dst, mean32, omega, omw, zc, s);
-            else if (ty == 8) launch_split_rb<8, 8>(d, sp, src, dst, mean32, omega, omw, zc, s);
-            else if (ty == 16) launch_split_rb<16, 8>(d, sp, src, dst, mean32, omega, omw, zc, s);
-            else launch_split_rb<32, 16>(d, sp, src, dst, mean32, omega, omw, zc, s);
+            if (two && it + 2 <= iters) {
+                if (ty == 16) launch_split_rb<16, 8, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else launch_split_rb<32, 16, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                it += 2;
+            } else {
+                if (ty == 4) launch_split_rb<4, 4, false>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else if (ty == 8) launch_split_rb<8, 8, false>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else if (ty == 16) launch_split_rb<16, 8, false>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else launch_split_rb<32, 16, false>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                it += 1;
+            }
             from = 1 - from;
         }
     }
@@ -725,14 +759,16 @@ float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float*
 
 const char* sor_kernel_name(const Dims& d, SorShape shape) {
     if (shape.ty == SOR_SPLIT) return "k_sor_split";
+    if (shape.ty <= -SOR_TWO) return "k_sor_split_rb2";
     if (is_split(shape)) return "k_sor_split_rb";
     return fused_shape(d, shape).nblocks > 0 ? "k_sor_fused" : "k_sor_half";
 }
 
-int sor_launches_per_iter(const Dims& d, SorShape shape) {
-    if (shape.ty == SOR_SPLIT) return 2;
-    if (is_split(shape)) return 1;
-    return fused_shape(d, shape).nblocks > 0 ? 1 : 2;
+long long sor_launches(const Dims& d, SorShape shape, int iters) {
+    if (shape.ty == SOR_SPLIT) return 2LL * iters;
+    if (shape.ty <= -SOR_TWO) return (iters + 1) / 2;
+    if (is_split(shape)) return iters;
+    return (fused_shape(d, shape).nblocks > 0 ? 1LL : 2LL) * iters;
 }
 
 int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
@@ -746,10 +782,15 @@ int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
         }
     if (n < max) out[n++] = SorShape{SOR_SPLIT, 0};
     static const int rb_rows[] = {4, 8, 16, 32};
-    static const int rb_zc[] = {4, 8, 16, 32};
+    static const int rb_zc[] = {8, 16, 32};
     for (int ty : rb_rows)
         for (int zc : rb_zc)
             if (n < max) out[n++] = SorShape{-ty, zc};
+    static const int rb2_rows[] = {16, 32};
+    static const int rb2_zc[] = {8, 16, 32};
+    for (int ty : rb2_rows)
+        for (int zc : rb2_zc)
+            if (n < max) out[n++] = SorShape{-(SOR_TWO + ty), zc};
     return n;
 }
 

@@ -120,7 +120,7 @@ hipError_t account_sweeps(const Dims& d, SorShape shape, int iters, SorProf* pf)
     float ms = 0.0f;
     E(hipEventElapsedTime(&ms, pf->a, pf->b));
     pf->ms += ms;
-    pf->launches += (long long)sor_launches_per_iter(d, shape) * iters;
+    pf->launches += sor_launches(d, shape, iters);
     return E.e;
 }
 }  // namespace

@@ -71,6 +71,7 @@ void launch_mean_finalize(const Dims& d, const double* part, float* mean32, doub
 // colour-split storage; ty == -r (r = 4, 8 or 16) selects one launch per red-black iteration on
 // colour-split storage with r rows by zc planes per workgroup; {0, 0} selects the built-in default.
 constexpr int SOR_SPLIT = -1;
+constexpr int SOR_TWO = 100;  // ty == -(SOR_TWO + r), r = 16 or 32: two iterations per launch on split storage
 struct SorShape {
     int ty, zc;
 };
@@ -101,8 +102,8 @@ int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const floa
 void launch_split_counts(const Dims& d, const uint8_t* nc, const SplitBufs& sp, hipStream_t s);
 // name of the sweep kernel a shape runs
 const char* sor_kernel_name(const Dims& d, SorShape shape);
-// kernel launches per SOR iteration (1 fused, 2 single-colour)
-int sor_launches_per_iter(const Dims& d, SorShape shape = SorShape{0, 0});
+// sweep-kernel launches of `iters` SOR iterations
+long long sor_launches(const Dims& d, SorShape shape, int iters);
 // fused-kernel shapes valid for this grid (LDS ring within 64 KiB); returns the count
 int sor_shape_candidates(const Dims& d, SorShape* out, int max);
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,

@@ -238,3 +238,222 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
 
 
 }  // namespace zq3d
+
+namespace zq3d {
+
+// Two full red-black iterations of one tile in one pass over memory: src pair -> dst pair.  Four
+// stages run at every plane step t, each one plane behind the previous and separated by a
+// workgroup barrier:
+//   red1 at plane t (tile + 3-cell halo) from the old black plane ring K (tile + 4),
+//   black1 at plane t-1 (tile + 2) from the red1 ring R1,
+//   red2 at plane t-2 (tile + 1) from the black1 ring B1 (written to d0 on the tile),
+//   black2 at plane t-3 (tile) from the red2 ring R2 (written to d1).
+// Each stage's own previous value is its colour's value one iteration earlier, two cells further
+// out in that colour's ring (old red comes from s0).  With a barrier between stages, rings of 4
+// (K) and 3 (R1, B1, R2) planes suffice: a plane is overwritten only after every stage that reads
+// it has passed the barrier that precedes the writer.  A stage runs only on the planes a later
+// stage needs (red1 z0-3 .. z1+2, black1 z0-2 .. z1+1, red2 z0-1 .. z1, black2 z0 .. z1-1), so no
+// stage reads a ring plane that was not written in this walk.  Per cell the arithmetic is that of
+// a single-colour sweep: the result is bit-identical to four single-colour sweeps.
+template <int TX, int TY, int NW>
+__device__ __forceinline__ void sor_split_rb2_tile(const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1,
+                                                   const float* __restrict__ b0, const float* __restrict__ b1, float m,
+                                                   const float* __restrict__ s0, const float* __restrict__ s1,
+                                                   float* __restrict__ d0, float* __restrict__ d1, const Dims& d,
+                                                   int Wh, int x0, int y0, int z0, int z1, float omega, float omw) {
+    constexpr int NT = 64 * NW;
+    // ring of halo h: (TY + 2h) rows x (TX + 2h) columns per plane, origin (x0 - h, y0 - h)
+    constexpr int XK = TX + 8, YK = TY + 8, X1 = TX + 6, Y1 = TY + 6, X2 = TX + 4, Y2 = TY + 4, X3 = TX + 2,
+                  Y3 = TY + 2;
+    constexpr int PK = XK * YK, P1 = X1 * Y1, P2 = X2 * Y2, P3 = X3 * Y3;
+    constexpr int NK = (PK + NT - 1) / NT, N1 = (P1 + NT - 1) / NT, N2 = (P2 + NT - 1) / NT,
+                  N3 = (P3 + NT - 1) / NT, N4 = (TX * TY + NT - 1) / NT;
+    __shared__ float Ks[4 * PK];
+    __shared__ float R1s[3 * P1];
+    __shared__ float B1s[3 * P2];
+    __shared__ float R2s[3 * P3];
+    __shared__ float rtab[8];
+    const int SP = split_row(Wh), SZ = SP * (d.H + 2), HW = d.H * Wh;
+    const int tw = min(TX, Wh - x0);
+    if (threadIdx.x < 8) rtab[threadIdx.x] = threadIdx.x >= 1 && threadIdx.x <= 6 ? rcp_count(threadIdx.x) : 0.0f;
+
+    // An item is a (row, column) of a stage's region, numbered row by row over the tile's own
+    // width, packed in one int: column in bits 0-7, row in bits 8-15, flags in bits 16-19 (1 / 2: a
+    // cell when the colour's parity is 0 / 1, 4: tile-owned, 8: ys odd); -1 when the thread has no
+    // such item.
+    static_assert(TX + 8 <= 256 && TY + 8 <= 256, "row and column fit 8 bits");
+    auto make = [&](int i, int h, int rows, int& r, int& c) {
+        const int w = tw + 2 * h;
+        const int f = threadIdx.x + NT * i;
+        r = f / w;
+        c = f % w;
+        return r < rows;
+    };
+    auto cell_item = [&](int i, int h) -> int {
+        int r, c;
+        if (!make(i, h, TY + 2 * h, r, c)) return -1;
+        const int xs = x0 - h + c, ys = y0 - h + r;
+        const bool in = xs >= 0 && xs < Wh && ys >= 0 && ys < d.H;
+        int fl = 0;
+        if (in && 2 * xs < d.W) fl |= 1;
+        if (in && 2 * xs + 1 < d.W) fl |= 2;
+        if (in && r >= h && r < TY + h && c >= h && c < tw + h) fl |= 4;
+        if (ys & 1) fl |= 8;
+        return c | (r << 8) | (fl << 16);
+    };
+    int kI[NK], i1[N1], i2[N2], i3[N3], i4[N4];
+#pragma unroll
+    for (int i = 0; i < NK; i++) {
+        int r, c;
+        const bool mine = make(i, 4, YK, r, c);
+        const int xs = x0 - 4 + c, ys = y0 - 4 + r;
+        const bool ok = mine && xs >= -1 && xs <= Wh && ys >= -1 && ys <= d.H;
+        kI[i] = mine ? c | (r << 8) | ((ok ? 1 : 0) << 16) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < N1; i++) i1[i] = cell_item(i, 3);
+#pragma unroll
+    for (int i = 0; i < N2; i++) i2[i] = cell_item(i, 2);
+#pragma unroll
+    for (int i = 0; i < N3; i++) i3[i] = cell_item(i, 1);
+#pragma unroll
+    for (int i = 0; i < N4; i++) i4[i] = cell_item(i, 0);
+    auto row = [](int it) { return (it >> 8) & 0xff; };
+    auto col = [](int it) { return it & 0xff; };
+    auto flags = [](int it) { return (it >> 16) & 0xf; };
+
+    // ---- loads (branch-free: a lane with nothing to fetch reads index 0, a zero padding slot) ----
+    auto load_k = [&](int z, float (&o)[NK]) {
+        const bool zin = z >= -1 && z <= d.D;
+#pragma unroll
+        for (int i = 0; i < NK; i++) {
+            const int xs = x0 - 4 + col(kI[i]), ys = y0 - 4 + row(kI[i]);
+            o[i] = s1[(zin && kI[i] >= 0 && flags(kI[i])) ? (z + 1) * SZ + (ys + 1) * SP + SPLIT_XOFF + xs : 0];
+        }
+    };
+    auto store_k = [&](int z, const float (&o)[NK]) {
+        float* K = Ks + ((z + 8) & 3) * PK;
+#pragma unroll
+        for (int i = 0; i < NK; i++)
+            if (kI[i] >= 0) K[row(kI[i]) * XK + col(kI[i])] = o[i];
+    };
+    // n, b (and for red1 the old value) of a stage's items at plane t; plane outside [lo, hi]: none
+    auto load_cells = [&](auto& items, auto& o, int h, int colour, int t, int lo, int hi, bool with_p) {
+        const bool tin = t >= lo && t <= hi && t >= 0 && t < d.D;
+        const uint8_t* nn = colour ? n1 : n0;
+        const float* bb = colour ? b1 : b0;
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(o) / sizeof(o[0])); i++) {
+            const int it = items[i];
+            const int xs = x0 - h + col(it), ys = y0 - h + row(it);
+            const int par = ((flags(it) >> 3) ^ t ^ colour) & 1;
+            const bool ok = tin && it >= 0 && ((flags(it) >> par) & 1);
+            const int oh = ok ? t * HW + ys * Wh + xs : 0;
+            const int n = nn[oh];
+            const float bv = bb[oh];
+            const float pv = with_p ? s0[ok ? (t + 1) * SZ + (ys + 1) * SP + SPLIT_XOFF + xs : 0] : 0.0f;
+            o[i] = SplitCellIn{pv, bv, ok ? n : -1};
+        }
+    };
+    // one cell of a stage: in = the other colour's ring (halo h+1) at planes t-1, t, t+1 (slot
+    // bases), own = the cell's previous value; writes the stage's ring (halo h) and, on the tile,
+    // the destination
+    auto update = [&](int it, int h, int colour, int t, const SplitCellIn& c, const float* Im, const float* I0,
+                      const float* Ip, int Xin, float own, float* out, int Xout, float* dst, bool to_dst) {
+        const int r = row(it), cc = col(it);
+        const int par = ((flags(it) >> 3) ^ t ^ colour) & 1;
+        const int base = (r + 1) * Xin + cc + 1;
+        const float sum = I0[base - 1 + par] + I0[base + par] + I0[base - Xin] + I0[base + Xin] + Im[base] + Ip[base];
+        const float bv = c.b - m;
+        const float pg = (sum - bv) * rtab[c.n & 7];
+        const float upd = omw * own + omega * pg;
+        const float pc = c.n > 0 ? upd : own;
+        if (out) out[r * Xout + cc] = c.n >= 0 ? pc : 0.0f;
+        if (to_dst && c.n >= 0 && (flags(it) & 4)) {
+            const int xs = x0 - h + cc, ys = y0 - h + r;
+            dst[(t + 1) * SZ + (ys + 1) * SP + SPLIT_XOFF + xs] = pc;
+        }
+    };
+    auto slot3 = [](int z) { return (z + 9) % 3; };
+    auto kslot = [](int z) { return (z + 8) & 3; };
+
+    // prefetched one step ahead: the first touches of a plane (old black, red1's and black1's cells);
+    // red2 and black2 read counts and right-hand sides red1 and black1 brought into L2 1-2 steps earlier
+    float kc[NK];
+    SplitCellIn c1[N1], c2[N2];
+    load_k(z0 - 4, kc);
+    store_k(z0 - 4, kc);
+    load_k(z0 - 3, kc);
+    store_k(z0 - 3, kc);
+    load_k(z0 - 2, kc);
+    store_k(z0 - 2, kc);
+    load_k(z0 - 1, kc);
+    load_cells(i1, c1, 3, 0, z0 - 3, z0 - 3, z1 + 2, true);
+    load_cells(i2, c2, 2, 1, z0 - 4, z0 - 2, z1 + 1, false);  // not a needed plane: no cells
+    // land the prologue's loads before the loop (see sor_split_rb_tile)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int t = z0 - 3; t <= z1 + 2; t++) {
+        float kn[NK];
+        SplitCellIn e1[N1], e2[N2];
+        load_k(t + 3, kn);
+        load_cells(i1, e1, 3, 0, t + 1, z0 - 3, z1 + 2, true);
+        load_cells(i2, e2, 2, 1, t, z0 - 2, z1 + 1, false);
+        store_k(t + 2, kc);
+        {  // red1 at t
+            const float *Im = Ks + kslot(t - 1) * PK, *I0 = Ks + kslot(t) * PK, *Ip = Ks + kslot(t + 1) * PK;
+            float* out = R1s + slot3(t) * P1;
+#pragma unroll
+            for (int i = 0; i < N1; i++)
+                if (i1[i] >= 0) update(i1[i], 3, 0, t, c1[i], Im, I0, Ip, XK, c1[i].p, out, X1, nullptr, false);
+        }
+        __syncthreads();
+        if (t - 1 >= z0 - 2 && t - 1 <= z1 + 1) {  // black1 at t-1; own: old black, K two cells out
+            const int tb = t - 1;
+            const float *Im = R1s + slot3(tb - 1) * P1, *I0 = R1s + slot3(tb) * P1, *Ip = R1s + slot3(tb + 1) * P1;
+            const float* own = Ks + kslot(tb) * PK;
+            float* out = B1s + slot3(tb) * P2;
+#pragma unroll
+            for (int i = 0; i < N2; i++)
+                if (i2[i] >= 0)
+                    update(i2[i], 2, 1, tb, c2[i], Im, I0, Ip, X1, own[(row(i2[i]) + 2) * XK + col(i2[i]) + 2], out,
+                           X2, nullptr, false);
+        }
+        __syncthreads();
+        if (t - 2 >= z0 - 1 && t - 2 <= z1) {  // red2 at t-2; own: red1, R1 two cells out
+            const int tr = t - 2;
+            const float *Im = B1s + slot3(tr - 1) * P2, *I0 = B1s + slot3(tr) * P2, *Ip = B1s + slot3(tr + 1) * P2;
+            const float* own = R1s + slot3(tr) * P1;
+            float* out = R2s + slot3(tr) * P3;
+            const bool owned_plane = tr >= z0 && tr < z1;
+            SplitCellIn c3[N3];
+            load_cells(i3, c3, 1, 0, tr, z0 - 1, z1, false);
+#pragma unroll
+            for (int i = 0; i < N3; i++)
+                if (i3[i] >= 0)
+                    update(i3[i], 1, 0, tr, c3[i], Im, I0, Ip, X2, own[(row(i3[i]) + 2) * X1 + col(i3[i]) + 2], out,
+                           X3, d0, owned_plane);
+        }
+        __syncthreads();
+        if (t - 3 >= z0 && t - 3 < z1) {  // black2 at t-3; own: black1, B1 two cells out
+            const int tb = t - 3;
+            const float *Im = R2s + slot3(tb - 1) * P3, *I0 = R2s + slot3(tb) * P3, *Ip = R2s + slot3(tb + 1) * P3;
+            const float* own = B1s + slot3(tb) * P2;
+            SplitCellIn c4[N4];
+            load_cells(i4, c4, 0, 1, tb, z0, z1 - 1, false);
+#pragma unroll
+            for (int i = 0; i < N4; i++)
+                if (i4[i] >= 0)
+                    update(i4[i], 0, 1, tb, c4[i], Im, I0, Ip, X3, own[(row(i4[i]) + 2) * X2 + col(i4[i]) + 2],
+                           nullptr, 0, d1, true);
+        }
+#pragma unroll
+        for (int i = 0; i < NK; i++) kc[i] = kn[i];
+#pragma unroll
+        for (int i = 0; i < N1; i++) c1[i] = e1[i];
+#pragma unroll
+        for (int i = 0; i < N2; i++) c2[i] = e2[i];
+    }
+}
+
+}  // namespace zq3d
