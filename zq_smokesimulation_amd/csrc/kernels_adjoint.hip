@@ -157,15 +157,18 @@ struct MacVelAdj {
 template <class R, class Vel>
 __device__ __forceinline__ int trace_path(const Vel& V, const uint8_t* __restrict__ occ, const Dims& d, R h,
                                           int steps, R dtt, R px, R py, R pz, R3<R>* q) {
+    OccCache oc;
+    bool open = trace_open_cached(occ, d, px, py, pz, oc);
     R3<R> vel = V(px, py, pz);
     int n = 0;
     int istep = 0;
     do {
-        if (!trace_open(occ, d, px, py, pz)) break;
+        if (!open) break;
         q[n++] = R3<R>{px, py, pz};
         px = px - (vel.x * dtt) / h;
         py = py - (vel.y * dtt) / h;
         pz = pz - (vel.z * dtt) / h;
+        open = trace_open_cached(occ, d, px, py, pz, oc);
         vel = V(px, py, pz);
         istep++;
     } while (istep < steps);
@@ -335,12 +338,14 @@ struct RevTile {
 template <class R, class Vel>
 __device__ __forceinline__ int trace_box(const Vel& V, const uint8_t* __restrict__ occ, const Dims& d, R h,
                                          int steps, R dtt, R px, R py, R pz, R3<R>* q, R3<R>* last, int* box) {
+    OccCache oc;
+    bool open = trace_open_cached(occ, d, px, py, pz, oc);
     R3<R> vel = V(px, py, pz);
     int n = 0;
     int istep = 0;
     *last = R3<R>{px, py, pz};
     do {
-        if (!trace_open(occ, d, px, py, pz)) break;
+        if (!open) break;
         const R3<R> p{px, py, pz};
         q[n++] = p;
         const Tri<R> t = cell_tri<R>(d, p);
@@ -349,6 +354,7 @@ __device__ __forceinline__ int trace_box(const Vel& V, const uint8_t* __restrict
         px = px - (vel.x * dtt) / h;
         py = py - (vel.y * dtt) / h;
         pz = pz - (vel.z * dtt) / h;
+        open = trace_open_cached(occ, d, px, py, pz, oc);
         vel = V(px, py, pz);
         istep++;
     } while (istep < steps);

@@ -209,32 +209,50 @@ struct MacVel {
     }
 };
 
+
 // True while the backtrace may take another step from (px,py,pz): inside [0,W]x[0,H]x[0,D] and
-// not in a solid cell (point sample, clamp).
+// not in a solid cell (point sample, clamp).  The occupancy byte of the cell last tested is kept:
+// consecutive backtrace points usually share a cell, so most tests load nothing.
+struct OccCache {
+    int o = -1;
+    bool open = false;
+};
+
 template <class R>
-__device__ __forceinline__ bool trace_open(const uint8_t* __restrict__ occ, const Dims& d, R px, R py, R pz) {
+__device__ __forceinline__ bool trace_open_cached(const uint8_t* __restrict__ occ, const Dims& d, R px, R py, R pz,
+                                                  OccCache& c) {
     if (!(px >= R(0) && px <= (R)d.W && py >= R(0) && py <= (R)d.H && pz >= R(0) && pz <= (R)d.D)) return false;
     const int oi = clampi((int)zfloor(px), 0, d.W - 1);
     const int oj = clampi((int)zfloor(py), 0, d.H - 1);
     const int ok = clampi((int)zfloor(pz), 0, d.D - 1);
-    return !occ[cidx(d, oi, oj, ok)];
+    const int o = cidx(d, oi, oj, ok);
+    if (o != c.o) {
+        c.o = o;
+        c.open = !occ[o];
+    }
+    return c.open;
 }
 
 // Semi-Lagrangian backtrace with the reference's loop (ZQ_CUDA_Advection3D.cu:87-109): stop when
 // the point leaves [0,W]x[0,H]x[0,D] or lands in a solid cell; otherwise remember it and step
-// back by vel*dtt/h.  The end point is the last accepted position.
+// back by vel*dtt/h.  The end point is the last accepted position.  The test of the next point is
+// issued before its velocity sample (both may load when the point enters another cell), which the
+// reference's order of decisions allows: the test is only read at the top of the next iteration.
 template <class R, class Vel>
 __device__ __forceinline__ R3<R> backtrace(const Vel& V, const uint8_t* __restrict__ occ, const Dims& d, R h,
                                            int steps, R dtt, R px, R py, R pz) {
     R3<R> last{px, py, pz};
+    OccCache oc;
+    bool open = trace_open_cached(occ, d, px, py, pz, oc);
     R3<R> vel = V(px, py, pz);
     int istep = 0;
     do {
-        if (!trace_open(occ, d, px, py, pz)) break;
+        if (!open) break;
         last = R3<R>{px, py, pz};
         px = px - (vel.x * dtt) / h;
         py = py - (vel.y * dtt) / h;
         pz = pz - (vel.z * dtt) / h;
+        open = trace_open_cached(occ, d, px, py, pz, oc);
         vel = V(px, py, pz);
         istep++;
     } while (istep < steps);
