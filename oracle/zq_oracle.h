@@ -130,6 +130,29 @@ int orc_num_threads(void);
 /* counter-based hash noise in [-amp, amp] per cell (identical in the product) */
 void orc_noise(int W, int H, int D, uint32_t seed, float amp, float* out);
 
+/* ---- coarse + sub-grid projection (COARSE_OPEN_POISSON_SOR, ZQ_SmokeSimulationUtils3D.cpp:3688-3795),
+ * in zq_oracle_coarse.cpp.  s divides W, H and D; coarse grid Wc = W/s etc., faces in the same layout.
+ * Pinned by: the face fractions, averages and coarse -> fine correction are integer/indexing work
+ * checked against brute-force numpy restatements and closed forms; the coarse solver reduces bit for
+ * bit to orc_sor with unit weights (s = 1) and converges to the dense solve of the same system;
+ * the sub-block operator matches the reference's mat.txt (s = 2).  Parity unpinned: the coarse
+ * solver's sweep order and omega (ZQlibCuda, absent) and the sub-block product's summation order
+ * (ZQ_CUDA_MatrixMul, absent). */
+void orc_coarse_setup(int W, int H, int D, int s, const uint8_t* occ, float* rU, float* rV, float* rW,
+                      uint8_t* cocc, uint8_t* regular);
+float orc_coarse_inv_diag(int Wc, int Hc, int Dc, int i, int j, int k, const float* rU, const float* rV,
+                          const float* rW, int open_mask);
+void orc_fine_to_coarse(int W, int H, int D, int s, const float* u, const float* v, const float* w, float* cu,
+                        float* cv, float* cw);
+void orc_coarse_project(int Wc, int Hc, int Dc, const float* cu, const float* cv, const float* cw,
+                        const uint8_t* cocc, const float* rU, const float* rV, const float* rW, int open_mask,
+                        float omega, int iters, float* p, float* du, float* dv, float* dw);
+void orc_coarse_to_fine(int W, int H, int D, int s, const float* du, const float* dv, const float* dw,
+                        const uint8_t* occ, float* u, float* v, float* w);
+/* M: R x F row-major float, R = 3 s^2 (s-1), F = 3 s^2 (s+1) */
+void orc_sub_project_regular(int W, int H, int D, int s, const float* M, const uint8_t* regular, float* u, float* v,
+                             float* w);
+
 /* ---- one forward step (ZQ_SmokeSimulation3D.cpp:156-243 order) ---- */
 typedef struct orc_step_params {
     int W, H, D;

@@ -95,6 +95,14 @@ def lib():
         L.orc_num_threads.restype = i
         L.orc_step.argtypes = [C.POINTER(StepParams), _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, vp, vp, vp]
         L.orc_step.restype = i
+        L.orc_coarse_setup.argtypes = [i, i, i, i, _u8p, _f32p, _f32p, _f32p, _u8p, _u8p]
+        L.orc_coarse_inv_diag.argtypes = [i, i, i, i, i, i, _f32p, _f32p, _f32p, i]
+        L.orc_coarse_inv_diag.restype = f
+        L.orc_fine_to_coarse.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.orc_coarse_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p, i, f, i,
+                                         _f32p, _f32p, _f32p, _f32p]
+        L.orc_coarse_to_fine.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p]
+        L.orc_sub_project_regular.argtypes = [i, i, i, i, _f32p, _u8p, _f32p, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -283,3 +291,212 @@ def step(params, u, v, w, T, rho, occ, extra_force=None, p64=None, boxes=None, s
     if rc:
         raise ValueError("orc_step failed")
     return stats
+
+
+# ---- coarse + sub-grid projection (zq_oracle_coarse.cpp; ZQ_SmokeSimulationUtils3D.cpp:3688-3795) ----
+def coarse_dims(occ, s):
+    W, H, D = _dims(occ)
+    if s < 1 or W % s or H % s or D % s:
+        raise ValueError("block size %d does not divide %dx%dx%d" % (s, W, H, D))
+    return W // s, H // s, D // s
+
+
+def coarse_setup(occ, s):
+    """Coarse face open fractions rU, rV, rW, coarse occupancy and per-block regular flags
+    (ZQ_InitCoarseGrid3D, ZQ_SmokeSimulationUtils3D.cpp:1666-1860; :3318-3340)."""
+    W, H, D = _dims(occ)
+    Wc, Hc, Dc = coarse_dims(occ, s)
+    rU, rV, rW = zeros_faces(Wc, Hc, Dc)
+    cocc = np.zeros((Dc, Hc, Wc), np.uint8)
+    regular = np.zeros((Dc, Hc, Wc), np.uint8)
+    lib().orc_coarse_setup(W, H, D, s, np.ascontiguousarray(occ), rU, rV, rW, cocc, regular)
+    return rU, rV, rW, cocc, regular
+
+
+def fine_to_coarse(u, v, w, s):
+    """ZQ_MapFine2Coarse3D (ZQ_SmokeSimulationUtils3D.cpp:2170-2258)."""
+    D, H, W1 = u.shape
+    W = W1 - 1
+    cu, cv, cw = zeros_faces(W // s, H // s, D // s)
+    lib().orc_fine_to_coarse(W, H, D, s, u, v, w, cu, cv, cw)
+    return cu, cv, cw
+
+
+def coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega=1.0, open_mask=ALL_FACES):
+    """Face-fraction red-black SOR on the coarse grid and the coarse delta (see zq_oracle_coarse.cpp).
+    Returns the padded pressure and du, dv, dw."""
+    Dc, Hc, Wc = cocc.shape
+    p = np.zeros((Dc + 2, Hc + 2, Wc + 2), np.float32)
+    du, dv, dw = zeros_faces(Wc, Hc, Dc)
+    lib().orc_coarse_project(Wc, Hc, Dc, cu, cv, cw, cocc, rU, rV, rW, open_mask, omega, iters, p, du, dv, dw)
+    return p, du, dv, dw
+
+
+def coarse_inv_diag(rU, rV, rW, i, j, k, open_mask=ALL_FACES):
+    Dc, Hc, Wc1 = rU.shape
+    return lib().orc_coarse_inv_diag(Wc1 - 1, Hc, Dc, i, j, k, rU, rV, rW, open_mask)
+
+
+def coarse_to_fine(du, dv, dw, occ, u, v, w, s):
+    """ZQ_MapCoarse2Fine3D (ZQ_SmokeSimulationUtils3D.cpp:3172-3296), in place."""
+    W, H, D = _dims(occ)
+    lib().orc_coarse_to_fine(W, H, D, s, du, dv, dw, np.ascontiguousarray(occ), u, v, w)
+
+
+def block_face_index(s):
+    """Flat face numbering of one s^3 block in the reference's order: u (sk, sj, si = 0..s), then v
+    (sk, sj = 0..s, si), then w (sk = 0..s, sj, si) (ZQ_SmokeSimulationUtils3D.cpp:1159-1161)."""
+    nu = s * s * (s + 1)
+    U = lambda i, j, k: k * s * (s + 1) + j * (s + 1) + i
+    V = lambda i, j, k: nu + k * (s + 1) * s + j * s + i
+    Wf = lambda i, j, k: 2 * nu + k * s * s + j * s + i
+    return U, V, Wf, 3 * nu
+
+
+def interior_rows(s):
+    """Block face numbers of the interior faces in the order of the operator's rows
+    (ZQ_SmokeSimulationUtils3D.cpp:1342-1383)."""
+    U, V, Wf, _ = block_face_index(s)
+    rows = [U(i, j, k) for k in range(s) for j in range(s) for i in range(1, s)]
+    rows += [V(i, j, k) for k in range(s) for j in range(1, s) for i in range(s)]
+    rows += [Wf(i, j, k) for k in range(1, s) for j in range(s) for i in range(s)]
+    return rows
+
+
+def sub_matrix(s):
+    """The regular-block operator of ZQ_InitSubMatPoisson3D (ZQ_SmokeSimulationUtils3D.cpp:1151-1391):
+    y = (I - D [0; pinv(P) V]) x on the interior faces, built in float64 and rounded to float32.
+    P is the block's Neumann Laplacian (-count on the diagonal, 1 per neighbour) without the column
+    of cell 0; V the divergence (n x F); D the interior-face gradient p_r - p_l (F x n).  The reference
+    inverts P with ZQlib's SVD (absent); numpy's SVD pseudo-inverse is the same matrix."""
+    n = s ** 3
+    U, V, Wf, F = block_face_index(s)
+    cell = lambda i, j, k: k * s * s + j * s + i
+    Lap = np.zeros((n, n))
+    Vm = np.zeros((n, F))
+    Dm = np.zeros((F, n))
+    for k in range(s):
+        for j in range(s):
+            for i in range(s):
+                c = cell(i, j, k)
+                Vm[c, U(i, j, k)] -= 1
+                Vm[c, U(i + 1, j, k)] += 1
+                Vm[c, V(i, j, k)] -= 1
+                Vm[c, V(i, j + 1, k)] += 1
+                Vm[c, Wf(i, j, k)] -= 1
+                Vm[c, Wf(i, j, k + 1)] += 1
+                for (a, b, cc, ok) in ((i - 1, j, k, i > 0), (i + 1, j, k, i < s - 1), (i, j - 1, k, j > 0),
+                                       (i, j + 1, k, j < s - 1), (i, j, k - 1, k > 0), (i, j, k + 1, k < s - 1)):
+                    if ok:
+                        Lap[c, cell(a, b, cc)] = 1
+                        Lap[c, c] -= 1
+                if i > 0:
+                    Dm[U(i, j, k), cell(i - 1, j, k)] = -1
+                    Dm[U(i, j, k), c] = 1
+                if j > 0:
+                    Dm[V(i, j, k), cell(i, j - 1, k)] = -1
+                    Dm[V(i, j, k), c] = 1
+                if k > 0:
+                    Dm[Wf(i, j, k), cell(i, j, k - 1)] = -1
+                    Dm[Wf(i, j, k), c] = 1
+    P = Lap[:, 1:]
+    invPV1 = np.zeros((n, F))
+    invPV1[1:] = np.linalg.pinv(P) @ Vm
+    full = np.eye(F) - Dm @ invPV1
+    return full[interior_rows(s)].astype(np.float32)
+
+
+def _block_faces(u, v, w, s, bi, bj, bk):
+    """Gather one block's faces in the reference's flat order (float32 vector of length F)."""
+    x0, y0, z0 = bi * s, bj * s, bk * s
+    return np.concatenate([u[z0:z0 + s, y0:y0 + s, x0:x0 + s + 1].reshape(-1),
+                           v[z0:z0 + s, y0:y0 + s + 1, x0:x0 + s].reshape(-1),
+                           w[z0:z0 + s + 1, y0:y0 + s, x0:x0 + s].reshape(-1)])
+
+
+def _scatter_interior(u, v, w, s, bi, bj, bk, y):
+    """Write the interior faces (operator row order) of one block back."""
+    x0, y0, z0 = bi * s, bj * s, bk * s
+    nu, nv = s * s * (s - 1), s * s * (s - 1)
+    u[z0:z0 + s, y0:y0 + s, x0 + 1:x0 + s] = y[:nu].reshape(s, s, s - 1)
+    v[z0:z0 + s, y0 + 1:y0 + s, x0:x0 + s] = y[nu:nu + nv].reshape(s, s - 1, s)
+    w[z0 + 1:z0 + s, y0:y0 + s, x0:x0 + s] = y[nu + nv:].reshape(s - 1, s, s)
+
+
+def sub_project_regular(u, v, w, s, M, regular):
+    """Regular blocks: interior faces = M x, one sequential float32 dot product per row (zq_oracle_coarse.cpp)."""
+    D, H, W1 = u.shape
+    lib().orc_sub_project_regular(W1 - 1, H, D, s, np.ascontiguousarray(M, np.float32),
+                                  np.ascontiguousarray(regular, np.uint8), u, v, w)
+
+
+def sub_solve_block(x, occ_b, s):
+    """ZQ_SolveSubGrid3D (ZQ_SmokeSimulationUtils3D.cpp:1393-1521) on one block with solid cells:
+    the closed Poisson system of ZQ_InitMatClosedPoisson3D (:603-747; rows all fluid cells, columns
+    all but the first, unit face fractions between fluid cells) solved in least squares (the
+    reference runs ZQlib's CG on the normal equations to 1e-12, absent: numpy's minimum-norm
+    least-squares solution is what it converges to), pressure rounded to float32 with the first
+    fluid cell at 0, interior faces between two fluid cells moved by -(p_r - p_l)
+    (ZQ_AdjustClosedVelocity3D, :2867-2927).  Returns the interior faces in operator row order."""
+    U, V, Wf, F = block_face_index(s)
+    occ_b = np.asarray(occ_b).reshape(s, s, s)
+    x = np.asarray(x, np.float32)
+    fluid = [(i, j, k) for k in range(s) for j in range(s) for i in range(s) if not occ_b[k, j, i]]
+    rows = interior_rows(s)
+    if len(fluid) < 2:
+        return x[rows].copy()
+    idx = {c: n for n, c in enumerate(fluid)}
+    A = np.zeros((len(fluid), len(fluid)))
+    b = np.zeros(len(fluid))
+    for (i, j, k), r in idx.items():
+        d = np.float32(x[U(i + 1, j, k)] - x[U(i, j, k)])
+        d = np.float32(d + np.float32(x[V(i, j + 1, k)] - x[V(i, j, k)]))
+        d = np.float32(d + np.float32(x[Wf(i, j, k + 1)] - x[Wf(i, j, k)]))
+        b[r] = d
+        for nb in ((i, j, k - 1), (i, j, k + 1), (i, j - 1, k), (i, j + 1, k), (i - 1, j, k), (i + 1, j, k)):
+            if nb in idx:
+                A[r, idx[nb]] = 1.0
+                A[r, r] -= 1.0
+    sol = np.linalg.lstsq(A[:, 1:], b, rcond=None)[0]
+    p = np.zeros((s, s, s), np.float32)
+    for (i, j, k), r in idx.items():
+        p[k, j, i] = 0.0 if r == 0 else np.float32(sol[r - 1])
+    out = x.copy()
+    for k in range(s):
+        for j in range(s):
+            for i in range(s):
+                if occ_b[k, j, i]:
+                    continue
+                if i > 0 and not occ_b[k, j, i - 1]:
+                    out[U(i, j, k)] = np.float32(x[U(i, j, k)] + np.float32(-(p[k, j, i] - p[k, j, i - 1])))
+                if j > 0 and not occ_b[k, j - 1, i]:
+                    out[V(i, j, k)] = np.float32(x[V(i, j, k)] + np.float32(-(p[k, j, i] - p[k, j - 1, i])))
+                if k > 0 and not occ_b[k - 1, j, i]:
+                    out[Wf(i, j, k)] = np.float32(x[Wf(i, j, k)] + np.float32(-(p[k, j, i] - p[k - 1, j, i])))
+    return out[rows]
+
+
+def sub_project_irregular(u, v, w, occ, s, regular):
+    """Blocks with a solid cell, each through sub_solve_block, in place."""
+    Dc, Hc, Wc = regular.shape
+    for bk in range(Dc):
+        for bj in range(Hc):
+            for bi in range(Wc):
+                if regular[bk, bj, bi]:
+                    continue
+                x = _block_faces(u, v, w, s, bi, bj, bk)
+                ob = occ[bk * s:(bk + 1) * s, bj * s:(bj + 1) * s, bi * s:(bi + 1) * s]
+                _scatter_interior(u, v, w, s, bi, bj, bk, sub_solve_block(x, ob, s))
+
+
+def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=None):
+    """ZQ_TotalProjection3D for COARSE_OPEN_POISSON_SOR (ZQ_SmokeSimulationUtils3D.cpp:3688-3793), in
+    place: fine -> coarse, coarse face-fraction solve, coarse -> fine, sub-projection (s > 1)."""
+    rU, rV, rW, cocc, regular = coarse_setup(occ, s)
+    cu, cv, cw = fine_to_coarse(u, v, w, s)
+    p, du, dv, dw = coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega, open_mask)
+    coarse_to_fine(du, dv, dw, occ, u, v, w, s)
+    if s > 1:
+        sub_project_regular(u, v, w, s, sub_matrix(s) if M is None else M, regular)
+        sub_project_irregular(u, v, w, occ, s, regular)
+    return p
