@@ -99,6 +99,26 @@ float SolveOpenPoissonRedBlackwithOccupy3D_MAC(float* mac_u, float* mac_v, float
 float SolveClosedPoissonRedBlackwithOccupy3D_MAC(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
                                                  const float div_per_volume, const int width, const int height,
                                                  const int depth, const int maxIter);
+/* replaces the ZQlibCuda call at ZQ_SmokeSimulationUtils3D.cpp:2630-2632 (the coarse grid of
+ * ZQ_SolveCoarseOpenPoisson3DSOR): the face-fraction Poisson problem of ZQ_InitMatOpenPoisson3D
+ * (:476-592; weight = open fraction of the face, p = 0 outside all six domain faces, cells with
+ * occupy set skipped), maxIter red-black Gauss-Seidel iterations from p = 0 (colour (i+j+k)&1,
+ * colour 0 first, neighbour sum x-, x+, y-, y+, z-, z+), then u -= p_r - p_l on every face with
+ * fraction > 0.  Grid sizes are the coarse grid's; faces in the MAC layout. */
+float SolveOpenPoissonRedBlackwithFaceRatio3D_MAC(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
+                                                 const float* unoccupyU, const float* unoccupyV,
+                                                 const float* unoccupyW, const int width, const int height,
+                                                 const int depth, const int maxIter);
+#ifdef __cplusplus
+}
+}
+namespace ZQ_CUDA_MatrixMul {
+extern "C" {
+#endif
+/* replaces ZQlibCuda's ZQ_CUDA_MatrixMul::MatrixMul, which SubProjection3DCuda wraps
+ * (ZQ_SmokeSimulationUtils3D.cpp:13-16; SampleMatrixMul/main.cpp:18): C (h1 x w2) = A (h1 x w1) *
+ * B (w1 x w2), row-major host arrays, each entry summed over k = 0..w1-1 in order. */
+float MatrixMul(const float* A, const float* B, const int h1, const int w1, const int w2, float* C);
 #ifdef __cplusplus
 }
 }
@@ -183,6 +203,12 @@ typedef struct zq3d_config {
     int adv_sca_type;      /* enum AdvectScalarType value */
     zq3d_sor_opts sor;     /* projection */
     int warm_start;        /* mixed: keep the pressure between steps */
+    /* 0: projection on the full grid (TOTAL_*_POISSON_SOR).  s in [1, 8] dividing width, height and
+     * depth: the reference's coarse + sub-grid projection (COARSE_OPEN_POISSON_SOR, subSize = s,
+     * ZQ_SmokeSimulationUtils3D.cpp:3688-3795): sor.iters face-fraction SOR iterations on the
+     * (W/s, H/s, D/s) grid with sor.omega and sor.open_mask, the coarse correction, then each s^3
+     * block's interior faces projected (s > 1).  Requires sor.mixed_precision = 0. */
+    int sub_size;
 } zq3d_config;
 
 typedef struct zq3d_sim zq3d_sim;

@@ -437,7 +437,9 @@ def sub_solve_block(x, occ_b, s):
     reference runs ZQlib's CG on the normal equations to 1e-12, absent: numpy's minimum-norm
     least-squares solution is what it converges to), pressure rounded to float32 with the first
     fluid cell at 0, interior faces between two fluid cells moved by -(p_r - p_l)
-    (ZQ_AdjustClosedVelocity3D, :2867-2927).  Returns the interior faces in operator row order."""
+    (ZQ_AdjustClosedVelocity3D, :2867-2927), every face then input + delta.  A block with fewer than two
+    fluid cells has no system and is returned unchanged (:1492-1497).  Returns the interior faces in
+    operator row order."""
     U, V, Wf, F = block_face_index(s)
     occ_b = np.asarray(occ_b).reshape(s, s, s)
     x = np.asarray(x, np.float32)
@@ -449,9 +451,12 @@ def sub_solve_block(x, occ_b, s):
     A = np.zeros((len(fluid), len(fluid)))
     b = np.zeros(len(fluid))
     for (i, j, k), r in idx.items():
+        # left to right, as the reference's expression (:2569-2571)
         d = np.float32(x[U(i + 1, j, k)] - x[U(i, j, k)])
-        d = np.float32(d + np.float32(x[V(i, j + 1, k)] - x[V(i, j, k)]))
-        d = np.float32(d + np.float32(x[Wf(i, j, k + 1)] - x[Wf(i, j, k)]))
+        d = np.float32(d + x[V(i, j + 1, k)])
+        d = np.float32(d - x[V(i, j, k)])
+        d = np.float32(d + x[Wf(i, j, k + 1)])
+        d = np.float32(d - x[Wf(i, j, k)])
         b[r] = d
         for nb in ((i, j, k - 1), (i, j, k + 1), (i, j - 1, k), (i, j + 1, k), (i - 1, j, k), (i + 1, j, k)):
             if nb in idx:
@@ -461,19 +466,20 @@ def sub_solve_block(x, occ_b, s):
     p = np.zeros((s, s, s), np.float32)
     for (i, j, k), r in idx.items():
         p[k, j, i] = 0.0 if r == 0 else np.float32(sol[r - 1])
-    out = x.copy()
+    # output = input + delta on every face, delta = 0 unless both cells are fluid (:1507-1512)
+    delta = np.zeros(F, np.float32)
     for k in range(s):
         for j in range(s):
             for i in range(s):
                 if occ_b[k, j, i]:
                     continue
                 if i > 0 and not occ_b[k, j, i - 1]:
-                    out[U(i, j, k)] = np.float32(x[U(i, j, k)] + np.float32(-(p[k, j, i] - p[k, j, i - 1])))
+                    delta[U(i, j, k)] = -(p[k, j, i] - p[k, j, i - 1])
                 if j > 0 and not occ_b[k, j - 1, i]:
-                    out[V(i, j, k)] = np.float32(x[V(i, j, k)] + np.float32(-(p[k, j, i] - p[k, j - 1, i])))
+                    delta[V(i, j, k)] = -(p[k, j, i] - p[k, j - 1, i])
                 if k > 0 and not occ_b[k - 1, j, i]:
-                    out[Wf(i, j, k)] = np.float32(x[Wf(i, j, k)] + np.float32(-(p[k, j, i] - p[k - 1, j, i])))
-    return out[rows]
+                    delta[Wf(i, j, k)] = -(p[k, j, i] - p[k - 1, j, i])
+    return (x + delta)[rows]
 
 
 def sub_project_irregular(u, v, w, occ, s, regular):

@@ -27,7 +27,8 @@ def declared_functions(header):
 def test_header_declares_reference_entry_points():
     names = declared_functions(os.path.join(ROOT, "include", "zq_smoke3d.h"))
     for ref in ("ZQ_Cuda_Prepare_Advection", "Advect_Velocity", "Advect_Scalar", "AddForce3D", "Attenuation3D",
-                "SolveOpenPoissonRedBlackwithOccupy3D_MAC", "SolveClosedPoissonRedBlackwithOccupy3D_MAC"):
+                "SolveOpenPoissonRedBlackwithOccupy3D_MAC", "SolveClosedPoissonRedBlackwithOccupy3D_MAC",
+                "SolveOpenPoissonRedBlackwithFaceRatio3D_MAC", "MatrixMul"):
         assert ref in names
 
 

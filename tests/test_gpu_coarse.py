@@ -1,0 +1,175 @@
+"""GPU parity of the coarse + sub-grid projection (zq3d_config.sub_size > 0, the reference's
+COARSE_OPEN_POISSON_SOR) and of its two C entry points, against the CPU oracle.
+
+Bars:
+- face fractions, averages, coarse sweeps, the coarse correction: bit-exact (same IEEE operations
+  in the same order).  They are visible bit for bit in the fine faces on coarse planes, which the
+  sub-projection does not touch, and in the whole result at s = 1;
+- block interiors: the reference applies a dense float32 operator (regular blocks) and an FP64
+  least-squares solve (blocks with solids); the kernels apply the same projections matrix-free
+  (DCT) and by FP64 CG.  Tolerance: 2e-5 of the largest face magnitude;
+- MatrixMul and the face-fraction solve: bit-exact.
+"""
+import numpy as np
+import pytest
+
+import zq_oracle as O
+from util import faces
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    bad = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32))
+    assert bad.size == 0, "%s: %d of %d differ, first at %d: %r vs %r" % (
+        what, bad.size, a.size, bad[0], a.reshape(-1)[bad[0]], b.reshape(-1)[bad[0]])
+
+
+def scene(rng, W, H, D, sprinkle=0.02, sphere=True):
+    occ = (rng.random((D, H, W)) < sprinkle).astype(np.uint8)
+    if sphere:
+        zz, yy, xx = np.mgrid[:D, :H, :W]
+        r = 0.18 * min(W, H, D)
+        occ[((xx - 0.45 * W) ** 2 + (yy - 0.55 * H) ** 2 + (zz - 0.5 * D) ** 2) < r * r] = 1
+    return occ
+
+
+def run_gpu(W, H, D, s, u, v, w, occ, iters, omega, open_mask):
+    from zq_smokesimulation_amd.sim import SmokeSim3D
+    import zq_smokesimulation_amd._lib as L
+    sim = SmokeSim3D(W, H, D, sub_size=s, sor_iters=iters, sor_omega=omega, sor_open_mask=open_mask)
+    try:
+        sim.set_state(u=u, v=v, w=w, occ=occ)
+        st = sim.project()
+        return sim.download(L.FIELD_U), sim.download(L.FIELD_V), sim.download(L.FIELD_W), st
+    finally:
+        sim.close()
+
+
+def check_against_oracle(W, H, D, s, seed, iters=40, omega=1.0, open_mask=O.ALL_FACES, sprinkle=0.02):
+    rng = np.random.default_rng(seed)
+    print("coarse", (W, H, D), "s", s, "seed", seed, flush=True)
+    u, v, w = faces(rng, W, H, D)
+    occ = scene(rng, W, H, D, sprinkle)
+    gu, gv, gw, st = run_gpu(W, H, D, s, u, v, w, occ, iters, omega, open_mask)
+    ru, rv, rw = u.copy(), v.copy(), w.copy()
+    O.coarse_projection(ru, rv, rw, occ, s, iters, omega, open_mask)
+    # faces on coarse planes: bit-exact (fractions, averages, sweeps, correction)
+    assert_same(gu[:, :, ::s], ru[:, :, ::s], "u on coarse planes")
+    assert_same(gv[:, ::s, :], rv[:, ::s, :], "v on coarse planes")
+    assert_same(gw[::s], rw[::s], "w on coarse planes")
+    scale = max(np.abs(ru).max(), np.abs(rv).max(), np.abs(rw).max())
+    for g, r, n in ((gu, ru, "u"), (gv, rv, "v"), (gw, rw, "w")):
+        np.testing.assert_allclose(g, r, rtol=0, atol=2e-5 * scale, err_msg=n)
+    return st
+
+
+def test_unit_blocks_are_bit_exact():
+    rng = np.random.default_rng(2)
+    W, H, D = 20, 14, 12
+    u, v, w = faces(rng, W, H, D)
+    occ = scene(rng, W, H, D, 0.1)
+    gu, gv, gw, _ = run_gpu(W, H, D, 1, u, v, w, occ, 30, 1.3, O.ALL_FACES)
+    ru, rv, rw = u.copy(), v.copy(), w.copy()
+    O.coarse_projection(ru, rv, rw, occ, 1, 30, 1.3, O.ALL_FACES)
+    for g, r, n in ((gu, ru, "u"), (gv, rv, "v"), (gw, rw, "w")):
+        assert_same(g, r, n)
+
+
+@pytest.mark.parametrize("s,shape", [(2, (24, 16, 12)), (4, (32, 24, 16)), (8, (32, 24, 24))])
+def test_coarse_projection_matches_oracle(s, shape):
+    check_against_oracle(*shape, s, seed=10 + s)
+
+
+@pytest.mark.parametrize("s,shape", [(3, (18, 12, 9)), (5, (20, 15, 10)), (6, (24, 18, 12)), (7, (21, 14, 14))])
+def test_every_block_size_matches_oracle(s, shape):
+    check_against_oracle(*shape, s, seed=20 + s, iters=25)
+
+
+def test_open_faces_and_relaxation_match_oracle():
+    check_against_oracle(32, 16, 16, 4, seed=5, iters=60, omega=1.6, open_mask=O.OPEN_YHI | O.OPEN_XLO)
+
+
+def test_no_solids_and_all_solid_blocks():
+    rng = np.random.default_rng(8)
+    W, H, D, s = 16, 16, 8, 4
+    u, v, w = faces(rng, W, H, D)
+    occ = np.zeros((D, H, W), np.uint8)
+    occ[:4, :4, :4] = 1          # one fully solid block
+    occ[4:, 8:12, 12:16] = 1     # another one, next to the domain face
+    occ[5, 13, 2] = 1            # a block with a single solid cell
+    gu, gv, gw, _ = run_gpu(W, H, D, s, u, v, w, occ, 20, 1.0, O.ALL_FACES)
+    ru, rv, rw = u.copy(), v.copy(), w.copy()
+    O.coarse_projection(ru, rv, rw, occ, s, 20, 1.0, O.ALL_FACES)
+    scale = max(np.abs(ru).max(), np.abs(rv).max(), np.abs(rw).max())
+    for g, r, n in ((gu, ru, "u"), (gv, rv, "v"), (gw, rw, "w")):
+        np.testing.assert_allclose(g, r, rtol=0, atol=2e-5 * scale, err_msg=n)
+    # a fully solid block keeps its faces
+    assert_same(gu[:4, :4, 1:4], u[:4, :4, 1:4], "solid block interior u")
+
+
+def test_blocks_end_with_uniform_divergence_at_fine256_block_size():
+    """Size-independent property at the fine256 block size (s = 8, 600 coarse iterations is the
+    config's own maxIter): every regular block leaves the projection with the same divergence in
+    each of its cells."""
+    W, H, D, s = 128, 96, 64, 8
+    rng = np.random.default_rng(42)
+    print("fine256 block property", (W, H, D), flush=True)
+    u, v, w = faces(rng, W, H, D)
+    occ = scene(rng, W, H, D, 0.0)
+    gu, gv, gw, st = run_gpu(W, H, D, s, u, v, w, occ, 600, 1.0, O.ALL_FACES)
+    d = ((gu[:, :, 1:].astype(np.float64) - gu[:, :, :-1]) + (gv[:, 1:] - gv[:, :-1]) + (gw[1:] - gw[:-1]))
+    Dc, Hc, Wc = D // s, H // s, W // s
+    blocks = d.reshape(Dc, s, Hc, s, Wc, s).transpose(0, 2, 4, 1, 3, 5).reshape(Dc, Hc, Wc, -1)
+    _, _, _, _, regular = O.coarse_setup(occ, s)
+    spread = blocks.max(axis=-1) - blocks.min(axis=-1)
+    assert spread[regular == 1].max() < 1e-4
+    assert st["max_div"] >= 0.0
+
+
+def test_face_ratio_solve_entry_point_is_bit_exact():
+    import zq_smokesimulation_amd as z
+    rng = np.random.default_rng(77)
+    W, H, D, s = 24, 16, 12, 2
+    occ = scene(rng, W, H, D, 0.05)
+    u, v, w = faces(rng, W, H, D)
+    rU, rV, rW, cocc, _ = O.coarse_setup(occ, s)
+    cu, cv, cw = O.fine_to_coarse(u, v, w, s)
+    p, du, dv, dw = O.coarse_project(cu, cv, cw, cocc, rU, rV, rW, 70, 1.0)
+    gu, gv, gw = cu.copy(), cv.copy(), cw.copy()
+    z.stages.solve_open_face_ratio(gu, gv, gw, cocc.astype(np.bool_), rU, rV, rW, 70)
+    # the expected projection u - (p_r - p_l) on faces with fraction > 0, from the oracle's pressure
+    P = p
+    eu = cu.copy()
+    m = rU > 0
+    eu[m] = (cu - (P[1:-1, 1:-1, 1:] - P[1:-1, 1:-1, :-1]))[m]
+    ev = cv.copy()
+    m = rV > 0
+    ev[m] = (cv - (P[1:-1, 1:, 1:-1] - P[1:-1, :-1, 1:-1]))[m]
+    ew = cw.copy()
+    m = rW > 0
+    ew[m] = (cw - (P[1:, 1:-1, 1:-1] - P[:-1, 1:-1, 1:-1]))[m]
+    assert_same(gu, eu, "u")
+    assert_same(gv, ev, "v")
+    assert_same(gw, ew, "w")
+    # delta = projected - original is the oracle's coarse delta
+    assert_same(gu - cu, du, "du")
+
+
+def test_matrix_mul_known_answer_and_order():
+    import zq_smokesimulation_amd as z
+    # SampleMatrixMul/main.cpp:5-18: 2x3 times 3x4 of 1..n
+    A = np.arange(1, 7, dtype=np.float32).reshape(2, 3)
+    B = np.arange(1, 13, dtype=np.float32).reshape(3, 4)
+    np.testing.assert_array_equal(z.stages.matrix_mul(A, B), [[38, 44, 50, 56], [83, 98, 113, 128]])
+    # the sub-block operator applied to a batch of blocks, summed in k order
+    rng = np.random.default_rng(4)
+    M = O.sub_matrix(3)
+    X = rng.standard_normal((M.shape[1], 300)).astype(np.float32)
+    acc = np.zeros((M.shape[0], X.shape[1]), np.float32)
+    for k in range(M.shape[1]):
+        acc = acc + M[:, k:k + 1] * X[k:k + 1, :]
+    assert_same(z.stages.matrix_mul(M, X), acc, "M X")

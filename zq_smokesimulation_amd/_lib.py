@@ -64,6 +64,7 @@ class Config(C.Structure):
         ("adv_sca_type", C.c_int),
         ("sor", SorOpts),
         ("warm_start", C.c_int),
+        ("sub_size", C.c_int),
     ]
 
 
@@ -141,6 +142,10 @@ def _load():
     L.ZQ3D_SolvePoissonRedBlack_ex.restype = f
     L.ZQ3D_MACtoRegular4.argtypes = [vp, vp, vp, vp, i, i, i]
     L.ZQ3D_MACtoRegular4.restype = f
+    L.SolveOpenPoissonRedBlackwithFaceRatio3D_MAC.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, i, i]
+    L.SolveOpenPoissonRedBlackwithFaceRatio3D_MAC.restype = f
+    L.MatrixMul.argtypes = [vp, vp, i, i, i, vp]
+    L.MatrixMul.restype = f
     L.zq3d_config_default.argtypes = [C.POINTER(Config), i, i, i]
     L.zq3d_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
     L.zq3d_destroy.argtypes = [vp]

@@ -643,6 +643,7 @@ struct zq3d_sim {
     SorProf sorprof;
     zq3d_timing tm{};
     SorShape shape{0, 0};  // fused sweep tile shape chosen by zq3d_autotune_sor / zq3d_set_sor_shape
+    CoarseState coarse;    // sub_size > 0: the coarse grid of the coarse + sub-grid projection
 };
 
 namespace {
@@ -710,6 +711,15 @@ int check_config(const zq3d_config& c, const char* where) {
         return ZQ3D_ERR_ARG;
     }
     if (c.adv_sca_type < 0 || c.adv_sca_type > 3) { set_err(where, "unknown adv_sca_type"); return ZQ3D_ERR_ARG; }
+    if (c.sub_size < 0 || c.sub_size > COARSE_MAX_S ||
+        (c.sub_size && (c.width % c.sub_size || c.height % c.sub_size || c.depth % c.sub_size))) {
+        set_err(where, "sub_size must be 0 or in [1, 8] dividing width, height and depth");
+        return ZQ3D_ERR_ARG;
+    }
+    if (c.sub_size && c.sor.mixed_precision) {
+        set_err(where, "sub_size > 0 runs the coarse FP32 SOR: sor.mixed_precision must be 0");
+        return ZQ3D_ERR_ARG;
+    }
     return check_sor_opts(c.sor, where);
 }
 
@@ -782,6 +792,12 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
         zq3d_destroy(s);
         return ZQ3D_ERR_HIP;
     }
+    if (cfg->sub_size > 0) {
+        if (int rc = coarse_create(s->d, cfg->sub_size, &s->coarse, where)) {
+            zq3d_destroy(s);
+            return rc;
+        }
+    }
     hipStream_t st = nullptr;
     launch_fill_f32(s->u, z.Nu, 0.0f, st);
     launch_fill_f32(s->v, z.Nv, 0.0f, st);
@@ -812,6 +828,7 @@ extern "C" void zq3d_destroy(zq3d_sim* s) {
         if (p) (void)hipFree(p);
     for (hipEvent_t e : {s->ev[0], s->ev[1], s->sorprof.a, s->sorprof.b})
         if (e) (void)hipEventDestroy(e);
+    coarse_destroy(&s->coarse);
     delete s;
 }
 
@@ -909,7 +926,9 @@ extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
     StageTimer t(s, &s->tm.ms_project);
     SorProf* pf = s->prof ? &s->sorprof : nullptr;
     int rc;
-    if (c.sor.mixed_precision) {
+    if (c.sub_size > 0) {
+        rc = project_coarse(s->coarse, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
+    } else if (c.sor.mixed_precision) {
         if (!c.warm_start || !s->p64_valid) launch_fill_f64(s->p64, Sizes(s->d).Np, 0.0, s->stream);
         rc = project_mixed(s->d, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
         s->p64_valid = (rc == ZQ3D_OK);

@@ -6,6 +6,7 @@
 
 #include <string>
 
+#include "zq3d_coarse.h"
 #include "zq3d_kernels.h"
 #include "zq_smoke3d.h"
 
@@ -97,6 +98,19 @@ int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
 // p64 holds the warm start (zeroed by the caller for a cold start).
 int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                   const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf = nullptr);
+
+// Coarse grid of the coarse + sub-grid projection (one device allocation).
+struct CoarseState {
+    CoarseDims cd{};
+    CoarseBufs B{};
+    void* mem = nullptr;
+};
+int coarse_create(const Dims& d, int s, CoarseState* out, const char* where);
+void coarse_destroy(CoarseState* c);
+// Coarse + sub-grid projection of the faces in place (o.iters coarse iterations with o.omega and
+// o.open_mask).  B provides the stats scratch (nc, b, p, part, sc); stats may be null.
+int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                   const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf);
 
 }  // namespace detail
 }  // namespace zq3d

@@ -95,6 +95,28 @@ def solve_closed(u, v, w, occ, div_per_volume, max_iter):
         int(max_iter)), "SolveClosed")
 
 
+def solve_open_face_ratio(u, v, w, occ, rU, rV, rW, max_iter):
+    """SolveOpenPoissonRedBlackwithFaceRatio3D_MAC (call site ZQ_SmokeSimulationUtils3D.cpp:2630):
+    the coarse grid's faces projected in place; rU, rV, rW are the face open fractions (face shapes)."""
+    W, H, D = _dims_from_occ(occ)
+    _check_faces(u, v, w, W, H, D)
+    _check_faces(rU, rV, rW, W, H, D)
+    return check(_lib.lib().SolveOpenPoissonRedBlackwithFaceRatio3D_MAC(
+        _p(u, np.float32), _p(v, np.float32), _p(w, np.float32), _occ(occ), _p(rU, np.float32), _p(rV, np.float32),
+        _p(rW, np.float32), W, H, D, int(max_iter)), "SolveOpenFaceRatio")
+
+
+def matrix_mul(A, B):
+    """ZQ_CUDA_MatrixMul::MatrixMul (called by SubProjection3DCuda, ZQ_SmokeSimulationUtils3D.cpp:13-16):
+    returns A @ B for float32 matrices, each entry summed over k in order."""
+    if A.ndim != 2 or B.ndim != 2 or A.shape[1] != B.shape[0]:
+        raise ValueError("shapes %s and %s do not multiply" % (A.shape, B.shape))
+    out = np.zeros((A.shape[0], B.shape[1]), np.float32)
+    check(_lib.lib().MatrixMul(_p(A, np.float32), _p(B, np.float32), A.shape[0], A.shape[1], B.shape[1],
+                               _p(out, np.float32)), "MatrixMul")
+    return out
+
+
 def solve_ex(u, v, w, occ, open_mask=_lib.OPEN_ALL, closed_mean=False, omega=1.0, iters=100, mixed=False,
              max_outer=0, tol=1e-6):
     """ZQ3D_SolvePoissonRedBlack_ex: returns (ms, stats dict)."""
