@@ -16,6 +16,10 @@ hand-written adjoint (each earlier segment recomputed from its checkpoint), for 
 1/2 |rho_64 - target|^2 with a target of 8 seeded Gaussian blobs and per-step control forces
 (zero); value = voxels * window / (forward + backward wall time).
 
+coarse_projection: the reference's default solver (COARSE_OPEN_POISSON_SOR) in a full step of its
+expsimu3D/fine256.txt scene (256^3, 8^3 blocks, 600 coarse iterations, internal sphere, box
+source), the scene parsed by the library's config reader; value = voxels * steps / wall time.
+
 ensemble: 64 independent grids with per-axis sizes drawn from {32..96} (seeded, anisotropic),
 alpha in [0.02, 0.1] and beta in [0.2, 1.0] per member, closed boxes with the same sphere
 source, plain red-black SOR (omega 1.8, 200 iterations), 100 steps replayed from one captured
@@ -54,6 +58,8 @@ def parse():
     ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
     ap.add_argument("--ens-members", type=int, default=64, help="ensemble members (0: skip the ensemble leg)")
     ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
+    ap.add_argument("--coarse-steps", type=int, default=5,
+                    help="timed steps of the coarse + sub-grid projection leg (0: skip)")
     ap.add_argument("--sor-shape", default=None,
                     help="sweep shape 'ty,zc' to use instead of autotuning it (a zq3d_autotune_sor candidate)")
     ap.add_argument("--traffic", default=None,
@@ -183,6 +189,69 @@ def ensemble_leg(args):
         "largest": max(sizes, key=lambda t: t[0] * t[1] * t[2]),
         "autotune": tune["best"],
         "density_finite": bool(np.isfinite(rho0).all()),
+    }
+
+
+# the reference's fine256 scene (expsimu3D/fine256.txt) with its default solver, COARSE_OPEN_POISSON_SOR
+FINE256 = ("resolution 256 256 256\nsubSize 8\nsource 0.5 0.1 0.5 0.1 0.05 0.1 0.8 20\nTamb 1\nconfineCoeff 30\n"
+           "step 30\nstepTime 0.03\nframeCount 200\nmaxIter 600\ninnerIter 20\nvelAttenCoeff 10\n"
+           "densityAttenCoeff 0\ntempAttenCoeff 0\nintObj sphere\nuseMovingObject false\n"
+           "advVelType inReg_outReg\nadvScaType Reg\nexportType di3\naddForceType slice\n"
+           "globalGridSolver COARSE_OPEN_POISSON_SOR\n")
+
+
+def coarse_leg(args):
+    """The reference's default projection (coarse + sub-grid, 8^3 blocks, 600 coarse iterations) in a
+    full step of its fine256 scene, parsed by the library's own config reader."""
+    import tempfile
+    import zq_smokesimulation_amd as z
+    from zq_smokesimulation_amd import scene
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "fine256.txt")
+        with open(path, "w") as f:
+            f.write(FINE256)
+        sc = scene.load_scene(path)
+        occ = scene.initial_occupancy(path)
+    c = sc.config
+    W, H, D = c.width, c.height, c.depth
+    over = {n: getattr(c, n) for n, _ in c._fields_ if n not in ("struct_size", "width", "height", "depth", "sor")}
+    over.update({"sor_" + n: getattr(c.sor, n) for n, _ in c.sor._fields_})
+    sim = z.SmokeSim3D(W, H, D, **over)
+    sweep = sim.coarse_sweep()
+    boxes = [list(sc.box_sources[i]) for i in range(sc.n_box_sources)]
+    sim.set_box_sources(boxes)
+    st = make_state(W, H, D)
+    st["T"][...] = c.Tamb
+    st["occ"] = occ
+    sim.set_state(**st)
+    for _ in range(args.warmup):
+        sim.step()
+    sim.synchronize()
+    sim.set_profiling(True)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.coarse_steps):
+        stats.append(sim.step())
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    tm = sim.timing()
+    rho = sim.download(z._lib.FIELD_RHO)
+    sim.close()
+    n = W * H * D
+    return {
+        "value": round(n * args.coarse_steps / wall / 1e6, 3),
+        "unit": "Mvoxel-steps/s",
+        "workload": "expsimu3D/fine256.txt: 256^3, COARSE_OPEN_POISSON_SOR, subSize 8 (32^3 coarse grid), "
+                    "maxIter 600, internal sphere, box source",
+        "steps": args.coarse_steps,
+        "ms_per_step": round(1e3 * wall / args.coarse_steps, 3),
+        "stages_ms_per_step": {k: round(tm[k] / max(tm["steps"], 1), 3) for k in
+                               ("ms_force", "ms_advect_velocity", "ms_project", "ms_sor", "ms_advect_scalar",
+                                "ms_reinject")},
+        "coarse_sweep_schedule": {"iterations_per_launch": sweep[0], "planes_per_workgroup": sweep[1]},
+        "coarse_sweep_launches_per_step": tm["sor_launches"] / max(tm["steps"], 1),
+        "max_div": float(max(s["max_div"] for s in stats)),
+        "density_finite": bool(np.isfinite(rho).all()),
     }
 
 
@@ -368,6 +437,9 @@ def main():
     if args.ens_members > 0:
         print("bench: ensemble of %d grids" % args.ens_members, file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_leg(args)
+    if args.coarse_steps > 0:
+        print("bench: coarse + sub-grid projection (fine256 scene)", file=sys.stderr, flush=True)
+        out["coarse_projection"] = coarse_leg(args)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean)

@@ -281,6 +281,12 @@ typedef struct zq3d_tune_report {
 int zq3d_autotune_sor(zq3d_sim* sim, int iters, zq3d_tune_report* report);
 /* explicit shape, one of the autotuner's candidates; (0, 0) restores the default */
 int zq3d_set_sor_shape(zq3d_sim* sim, int ty, int zc);
+/* Coarse + sub-grid projection (sub_size > 0; no reference counterpart): schedule of the coarse
+ * red-black sweeps.  k == 0: one launch per colour; k > 0: k iterations per launch, each workgroup
+ * owning zc z-planes of the coarse grid and recomputing a halo of 2k planes in LDS.  zq3d_create
+ * times the valid schedules and keeps the fastest; all produce the same bits. */
+int zq3d_set_coarse_sweep(zq3d_sim* sim, int k, int zc);
+int zq3d_get_coarse_sweep(zq3d_sim* sim, int* k, int* zc);
 /* name of the sweep kernel the current shape runs ("k_sor_split_rb", "k_sor_split", "k_sor_fused"
  * or "k_sor_half") */
 const char* zq3d_sor_kernel(zq3d_sim* sim);

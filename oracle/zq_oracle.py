@@ -506,3 +506,21 @@ def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=
         sub_project_regular(u, v, w, s, sub_matrix(s) if M is None else M, regular)
         sub_project_irregular(u, v, w, occ, s, regular)
     return p
+
+
+def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None):
+    """One step in the order of orc_step (ZQ_SmokeSimulation3D.cpp:156-243) with the coarse + sub-grid
+    projection in place of the full-grid solve (COARSE_*_POISSON_SOR): open on all six sides, FP32
+    SOR with params' omega and sor_iters on the coarse grid."""
+    P = dict(params)
+    W, H, D = _dims(occ)
+    add_force(u, v, w, occ, T, rho, P["h"], P["dt"], P["beta"], P["eps"], P["Tamb"], alpha=P.get("alpha", 0.0))
+    attenuation(u, v, w, T, rho, occ, P["aV"], P["aT"], P["aR"])
+    dtt = float(np.float32(P["dt"]) / np.float32(P["steps"]))
+    advect_velocity(u, v, w, occ, P["h"], P["steps"], dtt, P["adv_vel_type"], True)
+    coarse_projection(u, v, w, occ, s, P["sor_iters"], P["omega"], ALL_FACES)
+    To, Ro = advect_scalar(u, v, w, occ, T, rho, P["h"], P["steps"], dtt, P["adv_sca_type"])
+    T[...] = To
+    rho[...] = Ro
+    for b in boxes or []:
+        reinject_box(T, rho, b)

@@ -4,9 +4,12 @@
 //   device path  -- deviceResident (default): the fields live in HBM in a zq3d_sim and the
 //                   stages run back to back on one stream; the host grid is refreshed on demand.
 // Solvers: TOTAL_{OPEN,CLOSED}_POISSON_SOR are the full-grid red-black SOR this library builds.
-// COARSE_{OPEN,CLOSED}_POISSON_SOR (the reference's default, a coarse + sub-grid projection) is
-// run as the full-grid solve with the same boundary type; the flux, CPU PCG and octree solvers
-// are not built and Init fails for them.
+// COARSE_{OPEN,CLOSED}_POISSON_SOR (the reference's default) is the coarse + sub-grid projection
+// (zq3d_config.sub_size = subSize); both run the open face-fraction solve on the coarse grid, as
+// the reference's ZQ_TotalProjection3D does (ZQ_SmokeSimulationUtils3D.cpp:3730-3745).  In the
+// staged host path that projection is one device call on a projection-only simulation (the
+// fields cross PCIe as in the other stages).  The flux, CPU PCG and octree solvers are not built
+// and Init fails for them.
 #include "ZQ_SmokeSimulation3D.h"
 
 #include <chrono>
@@ -48,10 +51,11 @@ void add_noise(float* T, size_t n, unsigned seed, float amp) {
 }  // namespace
 
 ZQ_SmokeSimulation3D::ZQ_SmokeSimulation3D()
-    : fineGrid(nullptr), dev(nullptr), moving_frame(0), host_current(true), lastStats{} {}
+    : fineGrid(nullptr), dev(nullptr), proj(nullptr), moving_frame(0), host_current(true), lastStats{} {}
 
 ZQ_SmokeSimulation3D::~ZQ_SmokeSimulation3D() {
     if (dev) zq3d_destroy(dev);
+    if (proj) zq3d_destroy(proj);
     delete fineGrid;
     for (ZQ_SmokeMovingObject3D* o : mvobjs) delete o;
 }
@@ -64,8 +68,6 @@ bool ZQ_SmokeSimulation3D::Init(const char* file) {
                     (int)para.globalGridSolver);
         return false;
     }
-    if (para.globalGridSolver == COARSE_OPEN_POISSON_SOR || para.globalGridSolver == COARSE_CLOSED_POISSON_SOR)
-        std::printf("note: the coarse + sub-grid projection is run as the full-grid red-black SOR\n");
     if (para.useGuidingControl) {
         std::printf("guiding control is not built\n");
         return false;
@@ -128,6 +130,7 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
         c.sor.max_outer = para.maxOuter;
         c.sor.tol = para.sorTol;
         c.warm_start = para.warmStart ? 1 : 0;
+        if (is_coarse()) set_coarse(&c);
         if (zq3d_create(&c, &dev) != ZQ3D_OK) {
             std::printf("device simulation: %s\n", zq3d_last_error());
             return false;
@@ -155,8 +158,32 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
             return false;
         }
     }
+    if (!dev && is_coarse()) {
+        // projection-only device simulation of the staged path
+        zq3d_config c;
+        zq3d_config_default(&c, para.width, para.height, para.depth);
+        c.sor.omega = para.omega;
+        c.sor.iters = para.maxIter;
+        set_coarse(&c);
+        if (zq3d_create(&c, &proj) != ZQ3D_OK) {
+            std::printf("device projection: %s\n", zq3d_last_error());
+            return false;
+        }
+    }
     host_current = true;
     return true;
+}
+
+bool ZQ_SmokeSimulation3D::is_coarse() const {
+    return para.globalGridSolver == COARSE_OPEN_POISSON_SOR || para.globalGridSolver == COARSE_CLOSED_POISSON_SOR;
+}
+
+// the coarse solve of both COARSE_*_POISSON_SOR solvers: open face-fraction SOR, FP32, maxIter iterations
+void ZQ_SmokeSimulation3D::set_coarse(zq3d_config* c) const {
+    c->sub_size = para.subSize;
+    c->sor.open_mask = ZQ3D_OPEN_ALL;
+    c->sor.closed_mean = 0;
+    c->sor.mixed_precision = 0;
 }
 
 bool ZQ_SmokeSimulation3D::SyncToHost() {
@@ -239,6 +266,25 @@ bool ZQ_SmokeSimulation3D::_updateHost() {
 
 // _projection / _totalProjection (ZQ_SmokeSimulation3D.cpp:396-549) for the built solvers
 bool ZQ_SmokeSimulation3D::_projection() {
+    if (proj) {
+        const size_t N = (size_t)para.width * para.height * para.depth;
+        const size_t Nu = (size_t)(para.width + 1) * para.height * para.depth;
+        const size_t Nv = (size_t)para.width * (para.height + 1) * para.depth;
+        const size_t Nw = (size_t)para.width * para.height * (para.depth + 1);
+        std::vector<uint8_t> occ(N);
+        for (size_t o = 0; o < N; o++) occ[o] = fineGrid->GetOccupyPtr()[o] ? 1 : 0;
+        if (zq3d_upload(proj, ZQ3D_FIELD_OCC, occ.data(), N) ||
+            zq3d_upload(proj, ZQ3D_FIELD_U, fineGrid->GetVelocityUptr(), Nu * 4) ||
+            zq3d_upload(proj, ZQ3D_FIELD_V, fineGrid->GetVelocityVptr(), Nv * 4) ||
+            zq3d_upload(proj, ZQ3D_FIELD_W, fineGrid->GetVelocityWptr(), Nw * 4) || zq3d_project(proj, &lastStats) ||
+            zq3d_download(proj, ZQ3D_FIELD_U, fineGrid->GetVelocityUptr(), Nu * 4) ||
+            zq3d_download(proj, ZQ3D_FIELD_V, fineGrid->GetVelocityVptr(), Nv * 4) ||
+            zq3d_download(proj, ZQ3D_FIELD_W, fineGrid->GetVelocityWptr(), Nw * 4)) {
+            std::printf("coarse projection: %s\n", zq3d_last_error());
+            return false;
+        }
+        return true;
+    }
     if (para.IsClosedSolver()) return ZQ_SolveClosedPressure3DSOR(fineGrid, para);
     return ZQ_SolveOpenPressure3DSOR(fineGrid, para);
 }

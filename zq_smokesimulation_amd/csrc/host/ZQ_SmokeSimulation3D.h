@@ -37,10 +37,13 @@ private:
     ZQ_SmokeGrid3D* fineGrid;
     std::vector<ZQ_SmokeMovingObject3D*> mvobjs;
     zq3d_sim* dev;        // device-resident state (deviceResident and no file-based moving objects)
+    zq3d_sim* proj;       // staged path with a COARSE_* solver: projection-only device simulation
     int moving_frame;     // the reference keeps this in a function-static (ZQ_SmokeSimulation3D.cpp:161)
     bool host_current;    // host grid equals the device state
     zq3d_proj_stats lastStats;
 
+    bool is_coarse() const;
+    void set_coarse(zq3d_config* c) const;
     bool _initRegular();
     bool _projection();
     bool _updateResident();

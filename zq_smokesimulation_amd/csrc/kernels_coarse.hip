@@ -11,6 +11,7 @@
 // constant mode dropped (which is the mean removal), 12 s^4 flops per block instead of
 // 2 * 1344 * 1728.  Blocks with solid cells solve the block's closed Poisson problem
 // (ZQ_SolveSubGrid3D, :1393-1521) by FP64 conjugate gradients inside one workgroup.
+#include <algorithm>
 #include <climits>
 
 #include "zq3d_coarse.h"
@@ -38,6 +39,25 @@ __device__ __forceinline__ int c_p(const CG& g, int i, int j, int k) {
 }
 
 inline unsigned blocks_for(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// dst[e] = src[e] for e < n, by the workgroup's nt threads: U loads in flight per thread before
+// the LDS stores (a load-store loop would wait out the memory latency once per element)
+template <int U>
+__device__ __forceinline__ void stage(float* dst, const float* __restrict__ src, int n, int tid, int nt) {
+    for (int base = tid; base < n; base += U * nt) {
+        float t[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int e = base + q * nt;
+            t[q] = e < n ? src[e] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int e = base + q * nt;
+            if (e < n) dst[e] = t[q];
+        }
+    }
+}
 
 // ------------------------------------------------------------------------------ set-up
 // One thread per coarse face: 1 - (solid sub-faces)/s^2; a sub-face of an interior coarse face is
@@ -204,15 +224,93 @@ __global__ __launch_bounds__(256) void k_coarse_sweep(CG g, const float* __restr
     const int i = 2 * xh + ((colour + j + k) & 1);
     if (i >= g.Wc) return;
     const int o = c_c(g, i, j, k);
-    const float iv = inv[o];
-    if (iv == 0.0f) return;
     const int sy = g.Wc + 2, sz = (g.Wc + 2) * (g.Hc + 2);
     const int q = c_p(g, i, j, k);
+    // every operand is loaded before the test, so the loads share one memory latency
+    const float iv = inv[o];
     const float s = rU[c_u(g, i, j, k)] * p[q - 1] + rU[c_u(g, i + 1, j, k)] * p[q + 1] + rV[c_v(g, i, j, k)] * p[q - sy] +
                     rV[c_v(g, i, j + 1, k)] * p[q + sy] + rW[c_w(g, i, j, k)] * p[q - sz] +
                     rW[c_w(g, i, j, k + 1)] * p[q + sz];
     const float pg = (s - div[o]) * iv;
-    p[q] = omw * p[q] + omega * pg;
+    const float pn = omw * p[q] + omega * pg;
+    if (iv != 0.0f) p[q] = pn;
+}
+
+// k red-black iterations in one launch: a workgroup owns the z-planes [z0, z0 + zc) and keeps
+// pressure planes [z0 - 2k, z0 + zc + 2k) (whole padded x-y planes, ghosts included) in LDS, with
+// the fractions, divergence and 1/diagonal of the planes it updates.  Half-sweep h updates the
+// planes within 2k-1-h of its own range, so every value it reads is one a whole-grid sweep would
+// have produced (halo planes are recomputed redundantly, with the same operations), and the owned
+// planes end bit-identical to 2k single-colour launches.  src is read, dst written.
+struct SlabLayout {
+    int zlo, zhi, clo, chi;  // pressure planes [zlo, zhi), updated cell planes [clo, chi)
+    int NP, NC;
+};
+__host__ __device__ inline SlabLayout slab_layout(int Dc, int z0, int k, int zc) {
+    SlabLayout L;
+    L.zlo = z0 - 2 * k < -1 ? -1 : z0 - 2 * k;
+    L.zhi = z0 + zc + 2 * k > Dc + 1 ? Dc + 1 : z0 + zc + 2 * k;
+    L.clo = z0 - 2 * k + 1 < 0 ? 0 : z0 - 2 * k + 1;
+    L.chi = z0 + zc + 2 * k - 1 > Dc ? Dc : z0 + zc + 2 * k - 1;
+    L.NP = L.zhi - L.zlo;
+    L.NC = L.chi - L.clo;
+    return L;
+}
+// floats of LDS for planes NP (pressure) and NC (cells): padded pressure, u/v fractions of NC planes,
+// w fractions of NC+1 face planes, divergence and 1/diagonal of NC planes
+__host__ __device__ inline size_t slab_floats(int Wc, int Hc, int NP, int NC) {
+    const size_t PS = (size_t)(Wc + 2) * (Hc + 2), CS = (size_t)Wc * Hc;
+    return NP * PS + NC * ((size_t)(Wc + 1) * Hc + (size_t)Wc * (Hc + 1) + 2 * CS) + (NC + 1) * CS;
+}
+
+__global__ __launch_bounds__(1024) void k_coarse_slab(CG g, const float* __restrict__ rU, const float* __restrict__ rV,
+                                                     const float* __restrict__ rW, const float* __restrict__ inv,
+                                                     const float* __restrict__ div, const float* __restrict__ src,
+                                                     float* __restrict__ dst, float omega, float omw, int k, int zc) {
+    extern __shared__ float lds[];
+    const int z0 = blockIdx.x * zc;
+    const SlabLayout L = slab_layout(g.Dc, z0, k, zc);
+    const int PW = g.Wc + 2, PS = PW * (g.Hc + 2);
+    const int CS = g.Wc * g.Hc, US = (g.Wc + 1) * g.Hc, VS = g.Wc * (g.Hc + 1);
+    float* P = lds;
+    float* sU = P + L.NP * PS;
+    float* sV = sU + L.NC * US;
+    float* sW = sV + L.NC * VS;
+    float* sD = sW + (L.NC + 1) * CS;
+    float* sI = sD + L.NC * CS;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    stage<16>(P, src + (size_t)(L.zlo + 1) * PS, L.NP * PS, tid, nt);
+    stage<16>(sU, rU + (size_t)L.clo * US, L.NC * US, tid, nt);
+    stage<16>(sV, rV + (size_t)L.clo * VS, L.NC * VS, tid, nt);
+    stage<16>(sW, rW + (size_t)L.clo * CS, (L.NC + 1) * CS, tid, nt);
+    stage<16>(sD, div + (size_t)L.clo * CS, L.NC * CS, tid, nt);
+    stage<16>(sI, inv + (size_t)L.clo * CS, L.NC * CS, tid, nt);
+    __syncthreads();
+    const int Wh = (g.Wc + 1) >> 1;
+    for (int hs = 0; hs < 2 * k; hs++) {
+        const int c = hs & 1, r = 2 * k - 1 - hs;
+        const int a = max(z0 - r, 0), bnd = min(z0 + zc + r, g.Dc);
+        const int n = (bnd - a) * g.Hc * Wh;
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int xh = e % Wh, j = (e / Wh) % g.Hc, kk = a + e / (Wh * g.Hc);
+            const int i = 2 * xh + ((c + j + kk) & 1);
+            if (i >= g.Wc) continue;
+            const int lk = kk - L.clo;
+            const int o = lk * CS + j * g.Wc + i;
+            const float iv = sI[o];
+            if (iv == 0.0f) continue;
+            const int q = (kk - L.zlo) * PS + (j + 1) * PW + (i + 1);
+            const int fu = lk * US + j * (g.Wc + 1) + i, fv = lk * VS + j * g.Wc + i;
+            const float s = sU[fu] * P[q - 1] + sU[fu + 1] * P[q + 1] + sV[fv] * P[q - PW] + sV[fv + g.Wc] * P[q + PW] +
+                            sW[o] * P[q - PS] + sW[o + CS] * P[q + PS];
+            const float pg = (s - sD[o]) * iv;
+            P[q] = omw * P[q] + omega * pg;
+        }
+        __syncthreads();
+    }
+    const int zend = min(z0 + zc, g.Dc);
+    for (int e = threadIdx.x; e < (zend - z0) * PS; e += blockDim.x)
+        dst[(size_t)(z0 + 1) * PS + e] = P[(z0 - L.zlo) * PS + e];
 }
 
 // ------------------------------------------------------------------------------ coarse delta
@@ -340,112 +438,187 @@ struct SubTables {
     float lam[COARSE_MAX_S];
 };
 
-// NBX blocks along x per workgroup (one row of the tile per (y, z) of the block row).
+// A workgroup takes NB consecutive blocks of one block row (NB = 256 / s^2, so one thread per grid
+// line of each block: s^2 lines per block along each axis).  Faces are staged through LDS with
+// coalesced row loads; each thread runs the 1D transforms of its line in registers; the cell
+// buffers pad every x row to s+1 floats so lines along y and z hit distinct banks.
 template <int S>
 struct RegTile {
-    static constexpr int NBX = S == 2 ? 32 : S == 3 ? 8 : S == 4 ? 4 : S == 5 ? 2 : 1;
-    static constexpr int TW = S * NBX;
-    static constexpr int NCELL = S * S * TW;
+    static constexpr int NB = 256 / (S * S);
+    static constexpr int TW = NB * S;
+    static constexpr int RP = S + 1;                   // padded row
+    static constexpr int NCELL = NB * S * S * RP;      // padded cells of the tile
 };
+
+template <int S>
+__device__ __forceinline__ int rt_cell(int b, int z, int y, int x) {
+    return ((b * S + z) * S + y) * (S + 1) + x;
+}
 
 template <int S>
 __global__ __launch_bounds__(256) void k_sub_regular(CG g, SubTables tab, const uint8_t* __restrict__ regular, float* u,
                                                      float* v, float* w) {
-    constexpr int NBX = RegTile<S>::NBX, TW = RegTile<S>::TW, NC = RegTile<S>::NCELL;
+    constexpr int NB = RegTile<S>::NB, TW = RegTile<S>::TW, NC = RegTile<S>::NCELL;
     __shared__ float Us[S][S][TW + 1];
     __shared__ float Vs[S][S + 1][TW];
     __shared__ float Ws[S + 1][S][TW];
     __shared__ float A[NC], B[NC];
-    __shared__ float Q[S * S], L[S];
-    __shared__ uint8_t reg[NBX];
+    __shared__ uint8_t reg[NB];
     const int tid = threadIdx.x;
-    const int bi0 = blockIdx.x * NBX, bj = blockIdx.y, bk = blockIdx.z;
-    const int nb = min(NBX, g.Wc - bi0);
-    const int tw = nb * S;
+    const int bi0 = blockIdx.x * NB, bj = blockIdx.y, bk = blockIdx.z;
+    const int nb = min(NB, g.Wc - bi0);
     int mine = 0;
-    if (tid < nb) {
-        mine = regular[c_c(g, bi0 + tid, bj, bk)];
+    if (tid < NB) {
+        mine = tid < nb ? regular[c_c(g, bi0 + tid, bj, bk)] : 0;
         reg[tid] = (uint8_t)mine;
     }
-    if (tid < S * S) Q[tid] = tab.q[tid];
-    if (tid < S) L[tid] = tab.lam[tid];
     if (!__syncthreads_or(mine)) return;
     const int x0 = bi0 * S, y0 = bj * S, z0 = bk * S;
-    for (int e = tid; e < S * S * (tw + 1); e += 256) {
-        const int x = e % (tw + 1), r = e / (tw + 1);
-        Us[r / S][r % S][x] = u[f_u(g, x0 + x, y0 + r % S, z0 + r / S)];
-    }
-    for (int e = tid; e < S * (S + 1) * tw; e += 256) {
-        const int x = e % tw, r = e / tw;
-        Vs[r / (S + 1)][r % (S + 1)][x] = v[f_v(g, x0 + x, y0 + r % (S + 1), z0 + r / (S + 1))];
-    }
-    for (int e = tid; e < (S + 1) * S * tw; e += 256) {
-        const int x = e % tw, r = e / tw;
-        Ws[r / S][r % S][x] = w[f_w(g, x0 + x, y0 + r % S, z0 + r / S)];
-    }
-    __syncthreads();
-    const int ncell = S * S * tw;
-    // divergence, cell (x, y, z) at A[(z*S + y)*TW + x]
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, y = (e / tw) % S, z = e / (tw * S);
-        A[(z * S + y) * TW + x] = Us[z][y][x + 1] - Us[z][y][x] + Vs[z][y + 1][x] - Vs[z][y][x] + Ws[z + 1][y][x] -
-                                  Ws[z][y][x];
-    }
-    __syncthreads();
-    // forward transforms: x (A -> B), y (B -> A), z (A -> B)
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, row = e / tw, m = x % S, base = row * TW + (x - m);
-        float a = 0.0f;
-        for (int n = 0; n < S; n++) a += Q[m * S + n] * A[base + n];
-        B[row * TW + x] = a;
-    }
-    __syncthreads();
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, m = (e / tw) % S, z = e / (tw * S);
-        float a = 0.0f;
-        for (int n = 0; n < S; n++) a += Q[m * S + n] * B[(z * S + n) * TW + x];
-        A[(z * S + m) * TW + x] = a;
-    }
-    __syncthreads();
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, y = (e / tw) % S, m = e / (tw * S);
-        float a = 0.0f;
-        for (int n = 0; n < S; n++) a += Q[m * S + n] * A[(n * S + y) * TW + x];
-        const int mx = x % S;
-        const float lam = L[mx] + L[y] + L[m];
-        B[(m * S + y) * TW + x] = (mx | y | m) ? a / (-lam) : 0.0f;
+    const int tw = nb * S;
+    {
+        // all of a thread's face loads in flight before the LDS stores
+        constexpr int NU = S * S * (TW + 1), NV = S * (S + 1) * TW, NW = (S + 1) * S * TW;
+        constexpr int KU = (NU + 255) / 256, KV = (NV + 255) / 256, KW = (NW + 255) / 256;
+        float tu[KU], tv[KV], tw_[KW];
+#pragma unroll
+        for (int q = 0; q < KU; q++) {
+            const int e = tid + 256 * q, x = e % (TW + 1), r = e / (TW + 1);
+            tu[q] = (e < NU && x <= tw) ? u[f_u(g, x0 + x, y0 + r % S, z0 + r / S)] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < KV; q++) {
+            const int e = tid + 256 * q, x = e % TW, r = e / TW;
+            tv[q] = (e < NV && x < tw) ? v[f_v(g, x0 + x, y0 + r % (S + 1), z0 + r / (S + 1))] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < KW; q++) {
+            const int e = tid + 256 * q, x = e % TW, r = e / TW;
+            tw_[q] = (e < NW && x < tw) ? w[f_w(g, x0 + x, y0 + r % S, z0 + r / S)] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < KU; q++) {
+            const int e = tid + 256 * q;
+            if (e < NU) Us[(e / (TW + 1)) / S][(e / (TW + 1)) % S][e % (TW + 1)] = tu[q];
+        }
+#pragma unroll
+        for (int q = 0; q < KV; q++) {
+            const int e = tid + 256 * q;
+            if (e < NV) Vs[(e / TW) / (S + 1)][(e / TW) % (S + 1)][e % TW] = tv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < KW; q++) {
+            const int e = tid + 256 * q;
+            if (e < NW) Ws[(e / TW) / S][(e / TW) % S][e % TW] = tw_[q];
+        }
     }
     __syncthreads();
-    // inverse transforms: z (B -> A), y (A -> B), x (B -> A)
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, y = (e / tw) % S, n = e / (tw * S);
-        float a = 0.0f;
-        for (int m = 0; m < S; m++) a += Q[m * S + n] * B[(m * S + y) * TW + x];
-        A[(n * S + y) * TW + x] = a;
+    const int b = tid / (S * S), l = tid % (S * S);
+    const bool act = b < nb;
+    const int la = l % S, lb = l / S;
+    float q[S][S];
+#pragma unroll
+    for (int m = 0; m < S; m++)
+#pragma unroll
+        for (int n = 0; n < S; n++) q[m][n] = tab.q[m * S + n];
+    float in[S], out[S];
+    // x lines (y = la, z = lb): divergence, forward transform -> A
+    if (act) {
+        const int xb = b * S;
+#pragma unroll
+        for (int x = 0; x < S; x++)
+            in[x] = Us[lb][la][xb + x + 1] - Us[lb][la][xb + x] + Vs[lb][la + 1][xb + x] - Vs[lb][la][xb + x] +
+                    Ws[lb + 1][la][xb + x] - Ws[lb][la][xb + x];
+#pragma unroll
+        for (int m = 0; m < S; m++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int n = 0; n < S; n++) a += q[m][n] * in[n];
+            A[rt_cell<S>(b, lb, la, m)] = a;
+        }
     }
     __syncthreads();
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, n = (e / tw) % S, z = e / (tw * S);
-        float a = 0.0f;
-        for (int m = 0; m < S; m++) a += Q[m * S + n] * A[(z * S + m) * TW + x];
-        B[(z * S + n) * TW + x] = a;
+    // y lines (mx = la, z = lb): forward -> B
+    if (act) {
+#pragma unroll
+        for (int n = 0; n < S; n++) in[n] = A[rt_cell<S>(b, lb, n, la)];
+#pragma unroll
+        for (int m = 0; m < S; m++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int n = 0; n < S; n++) a += q[m][n] * in[n];
+            B[rt_cell<S>(b, lb, m, la)] = a;
+        }
     }
     __syncthreads();
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, row = e / tw, n = x % S, base = row * TW + (x - n);
-        float a = 0.0f;
-        for (int m = 0; m < S; m++) a += Q[m * S + n] * B[base + m];
-        A[row * TW + x] = a;
+    // z lines (mx = la, my = lb): forward, divide by the eigenvalue, inverse -> A
+    if (act) {
+#pragma unroll
+        for (int n = 0; n < S; n++) in[n] = B[rt_cell<S>(b, n, lb, la)];
+        const float lxy = tab.lam[la] + tab.lam[lb];
+#pragma unroll
+        for (int m = 0; m < S; m++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int n = 0; n < S; n++) a += q[m][n] * in[n];
+            const float lam = lxy + tab.lam[m];
+            out[m] = (la | lb | m) ? a / (-lam) : 0.0f;
+        }
+#pragma unroll
+        for (int n = 0; n < S; n++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int m = 0; m < S; m++) a += q[m][n] * out[m];
+            A[rt_cell<S>(b, n, lb, la)] = a;
+        }
     }
     __syncthreads();
-    // interior faces of the regular blocks: u -= p_r - p_l
-    for (int e = tid; e < ncell; e += 256) {
-        const int x = e % tw, y = (e / tw) % S, z = e / (tw * S);
-        if (!reg[x / S]) continue;
-        const float pc = A[(z * S + y) * TW + x];
-        if (x % S) u[f_u(g, x0 + x, y0 + y, z0 + z)] = Us[z][y][x] - (pc - A[(z * S + y) * TW + x - 1]);
-        if (y) v[f_v(g, x0 + x, y0 + y, z0 + z)] = Vs[z][y][x] - (pc - A[(z * S + y - 1) * TW + x]);
-        if (z) w[f_w(g, x0 + x, y0 + y, z0 + z)] = Ws[z][y][x] - (pc - A[((z - 1) * S + y) * TW + x]);
+    // y lines inverse -> B
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < S; m++) in[m] = A[rt_cell<S>(b, lb, m, la)];
+#pragma unroll
+        for (int n = 0; n < S; n++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int m = 0; m < S; m++) a += q[m][n] * in[m];
+            B[rt_cell<S>(b, lb, n, la)] = a;
+        }
+    }
+    __syncthreads();
+    // x lines inverse: the pressure -> A
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < S; m++) in[m] = B[rt_cell<S>(b, lb, la, m)];
+#pragma unroll
+        for (int n = 0; n < S; n++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int m = 0; m < S; m++) a += q[m][n] * in[m];
+            A[rt_cell<S>(b, lb, la, n)] = a;
+        }
+    }
+    __syncthreads();
+    // interior faces of the regular blocks: face -= p_r - p_l, written in row order
+    for (int e = tid; e < S * S * (TW + 1); e += 256) {
+        const int x = e % (TW + 1), r = e / (TW + 1), y = r % S, z = r / S;
+        const int bx = x / S, xi = x % S;
+        if (xi == 0 || x >= tw || !reg[bx]) continue;
+        u[f_u(g, x0 + x, y0 + y, z0 + z)] =
+            Us[z][y][x] - (A[rt_cell<S>(bx, z, y, xi)] - A[rt_cell<S>(bx, z, y, xi - 1)]);
+    }
+    for (int e = tid; e < S * (S + 1) * TW; e += 256) {
+        const int x = e % TW, r = e / TW, y = r % (S + 1), z = r / (S + 1);
+        const int bx = x / S, xi = x % S;
+        if (y == 0 || y == S || x >= tw || !reg[bx]) continue;
+        v[f_v(g, x0 + x, y0 + y, z0 + z)] =
+            Vs[z][y][x] - (A[rt_cell<S>(bx, z, y, xi)] - A[rt_cell<S>(bx, z, y - 1, xi)]);
+    }
+    for (int e = tid; e < (S + 1) * S * TW; e += 256) {
+        const int x = e % TW, r = e / TW, y = r % S, z = r / S;
+        const int bx = x / S, xi = x % S;
+        if (z == 0 || z == S || x >= tw || !reg[bx]) continue;
+        w[f_w(g, x0 + x, y0 + y, z0 + z)] =
+            Ws[z][y][x] - (A[rt_cell<S>(bx, z, y, xi)] - A[rt_cell<S>(bx, z - 1, y, xi)]);
     }
 }
 
@@ -502,17 +675,26 @@ __global__ __launch_bounds__(((S * S * S + 63) / 64) * 64) void k_sub_solids(CG 
     __shared__ int redi[NT / 64];
     const int t = threadIdx.x;
     const int x0 = bi * S, y0 = bj * S, z0 = bk * S;
-    for (int e = t; e < S * S * (S + 1); e += NT) {
-        const int x = e % (S + 1), r = e / (S + 1);
-        Us[r / S][r % S][x] = u[f_u(g, x0 + x, y0 + r % S, z0 + r / S)];
-    }
-    for (int e = t; e < S * (S + 1) * S; e += NT) {
-        const int x = e % S, r = e / S;
-        Vs[r / (S + 1)][r % (S + 1)][x] = v[f_v(g, x0 + x, y0 + r % (S + 1), z0 + r / (S + 1))];
-    }
-    for (int e = t; e < (S + 1) * S * S; e += NT) {
-        const int x = e % S, r = e / S;
-        Ws[r / S][r % S][x] = w[f_w(g, x0 + x, y0 + r % S, z0 + r / S)];
+    {
+        constexpr int NF = S * S * (S + 1), KF = (NF + NT - 1) / NT;
+        float tu[KF], tv[KF], tw_[KF];
+#pragma unroll
+        for (int q = 0; q < KF; q++) {
+            const int e = t + NT * q;
+            const int xu = e % (S + 1), ru = e / (S + 1), xv = e % S, rv = e / S;
+            tu[q] = e < NF ? u[f_u(g, x0 + xu, y0 + ru % S, z0 + ru / S)] : 0.0f;
+            tv[q] = e < NF ? v[f_v(g, x0 + xv, y0 + rv % (S + 1), z0 + rv / (S + 1))] : 0.0f;
+            tw_[q] = e < NF ? w[f_w(g, x0 + xv, y0 + rv % S, z0 + rv / S)] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < KF; q++) {
+            const int e = t + NT * q;
+            if (e < NF) {
+                Us[(e / (S + 1)) / S][(e / (S + 1)) % S][e % (S + 1)] = tu[q];
+                Vs[(e / S) / (S + 1)][(e / S) % (S + 1)][e % S] = tv[q];
+                Ws[(e / S) / S][(e / S) % S][e % S] = tw_[q];
+            }
+        }
     }
     const int ci = t % S, cj = (t / S) % S, ck = t / (S * S);
     const bool me = t < N && !occ[f_c(g, x0 + ci, y0 + cj, z0 + ck)];
@@ -651,9 +833,9 @@ SubTables make_tables(int s) {
 
 template <int S>
 void sub_launch(const CG& g, const CoarseBufs& B, const uint8_t* occ, float* u, float* v, float* w, hipStream_t s) {
-    constexpr int NBX = RegTile<S>::NBX;
+    constexpr int NB = RegTile<S>::NB;
     const SubTables tab = make_tables(S);
-    const dim3 gr((unsigned)((g.Wc + NBX - 1) / NBX), (unsigned)g.Hc, (unsigned)g.Dc);
+    const dim3 gr((unsigned)((g.Wc + NB - 1) / NB), (unsigned)g.Hc, (unsigned)g.Dc);
     hipLaunchKernelGGL(k_sub_regular<S>, gr, dim3(256), 0, s, g, tab, B.regular, u, v, w);
     constexpr int NT = ((S * S * S + 63) / 64) * 64;
     hipLaunchKernelGGL(k_sub_solids<S>, dim3((unsigned)g.Wc, (unsigned)g.Hc, (unsigned)g.Dc), dim3(NT), 0, s, g,
@@ -678,6 +860,7 @@ void launch_coarse_restrict(const CoarseDims& cd, const float* u, const float* v
     hipLaunchKernelGGL(k_coarse_div, dim3(blocks_for(cd.Nc(), 256)), dim3(256), 0, s, g, (const float*)B.cu,
                        (const float*)B.cv, (const float*)B.cw, (const uint8_t*)B.cocc, B.div);
     launch_fill_f32(B.p, cd.Np(), 0.0f, s);
+    launch_fill_f32(B.p2, cd.Np(), 0.0f, s);
 }
 
 void launch_coarse_prepare(const CoarseDims& cd, const float* u, const float* v, const float* w, int open_mask,
@@ -690,35 +873,90 @@ void launch_coarse_prepare(const CoarseDims& cd, const float* u, const float* v,
     hipLaunchKernelGGL(k_coarse_div, dim3(blocks_for(cd.Nc(), 256)), dim3(256), 0, s, g, u, v, w,
                        (const uint8_t*)B.cocc, B.div);
     launch_fill_f32(B.p, cd.Np(), 0.0f, s);
+    launch_fill_f32(B.p2, cd.Np(), 0.0f, s);
 }
 
-void launch_coarse_sor(const CoarseDims& cd, const CoarseBufs& B, float omega, int iters, hipStream_t s) {
+constexpr size_t kSlabMaxLds = 160 * 1024;
+
+// LDS of the largest slab a schedule launches (the first workgroup's halo may be clipped, an interior
+// one's is not)
+size_t coarse_slab_lds(const CoarseDims& cd, CoarseSweep sw) {
+    if (sw.k <= 0) return 0;
+    size_t most = 0;
+    for (int z0 = 0; z0 < cd.Dc; z0 += sw.zc) {
+        const SlabLayout L = slab_layout(cd.Dc, z0, sw.k, sw.zc);
+        most = std::max(most, slab_floats(cd.Wc, cd.Hc, L.NP, L.NC) * sizeof(float));
+    }
+    return most;
+}
+
+bool slab_lds_enabled() {
+    static const bool ok =
+        hipFuncSetAttribute((const void*)k_coarse_slab, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSlabMaxLds) ==
+        hipSuccess;
+    return ok;
+}
+
+bool coarse_sweep_valid(const CoarseDims& cd, CoarseSweep sw) {
+    if (sw.k == 0) return true;
+    if (sw.k < 0 || sw.zc <= 0) return false;
+    const size_t lds = coarse_slab_lds(cd, sw);
+    return lds <= kSlabMaxLds && (lds <= 64 * 1024 || slab_lds_enabled());
+}
+
+const float* launch_coarse_sor(const CoarseDims& cd, const CoarseBufs& B, float omega, int iters, CoarseSweep sw,
+                               hipStream_t s) {
     const CG g = make_cg(cd);
-    const size_t n = (size_t)((cd.Wc + 1) / 2) * cd.Hc * cd.Dc;
     const float omw = 1.0f - omega;
-    for (int it = 0; it < iters; it++)
-        for (int c = 0; c < 2; c++)
-            hipLaunchKernelGGL(k_coarse_sweep, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, (const float*)B.rU,
-                               (const float*)B.rV, (const float*)B.rW, (const float*)B.inv, (const float*)B.div, B.p,
-                               c, omega, omw);
+    if (sw.k <= 0) {
+        const size_t n = (size_t)((cd.Wc + 1) / 2) * cd.Hc * cd.Dc;
+        for (int it = 0; it < iters; it++)
+            for (int c = 0; c < 2; c++)
+                hipLaunchKernelGGL(k_coarse_sweep, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, (const float*)B.rU,
+                                   (const float*)B.rV, (const float*)B.rW, (const float*)B.inv, (const float*)B.div,
+                                   B.p, c, omega, omw);
+        return B.p;
+    }
+    float* src = B.p;
+    float* dst = B.p2;
+    const unsigned nwg = (unsigned)((cd.Dc + sw.zc - 1) / sw.zc);
+    for (int done = 0; done < iters;) {
+        const int k = std::min(sw.k, iters - done);
+        const CoarseSweep cur{k, sw.zc};
+        hipLaunchKernelGGL(k_coarse_slab, dim3(nwg), dim3(1024), coarse_slab_lds(cd, cur), s, g, (const float*)B.rU,
+                           (const float*)B.rV, (const float*)B.rW, (const float*)B.inv, (const float*)B.div,
+                           (const float*)src, dst, omega, omw, k, sw.zc);
+        std::swap(src, dst);
+        done += k;
+    }
+    return src;
 }
 
-void launch_coarse_prolong(const CoarseDims& cd, const CoarseBufs& B, int open_mask, const uint8_t* occ, float* u,
-                           float* v, float* w, hipStream_t s) {
+int coarse_sweep_candidates(const CoarseDims& cd, CoarseSweep* out, int max) {
+    static const CoarseSweep all[] = {{0, 0}, {1, 1}, {1, 2}, {2, 1}, {2, 2}, {2, 4}, {3, 1}, {3, 2}, {4, 1}, {4, 2},
+                                      {4, 4}, {6, 1}, {8, 1}};
+    int n = 0;
+    for (const CoarseSweep& c : all)
+        if (n < max && coarse_sweep_valid(cd, c)) out[n++] = c;
+    return n;
+}
+
+void launch_coarse_prolong(const CoarseDims& cd, const CoarseBufs& B, const float* p, int open_mask, const uint8_t* occ,
+                           float* u, float* v, float* w, hipStream_t s) {
     const CG g = make_cg(cd);
     const size_t n = (size_t)(cd.Wc + 1) * cd.f.H * cd.f.D + (size_t)cd.f.W * (cd.Hc + 1) * cd.f.D +
                      (size_t)cd.f.W * cd.f.H * (cd.Dc + 1);
     hipLaunchKernelGGL(k_coarse_prolong, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, (const float*)B.rU,
                        (const float*)B.rV, (const float*)B.rW, (const float*)B.cu, (const float*)B.cv,
-                       (const float*)B.cw, (const float*)B.p, open_mask, occ, u, v, w);
+                       (const float*)B.cw, p, open_mask, occ, u, v, w);
 }
 
-void launch_coarse_correct(const CoarseDims& cd, const CoarseBufs& B, int open_mask, float* u, float* v, float* w,
-                           hipStream_t s) {
+void launch_coarse_correct(const CoarseDims& cd, const CoarseBufs& B, const float* p, int open_mask, float* u, float* v,
+                           float* w, hipStream_t s) {
     const CG g = make_cg(cd);
     const size_t nf = cd.Nu() + cd.Nv() + cd.Nw();
     hipLaunchKernelGGL(k_coarse_correct, dim3(blocks_for(nf, 256)), dim3(256), 0, s, g, (const float*)B.rU,
-                       (const float*)B.rV, (const float*)B.rW, (const float*)B.p, open_mask, u, v, w);
+                       (const float*)B.rV, (const float*)B.rW, p, open_mask, u, v, w);
 }
 
 void launch_sub_project(const CoarseDims& cd, const CoarseBufs& B, const uint8_t* occ, float* u, float* v, float* w,

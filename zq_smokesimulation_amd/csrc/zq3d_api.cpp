@@ -941,6 +941,28 @@ extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
     return t.done("zq3d_project");
 }
 
+extern "C" int zq3d_set_coarse_sweep(zq3d_sim* s, int k, int zc) {
+    const char* where = "zq3d_set_coarse_sweep";
+    if (!s) { set_err(where, "null sim"); return ZQ3D_ERR_ARG; }
+    if (s->cfg.sub_size <= 0) { set_err(where, "the simulation has no coarse grid (sub_size == 0)"); return ZQ3D_ERR_STATE; }
+    const CoarseSweep sw{k, zc};
+    if (k < 0 || (k > 0 && !coarse_sweep_valid(s->coarse.cd, sw))) {
+        set_err(where, "k must be 0, or k > 0 and zc > 0 with the slab's planes within 160 KiB of LDS");
+        return ZQ3D_ERR_ARG;
+    }
+    s->coarse.sweep = k == 0 ? CoarseSweep{0, 0} : sw;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_get_coarse_sweep(zq3d_sim* s, int* k, int* zc) {
+    const char* where = "zq3d_get_coarse_sweep";
+    if (!s || !k || !zc) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    if (s->cfg.sub_size <= 0) { set_err(where, "the simulation has no coarse grid (sub_size == 0)"); return ZQ3D_ERR_STATE; }
+    *k = s->coarse.sweep.k;
+    *zc = s->coarse.sweep.zc;
+    return ZQ3D_OK;
+}
+
 extern "C" int zq3d_advect_scalar(zq3d_sim* s) {
     if (!s) { set_err("zq3d_advect_scalar", "null sim"); return ZQ3D_ERR_ARG; }
     const zq3d_config& c = s->cfg;

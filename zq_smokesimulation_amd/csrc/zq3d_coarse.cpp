@@ -36,6 +36,7 @@ size_t coarse_layout(const CoarseDims& cd, char* base, CoarseBufs* B) {
     f(&o->inv, cd.Nc());
     f(&o->div, cd.Nc());
     f(&o->p, cd.Np());
+    f(&o->p2, cd.Np());
     b(&o->cocc, cd.Nc());
     b(&o->regular, cd.Nc());
     return off;
@@ -55,15 +56,52 @@ int check_block(const Dims& d, int s, const char* where) {
 
 }  // namespace
 
+// Times every valid sweep schedule (iters iterations each, HIP events on the null stream) on the
+// coarse arrays as they stand and keeps the fastest; all schedules give the same bits.
+int tune_coarse_sweep(CoarseState* c, int iters, const char* where) {
+    CoarseSweep cand[16];
+    const int n = coarse_sweep_candidates(c->cd, cand, 16);
+    hipEvent_t a = nullptr, b = nullptr;
+    ZQ_HIP(hipEventCreate(&a), where);
+    ZQ_HIP(hipEventCreate(&b), where);
+    float best = 0.0f;
+    int bi = 0;
+    for (int i = 0; i < n; i++) {
+        (void)launch_coarse_sor(c->cd, c->B, 1.0f, 2, cand[i], nullptr);  // first-launch costs
+        (void)hipEventRecord(a, nullptr);
+        (void)launch_coarse_sor(c->cd, c->B, 1.0f, iters, cand[i], nullptr);
+        (void)hipEventRecord(b, nullptr);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (i == 0 || ms < best) {
+            best = ms;
+            bi = i;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ZQ_HIP(hipGetLastError(), where);
+    c->sweep = cand[bi];
+    return ZQ3D_OK;
+}
+
 int coarse_create(const Dims& d, int s, CoarseState* out, const char* where) {
     if (int rc = check_block(d, s, where)) return rc;
     out->cd = coarse_dims(d, s);
     const size_t bytes = coarse_layout(out->cd, nullptr, nullptr);
     ZQ_HIP(hipMalloc(&out->mem, bytes), where);
     coarse_layout(out->cd, (char*)out->mem, &out->B);
+    // a fluid coarse grid with unit fractions to time the schedules on
+    launch_fill_f32(out->B.rU, out->cd.Nu(), 1.0f, nullptr);
+    launch_fill_f32(out->B.rV, out->cd.Nv(), 1.0f, nullptr);
+    launch_fill_f32(out->B.rW, out->cd.Nw(), 1.0f, nullptr);
+    launch_fill_f32(out->B.inv, out->cd.Nc(), 1.0f / 6.0f, nullptr);
+    launch_fill_f32(out->B.div, out->cd.Nc(), 0.0f, nullptr);
     launch_fill_f32(out->B.p, out->cd.Np(), 0.0f, nullptr);
+    launch_fill_f32(out->B.p2, out->cd.Np(), 0.0f, nullptr);
     ZQ_HIP(hipStreamSynchronize(nullptr), where);
-    return ZQ3D_OK;
+    return tune_coarse_sweep(out, 20, where);
 }
 
 void coarse_destroy(CoarseState* c) {
@@ -78,16 +116,16 @@ int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uin
     launch_coarse_setup(c.cd, occ, o.open_mask, c.B, s);
     launch_coarse_restrict(c.cd, u, v, w, c.B, s);
     if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
-    launch_coarse_sor(c.cd, c.B, o.omega, o.iters, s);
+    const float* p = launch_coarse_sor(c.cd, c.B, o.omega, o.iters, c.sweep, s);
     if (pf) {
         ZQ_HIP(hipEventRecord(pf->b, s), where);
         ZQ_HIP(hipEventSynchronize(pf->b), where);
         float ms = 0.0f;
         ZQ_HIP(hipEventElapsedTime(&ms, pf->a, pf->b), where);
         pf->ms += ms;
-        pf->launches += 2LL * o.iters;
+        pf->launches += c.sweep.k > 0 ? (o.iters + c.sweep.k - 1) / c.sweep.k : 2LL * o.iters;
     }
-    launch_coarse_prolong(c.cd, c.B, o.open_mask, occ, u, v, w, s);
+    launch_coarse_prolong(c.cd, c.B, p, o.open_mask, occ, u, v, w, s);
     if (c.cd.s > 1) launch_sub_project(c.cd, c.B, occ, u, v, w, s);
     ZQ_HIP(hipGetLastError(), where);
     if (st) {
@@ -187,8 +225,9 @@ extern "C" float SolveOpenPoissonRedBlackwithFaceRatio3D_MAC(float* mac_u, float
     E(hipMemcpy(B.rW, unoccupyW, cd.Nw() * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(B.cocc, occupy, cd.Nc(), hipMemcpyHostToDevice));
     launch_coarse_prepare(cd, B.cu, B.cv, B.cw, ZQ3D_OPEN_ALL, B, 0);
-    launch_coarse_sor(cd, B, 1.0f, maxIter, 0);
-    launch_coarse_correct(cd, B, ZQ3D_OPEN_ALL, B.cu, B.cv, B.cw, 0);
+    const CoarseSweep sw = coarse_sweep_valid(cd, CoarseSweep{2, 1}) ? CoarseSweep{2, 1} : CoarseSweep{0, 0};
+    const float* p = launch_coarse_sor(cd, B, 1.0f, maxIter, sw, 0);
+    launch_coarse_correct(cd, B, p, ZQ3D_OPEN_ALL, B.cu, B.cv, B.cw, 0);
     E(hipGetLastError());
     E(hipMemcpy(mac_u, B.cu, cd.Nu() * 4, hipMemcpyDeviceToHost));
     E(hipMemcpy(mac_v, B.cv, cd.Nv() * 4, hipMemcpyDeviceToHost));

@@ -37,6 +37,7 @@ struct CoarseBufs {
     float* inv;              // 1/diagonal of the face-fraction operator per coarse cell (0: not updated)
     float* div;              // coarse divergence
     float* p;                // padded coarse pressure (zero ghost layer)
+    float* p2;               // second pressure the multi-iteration sweep ping-pongs with
     uint8_t* cocc;           // coarse occupancy
     uint8_t* regular;        // 1: the block holds no solid fine cell
 };
@@ -44,21 +45,30 @@ struct CoarseBufs {
 // fractions, coarse occupancy, 1/diagonal (open_mask: open domain faces) and block flags from the
 // fine occupancy
 void launch_coarse_setup(const CoarseDims& g, const uint8_t* occ, int open_mask, const CoarseBufs& B, hipStream_t s);
-// coarse faces = mean of their s x s fine sub-faces; coarse divergence; p = 0
+// coarse faces = mean of their s x s fine sub-faces; coarse divergence; p = p2 = 0
 void launch_coarse_restrict(const CoarseDims& g, const float* u, const float* v, const float* w, const CoarseBufs& B,
                             hipStream_t s);
-// `iters` red-black iterations on B.p (colour 0 first)
-void launch_coarse_sor(const CoarseDims& g, const CoarseBufs& B, float omega, int iters, hipStream_t s);
-// adds the coarse correction to the fine faces on coarse planes (fine cells beside them fluid)
-void launch_coarse_prolong(const CoarseDims& g, const CoarseBufs& B, int open_mask, const uint8_t* occ, float* u,
-                           float* v, float* w, hipStream_t s);
+// Sweep schedule of the coarse SOR: k == 0 launches each colour separately; k > 0 runs k red-black
+// iterations per launch on slabs of zc z-planes (halo planes recomputed).  All give the same bits.
+struct CoarseSweep {
+    int k, zc;
+};
+bool coarse_sweep_valid(const CoarseDims& g, CoarseSweep sw);
+// schedules valid for the grid (LDS of a slab within 160 KiB); returns the count
+int coarse_sweep_candidates(const CoarseDims& g, CoarseSweep* out, int max);
+// `iters` red-black iterations from B.p (colour 0 first); returns B.p or B.p2, whichever holds the result
+const float* launch_coarse_sor(const CoarseDims& g, const CoarseBufs& B, float omega, int iters, CoarseSweep sw,
+                               hipStream_t s);
+// adds the coarse correction (pressure p) to the fine faces on coarse planes (fine cells beside them fluid)
+void launch_coarse_prolong(const CoarseDims& g, const CoarseBufs& B, const float* p, int open_mask, const uint8_t* occ,
+                           float* u, float* v, float* w, hipStream_t s);
 // projects every block's interior faces; s in [2, COARSE_MAX_S]
 void launch_sub_project(const CoarseDims& g, const CoarseBufs& B, const uint8_t* occ, float* u, float* v, float* w,
                         hipStream_t s);
 // u -= grad p on the coarse faces with fraction > 0 (domain faces: open ones), in place
-void launch_coarse_correct(const CoarseDims& g, const CoarseBufs& B, int open_mask, float* u, float* v, float* w,
-                           hipStream_t s);
-// coarse divergence, 1/diagonal and p = 0 from caller-given coarse faces, occupancy and fractions
+void launch_coarse_correct(const CoarseDims& g, const CoarseBufs& B, const float* p, int open_mask, float* u, float* v,
+                           float* w, hipStream_t s);
+// coarse divergence, 1/diagonal and p = p2 = 0 from caller-given coarse faces, occupancy and fractions
 void launch_coarse_prepare(const CoarseDims& g, const float* u, const float* v, const float* w, int open_mask,
                            const CoarseBufs& B, hipStream_t s);
 // out (rows x n) = M (rows x cols) * in (cols x n), each output summed over k = 0..cols-1 in order

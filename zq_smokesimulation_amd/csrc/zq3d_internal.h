@@ -103,6 +103,7 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
 struct CoarseState {
     CoarseDims cd{};
     CoarseBufs B{};
+    CoarseSweep sweep{0, 0};  // schedule of the coarse SOR (timed at creation)
     void* mem = nullptr;
 };
 int coarse_create(const Dims& d, int s, CoarseState* out, const char* where);

@@ -175,5 +175,14 @@ class SmokeSim3D:
     def set_sor_shape(self, ty, zc):
         check(_lib.lib().zq3d_set_sor_shape(self._h, int(ty), int(zc)), "set_sor_shape")
 
+    def set_coarse_sweep(self, k, zc):
+        """Coarse SOR schedule of the coarse + sub-grid projection (k == 0: one launch per colour)."""
+        check(_lib.lib().zq3d_set_coarse_sweep(self._h, int(k), int(zc)), "set_coarse_sweep")
+
+    def coarse_sweep(self):
+        k, zc = C.c_int(), C.c_int()
+        check(_lib.lib().zq3d_get_coarse_sweep(self._h, C.byref(k), C.byref(zc)), "coarse_sweep")
+        return k.value, zc.value
+
     def set_stream(self, stream_handle):
         check(_lib.lib().zq3d_set_stream(self._h, stream_handle), "set_stream")
