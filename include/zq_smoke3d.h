@@ -46,7 +46,9 @@ extern "C" {
 /* replaces ZQ_CUDA_Advection3D.h:23 -- stores the grid and step constants used by the next calls */
 void ZQ_Cuda_Prepare_Advection(const unsigned int width, const unsigned int height, const unsigned int depth,
                                const float voxelSize, const unsigned int steps, const float deltatt);
-/* replaces ZQ_CUDA_Advection3D.h:31 -- u,v,w updated in place.  BFECC modes return -1 (not built). */
+/* replaces ZQ_CUDA_Advection3D.h:31 -- u,v,w updated in place, all six modes (the BFECC ones as
+ * ZQ_CUDA_Advection3D.cu:1110-1182, 1240-1336, 1389-1474, including the MAC variant leaving the
+ * corrected field on faces its final apply skips). */
 float Advect_Velocity(float* u, float* v, float* w, const bool* occupy, bool is_open, enum AdvectVelocityType type);
 /* replaces ZQ_CUDA_Advection3D.h:43-44.  The BFECC modes leave the outputs zero, as the
  * reference's switch does (ZQ_CUDA_Advection3D.cu:1782-1794). */

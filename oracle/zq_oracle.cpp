@@ -380,66 +380,154 @@ void orc_backtrace(int W, int H, int D, float h, int steps, float dtt, const flo
     backtrace(d, V, occ, h, steps, dtt, px, py, pz, out3[0], out3[1], out3[2]);
 }
 
-/* Advect_Velocity (ZQ_CUDA_Advection3D.cu:1686-1749) with the non-BFECC modes:
+/* Advect_Velocity (ZQ_CUDA_Advection3D.cu:1686-1749), non-BFECC modes:
  *  inReg_outReg (:76-113, 1061-1108): MAC->cell float4, backtrace per cell, sample the cell grid
  *      at lastpos, cell->MAC average, masked apply;
  *  inMAC_outMAC (:232-377, 1184-1238): per face, backtrace through the staggered grids, sample
  *      the face's own component;
  *  inReg_outMAC (:115-230, 1338-1387): per face, backtrace through the cell grid, sample the
- *      cell grid's matching component. */
+ *      cell grid's matching component.
+ * The BFECC modes compose these (below). */
+namespace {
+
+/* Input_Increment_Kernel (ZQ_CUDA_Advection3D.cu:56-64): input*1.5 - input_star*0.5 with double
+ * literals, so in double, rounded once to float.  The 4-channel kernel (:66-74) goes through float4
+ * operators of the absent ZQlibCudaDefines.cuh; the same double expression is used for it (equal to
+ * a contracted float evaluation except in double-rounding corner cases: parity unpinned there). */
+inline float bfecc_inc(float a, float b) { return (float)((double)a * 1.5 - (double)b * 0.5); }
+
+/* Cells: backtrace from the centre with V, sample the regular input grid In at the end point
+ * (Advect_Velocity_inRegular_outRegular_Kernel, :76-113; input and velocity textures differ under BFECC). */
+void adv_cells(const Dims& d, const VelSampler& V, const VelSampler& In, const uint8_t* occ, float h, int steps,
+               float dtt, float* out4) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < d.D; k++)
+        for (int j = 0; j < d.H; j++)
+            for (int i = 0; i < d.W; i++) {
+                float lx, ly, lz, sx, sy, sz;
+                backtrace(d, V, occ, h, steps, dtt, i + 0.5f, j + 0.5f, k + 0.5f, lx, ly, lz);
+                In.sample(lx, ly, lz, sx, sy, sz);
+                const size_t o = d.c(i, j, k);
+                out4[o * 4 + 0] = sx; out4[o * 4 + 1] = sy; out4[o * 4 + 2] = sz; out4[o * 4 + 3] = 0.0f;
+            }
+}
+
+/* Faces: backtrace from each face centre with V, the face's component of In at the end point
+ * (Advect_Velocity_inRegular_outMAC_*, :115-230; inMAC_outMAC_*, :232-377). */
+void adv_faces(const Dims& d, const VelSampler& V, const VelSampler& In, const uint8_t* occ, float h, int steps,
+               float dtt, float* ou, float* ov, float* ow) {
+    const int W = d.W, H = d.H, D = d.D;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < D; k++)
+        for (int j = 0; j < H; j++)
+            for (int i = 0; i <= W; i++) {
+                float lx, ly, lz, sx, sy, sz;
+                backtrace(d, V, occ, h, steps, dtt, (float)i, j + 0.5f, k + 0.5f, lx, ly, lz);
+                In.sample(lx, ly, lz, sx, sy, sz);
+                ou[d.fu(i, j, k)] = sx;
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < D; k++)
+        for (int j = 0; j <= H; j++)
+            for (int i = 0; i < W; i++) {
+                float lx, ly, lz, sx, sy, sz;
+                backtrace(d, V, occ, h, steps, dtt, i + 0.5f, (float)j, k + 0.5f, lx, ly, lz);
+                In.sample(lx, ly, lz, sx, sy, sz);
+                ov[d.fv(i, j, k)] = sy;
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k <= D; k++)
+        for (int j = 0; j < H; j++)
+            for (int i = 0; i < W; i++) {
+                float lx, ly, lz, sx, sy, sz;
+                backtrace(d, V, occ, h, steps, dtt, i + 0.5f, j + 0.5f, (float)k, lx, ly, lz);
+                In.sample(lx, ly, lz, sx, sy, sz);
+                ow[d.fw(i, j, k)] = sz;
+            }
+}
+
+std::vector<float> negated(const float* a, size_t n) {
+    std::vector<float> r(n);
+    for (size_t i = 0; i < n; i++) r[i] = -a[i];
+    return r;
+}
+
+}  // namespace
+
+/* Advect_Velocity (ZQ_CUDA_Advection3D.cu:1696-1742).  The BFECC modes follow
+ * cu_Advect_Velocity_inRegular_outRegular_BFECC (:1110-1182), ..._inMAC_outMAC_BFECC (:1240-1336) and
+ * ..._inRegular_outMAC_BFECC (:1389-1474): advect phi with v, advect that with -v (the velocity
+ * negated, :997-1019), phi~ = 1.5 phi - 0.5 phi^ (:56-74), advect phi~ with v, apply.  In the MAC
+ * variant phi is the caller's faces, overwritten by phi~, so faces the final apply skips keep phi~. */
 int orc_advect_velocity(int W, int H, int D, float h, int steps, float dtt, float* u, float* v, float* w,
                         const uint8_t* occ, int is_open, int type) {
     Dims d{W, H, D};
-    std::vector<float> ou(d.Nu(), 0.0f), ov(d.Nv(), 0.0f), ow(d.Nw(), 0.0f);
-    if (type == ORC_ADV_VEL_INREG_OUTREG) {
-        std::vector<float> vel4(d.N() * 4), adv4(d.N() * 4);
+    const size_t N = d.N(), Nu = d.Nu(), Nv = d.Nv(), Nw = d.Nw();
+    std::vector<float> ou(Nu, 0.0f), ov(Nv, 0.0f), ow(Nw, 0.0f);
+    switch (type) {
+    case ORC_ADV_VEL_INREG_OUTREG: {
+        std::vector<float> vel4(N * 4), adv4(N * 4);
         orc_mac_to_cell4(W, H, D, u, v, w, vel4.data());
-        VelSampler V = make_regular_sampler(d, vel4.data());
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < D; k++)
-            for (int j = 0; j < H; j++)
-                for (int i = 0; i < W; i++) {
-                    float lx, ly, lz;
-                    backtrace(d, V, occ, h, steps, dtt, i + 0.5f, j + 0.5f, k + 0.5f, lx, ly, lz);
-                    float sx, sy, sz;
-                    V.sample(lx, ly, lz, sx, sy, sz);
-                    size_t o = d.c(i, j, k);
-                    adv4[o * 4 + 0] = sx; adv4[o * 4 + 1] = sy; adv4[o * 4 + 2] = sz; adv4[o * 4 + 3] = 0.0f;
-                }
+        const VelSampler V = make_regular_sampler(d, vel4.data());
+        adv_cells(d, V, V, occ, h, steps, dtt, adv4.data());
         orc_cell4_to_mac(W, H, D, adv4.data(), ou.data(), ov.data(), ow.data());
-    } else if (type == ORC_ADV_VEL_INMAC_OUTMAC || type == ORC_ADV_VEL_INREG_OUTMAC) {
-        std::vector<float> vel4;
-        VelSampler V;
-        if (type == ORC_ADV_VEL_INMAC_OUTMAC) V = make_mac_sampler(d, u, v, w);
-        else { vel4.resize(d.N() * 4); orc_mac_to_cell4(W, H, D, u, v, w, vel4.data()); V = make_regular_sampler(d, vel4.data()); }
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < D; k++)
-            for (int j = 0; j < H; j++)
-                for (int i = 0; i <= W; i++) {
-                    float lx, ly, lz, sx, sy, sz;
-                    backtrace(d, V, occ, h, steps, dtt, (float)i, j + 0.5f, k + 0.5f, lx, ly, lz);
-                    V.sample(lx, ly, lz, sx, sy, sz);
-                    ou[d.fu(i, j, k)] = sx;
-                }
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < D; k++)
-            for (int j = 0; j <= H; j++)
-                for (int i = 0; i < W; i++) {
-                    float lx, ly, lz, sx, sy, sz;
-                    backtrace(d, V, occ, h, steps, dtt, i + 0.5f, (float)j, k + 0.5f, lx, ly, lz);
-                    V.sample(lx, ly, lz, sx, sy, sz);
-                    ov[d.fv(i, j, k)] = sy;
-                }
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k <= D; k++)
-            for (int j = 0; j < H; j++)
-                for (int i = 0; i < W; i++) {
-                    float lx, ly, lz, sx, sy, sz;
-                    backtrace(d, V, occ, h, steps, dtt, i + 0.5f, j + 0.5f, (float)k, lx, ly, lz);
-                    V.sample(lx, ly, lz, sx, sy, sz);
-                    ow[d.fw(i, j, k)] = sz;
-                }
-    } else {
+        break;
+    }
+    case ORC_ADV_VEL_INREG_OUTREG_BFECC: {
+        std::vector<float> vel4(N * 4), a4(N * 4), b4(N * 4);
+        orc_mac_to_cell4(W, H, D, u, v, w, vel4.data());
+        const VelSampler V = make_regular_sampler(d, vel4.data());
+        adv_cells(d, V, V, occ, h, steps, dtt, a4.data());
+        const std::vector<float> nvel = negated(vel4.data(), N * 4);
+        adv_cells(d, make_regular_sampler(d, nvel.data()), make_regular_sampler(d, a4.data()), occ, h, steps, dtt,
+                  b4.data());
+        for (size_t i = 0; i < N * 4; i++) b4[i] = bfecc_inc(vel4[i], b4[i]);
+        adv_cells(d, V, make_regular_sampler(d, b4.data()), occ, h, steps, dtt, a4.data());
+        orc_cell4_to_mac(W, H, D, a4.data(), ou.data(), ov.data(), ow.data());
+        break;
+    }
+    case ORC_ADV_VEL_INMAC_OUTMAC: {
+        const VelSampler V = make_mac_sampler(d, u, v, w);
+        adv_faces(d, V, V, occ, h, steps, dtt, ou.data(), ov.data(), ow.data());
+        break;
+    }
+    case ORC_ADV_VEL_INMAC_OUTMAC_BFECC: {
+        const VelSampler V = make_mac_sampler(d, u, v, w);
+        std::vector<float> fu(Nu), fv(Nv), fw(Nw);
+        adv_faces(d, V, V, occ, h, steps, dtt, fu.data(), fv.data(), fw.data());
+        const std::vector<float> nu = negated(u, Nu), nv = negated(v, Nv), nw = negated(w, Nw);
+        adv_faces(d, make_mac_sampler(d, nu.data(), nv.data(), nw.data()), make_mac_sampler(d, fu.data(), fv.data(), fw.data()),
+                  occ, h, steps, dtt, ou.data(), ov.data(), ow.data());
+        const std::vector<float> su(u, u + Nu), sv(v, v + Nv), sw(w, w + Nw);  // the velocity texture, before the increment
+        for (size_t i = 0; i < Nu; i++) u[i] = bfecc_inc(u[i], ou[i]);
+        for (size_t i = 0; i < Nv; i++) v[i] = bfecc_inc(v[i], ov[i]);
+        for (size_t i = 0; i < Nw; i++) w[i] = bfecc_inc(w[i], ow[i]);
+        adv_faces(d, make_mac_sampler(d, su.data(), sv.data(), sw.data()), make_mac_sampler(d, u, v, w), occ, h, steps,
+                  dtt, ou.data(), ov.data(), ow.data());
+        break;
+    }
+    case ORC_ADV_VEL_INREG_OUTMAC: {
+        std::vector<float> vel4(N * 4);
+        orc_mac_to_cell4(W, H, D, u, v, w, vel4.data());
+        const VelSampler V = make_regular_sampler(d, vel4.data());
+        adv_faces(d, V, V, occ, h, steps, dtt, ou.data(), ov.data(), ow.data());
+        break;
+    }
+    case ORC_ADV_VEL_INREG_OUTMAC_BFECC: {
+        std::vector<float> vel4(N * 4), c4(N * 4);
+        orc_mac_to_cell4(W, H, D, u, v, w, vel4.data());
+        const VelSampler V = make_regular_sampler(d, vel4.data());
+        adv_faces(d, V, V, occ, h, steps, dtt, ou.data(), ov.data(), ow.data());
+        orc_mac_to_cell4(W, H, D, ou.data(), ov.data(), ow.data(), c4.data());
+        const std::vector<float> nvel = negated(vel4.data(), N * 4);
+        adv_faces(d, make_regular_sampler(d, nvel.data()), make_regular_sampler(d, c4.data()), occ, h, steps, dtt,
+                  ou.data(), ov.data(), ow.data());
+        orc_mac_to_cell4(W, H, D, ou.data(), ov.data(), ow.data(), c4.data());
+        for (size_t i = 0; i < N * 4; i++) c4[i] = bfecc_inc(vel4[i], c4[i]);
+        adv_faces(d, V, make_regular_sampler(d, c4.data()), occ, h, steps, dtt, ou.data(), ov.data(), ow.data());
+        break;
+    }
+    default:
         return -1;
     }
     apply_advect_result(d, occ, is_open, ou.data(), ov.data(), ow.data(), u, v, w);

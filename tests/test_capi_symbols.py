@@ -61,7 +61,15 @@ def test_bad_arguments_fail_without_gpu():
     assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
     assert "dimensions" in L.last_error()
     L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
-    cfg.adv_vel_type = L.ADV_VEL_INREG_OUTREG_BFECC
+    cfg.adv_vel_type = 7
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.sub_size = 3  # does not divide 8
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    assert "sub_size" in L.last_error()
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.sub_size = 4
+    cfg.sor.mixed_precision = 1
     assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
     L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
     cfg.sor.omega = 2.5

@@ -110,6 +110,17 @@ def test_cli_coarse_projection(tmp_path, resident, solver):
     assert want[2].max() > 0.1
 
 
+@pytest.mark.parametrize("resident", ["false", "true"])
+@pytest.mark.parametrize("adv", ["inReg_outReg_BFECC", "inMAC_outMAC_BFECC", "inReg_outMAC_BFECC"])
+def test_cli_bfecc_advection_open(tmp_path, resident, adv):
+    text = BASE.replace("inReg_outReg", adv) + \
+        "frameCount 2\nintObj sphere\nglobalGridSolver TOTAL_OPEN_POISSON_SOR\ndeviceResident %s\n" % resident
+    got = run_cli(tmp_path, text, 2)
+    want = oracle_frames(str(tmp_path / "scene.txt"), 2)
+    for f in range(2):
+        np.testing.assert_array_equal(got[f], want[f], err_msg="frame %d" % f)
+
+
 def test_cli_mac_advection_open(tmp_path):
     text = BASE.replace("inReg_outReg", "inMAC_outMAC") + \
         "frameCount 2\nintObj null\nglobalGridSolver TOTAL_OPEN_POISSON_SOR\ndeviceResident false\n"

@@ -67,7 +67,7 @@ def test_mac_to_regular(zs, rng, shape):
     assert_same(zs.mac_to_regular4(u, v, w), O.mac_to_cell4(u, v, w), "vel4")
 
 
-@pytest.mark.parametrize("mode", [0, 2, 4])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_advect_velocity(zs, rng, shape, mode):
     W, H, D = shape
@@ -84,17 +84,21 @@ def test_advect_velocity(zs, rng, shape, mode):
         assert_same(a, b, "advect mode %d %s" % (mode, n))
 
 
-def test_advect_velocity_closed_apply(zs, rng):
+@pytest.mark.parametrize("mode", [0, 1, 3, 5])
+def test_advect_velocity_closed_apply(zs, rng, mode):
+    """Closed apply; for the MAC BFECC mode this includes the corrected field the reference leaves on
+    the faces its final apply skips."""
     W, H, D = 20, 18, 12
     h = 1.0 / W
+    print("advect closed", mode, flush=True)
     u, v, w = faces(rng, W, H, D, 0.4)
     occ = occupancy(rng, W, H, D, 0.1, shell=False)
     ref = copies(u, v, w)
-    O.advect_velocity(*ref, occ, h, 5, 0.01, 0, False)
+    O.advect_velocity(*ref, occ, h, 5, 0.01, mode, False)
     zs.prepare_advection(W, H, D, h, 5, 0.01)
-    zs.advect_velocity(u, v, w, occ, 0, False)
+    zs.advect_velocity(u, v, w, occ, mode, False)
     for a, b, n in zip((u, v, w), ref, "uvw"):
-        assert_same(a, b, "advect closed " + n)
+        assert_same(a, b, "advect closed %d %s" % (mode, n))
 
 
 @pytest.mark.parametrize("mode", [0, 2])
@@ -127,14 +131,14 @@ def test_advect_scalar_bfecc_zeroes(zs, rng):
         assert not T2.any() and not R2.any()
 
 
-def test_advect_bfecc_velocity_rejected(zs, rng):
+def test_advect_unknown_velocity_mode_rejected(zs, rng):
     import zq_smokesimulation_amd as z
     W, H, D = 8, 8, 8
     u, v, w = faces(rng, W, H, D)
     occ = np.zeros((D, H, W), np.uint8)
     zs.prepare_advection(W, H, D, 1.0 / W, 4, 0.01)
     with pytest.raises(z.ZQ3DError):
-        zs.advect_velocity(u, v, w, occ, 1, True)
+        zs.advect_velocity(u, v, w, occ, 6, True)
 
 
 @pytest.mark.parametrize("shape", SHAPES[:2])

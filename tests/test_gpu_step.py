@@ -69,6 +69,18 @@ def test_step_open_bit_exact():
         assert bad.size == 0, "%s: %d cells differ (max |d| %g)" % (name, bad.size, np.abs(a - b).max())
 
 
+@pytest.mark.parametrize("mode", [1, 3, 5])
+def test_step_open_bfecc_bit_exact(mode):
+    """Whole open-domain steps with each BFECC velocity mode (extra device scratch in the simulation)."""
+    W, H, D = 20, 16, 18
+    p = oracle_params(W, open_mask=O.ALL_FACES, closed_mean=0, sor_iters=40, adv_vel_type=mode)
+    ref, g, _, _ = run_both(W, H, D, p, 3, 0, SRC)
+    for name, a in zip(("u", "v", "w", "T", "rho"), ref):
+        b = g[name]
+        bad = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32))
+        assert bad.size == 0, "mode %d %s: %d cells differ (max |d| %g)" % (mode, name, bad.size, np.abs(a - b).max())
+
+
 def test_step_closed_config0_like():
     """Closed shell, alpha/beta buoyancy, confinement, 200 SOR iterations (config 0 at 32^3)."""
     W = H = D = 32

@@ -147,3 +147,67 @@ def test_mac_cell_roundtrip_linear():
     u2, _, _ = O.cell4_to_mac(vel4)
     assert np.array_equal(u2[:, :, 1:-1], u[:, :, 1:-1])
     assert np.array_equal(u2[:, :, 0], vel4[:, :, 0, 0]) and np.array_equal(u2[:, :, -1], vel4[:, :, -1, 0])
+
+
+# ---------------------------------------------------------------- BFECC velocity modes
+BFECC = [O.ADV_VEL_INREG_OUTREG_BFECC, O.ADV_VEL_INMAC_OUTMAC_BFECC, O.ADV_VEL_INREG_OUTMAC_BFECC]
+PLAIN_OF = {O.ADV_VEL_INREG_OUTREG_BFECC: O.ADV_VEL_INREG_OUTREG, O.ADV_VEL_INMAC_OUTMAC_BFECC: O.ADV_VEL_INMAC_OUTMAC,
+            O.ADV_VEL_INREG_OUTMAC_BFECC: O.ADV_VEL_INREG_OUTMAC}
+
+
+@pytest.mark.parametrize("mode", BFECC)
+def test_bfecc_zero_and_uniform_velocity_are_fixed_points(rng, mode):
+    """Zero velocity: nothing moves; a uniform velocity is a steady solution, so every face keeps it
+    (1.5 c - 0.5 c = c: the increment's weights sum to one)."""
+    W, H, D = 9, 7, 6
+    occ = (rng.random((D, H, W)) < 0.15).astype(np.uint8)
+    for c in ((0.0, 0.0, 0.0), (0.7, -0.4, 0.25)):
+        u, v, w = _uniform(W, H, D, c)
+        u2, v2, w2 = u.copy(), v.copy(), w.copy()
+        O.advect_velocity(u2, v2, w2, occ, 1.0 / W, 6, 0.05, mode, is_open=True)
+        np.testing.assert_allclose(u2, u, rtol=0, atol=1e-6)
+        np.testing.assert_allclose(v2, v, rtol=0, atol=1e-6)
+        np.testing.assert_allclose(w2, w, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("mode", BFECC)
+def test_bfecc_is_more_accurate_than_the_plain_mode(mode):
+    """u = U0 + eps sin(2 pi x / L) along x (v = w = 0): one step is a translation of the ripple by
+    (steps-1) U0 dtt / h cells up to O(eps^2).  Linear interpolation damps the ripple; BFECC's error
+    compensation must leave a clearly smaller error than the plain mode (a sign slip in the back
+    advection or in the increment makes it larger).  The regular-grid pair also averages faces to
+    cells and back, which scales the ripple by cos(pi/L)^2 in both modes; that is its target."""
+    W, H, D = 64, 6, 6
+    L, U0, eps = 8.0, 1.0, 0.01
+    h, steps, dtt = 1.0 / W, 5, 0.37 / (4 * W)   # shift 0.37 cells
+    shift = (steps - 1) * U0 * dtt / h
+    x = np.arange(W + 1, dtype=np.float64)
+    prof = U0 + eps * np.sin(2 * np.pi * x / L)
+    damp = np.cos(np.pi / L) ** 2 if mode == O.ADV_VEL_INREG_OUTREG_BFECC else 1.0
+    want = U0 + eps * damp * np.sin(2 * np.pi * (x - shift) / L)
+    occ = np.zeros((D, H, W), np.uint8)
+    errs = {}
+    for m in (mode, PLAIN_OF[mode]):
+        u = np.broadcast_to(prof.astype(np.float32), (D, H, W + 1)).copy()
+        v = np.zeros((D, H + 1, W), np.float32)
+        w = np.zeros((D + 1, H, W), np.float32)
+        O.advect_velocity(u, v, w, occ, h, steps, dtt, m, is_open=True)
+        errs[m] = np.abs(u[:, :, 16:48] - want[16:48]).max()
+    assert errs[mode] < 0.5 * errs[PLAIN_OF[mode]], errs
+
+
+def test_mac_bfecc_leaves_the_corrected_field_on_skipped_faces(rng):
+    """The MAC variant overwrites the caller's faces with phi~ = 1.5 phi - 0.5 phi^ before the final
+    apply (ZQ_CUDA_Advection3D.cu:1301-1307), so closed domain faces (never applied) change, while the
+    other modes leave them alone."""
+    W, H, D = 10, 8, 7
+    u, v, w = faces(rng, W, H, D, 0.5)
+    occ = np.zeros((D, H, W), np.uint8)
+    out = {}
+    for m in (O.ADV_VEL_INMAC_OUTMAC_BFECC, O.ADV_VEL_INMAC_OUTMAC, O.ADV_VEL_INREG_OUTMAC_BFECC):
+        u2, v2, w2 = u.copy(), v.copy(), w.copy()
+        O.advect_velocity(u2, v2, w2, occ, 1.0 / W, 6, 0.02, m, is_open=False)
+        out[m] = u2
+    assert not np.array_equal(out[O.ADV_VEL_INMAC_OUTMAC_BFECC][:, :, 0], u[:, :, 0])
+    assert np.array_equal(out[O.ADV_VEL_INMAC_OUTMAC][:, :, 0], u[:, :, 0])
+    assert np.array_equal(out[O.ADV_VEL_INREG_OUTMAC_BFECC][:, :, 0], u[:, :, 0])

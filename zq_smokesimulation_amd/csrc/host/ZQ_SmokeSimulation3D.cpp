@@ -106,14 +106,7 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
         c.temp_atten = std::exp(-para.stepTime * para.tempAttenCoeff);
         c.dens_atten = std::exp(-para.stepTime * para.densityAttenCoeff);
         ZQ_CUDA_Advection3D::AdvectVelocityType vt;
-        switch (para.advVelType) {
-        case ZQ_SmokeSimulationPara3D::ADV_VEL_INMAC_OUTMAC: vt = ZQ_CUDA_Advection3D::ADV_VEL_INMAC_OUTMAC; break;
-        case ZQ_SmokeSimulationPara3D::ADV_VEL_INREG_OUTMAC: vt = ZQ_CUDA_Advection3D::ADV_VEL_INREG_OUTMAC; break;
-        case ZQ_SmokeSimulationPara3D::ADV_VEL_INREG_OUTREG: vt = ZQ_CUDA_Advection3D::ADV_VEL_INREG_OUTREG; break;
-        default:
-            std::printf("BFECC velocity advection is not built\n");
-            return false;
-        }
+        if (!map_vel_type(para.advVelType, &vt)) return false;
         c.adv_vel_type = (int)vt;
         switch (para.advScaType) {
         case ZQ_SmokeSimulationPara3D::ADV_SCA_MAC: c.adv_sca_type = ZQ_CUDA_Advection3D::ADV_SCA_MAC; break;

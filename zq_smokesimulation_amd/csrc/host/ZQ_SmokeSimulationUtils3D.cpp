@@ -443,7 +443,6 @@ bool ZQ_Attenuation3D(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& 
     return true;
 }
 
-namespace {
 // ZQ_SmokeSimulationPara3D order -> ZQ_CUDA_Advection3D order (ZQ_SmokeSimulationUtils3D.cpp:2125-2148, 3802-3819)
 bool map_vel_type(ZQ_SmokeSimulationPara3D::AdvectVelocityType t, ZQ_CUDA_Advection3D::AdvectVelocityType* out) {
     switch (t) {
@@ -456,6 +455,7 @@ bool map_vel_type(ZQ_SmokeSimulationPara3D::AdvectVelocityType t, ZQ_CUDA_Advect
     }
     return false;
 }
+namespace {
 bool map_sca_type(ZQ_SmokeSimulationPara3D::AdvectScalarType t, ZQ_CUDA_Advection3D::AdvectScalarType* out) {
     switch (t) {
     case ZQ_SmokeSimulationPara3D::ADV_SCA_MAC: *out = ZQ_CUDA_Advection3D::ADV_SCA_MAC; return true;

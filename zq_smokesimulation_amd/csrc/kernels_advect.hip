@@ -6,6 +6,8 @@
 // of the 8 taps of a sample is one 16-byte load; the backtrace's taps are neighbours of each
 // other and of the neighbouring lanes', so they are served by L1/L2, and HBM sees roughly one
 // read of the float4 grid per kernel.
+#include <algorithm>
+
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
 #include "zq3d_stencils.h"
@@ -61,6 +63,61 @@ __global__ __launch_bounds__(BX* BY) void k_advect_faces(Vel V, const uint8_t* _
     out[fi] = AXIS == 0 ? s.x : (AXIS == 1 ? s.y : s.z);
 }
 
+// BFECC passes: backtrace with V, sample the input field I (another grid than the velocity) at the
+// end point.  Faces take I's matching component; cells take all of it, as float4 (w = 0) or as the
+// SoA scratch the face apply reads.
+template <class R, int AXIS, class Vel, class In>
+__global__ __launch_bounds__(BX* BY) void k_advect_faces_in(Vel V, In I, const uint8_t* __restrict__ occ,
+                                                            R* __restrict__ out, Dims d, R h, int steps, R dtt) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = d.W + (AXIS == 0), ny = d.H + (AXIS == 1);
+    if (i >= nx || j >= ny) return;
+    const R px = AXIS == 0 ? (R)i : i + R(0.5);
+    const R py = AXIS == 1 ? (R)j : j + R(0.5);
+    const R pz = AXIS == 2 ? (R)k : k + R(0.5);
+    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, px, py, pz);
+    const R3<R> s = I(last.x, last.y, last.z);
+    const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
+    out[fi] = AXIS == 0 ? s.x : (AXIS == 1 ? s.y : s.z);
+}
+
+template <class R, class Vel, class In, bool SOA>
+__global__ __launch_bounds__(BX* BY) void k_advect_cells_in(Vel V, In I, const uint8_t* __restrict__ occ,
+                                                            void* __restrict__ out, Dims d, R h, int steps, R dtt) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H) return;
+    const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
+    const R3<R> s = I(last.x, last.y, last.z);
+    const int o = cidx(d, i, j, k);
+    if (SOA) {
+        R* a = (R*)out;
+        const int N = d.W * d.H * d.D;
+        a[o] = s.x;
+        a[N + o] = s.y;
+        a[2 * N + o] = s.z;
+    } else {
+        vec4_t<R> r;
+        r.x = s.x, r.y = s.y, r.z = s.z, r.w = R(0);
+        ((vec4_t<R>*)out)[o] = r;
+    }
+}
+
+// Input_Increment (ZQ_CUDA_Advection3D.cu:56-74): phi~ = phi*1.5 - phi^*0.5 evaluated in double
+// (the literals are double), rounded once.  keep (nullable) receives phi first.
+template <class R>
+__global__ __launch_bounds__(256) void k_bfecc_increment(R* __restrict__ phi, const R* __restrict__ back,
+                                                         R* __restrict__ keep, R* __restrict__ out, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const R a = phi[t];
+    if (keep) keep[t] = a;
+    out[t] = (R)((double)a * 1.5 - (double)back[t] * 0.5);
+}
+
 // Masked copy scratch -> face (Apply_Advect_Velocity_Result_*, :471-650)
 template <class R, int AXIS>
 __global__ __launch_bounds__(BX* BY) void k_apply_faces(const R* __restrict__ src, const uint8_t* __restrict__ occ,
@@ -98,9 +155,107 @@ void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4
     hipLaunchKernelGGL(k_mac_to_cell4<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, vel4, d);
 }
 
+size_t bfecc_scratch_values(const Dims& d) {
+    const size_t N = (size_t)d.W * d.H * d.D;
+    const size_t NF = (size_t)(d.W + 1) * d.H * d.D + (size_t)d.W * (d.H + 1) * d.D + (size_t)d.W * d.H * (d.D + 1);
+    return std::max(8 * N, 2 * NF);
+}
+
+namespace {
+
+template <class R, class Vel, class In>
+void faces_in(const Dims& d, const Vel& V, const In& I, const uint8_t* occ, R* ou, R* ov, R* ow, const AdvParams& ap,
+              hipStream_t s) {
+    const dim3 blk(BX, BY);
+    const R h = (R)ap.h, dtt = (R)ap.dtt;
+    hipLaunchKernelGGL((k_advect_faces_in<R, 0, Vel, In>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, I, occ, ou, d, h,
+                       ap.steps, dtt);
+    hipLaunchKernelGGL((k_advect_faces_in<R, 1, Vel, In>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, I, occ, ov, d, h,
+                       ap.steps, dtt);
+    hipLaunchKernelGGL((k_advect_faces_in<R, 2, Vel, In>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, V, I, occ, ow, d, h,
+                       ap.steps, dtt);
+}
+
+template <class R, bool SOA, class Vel, class In>
+void cells_in(const Dims& d, const Vel& V, const In& I, const uint8_t* occ, void* out, const AdvParams& ap,
+              hipStream_t s) {
+    hipLaunchKernelGGL((k_advect_cells_in<R, Vel, In, SOA>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, V, I, occ,
+                       out, d, (R)ap.h, ap.steps, (R)ap.dtt);
+}
+
+template <class R>
+void increment(R* phi, const R* back, R* keep, R* out, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfecc_increment<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phi, back, keep, out,
+                       n);
+}
+
+// The three BFECC velocity modes (ZQ_CUDA_Advection3D.cu:1110-1182, 1240-1336, 1389-1474).  bf holds
+// bfecc_scratch_values(d) values; scratch the Nu+Nv+Nw face scratch; vel4 the cell velocity.
+template <class R>
+void advect_velocity_bfecc(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, int is_open, int mode,
+                           const AdvParams& ap, vec4_t<R>* vel4, R* scratch, R* bf, hipStream_t s) {
+    using V4 = vec4_t<R>;
+    const dim3 blk(BX, BY);
+    const size_t N = (size_t)d.W * d.H * d.D;
+    const size_t Nu = (size_t)(d.W + 1) * d.H * d.D, Nv = (size_t)d.W * (d.H + 1) * d.D;
+    const size_t Nw = (size_t)d.W * d.H * (d.D + 1), NF = Nu + Nv + Nw;
+    R* ou = scratch;
+    R* ov = scratch + Nu;
+    R* ow = scratch + Nu + Nv;
+    if (mode == 1) {  // inReg_outReg: vel4 = phi; A = adv(phi; v); B = adv(A; -v); B = phi~; adv(phi~; v) -> SoA
+        V4* A = (V4*)bf;
+        V4* B = (V4*)bf + N;
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        const RegularVelCached<R> Vf{vel4, d, {}};
+        const NegVel<R, RegularVelCached<R>> Vb{Vf};
+        cells_in<R, false>(d, Vf, RegularVel<R>{vel4, d}, occ, A, ap, s);
+        cells_in<R, false>(d, Vb, RegularVel<R>{A, d}, occ, B, ap, s);
+        increment<R>((R*)vel4, (const R*)B, nullptr, (R*)B, 4 * N, s);
+        R* adv = scratch;  // 3N values <= Nu+Nv+Nw
+        cells_in<R, true>(d, Vf, RegularVel<R>{B, d}, occ, adv, ap, s);
+        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
+                           is_open);
+        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 1>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, adv, occ, v, d,
+                           is_open);
+        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 2>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, adv, occ, w, d,
+                           is_open);
+        return;
+    }
+    if (mode == 3) {  // inMAC_outMAC: F1 = adv(u; u); F2 = adv(F1; -u); keep u, u = u~; adv(u~; kept) -> F1
+        R* F2 = bf;
+        R* K = bf + NF;
+        const MacVel<R> Vf{u, v, w, d};
+        faces_in<R>(d, Vf, Vf, occ, ou, ov, ow, ap, s);
+        faces_in<R>(d, NegVel<R, MacVel<R>>{Vf}, MacVel<R>{ou, ov, ow, d}, occ, F2, F2 + Nu, F2 + Nu + Nv, ap, s);
+        increment<R>(u, F2, K, u, Nu, s);
+        increment<R>(v, F2 + Nu, K + Nu, v, Nv, s);
+        increment<R>(w, F2 + Nu + Nv, K + Nu + Nv, w, Nw, s);
+        faces_in<R>(d, MacVel<R>{K, K + Nu, K + Nu + Nv, d}, MacVel<R>{u, v, w, d}, occ, ou, ov, ow, ap, s);
+    } else {  // mode 5, inReg_outMAC: F = adv(phi; v); C = cells(F); F = adv(C; -v); C = cells(F); C = phi~; F = adv(C; v)
+        V4* C = (V4*)bf;
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        const RegularVelCached<R> Vf{vel4, d, {}};
+        faces_in<R>(d, Vf, RegularVel<R>{vel4, d}, occ, ou, ov, ow, ap, s);
+        launch_mac_to_cell4<R>(d, ou, ov, ow, C, s);
+        faces_in<R>(d, NegVel<R, RegularVelCached<R>>{Vf}, RegularVel<R>{C, d}, occ, ou, ov, ow, ap, s);
+        launch_mac_to_cell4<R>(d, ou, ov, ow, C, s);
+        increment<R>((R*)vel4, (const R*)C, nullptr, (R*)C, 4 * N, s);
+        faces_in<R>(d, Vf, RegularVel<R>{C, d}, occ, ou, ov, ow, ap, s);
+    }
+    hipLaunchKernelGGL((k_apply_faces<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, ou, occ, u, d, is_open);
+    hipLaunchKernelGGL((k_apply_faces<R, 1>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, ov, occ, v, d, is_open);
+    hipLaunchKernelGGL((k_apply_faces<R, 2>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, ow, occ, w, d, is_open);
+}
+
+}  // namespace
+
 template <class R>
 void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, int is_open, int mode,
-                            const AdvParams& ap, vec4_t<R>* vel4, R* scratch, hipStream_t s) {
+                            const AdvParams& ap, vec4_t<R>* vel4, R* scratch, hipStream_t s, R* bfecc) {
+    if (mode == 1 || mode == 3 || mode == 5) {
+        advect_velocity_bfecc<R>(d, u, v, w, occ, is_open, mode, ap, vel4, scratch, bfecc, s);
+        return;
+    }
     const dim3 blk(BX, BY);
     const size_t Nu = (size_t)(d.W + 1) * d.H * d.D, Nv = (size_t)d.W * (d.H + 1) * d.D;
     R* ou = scratch;
@@ -163,7 +318,7 @@ void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, con
 #define ZQ3D_ADVECT_INSTANTIATE(R)                                                                                     \
     template void launch_mac_to_cell4<R>(const Dims&, const R*, const R*, const R*, vec4_t<R>*, hipStream_t);          \
     template void launch_advect_velocity<R>(const Dims&, R*, R*, R*, const uint8_t*, int, int, const AdvParams&,       \
-                                            vec4_t<R>*, R*, hipStream_t);                                              \
+                                            vec4_t<R>*, R*, hipStream_t, R*);                                          \
     template void launch_advect_scalar<R>(const Dims&, const R*, const R*, const R*, const uint8_t*, const R*,         \
                                           const R*, R*, R*, int, const AdvParams&, vec4_t<R>*, hipStream_t);
 ZQ3D_ADVECT_INSTANTIATE(float)

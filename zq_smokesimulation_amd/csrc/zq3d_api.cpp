@@ -50,7 +50,8 @@ int check_dims(int W, int H, int D, const char* where) {
 namespace {
 
 // ---------------- grow-only workspace for the host-pointer entry points ----------------
-enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_P2, S_B64, S_P64, S_PART, S_SCAL, S_COUNT };
+enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_P2, S_B64, S_P64, S_PART, S_SCAL,
+            S_BFECC, S_COUNT };
 
 struct Workspace {
     void* ptr[S_COUNT] = {};
@@ -293,10 +294,11 @@ extern "C" float Advect_Velocity(float* u, float* v, float* w, const bool* occup
     if (adv_ready(where)) return -1.0f;
     if (!u || !v || !w || !occupy) { set_err(where, "null pointer"); return -1.0f; }
     const int mode = (int)type;
-    if (mode != 0 && mode != 2 && mode != 4) {
-        set_err(where, "BFECC advection modes are not built");
+    if (mode < 0 || mode > 5) {
+        set_err(where, "unknown advection mode");
         return -1.0f;
     }
+    const bool bfecc = mode == 1 || mode == 3 || mode == 5;
     const Dims d{(int)g_adv.W, (int)g_adv.H, (int)g_adv.D};
     const Sizes z(d);
     Workspace& W = ws();
@@ -304,7 +306,8 @@ extern "C" float Advect_Velocity(float* u, float* v, float* w, const bool* occup
     Timer t;
     Errs E;
     if (W.ensure(S_U, z.Nu * 4) || W.ensure(S_V, z.Nv * 4) || W.ensure(S_W, z.Nw * 4) || W.ensure(S_OCC, z.N) ||
-        W.ensure(S_VEL4, z.N * 16) || W.ensure(S_SCR, (z.Nu + z.Nv + z.Nw) * 4)) {
+        W.ensure(S_VEL4, z.N * 16) || W.ensure(S_SCR, (z.Nu + z.Nv + z.Nw) * 4) ||
+        (bfecc && W.ensure(S_BFECC, bfecc_scratch_values(d) * 4))) {
         set_err(where, "device allocation failed");
         return -1.0f;
     }
@@ -318,7 +321,7 @@ extern "C" float Advect_Velocity(float* u, float* v, float* w, const bool* occup
     E(hipMemcpy(docc, occupy, z.N, hipMemcpyHostToDevice));
     const AdvParams ap{g_adv.h, g_adv.dtt, (int)g_adv.steps};
     launch_advect_velocity(d, du, dv, dw, docc, is_open ? 1 : 0, mode, ap, (float4*)W.ptr[S_VEL4],
-                           (float*)W.ptr[S_SCR], 0);
+                           (float*)W.ptr[S_SCR], 0, bfecc ? (float*)W.ptr[S_BFECC] : nullptr);
     E(hipMemcpy(u, du, z.Nu * 4, hipMemcpyDeviceToHost));
     E(hipMemcpy(v, dv, z.Nv * 4, hipMemcpyDeviceToHost));
     E(hipMemcpy(w, dw, z.Nw * 4, hipMemcpyDeviceToHost));
@@ -625,6 +628,7 @@ struct zq3d_sim {
     uint8_t* occ = nullptr;
     float4* vel4 = nullptr;       // also the vorticity scratch of the force stage
     float* scratch = nullptr;     // Nu+Nv+Nw floats: face scratch, cell force, advected cells
+    float* bfecc = nullptr;       // BFECC velocity modes: bfecc_scratch_values(d) floats
     uint8_t* nc = nullptr;
     float* b = nullptr;           // FP32 divergence / FP32 residual of the mixed solve
     float* p = nullptr;           // padded FP32 pressure / correction e of the mixed solve
@@ -706,10 +710,7 @@ int check_config(const zq3d_config& c, const char* where) {
         set_err(where, "voxel_size must be > 0 and step_time >= 0");
         return ZQ3D_ERR_ARG;
     }
-    if (c.adv_vel_type != 0 && c.adv_vel_type != 2 && c.adv_vel_type != 4) {
-        set_err(where, "adv_vel_type must be ADV_VEL_INREG_OUTREG, ADV_VEL_INMAC_OUTMAC or ADV_VEL_INREG_OUTMAC");
-        return ZQ3D_ERR_ARG;
-    }
+    if (c.adv_vel_type < 0 || c.adv_vel_type > 5) { set_err(where, "unknown adv_vel_type"); return ZQ3D_ERR_ARG; }
     if (c.adv_sca_type < 0 || c.adv_sca_type > 3) { set_err(where, "unknown adv_sca_type"); return ZQ3D_ERR_ARG; }
     if (c.sub_size < 0 || c.sub_size > COARSE_MAX_S ||
         (c.sub_size && (c.width % c.sub_size || c.height % c.sub_size || c.depth % c.sub_size))) {
@@ -787,6 +788,8 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
         A(dalloc(&s->b64, z.N * 8));
         A(dalloc(&s->p64, z.Np * 8));
     }
+    if (cfg->adv_vel_type == 1 || cfg->adv_vel_type == 3 || cfg->adv_vel_type == 5)
+        A(dalloc(&s->bfecc, bfecc_scratch_values(s->d) * 4));
     if (e != hipSuccess) {
         set_err(where, hipGetErrorString(e));
         zq3d_destroy(s);
@@ -821,7 +824,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
 
 extern "C" void zq3d_destroy(zq3d_sim* s) {
     if (!s) return;
-    void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->nc, s->b, s->p, s->p_alt, s->part,
+    void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->bfecc, s->nc, s->b, s->p, s->p_alt, s->part,
                     s->split_p, s->split_b, s->split_n,
                     s->b64, s->p64, s->sc};
     for (void* p : ptrs)
@@ -915,7 +918,8 @@ extern "C" int zq3d_advect_velocity(zq3d_sim* s) {
     const AdvParams ap{c.voxel_size, c.step_time / c.steps, c.steps};
     // the reference's stage always advects with is_open = true (ZQ_SmokeSimulationUtils3D.cpp:2163)
     StageTimer t(s, &s->tm.ms_advect_velocity);
-    launch_advect_velocity(s->d, s->u, s->v, s->w, s->occ, 1, c.adv_vel_type, ap, s->vel4, s->scratch, s->stream);
+    launch_advect_velocity(s->d, s->u, s->v, s->w, s->occ, 1, c.adv_vel_type, ap, s->vel4, s->scratch, s->stream,
+                           s->bfecc);
     ZQ_HIP(hipGetLastError(), "zq3d_advect_velocity");
     return t.done("zq3d_advect_velocity");
 }

@@ -210,6 +210,17 @@ struct MacVel {
 };
 
 
+// A velocity sampler negated (the BFECC back-advection, ZQ_CUDA_Advection3D.cu:997-1019, negates the
+// field before sampling it; negating the interpolated value gives the same bits).
+template <class R, class Vel>
+struct NegVel {
+    Vel v;
+    __device__ __forceinline__ R3<R> operator()(R px, R py, R pz) const {
+        const R3<R> r = v(px, py, pz);
+        return R3<R>{-r.x, -r.y, -r.z};
+    }
+};
+
 // True while the backtrace may take another step from (px,py,pz): inside [0,W]x[0,H]x[0,D] and
 // not in a solid cell (point sample, clamp).  The occupancy byte of the cell last tested is kept:
 // consecutive backtrace points usually share a cell, so most tests load nothing.

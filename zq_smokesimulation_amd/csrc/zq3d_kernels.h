@@ -38,10 +38,14 @@ struct AdvParams {
     float h, dtt;
     int steps;
 };
-// mode: 0 inReg_outReg, 2 inMAC_outMAC, 4 inReg_outMAC.  scratch: vel4 (N), faces (Nu+Nv+Nw values).
+// mode: AdvectVelocityType (0 inReg_outReg, 1 its BFECC, 2 inMAC_outMAC, 3 its BFECC, 4 inReg_outMAC,
+// 5 its BFECC).  scratch: vel4 (N), faces (Nu+Nv+Nw values); bfecc: bfecc_scratch_values(d) values,
+// needed by the BFECC modes only (nullable otherwise).
+size_t bfecc_scratch_values(const Dims& d);
 template <class R>
 void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, int is_open, int mode,
-                            const AdvParams& ap, vec4_t<R>* vel4, R* scratch_faces, hipStream_t s);
+                            const AdvParams& ap, vec4_t<R>* vel4, R* scratch_faces, hipStream_t s,
+                            R* bfecc = nullptr);
 // mac != 0: MAC velocity sampling (ADV_SCA_MAC); else cell-centred (ADV_SCA_REG).
 template <class R>
 void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, const uint8_t* occ, const R* Tin,
