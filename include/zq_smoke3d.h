@@ -289,6 +289,13 @@ int zq3d_set_sor_shape(zq3d_sim* sim, int ty, int zc);
  * times the valid schedules and keeps the fastest; all produce the same bits. */
 int zq3d_set_coarse_sweep(zq3d_sim* sim, int k, int zc);
 int zq3d_get_coarse_sweep(zq3d_sim* sim, int* k, int* zc);
+/* Guiding control of the coarse + sub-grid projection (ZQ_ApplyGuidingVelocity3D,
+ * ZQ_SmokeSimulationUtils3D.cpp:3139-3170, applied between the coarse solve and the coarse -> fine
+ * correction, :3748-3755): while guide faces are set, every coarse face's delta gets
+ * + coeff * (guide - averaged coarse face).  gu, gv, gw: host arrays of the coarse grid's faces
+ * ((Wc+1)*Hc*Dc, Wc*(Hc+1)*Dc, Wc*Hc*(Dc+1) floats: the first three arrays of a .guidedat file at the
+ * coarse resolution); gu == NULL turns guiding off.  Requires sub_size > 0. */
+int zq3d_set_guide(zq3d_sim* sim, float coeff, const float* gu, const float* gv, const float* gw);
 /* name of the sweep kernel the current shape runs ("k_sor_split_rb", "k_sor_split", "k_sor_fused"
  * or "k_sor_half") */
 const char* zq3d_sor_kernel(zq3d_sim* sim);

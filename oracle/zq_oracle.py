@@ -495,12 +495,23 @@ def sub_project_irregular(u, v, w, occ, s, regular):
                 _scatter_interior(u, v, w, s, bi, bj, bk, sub_solve_block(x, ob, s))
 
 
-def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=None):
+def apply_guide(du, dv, dw, cu, cv, cw, coeff, guide):
+    """ZQ_ApplyGuidingVelocity3D (ZQ_SmokeSimulationUtils3D.cpp:3139-3170): delta += coeff*(G - U) on
+    every coarse face, float32."""
+    c = np.float32(coeff)
+    for d, u, g in zip((du, dv, dw), (cu, cv, cw), guide):
+        d += c * (np.asarray(g, np.float32) - u)
+
+
+def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=None, guide=None, guide_coeff=0.0):
     """ZQ_TotalProjection3D for COARSE_OPEN_POISSON_SOR (ZQ_SmokeSimulationUtils3D.cpp:3688-3793), in
-    place: fine -> coarse, coarse face-fraction solve, coarse -> fine, sub-projection (s > 1)."""
+    place: fine -> coarse, coarse face-fraction solve, guiding (guide = coarse faces gu, gv, gw),
+    coarse -> fine, sub-projection (s > 1)."""
     rU, rV, rW, cocc, regular = coarse_setup(occ, s)
     cu, cv, cw = fine_to_coarse(u, v, w, s)
     p, du, dv, dw = coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega, open_mask)
+    if guide is not None:
+        apply_guide(du, dv, dw, cu, cv, cw, guide_coeff, guide)
     coarse_to_fine(du, dv, dw, occ, u, v, w, s)
     if s > 1:
         sub_project_regular(u, v, w, s, sub_matrix(s) if M is None else M, regular)
@@ -508,7 +519,7 @@ def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=
     return p
 
 
-def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None):
+def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None, guide=None, guide_coeff=0.0):
     """One step in the order of orc_step (ZQ_SmokeSimulation3D.cpp:156-243) with the coarse + sub-grid
     projection in place of the full-grid solve (COARSE_*_POISSON_SOR): open on all six sides, FP32
     SOR with params' omega and sor_iters on the coarse grid."""
@@ -518,7 +529,7 @@ def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None):
     attenuation(u, v, w, T, rho, occ, P["aV"], P["aT"], P["aR"])
     dtt = float(np.float32(P["dt"]) / np.float32(P["steps"]))
     advect_velocity(u, v, w, occ, P["h"], P["steps"], dtt, P["adv_vel_type"], True)
-    coarse_projection(u, v, w, occ, s, P["sor_iters"], P["omega"], ALL_FACES)
+    coarse_projection(u, v, w, occ, s, P["sor_iters"], P["omega"], ALL_FACES, guide=guide, guide_coeff=guide_coeff)
     To, Ro = advect_scalar(u, v, w, occ, T, rho, P["h"], P["steps"], dtt, P["adv_sca_type"])
     T[...] = To
     rho[...] = Ro

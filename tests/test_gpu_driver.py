@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import zq_oracle as O
+from util import faces
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +40,7 @@ def run_cli(tmp_path, text, frames):
     return [read_di3(str(out / ("frame%d.di3" % f))) for f in range(frames)]
 
 
-def oracle_frames(path, frames, moving=False, sub_size=0):
+def oracle_frames(path, frames, moving=False, sub_size=0, guides=None, guide_coeff=0.0):
     """The oracle's step driven by the parameters the config parses to (sub_size > 0: the coarse +
     sub-grid projection of the COARSE_*_POISSON_SOR solvers)."""
     from zq_smokesimulation_amd import scene
@@ -60,7 +61,8 @@ def oracle_frames(path, frames, moving=False, sub_size=0):
         if moving:
             occ = scene.apply_moving_objects(path, f, u, v, w)
         if sub_size:
-            O.coarse_step(p, u, v, w, T, R, occ, sub_size, boxes=boxes)
+            O.coarse_step(p, u, v, w, T, R, occ, sub_size, boxes=boxes,
+                          guide=guides[f] if guides else None, guide_coeff=guide_coeff)
         else:
             O.step(p, u, v, w, T, R, occ, boxes=boxes)
         out.append(R.copy())
@@ -119,6 +121,32 @@ def test_cli_bfecc_advection_open(tmp_path, resident, adv):
     want = oracle_frames(str(tmp_path / "scene.txt"), 2)
     for f in range(2):
         np.testing.assert_array_equal(got[f], want[f], err_msg="frame %d" % f)
+
+
+@pytest.mark.parametrize("resident", ["false", "true"])
+def test_cli_guiding_control(tmp_path, resident):
+    """useGuidingControl: frame f blends <guidingFold>/<f>.guidedat (coarse resolution) into the coarse delta."""
+    s = 4
+    W, H, D = 32, 24, 16
+    rng = np.random.default_rng(5)
+    gdir = tmp_path / "guide"
+    gdir.mkdir()
+    guides = []
+    for f in range(3):
+        gu, gv, gw = faces(rng, W // s, H // s, D // s, 0.2)
+        extra = np.zeros(2 * (W // s) * (H // s) * (D // s), np.float32)  # density, temperature
+        (gdir / ("%d.guidedat" % f)).write_bytes(np.concatenate([gu.ravel(), gv.ravel(), gw.ravel(), extra]).tobytes())
+        guides.append((gu, gv, gw))
+    text = BASE + ("frameCount 3\nintObj sphere\nsubSize %d\nglobalGridSolver COARSE_OPEN_POISSON_SOR\n"
+                   "useGuidingControl true\nguidingCoeff 0.3\nguidingFold %s\ndeviceResident %s\n" % (s, gdir, resident))
+    got = run_cli(tmp_path, text, 3)
+    want = oracle_frames(str(tmp_path / "scene.txt"), 3, sub_size=s, guides=guides, guide_coeff=0.3)
+    for f in range(3):
+        scale = max(1.0, float(np.abs(want[f]).max()))
+        np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-4 * scale, err_msg="frame %d" % f)
+    # the guide moved the flow: the unguided run differs
+    plain = oracle_frames(str(tmp_path / "scene.txt"), 3, sub_size=s)
+    assert np.abs(plain[2] - want[2]).max() > 1e-3
 
 
 def test_cli_mac_advection_open(tmp_path):

@@ -337,3 +337,23 @@ def test_whole_coarse_projection_removes_divergence():
             blk = d[bk * s:(bk + 1) * s, H - s:, bi * s:(bi + 1) * s]
             want = (np.float64(dv[bk, Hc - 1, bi]) - dv[bk, Hc, bi]) / s
             np.testing.assert_allclose(blk, want, atol=2e-4)
+
+
+def test_guiding_adds_the_blend_to_the_coarse_delta():
+    """With no coarse iterations the delta is zero, so at s = 1 (coarse = fine, no sub-projection) the
+    fluid faces become u + c (G - u) exactly and every other face stays."""
+    rng = np.random.default_rng(17)
+    W, H, D = 10, 8, 6
+    occ = (rng.random((D, H, W)) < 0.2).astype(np.uint8)
+    u, v, w = faces(rng, W, H, D)
+    G = faces(rng, W, H, D)
+    c = np.float32(0.3)
+    u1, v1, w1 = u.copy(), v.copy(), w.copy()
+    O.coarse_projection(u1, v1, w1, occ, 1, 0, guide=G, guide_coeff=0.3)
+    fl = occ == 0
+    mu = np.zeros(u.shape, bool)
+    mu[:, :, 1:-1] = fl[:, :, 1:] & fl[:, :, :-1]
+    mu[:, :, 0] = fl[:, :, 0]
+    mu[:, :, -1] = fl[:, :, -1]
+    want = np.where(mu, u + (np.float32(0) + c * (G[0] - u)), u)
+    assert np.array_equal(u1, want)

@@ -148,6 +148,7 @@ def _load():
     L.MatrixMul.restype = f
     L.zq3d_config_default.argtypes = [C.POINTER(Config), i, i, i]
     L.zq3d_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.zq3d_set_guide.argtypes = [vp, f, vp, vp, vp]
     L.zq3d_set_coarse_sweep.argtypes = [vp, i, i]
     L.zq3d_get_coarse_sweep.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
     L.zq3d_destroy.argtypes = [vp]

@@ -11,6 +11,7 @@
 // fields cross PCIe as in the other stages).  The flux, CPU PCG and octree solvers are not built
 // and Init fails for them.
 #include "ZQ_SmokeSimulation3D.h"
+#include "ZQ_SmokeGuideGrid3D.h"
 
 #include <chrono>
 #include <cmath>
@@ -66,10 +67,6 @@ bool ZQ_SmokeSimulation3D::Init(const char* file) {
     if (!solver_built(para.globalGridSolver)) {
         std::printf("globalGridSolver %d is not built: use TOTAL_OPEN_POISSON_SOR or TOTAL_CLOSED_POISSON_SOR\n",
                     (int)para.globalGridSolver);
-        return false;
-    }
-    if (para.useGuidingControl) {
-        std::printf("guiding control is not built\n");
         return false;
     }
     if (para.useMovingObject) {
@@ -202,9 +199,19 @@ bool ZQ_SmokeSimulation3D::SyncToHost() {
 
 bool ZQ_SmokeSimulation3D::Run(const char* fold) {
     char name[1024];
+    // one guide file per frame, <guidingFold>/<frame>.guidedat at the coarse resolution
+    // (ZQ_SmokeSimulation3D.cpp:110-124; the reference joins the path with a backslash)
+    ZQ_SmokeGuideGrid3D guideGrid(para.coarseWidth, para.coarseHeight, para.coarseDepth);
     for (int i = 0; i < para.frameCount; i++) {
         std::printf("frame [%d]\n", i);
-        if (!UpdateOneFrame(nullptr)) return false;
+        if (para.useGuidingControl) {
+            std::snprintf(name, sizeof(name), "%s/%d.guidedat", para.guidingFold, i);
+            if (!guideGrid.LoadFromFile(name)) {
+                std::printf("load guide file %s fail\n", name);
+                return false;
+            }
+        }
+        if (!UpdateOneFrame(para.useGuidingControl ? &guideGrid : nullptr)) return false;
         if (!SyncToHost()) return false;
         switch (para.exportType) {
         case ZQ_SmokeSimulationPara3D::EXPORT_DI3: std::snprintf(name, sizeof(name), "%s/frame%d.di3", fold, i); break;
@@ -225,8 +232,23 @@ bool ZQ_SmokeSimulation3D::Run(const char* fold) {
 }
 
 bool ZQ_SmokeSimulation3D::UpdateOneFrame(const ZQ_SmokeGuideGrid3D* guideGrid) {
-    (void)guideGrid;
     moving_frame += 1;
+    // guiding acts in the coarse projection only; the reference ignores it for the full-grid
+    // solvers (ZQ_SmokeSimulation3D.cpp:403-414)
+    zq3d_sim* target = dev ? dev : proj;
+    if (target && is_coarse()) {
+        const bool on = guideGrid && para.useGuidingControl;
+        if (on && ((int)guideGrid->GetWidth() != para.coarseWidth || (int)guideGrid->GetHeight() != para.coarseHeight ||
+                   (int)guideGrid->GetDepth() != para.coarseDepth)) {
+            std::printf("guide grid is not at the coarse resolution\n");
+            return false;
+        }
+        if (zq3d_set_guide(target, para.guidingCoeff, on ? guideGrid->GetUptr() : nullptr,
+                           on ? guideGrid->GetVptr() : nullptr, on ? guideGrid->GetWptr() : nullptr) != ZQ3D_OK) {
+            std::printf("guide: %s\n", zq3d_last_error());
+            return false;
+        }
+    }
     return dev ? _updateResident() : _updateHost();
 }
 

@@ -8,8 +8,7 @@
 #include "ZQ_SmokeMovingObject3D.h"
 #include "ZQ_SmokeSimulationUtils3D.h"
 
-// guide grids drive the guiding control of the coarse solver, which is not built; the type exists
-// so UpdateOneFrame keeps the reference's signature
+// guide grids drive the guiding control of the coarse solver (ZQ_SmokeGuideGrid3D.h)
 class ZQ_SmokeGuideGrid3D;
 
 class ZQ_SmokeSimulation3D {

@@ -345,12 +345,16 @@ __device__ __forceinline__ float delta_w(const CG& g, const float* rW, const flo
 // fluid (ZQ_MapCoarse2Fine3D, :3172-3296).  The reference's x = W faces test the cell at x = W, one
 // past the row (:3209); this reads the inside cell x = W-1, as its y and z faces do.  Its y = H
 // faces take the delta of coarse v row Hc-1 (:3251), which is kept.
+// With guide faces (gU non-null), each delta first gets + gcoef * (guide - coarse face)
+// (ZQ_ApplyGuidingVelocity3D, :3162-3167).
 __global__ __launch_bounds__(256) void k_coarse_prolong(CG g, const float* __restrict__ rU,
                                                         const float* __restrict__ rV, const float* __restrict__ rW,
                                                         const float* __restrict__ cu, const float* __restrict__ cv,
                                                         const float* __restrict__ cw, const float* __restrict__ p,
                                                         int om, const uint8_t* __restrict__ occ, float* u, float* v,
-                                                        float* w) {
+                                                        float* w, const float* __restrict__ gU,
+                                                        const float* __restrict__ gV, const float* __restrict__ gW,
+                                                        float gcoef) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int s = g.s;
     const size_t nu = (size_t)(g.Wc + 1) * g.H * g.D, nv = (size_t)g.W * (g.Hc + 1) * g.D;
@@ -363,8 +367,13 @@ __global__ __launch_bounds__(256) void k_coarse_prolong(CG g, const float* __res
         else if (ic == g.Wc) ok = !occ[f_c(g, g.W - 1, y, z)];
         else ok = !occ[f_c(g, x, y, z)] && !occ[f_c(g, x - 1, y, z)];
         if (ok) {
+            float dl = delta_u(g, rU, cu, p, om, ic, y / s, z / s);
+            if (gU) {
+                const int c = c_u(g, ic, y / s, z / s);
+                dl = dl + gcoef * (gU[c] - cu[c]);
+            }
             const int f = f_u(g, x, y, z);
-            u[f] = u[f] + delta_u(g, rU, cu, p, om, ic, y / s, z / s);
+            u[f] = u[f] + dl;
         }
         return;
     }
@@ -380,8 +389,13 @@ __global__ __launch_bounds__(256) void k_coarse_prolong(CG g, const float* __res
             row = g.Hc - 1;
         } else ok = !occ[f_c(g, x, y, z)] && !occ[f_c(g, x, y - 1, z)];
         if (ok) {
+            float dl = delta_v(g, rV, cv, p, om, x / s, row, z / s);
+            if (gV) {
+                const int c = c_v(g, x / s, row, z / s);
+                dl = dl + gcoef * (gV[c] - cv[c]);
+            }
             const int f = f_v(g, x, y, z);
-            v[f] = v[f] + delta_v(g, rV, cv, p, om, x / s, row, z / s);
+            v[f] = v[f] + dl;
         }
         return;
     }
@@ -394,8 +408,13 @@ __global__ __launch_bounds__(256) void k_coarse_prolong(CG g, const float* __res
         else if (kc == g.Dc) ok = !occ[f_c(g, x, y, g.D - 1)];
         else ok = !occ[f_c(g, x, y, z)] && !occ[f_c(g, x, y, z - 1)];
         if (ok) {
+            float dl = delta_w(g, rW, cw, p, om, x / s, y / s, kc);
+            if (gW) {
+                const int c = c_w(g, x / s, y / s, kc);
+                dl = dl + gcoef * (gW[c] - cw[c]);
+            }
             const int f = f_w(g, x, y, z);
-            w[f] = w[f] + delta_w(g, rW, cw, p, om, x / s, y / s, kc);
+            w[f] = w[f] + dl;
         }
     }
 }
@@ -942,13 +961,14 @@ int coarse_sweep_candidates(const CoarseDims& cd, CoarseSweep* out, int max) {
 }
 
 void launch_coarse_prolong(const CoarseDims& cd, const CoarseBufs& B, const float* p, int open_mask, const uint8_t* occ,
-                           float* u, float* v, float* w, hipStream_t s) {
+                           float* u, float* v, float* w, hipStream_t s, int guide, float guide_coeff) {
     const CG g = make_cg(cd);
     const size_t n = (size_t)(cd.Wc + 1) * cd.f.H * cd.f.D + (size_t)cd.f.W * (cd.Hc + 1) * cd.f.D +
                      (size_t)cd.f.W * cd.f.H * (cd.Dc + 1);
     hipLaunchKernelGGL(k_coarse_prolong, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, (const float*)B.rU,
                        (const float*)B.rV, (const float*)B.rW, (const float*)B.cu, (const float*)B.cv,
-                       (const float*)B.cw, p, open_mask, occ, u, v, w);
+                       (const float*)B.cw, p, open_mask, occ, u, v, w, guide ? (const float*)B.gU : nullptr,
+                       guide ? (const float*)B.gV : nullptr, guide ? (const float*)B.gW : nullptr, guide_coeff);
 }
 
 void launch_coarse_correct(const CoarseDims& cd, const CoarseBufs& B, const float* p, int open_mask, float* u, float* v,

@@ -958,6 +958,13 @@ extern "C" int zq3d_set_coarse_sweep(zq3d_sim* s, int k, int zc) {
     return ZQ3D_OK;
 }
 
+extern "C" int zq3d_set_guide(zq3d_sim* s, float coeff, const float* gu, const float* gv, const float* gw) {
+    const char* where = "zq3d_set_guide";
+    if (!s) { set_err(where, "null sim"); return ZQ3D_ERR_ARG; }
+    if (s->cfg.sub_size <= 0) { set_err(where, "guiding needs the coarse grid (sub_size > 0)"); return ZQ3D_ERR_STATE; }
+    return coarse_set_guide(&s->coarse, coeff, gu, gv, gw, s->stream, where);
+}
+
 extern "C" int zq3d_get_coarse_sweep(zq3d_sim* s, int* k, int* zc) {
     const char* where = "zq3d_get_coarse_sweep";
     if (!s || !k || !zc) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }

@@ -37,6 +37,9 @@ size_t coarse_layout(const CoarseDims& cd, char* base, CoarseBufs* B) {
     f(&o->div, cd.Nc());
     f(&o->p, cd.Np());
     f(&o->p2, cd.Np());
+    f(&o->gU, cd.Nu());
+    f(&o->gV, cd.Nv());
+    f(&o->gW, cd.Nw());
     b(&o->cocc, cd.Nc());
     b(&o->regular, cd.Nc());
     return off;
@@ -104,6 +107,25 @@ int coarse_create(const Dims& d, int s, CoarseState* out, const char* where) {
     return tune_coarse_sweep(out, 20, where);
 }
 
+int coarse_set_guide(CoarseState* c, float coeff, const float* gu, const float* gv, const float* gw, hipStream_t s,
+                     const char* where) {
+    if (!gu) {
+        c->guide = 0;
+        return ZQ3D_OK;
+    }
+    if (!gv || !gw) {
+        set_err(where, "gu, gv and gw must all be given (or gu == NULL to turn guiding off)");
+        return ZQ3D_ERR_ARG;
+    }
+    ZQ_HIP(hipMemcpyAsync(c->B.gU, gu, c->cd.Nu() * 4, hipMemcpyHostToDevice, s), where);
+    ZQ_HIP(hipMemcpyAsync(c->B.gV, gv, c->cd.Nv() * 4, hipMemcpyHostToDevice, s), where);
+    ZQ_HIP(hipMemcpyAsync(c->B.gW, gw, c->cd.Nw() * 4, hipMemcpyHostToDevice, s), where);
+    ZQ_HIP(hipStreamSynchronize(s), where);
+    c->guide = 1;
+    c->guide_coeff = coeff;
+    return ZQ3D_OK;
+}
+
 void coarse_destroy(CoarseState* c) {
     if (c && c->mem) (void)hipFree(c->mem);
     if (c) c->mem = nullptr;
@@ -125,7 +147,7 @@ int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uin
         pf->ms += ms;
         pf->launches += c.sweep.k > 0 ? (o.iters + c.sweep.k - 1) / c.sweep.k : 2LL * o.iters;
     }
-    launch_coarse_prolong(c.cd, c.B, p, o.open_mask, occ, u, v, w, s);
+    launch_coarse_prolong(c.cd, c.B, p, o.open_mask, occ, u, v, w, s, c.guide, c.guide_coeff);
     if (c.cd.s > 1) launch_sub_project(c.cd, c.B, occ, u, v, w, s);
     ZQ_HIP(hipGetLastError(), where);
     if (st) {

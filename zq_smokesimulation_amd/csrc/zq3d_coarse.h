@@ -40,6 +40,7 @@ struct CoarseBufs {
     float* p2;               // second pressure the multi-iteration sweep ping-pongs with
     uint8_t* cocc;           // coarse occupancy
     uint8_t* regular;        // 1: the block holds no solid fine cell
+    float *gU, *gV, *gW;     // guide faces (guiding control)
 };
 
 // fractions, coarse occupancy, 1/diagonal (open_mask: open domain faces) and block flags from the
@@ -59,9 +60,10 @@ int coarse_sweep_candidates(const CoarseDims& g, CoarseSweep* out, int max);
 // `iters` red-black iterations from B.p (colour 0 first); returns B.p or B.p2, whichever holds the result
 const float* launch_coarse_sor(const CoarseDims& g, const CoarseBufs& B, float omega, int iters, CoarseSweep sw,
                                hipStream_t s);
-// adds the coarse correction (pressure p) to the fine faces on coarse planes (fine cells beside them fluid)
+// adds the coarse correction (pressure p; with guide != 0 also guide_coeff * (guide - coarse face))
+// to the fine faces on coarse planes (fine cells beside them fluid)
 void launch_coarse_prolong(const CoarseDims& g, const CoarseBufs& B, const float* p, int open_mask, const uint8_t* occ,
-                           float* u, float* v, float* w, hipStream_t s);
+                           float* u, float* v, float* w, hipStream_t s, int guide = 0, float guide_coeff = 0.0f);
 // projects every block's interior faces; s in [2, COARSE_MAX_S]
 void launch_sub_project(const CoarseDims& g, const CoarseBufs& B, const uint8_t* occ, float* u, float* v, float* w,
                         hipStream_t s);

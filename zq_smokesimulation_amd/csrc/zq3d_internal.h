@@ -104,10 +104,15 @@ struct CoarseState {
     CoarseDims cd{};
     CoarseBufs B{};
     CoarseSweep sweep{0, 0};  // schedule of the coarse SOR (timed at creation)
+    int guide = 0;            // guide faces set (guiding control)
+    float guide_coeff = 0.0f;
     void* mem = nullptr;
 };
 int coarse_create(const Dims& d, int s, CoarseState* out, const char* where);
 void coarse_destroy(CoarseState* c);
+// host guide faces of the coarse grid (gu == NULL: guiding off)
+int coarse_set_guide(CoarseState* c, float coeff, const float* gu, const float* gv, const float* gw, hipStream_t s,
+                     const char* where);
 // Coarse + sub-grid projection of the faces in place (o.iters coarse iterations with o.omega and
 // o.open_mask).  B provides the stats scratch (nc, b, p, part, sc); stats may be null.
 int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,

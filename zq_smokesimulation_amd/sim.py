@@ -175,6 +175,21 @@ class SmokeSim3D:
     def set_sor_shape(self, ty, zc):
         check(_lib.lib().zq3d_set_sor_shape(self._h, int(ty), int(zc)), "set_sor_shape")
 
+    def set_guide(self, coeff, gu=None, gv=None, gw=None):
+        """Guiding control of the coarse projection: coarse-grid guide faces (None: off)."""
+        if gu is None:
+            check(_lib.lib().zq3d_set_guide(self._h, 0.0, None, None, None), "set_guide")
+            self._guide = None
+            return
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (gu, gv, gw)]
+        s = self.cfg.sub_size
+        Wc, Hc, Dc = self.W // s, self.H // s, self.D // s
+        if [a.shape for a in arrs] != [(Dc, Hc, Wc + 1), (Dc, Hc + 1, Wc), (Dc + 1, Hc, Wc)]:
+            raise ValueError("guide faces must have the coarse grid's face shapes")
+        check(_lib.lib().zq3d_set_guide(self._h, float(coeff), *(a.ctypes.data_as(C.c_void_p) for a in arrs)),
+              "set_guide")
+        self._guide = arrs
+
     def set_coarse_sweep(self, k, zc):
         """Coarse SOR schedule of the coarse + sub-grid projection (k == 0: one launch per colour)."""
         check(_lib.lib().zq3d_set_coarse_sweep(self._h, int(k), int(zc)), "set_coarse_sweep")
