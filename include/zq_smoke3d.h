@@ -209,8 +209,13 @@ typedef struct zq3d_config {
      * depth: the reference's coarse + sub-grid projection (COARSE_OPEN_POISSON_SOR, subSize = s,
      * ZQ_SmokeSimulationUtils3D.cpp:3688-3795): sor.iters face-fraction SOR iterations on the
      * (W/s, H/s, D/s) grid with sor.omega and sor.open_mask, the coarse correction, then each s^3
-     * block's interior faces projected (s > 1).  Requires sor.mixed_precision = 0. */
+     * block's interior faces projected (s > 1).  Requires sor.mixed_precision = 0.  With
+     * sor.closed_mean the divergence's mean per unit fluid volume is removed first (closed domains). */
     int sub_size;
+    /* sub_size > 0: 1 selects the flux form of the coarse solve (the *_FLUX_SOR solvers): an open
+     * domain face weighs by its own open fraction, as in the KKT system of ZQ_InitMatOpenFlux3D
+     * (ZQ_SmokeSimulationUtils3D.cpp:749-936), instead of 1 (ZQ_InitMatOpenPoisson3D, :476-592). */
+    int coarse_flux;
 } zq3d_config;
 
 typedef struct zq3d_sim zq3d_sim;

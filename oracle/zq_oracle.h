@@ -144,12 +144,13 @@ void orc_noise(int W, int H, int D, uint32_t seed, float amp, float* out);
 void orc_coarse_setup(int W, int H, int D, int s, const uint8_t* occ, float* rU, float* rV, float* rW,
                       uint8_t* cocc, uint8_t* regular);
 float orc_coarse_inv_diag(int Wc, int Hc, int Dc, int i, int j, int k, const float* rU, const float* rV,
-                          const float* rW, int open_mask);
+                          const float* rW, int open_mask, int flux);
 void orc_fine_to_coarse(int W, int H, int D, int s, const float* u, const float* v, const float* w, float* cu,
                         float* cv, float* cw);
 void orc_coarse_project(int Wc, int Hc, int Dc, const float* cu, const float* cv, const float* cw,
                         const uint8_t* cocc, const float* rU, const float* rV, const float* rW, int open_mask,
-                        float omega, int iters, float* p, float* du, float* dv, float* dw);
+                        float omega, int iters, float* p, float* du, float* dv, float* dw, const float* vol,
+                        int flux);
 void orc_coarse_to_fine(int W, int H, int D, int s, const float* du, const float* dv, const float* dw,
                         const uint8_t* occ, float* u, float* v, float* w);
 /* M: R x F row-major float, R = 3 s^2 (s-1), F = 3 s^2 (s+1) */

@@ -96,11 +96,11 @@ def lib():
         L.orc_step.argtypes = [C.POINTER(StepParams), _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, vp, vp, vp]
         L.orc_step.restype = i
         L.orc_coarse_setup.argtypes = [i, i, i, i, _u8p, _f32p, _f32p, _f32p, _u8p, _u8p]
-        L.orc_coarse_inv_diag.argtypes = [i, i, i, i, i, i, _f32p, _f32p, _f32p, i]
+        L.orc_coarse_inv_diag.argtypes = [i, i, i, i, i, i, _f32p, _f32p, _f32p, i, i]
         L.orc_coarse_inv_diag.restype = f
         L.orc_fine_to_coarse.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
         L.orc_coarse_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p, i, f, i,
-                                         _f32p, _f32p, _f32p, _f32p]
+                                         _f32p, _f32p, _f32p, _f32p, vp, i]
         L.orc_coarse_to_fine.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p]
         L.orc_sub_project_regular.argtypes = [i, i, i, i, _f32p, _u8p, _f32p, _f32p, _f32p]
         _lib = L
@@ -322,19 +322,30 @@ def fine_to_coarse(u, v, w, s):
     return cu, cv, cw
 
 
-def coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega=1.0, open_mask=ALL_FACES):
-    """Face-fraction red-black SOR on the coarse grid and the coarse delta (see zq_oracle_coarse.cpp).
+def coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega=1.0, open_mask=ALL_FACES, vol=None, flux=False):
+    """Face-fraction red-black SOR on the coarse grid and the coarse delta (see zq_oracle_coarse.cpp);
+    vol (closed domains) removes the divergence's mean per unit fluid volume first.
     Returns the padded pressure and du, dv, dw."""
     Dc, Hc, Wc = cocc.shape
     p = np.zeros((Dc + 2, Hc + 2, Wc + 2), np.float32)
     du, dv, dw = zeros_faces(Wc, Hc, Dc)
-    lib().orc_coarse_project(Wc, Hc, Dc, cu, cv, cw, cocc, rU, rV, rW, open_mask, omega, iters, p, du, dv, dw)
+    vv = None if vol is None else np.ascontiguousarray(vol, np.float32)
+    lib().orc_coarse_project(Wc, Hc, Dc, cu, cv, cw, cocc, rU, rV, rW, open_mask, omega, iters, p, du, dv, dw,
+                             _ptr(vv), int(flux))
     return p, du, dv, dw
 
 
-def coarse_inv_diag(rU, rV, rW, i, j, k, open_mask=ALL_FACES):
+def coarse_volume(occ, s):
+    """Fluid volume fraction per coarse cell, 1 - (solid fine cells)/s^3 in float32
+    (BuildVolumeUnOccupyRatio, ZQ_SmokeSimulationUtils3D.cpp:1860-1885)."""
+    D, H, W = occ.shape
+    n = (occ != 0).reshape(D // s, s, H // s, s, W // s, s).sum(axis=(1, 3, 5)).astype(np.float32)
+    return np.float32(1) - n / np.float32(s ** 3)
+
+
+def coarse_inv_diag(rU, rV, rW, i, j, k, open_mask=ALL_FACES, flux=False):
     Dc, Hc, Wc1 = rU.shape
-    return lib().orc_coarse_inv_diag(Wc1 - 1, Hc, Dc, i, j, k, rU, rV, rW, open_mask)
+    return lib().orc_coarse_inv_diag(Wc1 - 1, Hc, Dc, i, j, k, rU, rV, rW, open_mask, int(flux))
 
 
 def coarse_to_fine(du, dv, dw, occ, u, v, w, s):
@@ -503,13 +514,15 @@ def apply_guide(du, dv, dw, cu, cv, cw, coeff, guide):
         d += c * (np.asarray(g, np.float32) - u)
 
 
-def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=None, guide=None, guide_coeff=0.0):
+def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=None, guide=None, guide_coeff=0.0,
+                      closed_mean=False, flux=False):
     """ZQ_TotalProjection3D for COARSE_OPEN_POISSON_SOR (ZQ_SmokeSimulationUtils3D.cpp:3688-3793), in
     place: fine -> coarse, coarse face-fraction solve, guiding (guide = coarse faces gu, gv, gw),
     coarse -> fine, sub-projection (s > 1)."""
     rU, rV, rW, cocc, regular = coarse_setup(occ, s)
     cu, cv, cw = fine_to_coarse(u, v, w, s)
-    p, du, dv, dw = coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega, open_mask)
+    p, du, dv, dw = coarse_project(cu, cv, cw, cocc, rU, rV, rW, iters, omega, open_mask,
+                                   coarse_volume(occ, s) if closed_mean else None, flux)
     if guide is not None:
         apply_guide(du, dv, dw, cu, cv, cw, guide_coeff, guide)
     coarse_to_fine(du, dv, dw, occ, u, v, w, s)
@@ -519,7 +532,8 @@ def coarse_projection(u, v, w, occ, s, iters, omega=1.0, open_mask=ALL_FACES, M=
     return p
 
 
-def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None, guide=None, guide_coeff=0.0):
+def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None, guide=None, guide_coeff=0.0, open_mask=ALL_FACES,
+                closed_mean=False, flux=False):
     """One step in the order of orc_step (ZQ_SmokeSimulation3D.cpp:156-243) with the coarse + sub-grid
     projection in place of the full-grid solve (COARSE_*_POISSON_SOR): open on all six sides, FP32
     SOR with params' omega and sor_iters on the coarse grid."""
@@ -529,7 +543,8 @@ def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None, guide=None, guide_c
     attenuation(u, v, w, T, rho, occ, P["aV"], P["aT"], P["aR"])
     dtt = float(np.float32(P["dt"]) / np.float32(P["steps"]))
     advect_velocity(u, v, w, occ, P["h"], P["steps"], dtt, P["adv_vel_type"], True)
-    coarse_projection(u, v, w, occ, s, P["sor_iters"], P["omega"], ALL_FACES, guide=guide, guide_coeff=guide_coeff)
+    coarse_projection(u, v, w, occ, s, P["sor_iters"], P["omega"], open_mask, guide=guide, guide_coeff=guide_coeff,
+                      closed_mean=closed_mean, flux=flux)
     To, Ro = advect_scalar(u, v, w, occ, T, rho, P["h"], P["steps"], dtt, P["adv_sca_type"])
     T[...] = To
     rho[...] = Ro

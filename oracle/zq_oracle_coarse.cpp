@@ -106,8 +106,20 @@ void orc_coarse_setup(int W, int H, int D, int s, const uint8_t* occ, float* rU,
 // the in-domain sides, in that order (ZQ_InitMatOpenPoisson3D, ZQ_SmokeSimulationUtils3D.cpp:570-589,
 // where every out-of-domain side is open).  A solid neighbour's face has fraction 0.
 float orc_coarse_inv_diag(int Wc, int Hc, int Dc, int i, int j, int k, const float* rU, const float* rV,
-                          const float* rW, int open_mask) {
+                          const float* rW, int open_mask, int flux) {
     const CD d{Wc, Hc, Dc, 1, Wc, Hc, Dc};
+    if (flux) {
+        // the flux KKT systems (ZQ_InitMatOpenFlux3D :749-936, ZQ_InitMatClosedFlux3D :937-1150):
+        // every side by its own fraction, an out-of-domain side only when it is open
+        float c = 0.0f;
+        if (i > 0 || (open_mask & ORC_OPEN_XLO)) c += rU[d.cu(i, j, k)];
+        if (i < Wc - 1 || (open_mask & ORC_OPEN_XHI)) c += rU[d.cu(i + 1, j, k)];
+        if (j > 0 || (open_mask & ORC_OPEN_YLO)) c += rV[d.cv(i, j, k)];
+        if (j < Hc - 1 || (open_mask & ORC_OPEN_YHI)) c += rV[d.cv(i, j + 1, k)];
+        if (k > 0 || (open_mask & ORC_OPEN_ZLO)) c += rW[d.cw(i, j, k)];
+        if (k < Dc - 1 || (open_mask & ORC_OPEN_ZHI)) c += rW[d.cw(i, j, k + 1)];
+        return c > 0.0f ? 1.0f / c : 0.0f;
+    }
     int nopen = 0;
     nopen += (i == 0) && (open_mask & ORC_OPEN_XLO);
     nopen += (i == Wc - 1) && (open_mask & ORC_OPEN_XHI);
@@ -173,10 +185,14 @@ void orc_fine_to_coarse(int W, int H, int D, int s, const float* u, const float*
 // projection then moves a face with fraction > 0 by g = p_r - p_l (p = 0 outside an open domain
 // face; closed domain faces and faces with fraction 0 stay), as ZQ_AdjustOpenVelocity3D does
 // (:2794-2865), and delta = (u - g) - u.  open_mask selects the open domain faces (the reference:
-// all six).  p: (Wc+2)(Hc+2)(Dc+2), zero on entry.
+// all six).  p: (Wc+2)(Hc+2)(Dc+2), zero on entry.  vol (nullable): closed domains, the
+// divergence's mean per unit fluid volume is removed first (the *_CLOSED_FLUX_SOR solvers,
+// :2465-2520, 2734-2792, whose ZQlibCuda solves are absent: the weighting is this restatement's).
+// flux != 0 takes the flux form's diagonal (orc_coarse_inv_diag).
 void orc_coarse_project(int Wc, int Hc, int Dc, const float* cu, const float* cv, const float* cw,
                         const uint8_t* cocc, const float* rU, const float* rV, const float* rW, int open_mask,
-                        float omega, int iters, float* p, float* du, float* dv, float* dw) {
+                        float omega, int iters, float* p, float* du, float* dv, float* dw, const float* vol,
+                        int flux) {
     const CD d{Wc, Hc, Dc, 1, Wc, Hc, Dc};
     const size_t N = (size_t)Wc * Hc * Dc;
     std::vector<float> div(N, 0.0f), inv(N, 0.0f);
@@ -187,8 +203,20 @@ void orc_coarse_project(int Wc, int Hc, int Dc, const float* cu, const float* cv
                 if (cocc[o]) continue;
                 div[o] = cu[d.cu(i + 1, j, k)] - cu[d.cu(i, j, k)] + cv[d.cv(i, j + 1, k)] - cv[d.cv(i, j, k)] +
                          cw[d.cw(i, j, k + 1)] - cw[d.cw(i, j, k)];
-                inv[o] = orc_coarse_inv_diag(Wc, Hc, Dc, i, j, k, rU, rV, rW, open_mask);
+                inv[o] = orc_coarse_inv_diag(Wc, Hc, Dc, i, j, k, rU, rV, rW, open_mask, flux);
             }
+    if (vol) {
+        // closed domains: div -= mean * vol, mean = sum(div) / sum(vol) over non-solid cells, FP64
+        double sd = 0.0, sv = 0.0;
+        for (size_t o = 0; o < N; o++)
+            if (!cocc[o]) {
+                sd += (double)div[o];
+                sv += (double)vol[o];
+            }
+        const double mean = sv > 0.0 ? sd / sv : 0.0;
+        for (size_t o = 0; o < N; o++)
+            if (!cocc[o]) div[o] = (float)((double)div[o] - mean * (double)vol[o]);
+    }
     const size_t sy = (size_t)(Wc + 2), sz = (size_t)(Wc + 2) * (Hc + 2);
     const float omw = 1.0f - omega;
     for (int it = 0; it < iters; it++)

@@ -72,6 +72,10 @@ def test_bad_arguments_fail_without_gpu():
     cfg.sor.mixed_precision = 1
     assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
     L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.coarse_flux = 1  # without a coarse grid
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    assert "coarse_flux" in L.last_error()
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
     cfg.sor.omega = 2.5
     assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
     assert L.lib().Attenuation3D(None, None, None, None, None, None, 1.0, 1.0, 1.0, 8, 8, 8) < 0

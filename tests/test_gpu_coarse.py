@@ -242,3 +242,54 @@ def test_guiding_matches_oracle():
     qu, qv, qw = u.copy(), v.copy(), w.copy()
     O.coarse_projection(qu, qv, qw, occ, s, 30, 1.0, O.ALL_FACES)
     assert_same(pu[:, :, ::s], qu[:, :, ::s], "u coarse planes without guide")
+
+
+def run_gpu_flux(W, H, D, s, u, v, w, occ, iters, omega, closed):
+    from zq_smokesimulation_amd.sim import SmokeSim3D
+    import zq_smokesimulation_amd._lib as L
+    sim = SmokeSim3D(W, H, D, sub_size=s, coarse_flux=1, sor_iters=iters, sor_omega=omega,
+                     sor_open_mask=0 if closed else O.ALL_FACES, sor_closed_mean=1 if closed else 0)
+    try:
+        sim.set_state(u=u, v=v, w=w, occ=occ)
+        sim.project()
+        return sim.download(L.FIELD_U), sim.download(L.FIELD_V), sim.download(L.FIELD_W)
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("closed", [False, True], ids=["open", "closed"])
+@pytest.mark.parametrize("s", [1, 2, 4])
+def test_flux_form_matches_oracle(s, closed):
+    """The *_FLUX_SOR solvers (TOTAL_* at s = 1): open faces bit-exact on coarse planes; closed
+    domains subtract the FP64 mean divergence per unit volume, summed in another order on the device,
+    so they agree to rounding."""
+    W, H, D = 32, 24, 16
+    rng = np.random.default_rng(40 + s + 10 * closed)
+    print("flux", "closed" if closed else "open", "s", s, flush=True)
+    u, v, w = faces(rng, W, H, D)
+    occ = scene(rng, W, H, D, 0.03)
+    if closed:
+        # solid cells on the domain faces make the flux form differ from the Poisson form
+        occ[:, 0, : W // 2] = 1
+    gu, gv, gw = run_gpu_flux(W, H, D, s, u, v, w, occ, 40, 1.2, closed)
+    ru, rv, rw = u.copy(), v.copy(), w.copy()
+    O.coarse_projection(ru, rv, rw, occ, s, 40, 1.2, 0 if closed else O.ALL_FACES, closed_mean=closed, flux=True)
+    scale = max(np.abs(ru).max(), np.abs(rv).max(), np.abs(rw).max())
+    if not closed:
+        assert_same(gu[:, :, ::s], ru[:, :, ::s], "u on coarse planes")
+        assert_same(gv[:, ::s, :], rv[:, ::s, :], "v on coarse planes")
+        assert_same(gw[::s], rw[::s], "w on coarse planes")
+    for g, r, n in ((gu, ru, "u"), (gv, rv, "v"), (gw, rw, "w")):
+        np.testing.assert_allclose(g, r, rtol=0, atol=2e-5 * scale, err_msg=n)
+    if closed:
+        # closed walls keep their values, except y = H: the reference's coarse -> fine map adds the
+        # delta of coarse v row Hc-1 there (ZQ_SmokeSimulationUtils3D.cpp:3251), walls included
+        assert_same(gu[:, :, [0, W]], u[:, :, [0, W]], "u walls")
+        assert_same(gv[:, 0, :], v[:, 0, :], "v wall y = 0")
+        assert_same(gw[[0, D]], w[[0, D]], "w walls")
+    elif s > 1:
+        # the flux weighting of partly solid open domain faces is in effect: the Poisson form differs
+        # (at s = 1 every fraction is 0 or 1 and the two forms coincide)
+        pu, pv, pw = u.copy(), v.copy(), w.copy()
+        O.coarse_projection(pu, pv, pw, occ, s, 40, 1.2, O.ALL_FACES)
+        assert np.abs(pu - ru).max() > 1e-4 * scale

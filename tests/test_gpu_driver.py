@@ -60,7 +60,11 @@ def oracle_frames(path, frames, moving=False, sub_size=0, guides=None, guide_coe
     for f in range(frames):
         if moving:
             occ = scene.apply_moving_objects(path, f, u, v, w)
-        if sub_size:
+        if c.coarse_flux:
+            # the *_FLUX_SOR solvers: the flux form on the coarse grid (s = 1 for TOTAL_*)
+            O.coarse_step(p, u, v, w, T, R, occ, c.sub_size, boxes=boxes, open_mask=c.sor.open_mask,
+                          closed_mean=bool(c.sor.closed_mean), flux=True)
+        elif sub_size:
             O.coarse_step(p, u, v, w, T, R, occ, sub_size, boxes=boxes,
                           guide=guides[f] if guides else None, guide_coeff=guide_coeff)
         else:
@@ -109,6 +113,29 @@ def test_cli_coarse_projection(tmp_path, resident, solver):
     for f in range(3):
         scale = max(1.0, float(np.abs(want[f]).max()))
         np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-4 * scale, err_msg="frame %d" % f)
+    assert want[2].max() > 0.1
+
+
+@pytest.mark.parametrize("resident", ["false", "true"])
+@pytest.mark.parametrize("solver", ["TOTAL_OPEN_FLUX_SOR", "TOTAL_CLOSED_FLUX_SOR", "COARSE_OPEN_FLUX_SOR",
+                                    "COARSE_CLOSED_FLUX_SOR"])
+def test_cli_flux_solvers(tmp_path, resident, solver):
+    """The flux solvers: (maxIter-1)/innerIter+1 passes of innerIter iterations (40, 15 -> 45).
+    TOTAL_OPEN_FLUX_SOR (s = 1, open) is bit-exact; block interiors (COARSE_*) and the closed
+    mean (summed in another order on the device) agree to 1e-4 of the density scale."""
+    from zq_smokesimulation_amd import scene
+    text = BASE + "frameCount 3\nintObj sphere\nsubSize 4\ninnerIter 15\nglobalGridSolver %s\ndeviceResident %s\n" % (
+        solver, resident)
+    got = run_cli(tmp_path, text, 3)
+    c = scene.load_scene(str(tmp_path / "scene.txt")).config
+    assert c.coarse_flux == 1 and c.sor.iters == 45 and c.sub_size == (1 if solver.startswith("TOTAL") else 4)
+    want = oracle_frames(str(tmp_path / "scene.txt"), 3)
+    for f in range(3):
+        if solver == "TOTAL_OPEN_FLUX_SOR":
+            np.testing.assert_array_equal(got[f], want[f], err_msg="frame %d" % f)
+        else:
+            scale = max(1.0, float(np.abs(want[f]).max()))
+            np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-4 * scale, err_msg="frame %d" % f)
     assert want[2].max() > 0.1
 
 

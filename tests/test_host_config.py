@@ -108,6 +108,23 @@ def test_closed_solver_sets_closed_mean(tmp_path):
     assert s.wall_faces == O.ALL_FACES
 
 
+@pytest.mark.parametrize("solver, sub, closed", [("TOTAL_OPEN_FLUX_SOR", 1, 0), ("TOTAL_CLOSED_FLUX_SOR", 1, 1),
+                                                 ("COARSE_OPEN_FLUX_SOR", 4, 0), ("COARSE_CLOSED_FLUX_SOR", 4, 1)])
+def test_flux_solvers_map_to_the_coarse_flux_form(tmp_path, solver, sub, closed):
+    # (maxIter-1)/innerIter+1 passes of innerIter iterations: 40, 15 -> 3 passes, 45 iterations
+    p = write_scene(tmp_path, "resolution 16 12 8\nsubSize 4\nmaxIter 40\ninnerIter 15\nglobalGridSolver %s\n" % solver)
+    c = scene.load_scene(p).config
+    assert (c.sub_size, c.coarse_flux, c.sor.iters, c.sor.mixed_precision) == (sub, 1, 45, 0)
+    assert (c.sor.open_mask, c.sor.closed_mean) == ((0, 1) if closed else (O.ALL_FACES, 0))
+
+
+def test_coarse_poisson_solvers_keep_max_iter(tmp_path):
+    for solver in ("COARSE_OPEN_POISSON_SOR", "COARSE_CLOSED_POISSON_SOR"):
+        p = write_scene(tmp_path, "resolution 16 12 8\nsubSize 4\nmaxIter 40\ninnerIter 15\nglobalGridSolver %s\n" % solver)
+        c = scene.load_scene(p).config
+        assert (c.sub_size, c.coarse_flux, c.sor.iters, c.sor.open_mask) == (4, 0, 40, O.ALL_FACES)
+
+
 def test_box_object_closed_form():
     # |k+.5-D/2| < D/20, |j+.5-H/2| < H/20, |i+.5-W/2| < W/10 with integer D/20 etc.: at 64^3
     # k, j in 29..34 and i in 26..37

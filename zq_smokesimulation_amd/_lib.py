@@ -65,6 +65,7 @@ class Config(C.Structure):
         ("sor", SorOpts),
         ("warm_start", C.c_int),
         ("sub_size", C.c_int),
+        ("coarse_flux", C.c_int),
     ]
 
 

@@ -6,10 +6,11 @@
 // Solvers: TOTAL_{OPEN,CLOSED}_POISSON_SOR are the full-grid red-black SOR this library builds.
 // COARSE_{OPEN,CLOSED}_POISSON_SOR (the reference's default) is the coarse + sub-grid projection
 // (zq3d_config.sub_size = subSize); both run the open face-fraction solve on the coarse grid, as
-// the reference's ZQ_TotalProjection3D does (ZQ_SmokeSimulationUtils3D.cpp:3730-3745).  In the
-// staged host path that projection is one device call on a projection-only simulation (the
-// fields cross PCIe as in the other stages).  The flux, CPU PCG and octree solvers are not built
-// and Init fails for them.
+// the reference's ZQ_TotalProjection3D does (ZQ_SmokeSimulationUtils3D.cpp:3730-3745).  The four
+// *_FLUX_SOR solvers run the flux form of that solve (TOTAL_* on the fine grid, s = 1).  In the
+// staged host path those projections are one device call on a projection-only simulation (the
+// fields cross PCIe as in the other stages).  The CPU PCG and octree solvers are not built and
+// Init fails for them.
 #include "ZQ_SmokeSimulation3D.h"
 #include "ZQ_SmokeGuideGrid3D.h"
 
@@ -27,7 +28,8 @@ double now_s() {
 
 bool solver_built(GlobalSolverType3D s) {
     return s == TOTAL_OPEN_POISSON_SOR || s == TOTAL_CLOSED_POISSON_SOR || s == COARSE_OPEN_POISSON_SOR ||
-           s == COARSE_CLOSED_POISSON_SOR;
+           s == COARSE_CLOSED_POISSON_SOR || s == COARSE_OPEN_FLUX_SOR || s == COARSE_CLOSED_FLUX_SOR ||
+           s == TOTAL_OPEN_FLUX_SOR || s == TOTAL_CLOSED_FLUX_SOR;
 }
 
 // counter-based hash noise, same as the device-resident test fixtures: lowbias32 of (index, seed)
@@ -153,7 +155,6 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
         zq3d_config c;
         zq3d_config_default(&c, para.width, para.height, para.depth);
         c.sor.omega = para.omega;
-        c.sor.iters = para.maxIter;
         set_coarse(&c);
         if (zq3d_create(&c, &proj) != ZQ3D_OK) {
             std::printf("device projection: %s\n", zq3d_last_error());
@@ -165,16 +166,12 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
 }
 
 bool ZQ_SmokeSimulation3D::is_coarse() const {
-    return para.globalGridSolver == COARSE_OPEN_POISSON_SOR || para.globalGridSolver == COARSE_CLOSED_POISSON_SOR;
+    zq3d_config c{};
+    return map_coarse_solver(para, &c);
 }
 
-// the coarse solve of both COARSE_*_POISSON_SOR solvers: open face-fraction SOR, FP32, maxIter iterations
-void ZQ_SmokeSimulation3D::set_coarse(zq3d_config* c) const {
-    c->sub_size = para.subSize;
-    c->sor.open_mask = ZQ3D_OPEN_ALL;
-    c->sor.closed_mean = 0;
-    c->sor.mixed_precision = 0;
-}
+// the coarse-grid solves (COARSE_* and the flux solvers), map_coarse_solver
+void ZQ_SmokeSimulation3D::set_coarse(zq3d_config* c) const { map_coarse_solver(para, c); }
 
 bool ZQ_SmokeSimulation3D::SyncToHost() {
     if (!dev || host_current) return true;

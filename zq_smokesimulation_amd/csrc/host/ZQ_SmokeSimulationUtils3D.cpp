@@ -443,6 +443,21 @@ bool ZQ_Attenuation3D(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& 
     return true;
 }
 
+bool map_coarse_solver(const ZQ_SmokeSimulationPara3D& para, zq3d_config* c) {
+    const GlobalSolverType3D t = para.globalGridSolver;
+    const bool flux = t == COARSE_OPEN_FLUX_SOR || t == COARSE_CLOSED_FLUX_SOR || t == TOTAL_OPEN_FLUX_SOR ||
+                      t == TOTAL_CLOSED_FLUX_SOR;
+    if (!flux && t != COARSE_OPEN_POISSON_SOR && t != COARSE_CLOSED_POISSON_SOR) return false;
+    const bool closed = t == COARSE_CLOSED_FLUX_SOR || t == TOTAL_CLOSED_FLUX_SOR;
+    c->sub_size = (t == TOTAL_OPEN_FLUX_SOR || t == TOTAL_CLOSED_FLUX_SOR) ? 1 : para.subSize;
+    c->coarse_flux = flux ? 1 : 0;
+    c->sor.open_mask = closed ? 0 : ZQ3D_OPEN_ALL;
+    c->sor.closed_mean = closed ? 1 : 0;
+    c->sor.mixed_precision = 0;
+    c->sor.iters = flux ? ((para.maxIter - 1) / para.innerIter + 1) * para.innerIter : para.maxIter;
+    return true;
+}
+
 // ZQ_SmokeSimulationPara3D order -> ZQ_CUDA_Advection3D order (ZQ_SmokeSimulationUtils3D.cpp:2125-2148, 3802-3819)
 bool map_vel_type(ZQ_SmokeSimulationPara3D::AdvectVelocityType t, ZQ_CUDA_Advection3D::AdvectVelocityType* out) {
     switch (t) {

@@ -130,6 +130,12 @@ bool ZQ_ApplyMovingObject3D(ZQ_SmokeGrid3D* fineGrid, const std::vector<ZQ_Smoke
 bool ZQ_AddForce3D(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& para);
 bool ZQ_Attenuation3D(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& para);
 bool ZQ_AdvectVelocity3D(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& para);
+// The projection settings of the solvers that run on the coarse-grid machinery: COARSE_*_POISSON_SOR
+// (both the open face-fraction solve, as ZQ_TotalProjection3D calls it, :3730-3745), COARSE_*_FLUX_SOR
+// and TOTAL_*_FLUX_SOR (the flux form; TOTAL_* on the fine grid itself, s = 1), the flux solvers with
+// (maxIter-1)/innerIter+1 passes of innerIter iterations (:2452-2458).  Returns false (c untouched)
+// for the other solvers.
+bool map_coarse_solver(const ZQ_SmokeSimulationPara3D& para, zq3d_config* c);
 // the config's velocity advection type in ZQ_CUDA_Advection3D's numbering (false: unknown)
 bool map_vel_type(ZQ_SmokeSimulationPara3D::AdvectVelocityType t, ZQ_CUDA_Advection3D::AdvectVelocityType* out);
 bool ZQ_SolveOpenPressure3DSOR(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& para);

@@ -38,13 +38,7 @@ extern "C" int zq3d_scene_load(const char* path, zq3d_scene* out) {
     c.sor.max_outer = p.maxOuter;
     c.sor.tol = p.sorTol;
     c.warm_start = p.warmStart ? 1 : 0;
-    if (p.globalGridSolver == COARSE_OPEN_POISSON_SOR || p.globalGridSolver == COARSE_CLOSED_POISSON_SOR) {
-        // both run the open face-fraction solve on the coarse grid (ZQ_SmokeSimulationUtils3D.cpp:3730-3745)
-        c.sub_size = p.subSize;
-        c.sor.open_mask = ZQ3D_OPEN_ALL;
-        c.sor.closed_mean = 0;
-        c.sor.mixed_precision = 0;
-    }
+    map_coarse_solver(p, &c);
     out->frame_count = p.frameCount;
     out->solver = (int)p.globalGridSolver;
     out->wall_faces = p.WallFaceMask();

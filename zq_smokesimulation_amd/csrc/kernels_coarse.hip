@@ -118,13 +118,14 @@ __global__ __launch_bounds__(256) void k_coarse_fractions(CG g, const uint8_t* _
 
 // One thread per coarse cell: solid when all six fractions are 0 (:1837-1858); 1/c of the
 // face-fraction operator with c = (open out-of-domain sides) + r(x-) + r(x+) + r(y-) + r(y+) +
-// r(z-) + r(z+) over the in-domain sides (the system of ZQ_InitMatOpenPoisson3D, :570-589), 0 for
-// solid cells and c == 0; regular = no solid fine cell in the block (:3318-3340).  derive == 0 takes
+// r(z-) + r(z+) over the in-domain sides (the system of ZQ_InitMatOpenPoisson3D, :570-589), or with
+// flux != 0 the sum of all six fractions, out-of-domain sides only when open (the flux KKT systems,
+// :749-1150); 0 for solid cells and c == 0; regular = no solid fine cell in the block (:3318-3340).  derive == 0 takes
 // the coarse occupancy as given in cocc instead (the C ABI solve's caller passes it).
 __global__ __launch_bounds__(256) void k_coarse_cells(CG g, const uint8_t* __restrict__ occ, int open_mask,
                                                       const float* __restrict__ rU, const float* __restrict__ rV,
                                                       const float* __restrict__ rW, uint8_t* cocc, float* inv,
-                                                      uint8_t* regular, int derive) {
+                                                      uint8_t* regular, float* vol, int derive, int flux) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)g.Wc * g.Hc * g.Dc) return;
     const int i = (int)(t % g.Wc), j = (int)((t / g.Wc) % g.Hc), k = (int)(t / ((size_t)g.Wc * g.Hc));
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void k_coarse_cells(CG g, const uint8_t* __res
     const float r4 = rW[c_w(g, i, j, k)], r5 = rW[c_w(g, i, j, k + 1)];
     const bool solid = derive ? (r0 == 0 && r1 == 0 && r2 == 0 && r3 == 0 && r4 == 0 && r5 == 0) : cocc[t] != 0;
     float iv = 0.0f;
-    if (!solid) {
+    if (!solid && !flux) {
         int nopen = 0;
         nopen += (i == 0) && (open_mask & 1);
         nopen += (i == g.Wc - 1) && (open_mask & 2);
@@ -149,17 +150,62 @@ __global__ __launch_bounds__(256) void k_coarse_cells(CG g, const uint8_t* __res
         if (k > 0) c += r4;
         if (k < g.Dc - 1) c += r5;
         iv = c > 0.0f ? 1.0f / c : 0.0f;
+    } else if (!solid) {
+        // flux form: every side by its own fraction, an out-of-domain side only when open
+        float c = 0.0f;
+        if (i > 0 || (open_mask & 1)) c += r0;
+        if (i < g.Wc - 1 || (open_mask & 2)) c += r1;
+        if (j > 0 || (open_mask & 4)) c += r2;
+        if (j < g.Hc - 1 || (open_mask & 8)) c += r3;
+        if (k > 0 || (open_mask & 16)) c += r4;
+        if (k < g.Dc - 1 || (open_mask & 32)) c += r5;
+        iv = c > 0.0f ? 1.0f / c : 0.0f;
     }
     if (derive) cocc[t] = solid;
     inv[t] = iv;
     if (regular) {
         const int s = g.s;
-        int any = 0;
+        int n = 0;
         for (int sk = 0; sk < s; sk++)
             for (int sj = 0; sj < s; sj++)
-                for (int si = 0; si < s; si++) any |= occ[f_c(g, i * s + si, j * s + sj, k * s + sk)];
-        regular[t] = !any;
+                for (int si = 0; si < s; si++) n += occ[f_c(g, i * s + si, j * s + sj, k * s + sk)] != 0;
+        regular[t] = n == 0;
+        // BuildVolumeUnOccupyRatio (:1860-1885)
+        vol[t] = 1.0f - (float)n / (float)(s * s * s);
     }
+}
+
+// Closed domains: mean = sum(div) / sum(vol) over the non-solid coarse cells, FP64, one workgroup
+// of 1024 threads (thread t sums cells t, t+1024, ... in order, then a fixed tree), then
+// div = (float)(div - mean * vol) on those cells.
+__global__ __launch_bounds__(1024) void k_coarse_mean(int n, const float* __restrict__ div, const float* __restrict__ vol,
+                                                      const uint8_t* __restrict__ cocc, double* mean) {
+    __shared__ double sd[1024], sv[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int c = t; c < n; c += 1024)
+        if (!cocc[c]) {
+            a += (double)div[c];
+            b += (double)vol[c];
+        }
+    sd[t] = a;
+    sv[t] = b;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) {
+            sd[t] += sd[t + h];
+            sv[t] += sv[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) *mean = sv[0] > 0.0 ? sd[0] / sv[0] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_coarse_sub_mean(int n, float* __restrict__ div, const float* __restrict__ vol,
+                                                         const uint8_t* __restrict__ cocc, const double* mean) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || cocc[c]) return;
+    div[c] = (float)((double)div[c] - *mean * (double)vol[c]);
 }
 
 // ------------------------------------------------------------------------------ averaging
@@ -863,12 +909,14 @@ void sub_launch(const CG& g, const CoarseBufs& B, const uint8_t* occ, float* u, 
 
 }  // namespace
 
-void launch_coarse_setup(const CoarseDims& cd, const uint8_t* occ, int open_mask, const CoarseBufs& B, hipStream_t s) {
+void launch_coarse_setup(const CoarseDims& cd, const uint8_t* occ, int open_mask, const CoarseBufs& B, hipStream_t s,
+                         int flux) {
     const CG g = make_cg(cd);
     const size_t nf = cd.Nu() + cd.Nv() + cd.Nw();
     hipLaunchKernelGGL(k_coarse_fractions, dim3(blocks_for(nf, 256)), dim3(256), 0, s, g, occ, B.rU, B.rV, B.rW);
     hipLaunchKernelGGL(k_coarse_cells, dim3(blocks_for(cd.Nc(), 256)), dim3(256), 0, s, g, occ, open_mask,
-                       (const float*)B.rU, (const float*)B.rV, (const float*)B.rW, B.cocc, B.inv, B.regular, 1);
+                       (const float*)B.rU, (const float*)B.rV, (const float*)B.rW, B.cocc, B.inv, B.regular, B.vol, 1,
+                       flux);
 }
 
 void launch_coarse_restrict(const CoarseDims& cd, const float* u, const float* v, const float* w, const CoarseBufs& B,
@@ -888,7 +936,7 @@ void launch_coarse_prepare(const CoarseDims& cd, const float* u, const float* v,
     // 1/diagonal from the caller's fractions and coarse occupancy (B.rU/rV/rW, B.cocc)
     hipLaunchKernelGGL(k_coarse_cells, dim3(blocks_for(cd.Nc(), 256)), dim3(256), 0, s, g, (const uint8_t*)nullptr,
                        open_mask, (const float*)B.rU, (const float*)B.rV, (const float*)B.rW, B.cocc, B.inv,
-                       (uint8_t*)nullptr, 0);
+                       (uint8_t*)nullptr, (float*)nullptr, 0, 0);
     hipLaunchKernelGGL(k_coarse_div, dim3(blocks_for(cd.Nc(), 256)), dim3(256), 0, s, g, u, v, w,
                        (const uint8_t*)B.cocc, B.div);
     launch_fill_f32(B.p, cd.Np(), 0.0f, s);
@@ -899,6 +947,14 @@ constexpr size_t kSlabMaxLds = 160 * 1024;
 
 // LDS of the largest slab a schedule launches (the first workgroup's halo may be clipped, an interior
 // one's is not)
+void launch_coarse_remove_mean(const CoarseDims& cd, const CoarseBufs& B, hipStream_t s) {
+    const int n = (int)cd.Nc();
+    hipLaunchKernelGGL(k_coarse_mean, dim3(1), dim3(1024), 0, s, n, (const float*)B.div, (const float*)B.vol,
+                       (const uint8_t*)B.cocc, B.mean);
+    hipLaunchKernelGGL(k_coarse_sub_mean, dim3(blocks_for((size_t)n, 256)), dim3(256), 0, s, n, B.div,
+                       (const float*)B.vol, (const uint8_t*)B.cocc, (const double*)B.mean);
+}
+
 size_t coarse_slab_lds(const CoarseDims& cd, CoarseSweep sw) {
     if (sw.k <= 0) return 0;
     size_t most = 0;

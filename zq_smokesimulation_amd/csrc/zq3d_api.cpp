@@ -717,6 +717,10 @@ int check_config(const zq3d_config& c, const char* where) {
         set_err(where, "sub_size must be 0 or in [1, 8] dividing width, height and depth");
         return ZQ3D_ERR_ARG;
     }
+    if (c.coarse_flux && !c.sub_size) {
+        set_err(where, "coarse_flux selects the flux form of the coarse solve: it needs sub_size > 0");
+        return ZQ3D_ERR_ARG;
+    }
     if (c.sub_size && c.sor.mixed_precision) {
         set_err(where, "sub_size > 0 runs the coarse FP32 SOR: sor.mixed_precision must be 0");
         return ZQ3D_ERR_ARG;
@@ -800,6 +804,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
             zq3d_destroy(s);
             return rc;
         }
+        s->coarse.flux = cfg->coarse_flux ? 1 : 0;
     }
     hipStream_t st = nullptr;
     launch_fill_f32(s->u, z.Nu, 0.0f, st);

@@ -37,6 +37,9 @@ size_t coarse_layout(const CoarseDims& cd, char* base, CoarseBufs* B) {
     f(&o->div, cd.Nc());
     f(&o->p, cd.Np());
     f(&o->p2, cd.Np());
+    f(&o->vol, cd.Nc());
+    if (base) o->mean = (double*)(base + off);
+    off += 256;
     f(&o->gU, cd.Nu());
     f(&o->gV, cd.Nv());
     f(&o->gW, cd.Nw());
@@ -135,8 +138,9 @@ int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uin
                    const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf) {
     const char* where = "project_coarse";
     // the fractions follow the occupancy, which callers may change between projections
-    launch_coarse_setup(c.cd, occ, o.open_mask, c.B, s);
+    launch_coarse_setup(c.cd, occ, o.open_mask, c.B, s, c.flux);
     launch_coarse_restrict(c.cd, u, v, w, c.B, s);
+    if (o.closed_mean) launch_coarse_remove_mean(c.cd, c.B, s);
     if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
     const float* p = launch_coarse_sor(c.cd, c.B, o.omega, o.iters, c.sweep, s);
     if (pf) {

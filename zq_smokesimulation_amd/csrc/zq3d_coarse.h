@@ -41,11 +41,14 @@ struct CoarseBufs {
     uint8_t* cocc;           // coarse occupancy
     uint8_t* regular;        // 1: the block holds no solid fine cell
     float *gU, *gV, *gW;     // guide faces (guiding control)
+    float* vol;              // fluid volume fraction 1 - (solid fine cells)/s^3 per coarse cell
+    double* mean;            // closed domains: mean divergence per unit fluid volume
 };
 
-// fractions, coarse occupancy, 1/diagonal (open_mask: open domain faces) and block flags from the
-// fine occupancy
-void launch_coarse_setup(const CoarseDims& g, const uint8_t* occ, int open_mask, const CoarseBufs& B, hipStream_t s);
+// fractions, coarse occupancy, 1/diagonal (open_mask: open domain faces; flux: the flux form's
+// diagonal), volume fractions and block flags from the fine occupancy
+void launch_coarse_setup(const CoarseDims& g, const uint8_t* occ, int open_mask, const CoarseBufs& B, hipStream_t s,
+                         int flux = 0);
 // coarse faces = mean of their s x s fine sub-faces; coarse divergence; p = p2 = 0
 void launch_coarse_restrict(const CoarseDims& g, const float* u, const float* v, const float* w, const CoarseBufs& B,
                             hipStream_t s);
@@ -57,6 +60,9 @@ struct CoarseSweep {
 bool coarse_sweep_valid(const CoarseDims& g, CoarseSweep sw);
 // schedules valid for the grid (LDS of a slab within 160 KiB); returns the count
 int coarse_sweep_candidates(const CoarseDims& g, CoarseSweep* out, int max);
+// closed domains: div -= mean * vol on every non-solid coarse cell, mean = sum(div) / sum(vol) over
+// them in FP64 (one workgroup, fixed summation order, so repeatable)
+void launch_coarse_remove_mean(const CoarseDims& g, const CoarseBufs& B, hipStream_t s);
 // `iters` red-black iterations from B.p (colour 0 first); returns B.p or B.p2, whichever holds the result
 const float* launch_coarse_sor(const CoarseDims& g, const CoarseBufs& B, float omega, int iters, CoarseSweep sw,
                                hipStream_t s);

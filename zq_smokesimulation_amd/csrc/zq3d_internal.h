@@ -104,6 +104,7 @@ struct CoarseState {
     CoarseDims cd{};
     CoarseBufs B{};
     CoarseSweep sweep{0, 0};  // schedule of the coarse SOR (timed at creation)
+    int flux = 0;             // the flux form's diagonal (zq3d_config.coarse_flux)
     int guide = 0;            // guide faces set (guiding control)
     float guide_coeff = 0.0f;
     void* mem = nullptr;
