@@ -179,6 +179,27 @@ typedef struct orc_step_params {
 int orc_step(const orc_step_params* P, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
              const float* extra_force, double* p64_pad, double* stats);
 
+/* ---- octree projection with the red-black solver (OPEN_OCTREE_POISSON_SOR, CLOSED_OCTREE_POISSON_SOR,
+ * ZQ_SmokeOctreeGrid3D.cpp), in zq_oracle_octree.cpp.  W, H, D are multiples of 8; level L holds
+ * (W >> L, H >> L, D >> L) cells, x fastest.  Pinned by: the distance field against a brute-force
+ * taxicab distance, the leaf flags against their closed form, the neighbour lists against brute-force
+ * face adjacency, the converged solve against a direct sparse solve of the same system and against
+ * zero net flux through every leaf after the correction.  Parity unpinned: ZQlibCuda's closed octree
+ * solver (absent) -- the closed right-hand side div - mean * volume is defined here. */
+void orc_oct_distance(int W, int H, int D, const uint8_t* occ, int32_t* dist);
+void orc_oct_leaves(int W, int H, int D, const uint8_t* occ, const int32_t* dist, int t1, int t2, int t3, uint8_t* l0,
+                    uint8_t* l1, uint8_t* l2, uint8_t* l3);
+int orc_oct_neighbors(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, const uint8_t* l2, const uint8_t* l3,
+                      int L, int x, int y, int z, int32_t* out);
+void orc_oct_divergence(int W, int H, int D, const float* u, const float* v, const float* w, float* d0, float* d1,
+                        float* d2, float* d3);
+float orc_oct_closed_mean(int W, int H, int D, const uint8_t* occ, const float* u, const float* v, const float* w);
+void orc_oct_solve(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, const uint8_t* l2, const uint8_t* l3,
+                   const float* d0, const float* d1, const float* d2, const float* d3, int iters, int closed,
+                   float mean, float* p0, float* p1, float* p2, float* p3);
+void orc_oct_adjust(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, const uint8_t* l2, const uint8_t* l3,
+                    const float* p0, const float* p1, const float* p2, const float* p3, float* u, float* v, float* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ ALL_FACES = 63
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
 class StepParams(C.Structure):
@@ -103,6 +104,17 @@ def lib():
                                          _f32p, _f32p, _f32p, _f32p, vp, i]
         L.orc_coarse_to_fine.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p]
         L.orc_sub_project_regular.argtypes = [i, i, i, i, _f32p, _u8p, _f32p, _f32p, _f32p]
+        L.orc_oct_distance.argtypes = [i, i, i, _u8p, _i32p]
+        L.orc_oct_leaves.argtypes = [i, i, i, _u8p, _i32p, i, i, i, _u8p, _u8p, _u8p, _u8p]
+        L.orc_oct_neighbors.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, i, i, i, i, _i32p]
+        L.orc_oct_neighbors.restype = i
+        L.orc_oct_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.orc_oct_closed_mean.argtypes = [i, i, i, _u8p, _f32p, _f32p, _f32p]
+        L.orc_oct_closed_mean.restype = f
+        L.orc_oct_solve.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, i, i, f,
+                                    _f32p, _f32p, _f32p, _f32p]
+        L.orc_oct_adjust.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                     _f32p]
         _lib = L
     return _lib
 
@@ -550,3 +562,86 @@ def coarse_step(params, u, v, w, T, rho, occ, s, boxes=None, guide=None, guide_c
     rho[...] = Ro
     for b in boxes or []:
         reinject_box(T, rho, b)
+
+
+# ---- octree projection (OPEN_OCTREE_POISSON_SOR / CLOSED_OCTREE_POISSON_SOR) ------------------
+OCTREE_THRESH = (2, 4, 16)  # octree_thresh1..3 defaults (ZQ_SmokeSimulationUtils3D.cpp:52-54)
+
+
+def _level_shape(W, H, D, L):
+    return (D >> L, H >> L, W >> L)
+
+
+def octree_distance(occ):
+    W, H, D = _dims(occ)
+    dist = np.zeros((D, H, W), np.int32)
+    lib().orc_oct_distance(W, H, D, np.ascontiguousarray(occ, np.uint8), dist)
+    return dist
+
+
+def octree_leaves(occ, thresh=OCTREE_THRESH):
+    """Leaf flags (l0, l1, l2, l3) of _buildOctreeInfo (ZQ_SmokeOctreeGrid3D.cpp:5328-5609)."""
+    W, H, D = _dims(occ)
+    occ = np.ascontiguousarray(occ, np.uint8)
+    dist = octree_distance(occ)
+    ls = [np.zeros(_level_shape(W, H, D, L), np.uint8) for L in range(4)]
+    lib().orc_oct_leaves(W, H, D, occ, dist, int(thresh[0]), int(thresh[1]), int(thresh[2]), *ls)
+    return ls
+
+
+def octree_neighbors(leaves, L, x, y, z):
+    """The neighbour list of the level-L cell (x, y, z): rows (level, x, y, z) in the reference's order."""
+    D, H, W = leaves[0].shape
+    out = np.zeros((6 * 4 ** L, 4), np.int32)
+    n = lib().orc_oct_neighbors(W, H, D, *leaves, L, x, y, z, out)
+    return out[:n]
+
+
+def octree_divergence(u, v, w):
+    D, H, W1 = u.shape
+    W = W1 - 1
+    ds = [np.zeros(_level_shape(W, H, D, L), np.float32) for L in range(4)]
+    lib().orc_oct_divergence(W, H, D, u, v, w, *ds)
+    return ds
+
+
+def octree_solve(leaves, divs, iters, closed=False, mean=0.0):
+    D, H, W = leaves[0].shape
+    ps = [np.zeros(_level_shape(W, H, D, L), np.float32) for L in range(4)]
+    lib().orc_oct_solve(W, H, D, *leaves, *divs, int(iters), 1 if closed else 0, float(mean), *ps)
+    return ps
+
+
+def octree_adjust(leaves, ps, u, v, w):
+    D, H, W = leaves[0].shape
+    lib().orc_oct_adjust(W, H, D, *leaves, *ps, u, v, w)
+
+
+def octree_sub_project(u, v, w, leaves, mats=None):
+    """SubProjection (ZQ_SmokeOctreeGrid3D.cpp:840-1245): every level-L leaf's interior faces (L >= 1)
+    through the regular-block operator of ZQ_InitSubMatPoisson3D with s = 2^L (:4994-5032).  At level 1
+    the reference's write-back of the w rows never advances its row (:1125-1135), so every interior w
+    face of a level-1 leaf takes the first w row, the face (sk = 1, sj = 0, si = 0); kept."""
+    for L in (1, 2, 3):
+        s = 1 << L
+        if not leaves[L].any():
+            continue
+        M = sub_matrix(s) if mats is None else mats[L]
+        sub_project_regular(u, v, w, s, M, leaves[L])
+        if L == 1:
+            for bk, bj, bi in zip(*np.nonzero(leaves[1])):
+                z, y, x = 2 * bk + 1, 2 * bj, 2 * bi
+                w[z, y:y + 2, x:x + 2] = w[z, y, x]
+
+
+def octree_projection(u, v, w, occ, iters, thresh=OCTREE_THRESH, closed=False, mats=None):
+    """One octree projection in place (ZQ_SmokeSimulation3D.cpp:478-513 without guiding): build the
+    leaves, red-black solve over the levels (iters = maxIter), face correction, sub-projection of the
+    leaf blocks.  Returns the leaves and the pressures."""
+    leaves = octree_leaves(occ, thresh)
+    W, H, D = _dims(occ)
+    mean = lib().orc_oct_closed_mean(W, H, D, np.ascontiguousarray(occ, np.uint8), u, v, w) if closed else 0.0
+    ps = octree_solve(leaves, octree_divergence(u, v, w), iters, closed, mean)
+    octree_adjust(leaves, ps, u, v, w)
+    octree_sub_project(u, v, w, leaves, mats)
+    return leaves, ps
