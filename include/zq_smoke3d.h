@@ -111,6 +111,41 @@ float SolveOpenPoissonRedBlackwithFaceRatio3D_MAC(float* mac_u, float* mac_v, fl
                                                  const float* unoccupyU, const float* unoccupyV,
                                                  const float* unoccupyW, const int width, const int height,
                                                  const int depth, const int maxIter);
+/* replace the ZQlibCuda calls at ZQ_SmokeOctreeGrid3D.cpp:2897-2907 and 3000-3010 (signature as
+ * CPU_SolveOpenOctreePoissonRedBlack3_MAC, CPU_PoissonSolver3D_OpenOctreePoisson.h:5-14): maxIter
+ * passes of red-black Gauss-Seidel over the four leaf levels from p = 0, then the face correction
+ * between leaves (no sub-projection: the caller's SubProjection follows).  W, H, D multiples of 8.
+ * The index / neighbour lists are what _selectInfos_for_CUDAOpenPoissonRedBlack3 builds from the
+ * leaf flags; the same neighbours are derived from the flags on the device, so the lists are not
+ * read.  The closed form subtracts div_per_volume times each leaf's volume from its divergence. */
+float SolveOpenOctreePoissonRedBlack3_3D_MAC(
+    float* mac_u, float* mac_v, float* mac_w, const bool* leaf0, const bool* leaf1, const bool* leaf2,
+    const bool* leaf3, const int width, const int height, const int depth, const int maxIter,
+    const int level0_num_red, const int* level0_index_red, const int level0_info_len_red,
+    const int* level0_neighborinfo_red, const int level0_num_black, const int* level0_index_black,
+    const int level0_info_len_black, const int* level0_neighborinfo_black, const int level1_num_red,
+    const int* level1_index_red, const int level1_info_len_red, const int* level1_neighborinfo_red,
+    const int level1_num_black, const int* level1_index_black, const int level1_info_len_black,
+    const int* level1_neighborinfo_black, const int level2_num_red, const int* level2_index_red,
+    const int level2_info_len_red, const int* level2_neighborinfo_red, const int level2_num_black,
+    const int* level2_index_black, const int level2_info_len_black, const int* level2_neighborinfo_black,
+    const int level3_num_red, const int* level3_index_red, const int level3_info_len_red,
+    const int* level3_neighborinfo_red, const int level3_num_black, const int* level3_index_black,
+    const int level3_info_len_black, const int* level3_neighborinfo_black);
+float SolveClosedOctreePoissonRedBlack3_3D_MAC(
+    float* mac_u, float* mac_v, float* mac_w, const bool* leaf0, const bool* leaf1, const bool* leaf2,
+    const bool* leaf3, const float div_per_volume, const int width, const int height, const int depth,
+    const int maxIter, const int level0_num_red, const int* level0_index_red, const int level0_info_len_red,
+    const int* level0_neighborinfo_red, const int level0_num_black, const int* level0_index_black,
+    const int level0_info_len_black, const int* level0_neighborinfo_black, const int level1_num_red,
+    const int* level1_index_red, const int level1_info_len_red, const int* level1_neighborinfo_red,
+    const int level1_num_black, const int* level1_index_black, const int level1_info_len_black,
+    const int* level1_neighborinfo_black, const int level2_num_red, const int* level2_index_red,
+    const int level2_info_len_red, const int* level2_neighborinfo_red, const int level2_num_black,
+    const int* level2_index_black, const int level2_info_len_black, const int* level2_neighborinfo_black,
+    const int level3_num_red, const int* level3_index_red, const int level3_info_len_red,
+    const int* level3_neighborinfo_red, const int level3_num_black, const int* level3_index_black,
+    const int level3_info_len_black, const int* level3_neighborinfo_black);
 #ifdef __cplusplus
 }
 }
@@ -216,6 +251,17 @@ typedef struct zq3d_config {
      * domain face weighs by its own open fraction, as in the KKT system of ZQ_InitMatOpenFlux3D
      * (ZQ_SmokeSimulationUtils3D.cpp:749-936), instead of 1 (ZQ_InitMatOpenPoisson3D, :476-592). */
     int coarse_flux;
+    /* 1: the octree projection with the red-black solver (OPEN_OCTREE_POISSON_SOR /
+     * CLOSED_OCTREE_POISSON_SOR, ZQ_SmokeOctreeGrid3D.cpp): leaves of 1, 2^3, 4^3 and 8^3 cells chosen
+     * from the taxicab distance to the solid cells with thresholds octree_thresh (octree_thresh1..3,
+     * defaults 2, 4, 16), sor.iters Gauss-Seidel passes over the levels from p = 0, the face
+     * correction, then each 2^3 / 4^3 / 8^3 leaf's interior faces projected.  Every domain face beside
+     * a leaf is open (a ghost pressure 0 four cells out; sor.open_mask is not used -- closed domains
+     * carry a wall shell in the occupancy); with sor.closed_mean the mean divergence per fluid cell
+     * times each leaf's volume is removed.  Requires width, height, depth multiples of 8, sub_size 0
+     * and sor.mixed_precision 0. */
+    int octree;
+    int octree_thresh[3];
 } zq3d_config;
 
 typedef struct zq3d_sim zq3d_sim;

@@ -645,3 +645,20 @@ def octree_projection(u, v, w, occ, iters, thresh=OCTREE_THRESH, closed=False, m
     octree_adjust(leaves, ps, u, v, w)
     octree_sub_project(u, v, w, leaves, mats)
     return leaves, ps
+
+
+def octree_step(params, u, v, w, T, rho, occ, boxes=None, thresh=OCTREE_THRESH, closed=False, mats=None):
+    """One step in the order of orc_step (ZQ_SmokeSimulation3D.cpp:156-243) with the octree projection
+    (OPEN_OCTREE_POISSON_SOR / CLOSED_OCTREE_POISSON_SOR, params' sor_iters passes) in place of the
+    full-grid solve."""
+    P = dict(params)
+    add_force(u, v, w, occ, T, rho, P["h"], P["dt"], P["beta"], P["eps"], P["Tamb"], alpha=P.get("alpha", 0.0))
+    attenuation(u, v, w, T, rho, occ, P["aV"], P["aT"], P["aR"])
+    dtt = float(np.float32(P["dt"]) / np.float32(P["steps"]))
+    advect_velocity(u, v, w, occ, P["h"], P["steps"], dtt, P["adv_vel_type"], True)
+    octree_projection(u, v, w, occ, P["sor_iters"], thresh, closed, mats)
+    To, Ro = advect_scalar(u, v, w, occ, T, rho, P["h"], P["steps"], dtt, P["adv_sca_type"])
+    T[...] = To
+    rho[...] = Ro
+    for b in boxes or []:
+        reinject_box(T, rho, b)

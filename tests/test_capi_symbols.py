@@ -28,7 +28,8 @@ def test_header_declares_reference_entry_points():
     names = declared_functions(os.path.join(ROOT, "include", "zq_smoke3d.h"))
     for ref in ("ZQ_Cuda_Prepare_Advection", "Advect_Velocity", "Advect_Scalar", "AddForce3D", "Attenuation3D",
                 "SolveOpenPoissonRedBlackwithOccupy3D_MAC", "SolveClosedPoissonRedBlackwithOccupy3D_MAC",
-                "SolveOpenPoissonRedBlackwithFaceRatio3D_MAC", "MatrixMul"):
+                "SolveOpenPoissonRedBlackwithFaceRatio3D_MAC", "MatrixMul", "SolveOpenOctreePoissonRedBlack3_3D_MAC",
+                "SolveClosedOctreePoissonRedBlack3_3D_MAC"):
         assert ref in names
 
 
@@ -49,6 +50,7 @@ def test_config_struct_size_matches_library():
     assert cfg.struct_size == C.sizeof(L.Config)
     assert (cfg.width, cfg.height, cfg.depth) == (64, 32, 16)
     assert cfg.steps == 20 and cfg.Tamb == 10.0 and cfg.sor.open_mask == 63
+    assert cfg.octree == 0 and list(cfg.octree_thresh) == [2, 4, 16]
     assert abs(cfg.voxel_size - 1.0 / 64) < 1e-9
 
 
@@ -75,6 +77,22 @@ def test_bad_arguments_fail_without_gpu():
     cfg.coarse_flux = 1  # without a coarse grid
     assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
     assert "coarse_flux" in L.last_error()
+    L.lib().zq3d_config_default(C.byref(cfg), 12, 8, 8)
+    cfg.octree = 1  # 12 is no multiple of 8
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    assert "octree" in L.last_error()
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.octree = 1
+    cfg.sub_size = 2
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    cfg.octree = 1
+    cfg.octree_thresh = (1, -1, 4)
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    lists = [0, None, 0, None] * 8
+    assert L.lib().SolveOpenOctreePoissonRedBlack3_3D_MAC(None, None, None, None, None, None, None, 12, 8, 8, 5,
+                                                          *lists) < 0
+    assert "multiples of 8" in L.last_error()
     L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
     cfg.sor.omega = 2.5
     assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG

@@ -60,7 +60,10 @@ def oracle_frames(path, frames, moving=False, sub_size=0, guides=None, guide_coe
     for f in range(frames):
         if moving:
             occ = scene.apply_moving_objects(path, f, u, v, w)
-        if c.coarse_flux:
+        if c.octree:
+            O.octree_step(p, u, v, w, T, R, occ, boxes=boxes, thresh=tuple(c.octree_thresh),
+                          closed=bool(c.sor.closed_mean))
+        elif c.coarse_flux:
             # the *_FLUX_SOR solvers: the flux form on the coarse grid (s = 1 for TOTAL_*)
             O.coarse_step(p, u, v, w, T, R, occ, c.sub_size, boxes=boxes, open_mask=c.sor.open_mask,
                           closed_mean=bool(c.sor.closed_mean), flux=True)
@@ -136,6 +139,25 @@ def test_cli_flux_solvers(tmp_path, resident, solver):
         else:
             scale = max(1.0, float(np.abs(want[f]).max()))
             np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-4 * scale, err_msg="frame %d" % f)
+    assert want[2].max() > 0.1
+
+
+@pytest.mark.parametrize("resident", ["false", "true"])
+@pytest.mark.parametrize("solver", ["OPEN_OCTREE_POISSON_SOR", "CLOSED_OCTREE_POISSON_SOR"])
+def test_cli_octree_solvers(tmp_path, resident, solver):
+    """The octree solvers (leaves from octree_thresh1..3, maxIter passes): leaf interiors go through a
+    matrix-free solve on the device and a dense operator in the oracle, and the closed mean is summed
+    in another order, so frames agree to 1e-4 of the density scale."""
+    from zq_smokesimulation_amd import scene
+    text = BASE + ("frameCount 3\nintObj sphere\noctree_thresh1 1\noctree_thresh2 2\noctree_thresh3 4\n"
+                   "globalGridSolver %s\ndeviceResident %s\n" % (solver, resident))
+    got = run_cli(tmp_path, text, 3)
+    c = scene.load_scene(str(tmp_path / "scene.txt")).config
+    assert c.octree == 1 and list(c.octree_thresh) == [1, 2, 4] and c.sor.iters == 40
+    want = oracle_frames(str(tmp_path / "scene.txt"), 3)
+    for f in range(3):
+        scale = max(1.0, float(np.abs(want[f]).max()))
+        np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-4 * scale, err_msg="frame %d" % f)
     assert want[2].max() > 0.1
 
 

@@ -118,6 +118,24 @@ def test_flux_solvers_map_to_the_coarse_flux_form(tmp_path, solver, sub, closed)
     assert (c.sor.open_mask, c.sor.closed_mean) == ((0, 1) if closed else (O.ALL_FACES, 0))
 
 
+@pytest.mark.parametrize("solver, closed", [("OPEN_OCTREE_POISSON_SOR", 0), ("CLOSED_OCTREE_POISSON_SOR", 1)])
+def test_octree_solvers_map_to_the_octree_projection(tmp_path, solver, closed):
+    p = write_scene(tmp_path, "resolution 16 24 8\nmaxIter 33\noctree_thresh1 3\noctree_thresh3 9\n"
+                    "globalGridSolver %s\n" % solver)
+    s = scene.load_scene(p)
+    c = s.config
+    assert (c.octree, list(c.octree_thresh), c.sor.iters, c.sub_size, c.sor.mixed_precision) == (1, [3, 4, 9], 33, 0, 0)
+    assert c.sor.closed_mean == closed
+    # the closed solver's domain is walled by a shell of solid cells (its occupancy), not by open_mask
+    assert s.wall_faces == (O.ALL_FACES if closed else 0)
+
+
+def test_octree_pcg_solvers_stay_unmapped(tmp_path):
+    for solver in ("OPEN_OCTREE_POISSON", "CLOSED_OCTREE_FLUX"):
+        c = scene.load_scene(write_scene(tmp_path, "resolution 16 16 16\nglobalGridSolver %s\n" % solver)).config
+        assert c.octree == 0 and c.sub_size == 0
+
+
 def test_coarse_poisson_solvers_keep_max_iter(tmp_path):
     for solver in ("COARSE_OPEN_POISSON_SOR", "COARSE_CLOSED_POISSON_SOR"):
         p = write_scene(tmp_path, "resolution 16 12 8\nsubSize 4\nmaxIter 40\ninnerIter 15\nglobalGridSolver %s\n" % solver)

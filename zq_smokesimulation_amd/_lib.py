@@ -66,6 +66,8 @@ class Config(C.Structure):
         ("warm_start", C.c_int),
         ("sub_size", C.c_int),
         ("coarse_flux", C.c_int),
+        ("octree", C.c_int),
+        ("octree_thresh", C.c_int * 3),
     ]
 
 
@@ -147,6 +149,11 @@ def _load():
     L.SolveOpenPoissonRedBlackwithFaceRatio3D_MAC.restype = f
     L.MatrixMul.argtypes = [vp, vp, i, i, i, vp]
     L.MatrixMul.restype = f
+    lists = [i, vp, i, vp] * 8
+    L.SolveOpenOctreePoissonRedBlack3_3D_MAC.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, i, i] + lists
+    L.SolveOpenOctreePoissonRedBlack3_3D_MAC.restype = f
+    L.SolveClosedOctreePoissonRedBlack3_3D_MAC.argtypes = [vp, vp, vp, vp, vp, vp, vp, f, i, i, i, i] + lists
+    L.SolveClosedOctreePoissonRedBlack3_3D_MAC.restype = f
     L.zq3d_config_default.argtypes = [C.POINTER(Config), i, i, i]
     L.zq3d_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
     L.zq3d_set_guide.argtypes = [vp, f, vp, vp, vp]

@@ -7,10 +7,11 @@
 // COARSE_{OPEN,CLOSED}_POISSON_SOR (the reference's default) is the coarse + sub-grid projection
 // (zq3d_config.sub_size = subSize); both run the open face-fraction solve on the coarse grid, as
 // the reference's ZQ_TotalProjection3D does (ZQ_SmokeSimulationUtils3D.cpp:3730-3745).  The four
-// *_FLUX_SOR solvers run the flux form of that solve (TOTAL_* on the fine grid, s = 1).  In the
-// staged host path those projections are one device call on a projection-only simulation (the
-// fields cross PCIe as in the other stages).  The CPU PCG and octree solvers are not built and
-// Init fails for them.
+// *_FLUX_SOR solvers run the flux form of that solve (TOTAL_* on the fine grid, s = 1), and
+// {OPEN,CLOSED}_OCTREE_POISSON_SOR the octree projection (zq3d_config.octree).  In the staged host
+// path those projections are one device call on a projection-only simulation (the fields cross
+// PCIe as in the other stages).  The CPU PCG solvers (plain and octree) and guiding with the octree
+// are not built and Init fails for them.
 #include "ZQ_SmokeSimulation3D.h"
 #include "ZQ_SmokeGuideGrid3D.h"
 
@@ -29,7 +30,8 @@ double now_s() {
 bool solver_built(GlobalSolverType3D s) {
     return s == TOTAL_OPEN_POISSON_SOR || s == TOTAL_CLOSED_POISSON_SOR || s == COARSE_OPEN_POISSON_SOR ||
            s == COARSE_CLOSED_POISSON_SOR || s == COARSE_OPEN_FLUX_SOR || s == COARSE_CLOSED_FLUX_SOR ||
-           s == TOTAL_OPEN_FLUX_SOR || s == TOTAL_CLOSED_FLUX_SOR;
+           s == TOTAL_OPEN_FLUX_SOR || s == TOTAL_CLOSED_FLUX_SOR || s == OPEN_OCTREE_POISSON_SOR ||
+           s == CLOSED_OCTREE_POISSON_SOR;
 }
 
 // counter-based hash noise, same as the device-resident test fixtures: lowbias32 of (index, seed)
@@ -67,9 +69,16 @@ bool ZQ_SmokeSimulation3D::Init(const char* file) {
     const double t0 = now_s();
     if (!para.LoadFromFile(file, true)) return false;
     if (!solver_built(para.globalGridSolver)) {
-        std::printf("globalGridSolver %d is not built: use TOTAL_OPEN_POISSON_SOR or TOTAL_CLOSED_POISSON_SOR\n",
+        std::printf("globalGridSolver %d is not built (the CPU PCG and octree PCG solvers): use a *_SOR solver\n",
                     (int)para.globalGridSolver);
         return false;
+    }
+    {
+        zq3d_config c{};
+        if (map_octree_solver(para, &c) && para.useGuidingControl) {
+            std::printf("guiding control with the octree solvers is not built\n");
+            return false;
+        }
     }
     if (para.useMovingObject) {
         // the reference resolves the file relative to the working directory
@@ -122,7 +131,7 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
         c.sor.max_outer = para.maxOuter;
         c.sor.tol = para.sorTol;
         c.warm_start = para.warmStart ? 1 : 0;
-        if (is_coarse()) set_coarse(&c);
+        if (device_projection()) set_projection(&c);
         if (zq3d_create(&c, &dev) != ZQ3D_OK) {
             std::printf("device simulation: %s\n", zq3d_last_error());
             return false;
@@ -150,12 +159,12 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
             return false;
         }
     }
-    if (!dev && is_coarse()) {
+    if (!dev && device_projection()) {
         // projection-only device simulation of the staged path
         zq3d_config c;
         zq3d_config_default(&c, para.width, para.height, para.depth);
         c.sor.omega = para.omega;
-        set_coarse(&c);
+        set_projection(&c);
         if (zq3d_create(&c, &proj) != ZQ3D_OK) {
             std::printf("device projection: %s\n", zq3d_last_error());
             return false;
@@ -170,8 +179,15 @@ bool ZQ_SmokeSimulation3D::is_coarse() const {
     return map_coarse_solver(para, &c);
 }
 
-// the coarse-grid solves (COARSE_* and the flux solvers), map_coarse_solver
-void ZQ_SmokeSimulation3D::set_coarse(zq3d_config* c) const { map_coarse_solver(para, c); }
+bool ZQ_SmokeSimulation3D::device_projection() const {
+    zq3d_config c{};
+    return map_coarse_solver(para, &c) || map_octree_solver(para, &c);
+}
+
+// the coarse-grid solves (COARSE_* and the flux solvers) and the octree solves
+void ZQ_SmokeSimulation3D::set_projection(zq3d_config* c) const {
+    if (!map_coarse_solver(para, c)) map_octree_solver(para, c);
+}
 
 bool ZQ_SmokeSimulation3D::SyncToHost() {
     if (!dev || host_current) return true;

@@ -41,8 +41,9 @@ private:
     bool host_current;    // host grid equals the device state
     zq3d_proj_stats lastStats;
 
-    bool is_coarse() const;
-    void set_coarse(zq3d_config* c) const;
+    bool is_coarse() const;                    // a coarse-grid solve (guiding applies)
+    bool device_projection() const;            // the coarse-grid and octree solves
+    void set_projection(zq3d_config* c) const;
     bool _initRegular();
     bool _projection();
     bool _updateResident();

@@ -458,6 +458,22 @@ bool map_coarse_solver(const ZQ_SmokeSimulationPara3D& para, zq3d_config* c) {
     return true;
 }
 
+bool map_octree_solver(const ZQ_SmokeSimulationPara3D& para, zq3d_config* c) {
+    const GlobalSolverType3D t = para.globalGridSolver;
+    if (t != OPEN_OCTREE_POISSON_SOR && t != CLOSED_OCTREE_POISSON_SOR) return false;
+    c->octree = 1;
+    c->octree_thresh[0] = para.octree_thresh1;
+    c->octree_thresh[1] = para.octree_thresh2;
+    c->octree_thresh[2] = para.octree_thresh3;
+    c->sub_size = 0;
+    c->coarse_flux = 0;
+    c->sor.open_mask = ZQ3D_OPEN_ALL;
+    c->sor.closed_mean = t == CLOSED_OCTREE_POISSON_SOR ? 1 : 0;
+    c->sor.mixed_precision = 0;
+    c->sor.iters = para.maxIter;
+    return true;
+}
+
 // ZQ_SmokeSimulationPara3D order -> ZQ_CUDA_Advection3D order (ZQ_SmokeSimulationUtils3D.cpp:2125-2148, 3802-3819)
 bool map_vel_type(ZQ_SmokeSimulationPara3D::AdvectVelocityType t, ZQ_CUDA_Advection3D::AdvectVelocityType* out) {
     switch (t) {

@@ -136,6 +136,11 @@ bool ZQ_AdvectVelocity3D(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3
 // (maxIter-1)/innerIter+1 passes of innerIter iterations (:2452-2458).  Returns false (c untouched)
 // for the other solvers.
 bool map_coarse_solver(const ZQ_SmokeSimulationPara3D& para, zq3d_config* c);
+// The octree solvers with the red-black solve, OPEN_OCTREE_POISSON_SOR and CLOSED_OCTREE_POISSON_SOR
+// (ZQ_SmokeOctreeGrid3D.cpp:1300-1353, 2850-3018): zq3d_config.octree with octree_thresh1..3 and
+// maxIter passes; the closed one removes the mean divergence.  Returns false (c untouched) for the
+// other solvers.
+bool map_octree_solver(const ZQ_SmokeSimulationPara3D& para, zq3d_config* c);
 // the config's velocity advection type in ZQ_CUDA_Advection3D's numbering (false: unknown)
 bool map_vel_type(ZQ_SmokeSimulationPara3D::AdvectVelocityType t, ZQ_CUDA_Advection3D::AdvectVelocityType* out);
 bool ZQ_SolveOpenPressure3DSOR(ZQ_SmokeGrid3D* fineGrid, const ZQ_SmokeSimulationPara3D& para);

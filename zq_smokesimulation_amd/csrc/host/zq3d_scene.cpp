@@ -39,6 +39,7 @@ extern "C" int zq3d_scene_load(const char* path, zq3d_scene* out) {
     c.sor.tol = p.sorTol;
     c.warm_start = p.warmStart ? 1 : 0;
     map_coarse_solver(p, &c);
+    map_octree_solver(p, &c);
     out->frame_count = p.frameCount;
     out->solver = (int)p.globalGridSolver;
     out->wall_faces = p.WallFaceMask();

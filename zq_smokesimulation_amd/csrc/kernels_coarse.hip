@@ -13,6 +13,7 @@
 // (ZQ_SolveSubGrid3D, :1393-1521) by FP64 conjugate gradients inside one workgroup.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "zq3d_coarse.h"
 #include "zq3d_kernels.h"
@@ -520,9 +521,11 @@ __device__ __forceinline__ int rt_cell(int b, int z, int y, int x) {
     return ((b * S + z) * S + y) * (S + 1) + x;
 }
 
+// w_first != 0: every interior w face of a block takes the block's first one (z = 1, y = 0, x = 0),
+// as the octree's level-1 write-back does
 template <int S>
 __global__ __launch_bounds__(256) void k_sub_regular(CG g, SubTables tab, const uint8_t* __restrict__ regular, float* u,
-                                                     float* v, float* w) {
+                                                     float* v, float* w, int w_first) {
     constexpr int NB = RegTile<S>::NB, TW = RegTile<S>::TW, NC = RegTile<S>::NCELL;
     __shared__ float Us[S][S][TW + 1];
     __shared__ float Vs[S][S + 1][TW];
@@ -682,8 +685,9 @@ __global__ __launch_bounds__(256) void k_sub_regular(CG g, SubTables tab, const 
         const int x = e % TW, r = e / TW, y = r % S, z = r / S;
         const int bx = x / S, xi = x % S;
         if (z == 0 || z == S || x >= tw || !reg[bx]) continue;
-        w[f_w(g, x0 + x, y0 + y, z0 + z)] =
-            Ws[z][y][x] - (A[rt_cell<S>(bx, z, y, xi)] - A[rt_cell<S>(bx, z - 1, y, xi)]);
+        const float val = w_first ? Ws[1][0][bx * S] - (A[rt_cell<S>(bx, 1, 0, 0)] - A[rt_cell<S>(bx, 0, 0, 0)])
+                                  : Ws[z][y][x] - (A[rt_cell<S>(bx, z, y, xi)] - A[rt_cell<S>(bx, z - 1, y, xi)]);
+        w[f_w(g, x0 + x, y0 + y, z0 + z)] = val;
     }
 }
 
@@ -901,7 +905,7 @@ void sub_launch(const CG& g, const CoarseBufs& B, const uint8_t* occ, float* u, 
     constexpr int NB = RegTile<S>::NB;
     const SubTables tab = make_tables(S);
     const dim3 gr((unsigned)((g.Wc + NB - 1) / NB), (unsigned)g.Hc, (unsigned)g.Dc);
-    hipLaunchKernelGGL(k_sub_regular<S>, gr, dim3(256), 0, s, g, tab, B.regular, u, v, w);
+    hipLaunchKernelGGL(k_sub_regular<S>, gr, dim3(256), 0, s, g, tab, B.regular, u, v, w, 0);
     constexpr int NT = ((S * S * S + 63) / 64) * 64;
     hipLaunchKernelGGL(k_sub_solids<S>, dim3((unsigned)g.Wc, (unsigned)g.Hc, (unsigned)g.Dc), dim3(NT), 0, s, g,
                        B.regular, occ, u, v, w);
@@ -1046,6 +1050,23 @@ void launch_sub_project(const CoarseDims& cd, const CoarseBufs& B, const uint8_t
     case 6: sub_launch<6>(g, B, occ, u, v, w, s); break;
     case 7: sub_launch<7>(g, B, occ, u, v, w, s); break;
     case 8: sub_launch<8>(g, B, occ, u, v, w, s); break;
+    default: break;
+    }
+}
+
+void launch_sub_regular_mask(const Dims& f, int bs, const uint8_t* mask, float* u, float* v, float* w, int w_first,
+                             hipStream_t s) {
+    const CG g = make_cg(coarse_dims(f, bs));
+    auto go = [&](auto tag) {
+        constexpr int S = decltype(tag)::value;
+        constexpr int NB = RegTile<S>::NB;
+        const dim3 gr((unsigned)((g.Wc + NB - 1) / NB), (unsigned)g.Hc, (unsigned)g.Dc);
+        hipLaunchKernelGGL(k_sub_regular<S>, gr, dim3(256), 0, s, g, make_tables(S), mask, u, v, w, w_first);
+    };
+    switch (bs) {
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
     default: break;
     }
 }

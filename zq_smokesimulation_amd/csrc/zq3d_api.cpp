@@ -648,6 +648,7 @@ struct zq3d_sim {
     zq3d_timing tm{};
     SorShape shape{0, 0};  // fused sweep tile shape chosen by zq3d_autotune_sor / zq3d_set_sor_shape
     CoarseState coarse;    // sub_size > 0: the coarse grid of the coarse + sub-grid projection
+    OctreeState octree;    // octree != 0: the leaves and level arrays of the octree projection
 };
 
 namespace {
@@ -721,6 +722,20 @@ int check_config(const zq3d_config& c, const char* where) {
         set_err(where, "coarse_flux selects the flux form of the coarse solve: it needs sub_size > 0");
         return ZQ3D_ERR_ARG;
     }
+    if (c.octree) {
+        if (c.width % 8 || c.height % 8 || c.depth % 8) {
+            set_err(where, "octree: width, height and depth must be multiples of 8");
+            return ZQ3D_ERR_ARG;
+        }
+        if (c.sub_size || c.sor.mixed_precision) {
+            set_err(where, "octree runs its own FP32 solve: sub_size and sor.mixed_precision must be 0");
+            return ZQ3D_ERR_ARG;
+        }
+        if (c.octree_thresh[0] < 0 || c.octree_thresh[1] < 0 || c.octree_thresh[2] < 0) {
+            set_err(where, "octree_thresh must be >= 0");
+            return ZQ3D_ERR_ARG;
+        }
+    }
     if (c.sub_size && c.sor.mixed_precision) {
         set_err(where, "sub_size > 0 runs the coarse FP32 SOR: sor.mixed_precision must be 0");
         return ZQ3D_ERR_ARG;
@@ -755,6 +770,9 @@ extern "C" int zq3d_config_default(zq3d_config* c, int width, int height, int de
     c->sor.max_outer = 0;
     c->sor.tol = 1e-6;
     c->warm_start = 1;
+    c->octree_thresh[0] = 2;  // octree_thresh1..3 (ZQ_SmokeSimulationUtils3D.cpp:52-54)
+    c->octree_thresh[1] = 4;
+    c->octree_thresh[2] = 16;
     return ZQ3D_OK;
 }
 
@@ -806,6 +824,12 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
         }
         s->coarse.flux = cfg->coarse_flux ? 1 : 0;
     }
+    if (cfg->octree) {
+        if (int rc = octree_create(s->d, cfg->octree_thresh, &s->octree, where)) {
+            zq3d_destroy(s);
+            return rc;
+        }
+    }
     hipStream_t st = nullptr;
     launch_fill_f32(s->u, z.Nu, 0.0f, st);
     launch_fill_f32(s->v, z.Nv, 0.0f, st);
@@ -837,6 +861,7 @@ extern "C" void zq3d_destroy(zq3d_sim* s) {
     for (hipEvent_t e : {s->ev[0], s->ev[1], s->sorprof.a, s->sorprof.b})
         if (e) (void)hipEventDestroy(e);
     coarse_destroy(&s->coarse);
+    octree_destroy(&s->octree);
     delete s;
 }
 
@@ -937,6 +962,8 @@ extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
     int rc;
     if (c.sub_size > 0) {
         rc = project_coarse(s->coarse, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
+    } else if (c.octree) {
+        rc = project_octree(s->octree, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
     } else if (c.sor.mixed_precision) {
         if (!c.warm_start || !s->p64_valid) launch_fill_f64(s->p64, Sizes(s->d).Np, 0.0, s->stream);
         rc = project_mixed(s->d, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);

@@ -79,6 +79,10 @@ void launch_coarse_correct(const CoarseDims& g, const CoarseBufs& B, const float
 // coarse divergence, 1/diagonal and p = p2 = 0 from caller-given coarse faces, occupancy and fractions
 void launch_coarse_prepare(const CoarseDims& g, const float* u, const float* v, const float* w, int open_mask,
                            const CoarseBufs& B, hipStream_t s);
+// the regular-block projection of the interior faces of every bs^3 block (bs = 2, 4 or 8) whose mask
+// byte (a (W/bs, H/bs, D/bs) array) is set; w_first: see k_sub_regular
+void launch_sub_regular_mask(const Dims& f, int bs, const uint8_t* mask, float* u, float* v, float* w, int w_first,
+                             hipStream_t s);
 // out (rows x n) = M (rows x cols) * in (cols x n), each output summed over k = 0..cols-1 in order
 void launch_batched_matvec(int rows, int cols, int n, const float* M, const float* in, float* out, hipStream_t s);
 

@@ -8,6 +8,7 @@
 
 #include "zq3d_coarse.h"
 #include "zq3d_kernels.h"
+#include "zq3d_octree.h"
 #include "zq_smoke3d.h"
 
 namespace zq3d {
@@ -117,6 +118,21 @@ int coarse_set_guide(CoarseState* c, float coeff, const float* gu, const float* 
 // Coarse + sub-grid projection of the faces in place (o.iters coarse iterations with o.omega and
 // o.open_mask).  B provides the stats scratch (nc, b, p, part, sc); stats may be null.
 int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                   const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf);
+
+// Octree of the octree projection (one device allocation).
+struct OctreeState {
+    Dims d{};
+    int thresh[3] = {2, 4, 16};  // octree_thresh1..3
+    OctreeBufs B{};
+    void* mem = nullptr;
+};
+int octree_create(const Dims& d, const int thresh[3], OctreeState* out, const char* where);
+void octree_destroy(OctreeState* o);
+// Octree projection of the faces in place: the leaves from occ, o.iters red-black passes over the
+// levels, the face correction, the leaf sub-projection; o.closed_mean subtracts the mean divergence
+// per fine cell times each leaf's volume.  B provides the mean and stats scratch; stats may be null.
+int project_octree(const OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                    const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf);
 
 }  // namespace detail

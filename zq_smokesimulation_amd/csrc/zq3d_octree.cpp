@@ -1,0 +1,188 @@
+// zq3d_octree.cpp -- the octree projection on the device-resident simulation, and the two
+// ZQlibCuda octree solver calls of ZQ_SmokeOctreeGrid3D (host arrays).
+#include <cstring>
+#include <mutex>
+
+#include "zq3d_internal.h"
+
+namespace zq3d {
+namespace detail {
+
+int octree_create(const Dims& d, const int thresh[3], OctreeState* out, const char* where) {
+    out->d = d;
+    for (int i = 0; i < 3; i++) out->thresh[i] = thresh[i];
+    ZQ_HIP(hipMalloc(&out->mem, octree_bytes(d)), where);
+    octree_layout(d, (char*)out->mem, &out->B);
+    return ZQ3D_OK;
+}
+
+void octree_destroy(OctreeState* o) {
+    if (o && o->mem) (void)hipFree(o->mem);
+    if (o) o->mem = nullptr;
+}
+
+int project_octree(const OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+                   const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf) {
+    const char* where = "project_octree";
+    // the leaves follow the occupancy, which callers may change between projections
+    // (ReBuildFromGlobalGrid each frame with moving objects, ZQ_SmokeSimulation3D.cpp:482-491)
+    launch_octree_build(t.d, occ, t.thresh[0], t.thresh[1], t.thresh[2], t.B, s);
+    const float* mean = nullptr;
+    if (o.closed_mean) {
+        // the mean divergence over the fluid cells (_solveGlobalClosedPoissonSOR, ZQ_SmokeOctreeGrid3D.cpp:2935-2964)
+        launch_proj_setup(t.d, u, v, w, occ, 0, B.nc, B.b, nullptr, B.p, nullptr, B.part, s);
+        launch_mean_finalize(t.d, B.part, &B.sc->mean32, &B.sc->mean64, 1, s);
+        mean = &B.sc->mean32;
+    }
+    launch_octree_prepare(t.d, u, v, w, t.B, s);
+    if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
+    launch_octree_sweeps(t.d, t.B, o.iters, mean, s);
+    if (pf) {
+        ZQ_HIP(hipEventRecord(pf->b, s), where);
+        ZQ_HIP(hipEventSynchronize(pf->b), where);
+        float ms = 0.0f;
+        ZQ_HIP(hipEventElapsedTime(&ms, pf->a, pf->b), where);
+        pf->ms += ms;
+        pf->launches += 8LL * o.iters;
+    }
+    launch_octree_adjust(t.d, t.B, u, v, w, s);
+    launch_octree_sub_project(t.d, t.B, u, v, w, s);
+    ZQ_HIP(hipGetLastError(), where);
+    if (st) {
+        Scalars* sc = B.sc;
+        launch_fill_f64(&sc->mean64, 1, 0.0, s);
+        launch_proj_setup(t.d, u, v, w, occ, ZQ3D_OPEN_ALL, B.nc, B.b, nullptr, B.p, nullptr, nullptr, s);
+        launch_max_div64(t.d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+        double md = 0.0;
+        if (int rc = read_double_bits(&sc->dmax, &md, s, where)) return rc;
+        st->outer = 0;
+        st->sweeps = o.iters;
+        st->max_residual = -1.0;
+        st->max_div = md;
+        st->mean = 0.0;
+    }
+    return ZQ3D_OK;
+}
+
+}  // namespace detail
+}  // namespace zq3d
+
+using namespace zq3d;
+using namespace zq3d::detail;
+
+namespace {
+
+size_t up256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// grow-only device arena of the two host-pointer entry points below
+struct OctArena {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    std::mutex mu;
+    hipError_t ensure(size_t bytes) {
+        if (cap >= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&ptr, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+};
+
+OctArena& oct_arena() {
+    static OctArena a;
+    return a;
+}
+
+// One octree solve on host arrays: faces and leaf flags in, corrected faces out.  The index and
+// neighbour lists the reference passes are a function of the leaf flags alone (its
+// _selectInfos_for_CUDAOpenPoissonRedBlack3 builds them from nothing else); the kernels derive the
+// same neighbours, in the same order, from the flags, so the lists are accepted and not read.
+float octree_call(const char* where, float* mac_u, float* mac_v, float* mac_w, const bool* leaf0, const bool* leaf1,
+                  const bool* leaf2, const bool* leaf3, const float* div_per_volume, int width, int height, int depth,
+                  int maxIter) {
+    if (check_dims(width, height, depth, where)) return -1.0f;
+    if (width % 8 || height % 8 || depth % 8) {
+        set_err(where, "width, height and depth must be multiples of 8 (the octree's 8^3 blocks)");
+        return -1.0f;
+    }
+    if (!mac_u || !mac_v || !mac_w || !leaf0 || !leaf1 || !leaf2 || !leaf3) {
+        set_err(where, "null pointer");
+        return -1.0f;
+    }
+    if (maxIter < 0) { set_err(where, "maxIter must be >= 0"); return -1.0f; }
+    static_assert(sizeof(bool) == 1, "leaf flags are one byte per cell");
+    const Dims d{width, height, depth};
+    const Sizes z(d);
+    const size_t n1 = z.N / 8, n2 = z.N / 64, n3 = z.N / 512;
+    const size_t o_u = 0, o_v = o_u + up256(z.Nu * 4), o_w = o_v + up256(z.Nv * 4), o_l0 = o_w + up256(z.Nw * 4);
+    const size_t o_l1 = o_l0 + up256(z.N), o_l2 = o_l1 + up256(n1), o_l3 = o_l2 + up256(n2), o_m = o_l3 + up256(n3);
+    const size_t o_t = o_m + 256, total = o_t + octree_bytes(d);
+    OctArena& A = oct_arena();
+    std::lock_guard<std::mutex> lk(A.mu);
+    if (A.ensure(total) != hipSuccess) { set_err(where, "device allocation failed"); return -1.0f; }
+    char* base = (char*)A.ptr;
+    float *du = (float*)(base + o_u), *dv = (float*)(base + o_v), *dw = (float*)(base + o_w);
+    uint8_t *l0 = (uint8_t*)(base + o_l0), *l1 = (uint8_t*)(base + o_l1), *l2 = (uint8_t*)(base + o_l2);
+    uint8_t* l3 = (uint8_t*)(base + o_l3);
+    float* mean = (float*)(base + o_m);
+    OctreeBufs B{};
+    octree_layout(d, base + o_t, &B);
+    hipEvent_t a = nullptr, b = nullptr;
+    Errs E;
+    E(hipEventCreate(&a));
+    E(hipEventCreate(&b));
+    E(hipEventRecord(a, 0));
+    E(hipMemcpy(du, mac_u, z.Nu * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(dv, mac_v, z.Nv * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(dw, mac_w, z.Nw * 4, hipMemcpyHostToDevice));
+    E(hipMemcpy(l0, leaf0, z.N, hipMemcpyHostToDevice));
+    E(hipMemcpy(l1, leaf1, n1, hipMemcpyHostToDevice));
+    E(hipMemcpy(l2, leaf2, n2, hipMemcpyHostToDevice));
+    E(hipMemcpy(l3, leaf3, n3, hipMemcpyHostToDevice));
+    if (div_per_volume) E(hipMemcpy(mean, div_per_volume, 4, hipMemcpyHostToDevice));
+    if (E.e == hipSuccess) {
+        launch_octree_from_leaves(d, l0, l1, l2, l3, B, 0);
+        launch_octree_prepare(d, du, dv, dw, B, 0);
+        launch_octree_sweeps(d, B, maxIter, div_per_volume ? mean : nullptr, 0);
+        launch_octree_adjust(d, B, du, dv, dw, 0);
+        E(hipGetLastError());
+    }
+    E(hipMemcpy(mac_u, du, z.Nu * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_v, dv, z.Nv * 4, hipMemcpyDeviceToHost));
+    E(hipMemcpy(mac_w, dw, z.Nw * 4, hipMemcpyDeviceToHost));
+    E(hipEventRecord(b, 0));
+    E(hipEventSynchronize(b));
+    float ms = 0.0f;
+    E(hipEventElapsedTime(&ms, a, b));
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (E.e != hipSuccess) { set_err(where, hipGetErrorString(E.e)); return -1.0f; }
+    return ms;
+}
+
+}  // namespace
+
+#define ZQ3D_OCT_LISTS                                                                                          \
+    const int, const int*, const int, const int*, const int, const int*, const int, const int*, const int,     \
+        const int*, const int, const int*, const int, const int*, const int, const int*, const int, const int*, \
+        const int, const int*, const int, const int*, const int, const int*, const int, const int*, const int,  \
+        const int*, const int, const int*, const int, const int*
+
+extern "C" float SolveOpenOctreePoissonRedBlack3_3D_MAC(float* mac_u, float* mac_v, float* mac_w, const bool* leaf0,
+                                                       const bool* leaf1, const bool* leaf2, const bool* leaf3,
+                                                       const int width, const int height, const int depth,
+                                                       const int maxIter, ZQ3D_OCT_LISTS) {
+    return octree_call("SolveOpenOctreePoissonRedBlack3_3D_MAC", mac_u, mac_v, mac_w, leaf0, leaf1, leaf2, leaf3,
+                       nullptr, width, height, depth, maxIter);
+}
+
+extern "C" float SolveClosedOctreePoissonRedBlack3_3D_MAC(float* mac_u, float* mac_v, float* mac_w, const bool* leaf0,
+                                                         const bool* leaf1, const bool* leaf2, const bool* leaf3,
+                                                         const float div_per_volume, const int width,
+                                                         const int height, const int depth, const int maxIter,
+                                                         ZQ3D_OCT_LISTS) {
+    return octree_call("SolveClosedOctreePoissonRedBlack3_3D_MAC", mac_u, mac_v, mac_w, leaf0, leaf1, leaf2, leaf3,
+                       &div_per_volume, width, height, depth, maxIter);
+}

@@ -106,6 +106,29 @@ def solve_open_face_ratio(u, v, w, occ, rU, rV, rW, max_iter):
         _p(rW, np.float32), W, H, D, int(max_iter)), "SolveOpenFaceRatio")
 
 
+def solve_octree(u, v, w, leaves, max_iter, div_per_volume=None):
+    """SolveOpenOctreePoissonRedBlack3_3D_MAC / SolveClosedOctreePoissonRedBlack3_3D_MAC (call sites
+    ZQ_SmokeOctreeGrid3D.cpp:2897, 3000): the faces corrected in place from the leaf flags of the four
+    levels (uint8 arrays of shapes (D >> L, H >> L, W >> L)); div_per_volume selects the closed form.
+    The index / neighbour lists are not read by the library: empty ones are passed."""
+    l0 = leaves[0]
+    D, H, W = l0.shape
+    _check_faces(u, v, w, W, H, D)
+    for L, a in enumerate(leaves):
+        if a.shape != (D >> L, H >> L, W >> L):
+            raise ValueError("leaf level %d has shape %s" % (L, a.shape))
+    ls = [np.ascontiguousarray(a, np.uint8) for a in leaves]
+    lp = [a.ctypes.data_as(C.c_void_p) for a in ls]
+    empty = [0, None, 0, None] * 8
+    fp = (_p(u, np.float32), _p(v, np.float32), _p(w, np.float32))
+    if div_per_volume is None:
+        ms = _lib.lib().SolveOpenOctreePoissonRedBlack3_3D_MAC(*fp, *lp, W, H, D, int(max_iter), *empty)
+    else:
+        ms = _lib.lib().SolveClosedOctreePoissonRedBlack3_3D_MAC(*fp, *lp, float(div_per_volume), W, H, D,
+                                                                 int(max_iter), *empty)
+    return check(ms, "SolveOctree")
+
+
 def matrix_mul(A, B):
     """ZQ_CUDA_MatrixMul::MatrixMul (called by SubProjection3DCuda, ZQ_SmokeSimulationUtils3D.cpp:13-16):
     returns A @ B for float32 matrices, each entry summed over k in order."""
