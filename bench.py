@@ -20,6 +20,10 @@ coarse_projection: the reference's default solver (COARSE_OPEN_POISSON_SOR) in a
 expsimu3D/fine256.txt scene (256^3, 8^3 blocks, 600 coarse iterations, internal sphere, box
 source), the scene parsed by the library's config reader; value = voxels * steps / wall time.
 
+octree_projection: the same scene with globalGridSolver OPEN_OCTREE_POISSON_SOR (leaves of 1, 2^3,
+4^3 and 8^3 cells from the distance to the sphere, thresholds 2/4/16, 600 red-black passes over
+the levels); value = voxels * steps / wall time.
+
 ensemble: 64 independent grids with per-axis sizes drawn from {32..96} (seeded, anisotropic),
 alpha in [0.02, 0.1] and beta in [0.2, 1.0] per member, closed boxes with the same sphere
 source, plain red-black SOR (omega 1.8, 200 iterations), 100 steps replayed from one captured
@@ -60,6 +64,8 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
     ap.add_argument("--coarse-steps", type=int, default=5,
                     help="timed steps of the coarse + sub-grid projection leg (0: skip)")
+    ap.add_argument("--octree-steps", type=int, default=3,
+                    help="timed steps of the octree projection leg (0: skip)")
     ap.add_argument("--sor-shape", default=None,
                     help="sweep shape 'ty,zc' to use instead of autotuning it (a zq3d_autotune_sor candidate)")
     ap.add_argument("--traffic", default=None,
@@ -200,37 +206,38 @@ FINE256 = ("resolution 256 256 256\nsubSize 8\nsource 0.5 0.1 0.5 0.1 0.05 0.1 0
            "globalGridSolver COARSE_OPEN_POISSON_SOR\n")
 
 
-def coarse_leg(args):
-    """The reference's default projection (coarse + sub-grid, 8^3 blocks, 600 coarse iterations) in a
-    full step of its fine256 scene, parsed by the library's own config reader."""
+def scene_leg(text, steps, warmup, workload):
+    """A full step of a reference scene, parsed by the library's own config reader, timed over
+    `steps` steps after `warmup`; returns the leg's record and the simulation's stage timing."""
     import tempfile
     import zq_smokesimulation_amd as z
     from zq_smokesimulation_amd import scene
     with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "fine256.txt")
+        path = os.path.join(td, "scene.txt")
         with open(path, "w") as f:
-            f.write(FINE256)
+            f.write(text)
         sc = scene.load_scene(path)
         occ = scene.initial_occupancy(path)
     c = sc.config
     W, H, D = c.width, c.height, c.depth
     over = {n: getattr(c, n) for n, _ in c._fields_ if n not in ("struct_size", "width", "height", "depth", "sor")}
     over.update({"sor_" + n: getattr(c.sor, n) for n, _ in c.sor._fields_})
+    over["octree_thresh"] = tuple(c.octree_thresh)
     sim = z.SmokeSim3D(W, H, D, **over)
-    sweep = sim.coarse_sweep()
+    sweep = sim.coarse_sweep() if c.sub_size > 0 else None
     boxes = [list(sc.box_sources[i]) for i in range(sc.n_box_sources)]
     sim.set_box_sources(boxes)
     st = make_state(W, H, D)
     st["T"][...] = c.Tamb
     st["occ"] = occ
     sim.set_state(**st)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         sim.step()
     sim.synchronize()
     sim.set_profiling(True)
     stats = []
     t0 = time.perf_counter()
-    for _ in range(args.coarse_steps):
+    for _ in range(steps):
         stats.append(sim.step())
     sim.synchronize()
     wall = time.perf_counter() - t0
@@ -238,21 +245,43 @@ def coarse_leg(args):
     rho = sim.download(z._lib.FIELD_RHO)
     sim.close()
     n = W * H * D
-    return {
-        "value": round(n * args.coarse_steps / wall / 1e6, 3),
+    rec = {
+        "value": round(n * steps / wall / 1e6, 3),
         "unit": "Mvoxel-steps/s",
-        "workload": "expsimu3D/fine256.txt: 256^3, COARSE_OPEN_POISSON_SOR, subSize 8 (32^3 coarse grid), "
-                    "maxIter 600, internal sphere, box source",
-        "steps": args.coarse_steps,
-        "ms_per_step": round(1e3 * wall / args.coarse_steps, 3),
+        "workload": workload,
+        "steps": steps,
+        "ms_per_step": round(1e3 * wall / steps, 3),
         "stages_ms_per_step": {k: round(tm[k] / max(tm["steps"], 1), 3) for k in
                                ("ms_force", "ms_advect_velocity", "ms_project", "ms_sor", "ms_advect_scalar",
                                 "ms_reinject")},
-        "coarse_sweep_schedule": {"iterations_per_launch": sweep[0], "planes_per_workgroup": sweep[1]},
-        "coarse_sweep_launches_per_step": tm["sor_launches"] / max(tm["steps"], 1),
         "max_div": float(max(s["max_div"] for s in stats)),
         "density_finite": bool(np.isfinite(rho).all()),
     }
+    return rec, tm, sweep
+
+
+def coarse_leg(args):
+    """The reference's default projection (coarse + sub-grid, 8^3 blocks, 600 coarse iterations) in a
+    full step of its fine256 scene."""
+    rec, tm, sweep = scene_leg(FINE256, args.coarse_steps, args.warmup,
+                               "expsimu3D/fine256.txt: 256^3, COARSE_OPEN_POISSON_SOR, subSize 8 (32^3 coarse "
+                               "grid), maxIter 600, internal sphere, box source")
+    rec["coarse_sweep_schedule"] = {"iterations_per_launch": sweep[0], "planes_per_workgroup": sweep[1]}
+    rec["coarse_sweep_launches_per_step"] = tm["sor_launches"] / max(tm["steps"], 1)
+    return rec
+
+
+def octree_leg(args):
+    """The octree projection (OPEN_OCTREE_POISSON_SOR, octree_thresh 2/4/16, 600 passes over the four
+    leaf levels) in a full step of the same fine256 scene."""
+    text = FINE256.replace("globalGridSolver COARSE_OPEN_POISSON_SOR", "globalGridSolver OPEN_OCTREE_POISSON_SOR")
+    rec, tm, _ = scene_leg(text, args.octree_steps, min(args.warmup, 3),
+                           "expsimu3D/fine256.txt with globalGridSolver OPEN_OCTREE_POISSON_SOR: 256^3, leaves of "
+                           "1, 2^3, 4^3, 8^3 cells (octree_thresh 2, 4, 16), maxIter 600, internal sphere, box source")
+    launches = tm["sor_launches"] / max(tm["steps"], 1)
+    rec["octree_sweep_launches_per_step"] = launches
+    rec["octree_us_per_pass"] = round(1e3 * tm["ms_sor"] / max(tm["steps"], 1) / 600, 3)
+    return rec
 
 
 def adjoint_window(state, W, H, D, args, fwd_ms_per_step, shape):
@@ -440,6 +469,9 @@ def main():
     if args.coarse_steps > 0:
         print("bench: coarse + sub-grid projection (fine256 scene)", file=sys.stderr, flush=True)
         out["coarse_projection"] = coarse_leg(args)
+    if args.octree_steps > 0:
+        print("bench: octree projection (fine256 scene)", file=sys.stderr, flush=True)
+        out["octree_projection"] = octree_leg(args)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean)

@@ -14,7 +14,12 @@
 // the face in exactly the order the reference's lists hold them (same level or coarser first,
 // else the finer leaves square by square), and a level-m leaf's cells form one aligned run of 4^m
 // positions, skipped at once.  The sums coeff += w, sigma += w * p run in that order, so the
-// result is the lists' bit for bit.  One launch per level and colour, the levels in order 0..3.
+// result is the lists' bit for bit.  The walks run once per projection: they fill compact
+// per-level, per-colour cell lists and, for the 2^3 to 8^3 leaves, their neighbour entries (counted,
+// scanned, written), so that each of the iters x 8 sweep launches is a gather over a short list
+// (a scene far from its obstacles is mostly 8^3 leaves, and the launches touch only those).
+#include <hipcub/hipcub.hpp>
+
 #include "zq3d_coarse.h"
 #include "zq3d_kernels.h"
 #include "zq3d_octree.h"
@@ -177,86 +182,221 @@ __global__ __launch_bounds__(256) void k_oct_restrict(OG g, int L, const float* 
 }
 
 // ---------------------------------------------------------------------------- Gauss-Seidel
-// one neighbour of level m at fine cell (x, y, z): coeff += w, sigma += w * p_m
-__device__ __forceinline__ void oct_take(const OG& g, const OctW& wt, const LevelPtrs& P, int L, int m, int x, int y,
-                                         int z, float& coeff, float& sigma) {
-    const float wg = wt.w[L][m];
-    coeff += wg;
-    sigma += wg * P.p[m][oat(g, m, x >> m, y >> m, z >> m)];
+// Wave-aggregated append of `idx` to `list` where `pred` holds: one atomic per wave and list.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* list, uint32_t* count) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = idx;
+}
+
+// The cells each sweep updates, per level and colour (red: even x + y + z at the level): every
+// level-0..2 leaf, and every level-3 leaf whose column's block at z = 0 is a leaf (the reference's
+// level-3 list test, ZQ_SmokeOctreeGrid3D.cpp:8416; the blocks that test takes that are no leaf are
+// never read, so they are left out).  Within a list the order is free: cells of one colour and level
+// never neighbour each other.
+__global__ __launch_bounds__(256) void k_oct_lists(OG g, const uint8_t* __restrict__ lev, uint32_t* l00, uint32_t* l01,
+                                                   uint32_t* l10, uint32_t* l11, uint32_t* l20, uint32_t* l21,
+                                                   uint32_t* l30, uint32_t* l31, uint32_t* count) {
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t N = (size_t)g.W * g.H * g.D;
+    const bool valid = c < N;
+    int x = 0, y = 0, z = 0, m = -1;
+    if (valid) {
+        x = (int)(c % g.W);
+        y = (int)((c / g.W) % g.H);
+        z = (int)(c / ((size_t)g.W * g.H));
+        m = lev[c];
+    }
+    uint32_t* lists[4][2] = {{l00, l01}, {l10, l11}, {l20, l21}, {l30, l31}};
+#pragma unroll
+    for (int L = 0; L < 4; L++) {
+        const int mask = (1 << L) - 1;
+        bool take = valid && m == L && ((x | y | z) & mask) == 0;
+        if (L == 3 && take) take = lev[oat(g, 0, x, y, 0)] == 3;
+        const int col = ((x >> L) + (y >> L) + (z >> L)) & 1;
+        const uint32_t idx = (uint32_t)oat(g, L, x >> L, y >> L, z >> L);
+        wave_append(take && col == 0, idx, lists[L][0], count + 2 * L);
+        wave_append(take && col == 1, idx, lists[L][1], count + 2 * L + 1);
+    }
+}
+
+__global__ void k_oct_zero(uint32_t* count) {
+    if (threadIdx.x < 8) count[threadIdx.x] = 0u;
 }
 
 // level-0 leaves of one colour: the six face neighbours read straight off the level map
 __global__ __launch_bounds__(256) void k_oct_gs0(OG g, OctW wt, LevelPtrs P, const uint8_t* __restrict__ lev,
-                                                 const float* __restrict__ d0, float* __restrict__ p0, int colour,
+                                                 const float* __restrict__ d0, float* __restrict__ p0,
+                                                 const uint32_t* __restrict__ list, uint32_t n,
                                                  const float* __restrict__ mean) {
-    const int hx = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
-    const int x = 2 * hx + ((y + z + colour) & 1);
-    if (x >= g.W || y >= g.H) return;
-    const size_t c = oat(g, 0, x, y, z);
-    if (lev[c] != 0) return;
-    float coeff = 0.0f, sigma = 0.0f;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = list[i];
+    const int x = (int)(c % (uint32_t)g.W), y = (int)((c / (uint32_t)g.W) % (uint32_t)g.H);
+    const int z = (int)(c / ((uint32_t)g.W * (uint32_t)g.H));
     const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
+    int m[6];
 #pragma unroll
     for (int f = 0; f < 6; f++) {
-        if (nx[f] < 0 || ny[f] < 0 || nz[f] < 0 || nx[f] >= g.W || ny[f] >= g.H || nz[f] >= g.D) {
+        const bool out = nx[f] < 0 || ny[f] < 0 || nz[f] < 0 || nx[f] >= g.W || ny[f] >= g.H || nz[f] >= g.D;
+        m[f] = out ? -1 : lev[oat(g, 0, nx[f], ny[f], nz[f])];
+    }
+    float pv[6];
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        const int mm = m[f];
+        pv[f] = (mm >= 0 && mm != OCT_SOLID) ? P.p[mm][oat(g, mm, nx[f] >> mm, ny[f] >> mm, nz[f] >> mm)] : 0.0f;
+    }
+    float coeff = 0.0f, sigma = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        if (m[f] < 0) {
             coeff += wt.w[0][3];  // open domain face
-            continue;
+        } else if (m[f] != OCT_SOLID) {
+            const float wg = wt.w[0][m[f]];
+            coeff += wg;
+            sigma += wg * pv[f];
         }
-        const int m = lev[oat(g, 0, nx[f], ny[f], nz[f])];
-        if (m != OCT_SOLID) oct_take(g, wt, P, 0, m, nx[f], ny[f], nz[f], coeff, sigma);
     }
     const float dd = d0[c];
     sigma -= mean ? dd - *mean : dd;
     p0[c] = sigma / coeff;
 }
 
-// level L >= 1 leaves of one colour: each face's fine cells walked in interleaved order
-__global__ __launch_bounds__(64) void k_oct_gsL(OG g, OctW wt, LevelPtrs P, const uint8_t* __restrict__ lev, int L,
-                                                const float* __restrict__ dl, float* __restrict__ pl, int colour,
-                                                const float* __restrict__ mean) {
-    const int wl = ow(g, L), hl = oh(g, L), dlv = od(g, L);
-    const int hx = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
-    const int x = 2 * hx + ((y + z + colour) & 1);
-    if (x >= wl) return;
+// A neighbour entry: level in bits 30-31, the linear index at that level below; OCT_OPEN marks an
+// open domain face (its weight enters the diagonal only).
+constexpr uint32_t OCT_OPEN = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t oct_entry(const OG& g, int m, int x, int y, int z) {
+    return ((uint32_t)m << 30) | (uint32_t)oat(g, m, x >> m, y >> m, z >> m);
+}
+
+// The neighbour entries of the level-L cell (x, y, z), in the reference's order (WRITE: stored to
+// out); returns their number.  Per face: an open domain face; else the leaf behind the face's first
+// fine cell when it is as large as this one; else the face's fine cells in interleaved order (slower
+// axis high), one entry per leaf met (its run of 4^m positions skipped), solid cells skipped.
+template <bool WRITE>
+__device__ int oct_walk(const OG& g, const uint8_t* __restrict__ lev, int L, int x, int y, int z, uint32_t* out) {
     const int S = 1 << L;
-    const int X0 = x * S, Y0 = y * S, Z0 = z * S;
-    if (lev[oat(g, 0, X0, Y0, Z0)] != L) return;
-    // level 3: the reference's lists take a block when the block at z = 0 of its column is a leaf
-    // (ZQ_SmokeOctreeGrid3D.cpp:8416); blocks it takes that are no leaf are never read
-    if (L == 3 && lev[oat(g, 0, X0, Y0, 0)] != 3) return;
-    float coeff = 0.0f, sigma = 0.0f;
-    const int cl[3] = {x, y, z}, dims[3] = {wl, hl, dlv}, base[3] = {X0, Y0, Z0};
-    for (int ax = 0; ax < 3; ax++) {
+    const int cl[3] = {x, y, z}, dims[3] = {ow(g, L), oh(g, L), od(g, L)}, base[3] = {x * S, y * S, z * S};
+    int k = 0;
+    for (int f = 0; f < 6; f++) {
+        const int ax = f >> 1, sd = (f & 1) ? 1 : -1;
+        if ((sd < 0 && cl[ax] == 0) || (sd > 0 && cl[ax] == dims[ax] - 1)) {
+            if (WRITE) out[k] = OCT_OPEN;
+            k++;
+            continue;
+        }
         const int slow = ax == 2 ? 1 : 2, fast = ax == 0 ? 1 : 0;
-        for (int sd = -1; sd <= 1; sd += 2) {
-            if ((sd < 0 && cl[ax] == 0) || (sd > 0 && cl[ax] == dims[ax] - 1)) {
-                coeff += wt.w[L][3];  // open domain face
+        int q0[3] = {base[0], base[1], base[2]};
+        q0[ax] = sd < 0 ? base[ax] - 1 : base[ax] + S;
+        const int m0 = lev[oat(g, 0, q0[0], q0[1], q0[2])];
+        if (m0 != OCT_SOLID && m0 >= L) {  // one leaf as large as this one covers the face
+            if (WRITE) out[k] = oct_entry(g, m0, q0[0], q0[1], q0[2]);
+            k++;
+            continue;
+        }
+        for (int t = 0; t < S * S;) {
+            int a = 0, b = 0;
+            for (int j = 0; j < L; j++) {
+                b |= ((t >> (2 * j)) & 1) << j;
+                a |= ((t >> (2 * j + 1)) & 1) << j;
+            }
+            int q[3] = {q0[0], q0[1], q0[2]};
+            q[slow] += a;
+            q[fast] += b;
+            const int m = lev[oat(g, 0, q[0], q[1], q[2])];
+            if (m == OCT_SOLID) {
+                t += 1;
                 continue;
             }
-            const int f = sd < 0 ? base[ax] - 1 : base[ax] + S;
-            for (int t = 0; t < S * S;) {
-                int a = 0, b = 0;
-                for (int k = 0; k < L; k++) {
-                    b |= ((t >> (2 * k)) & 1) << k;
-                    a |= ((t >> (2 * k + 1)) & 1) << k;
-                }
-                int q[3];
-                q[ax] = f;
-                q[slow] = base[slow] + a;
-                q[fast] = base[fast] + b;
-                const int m = lev[oat(g, 0, q[0], q[1], q[2])];
-                if (m == OCT_SOLID) {
-                    t += 1;
-                    continue;
-                }
-                oct_take(g, wt, P, L, m, q[0], q[1], q[2], coeff, sigma);
-                if (m >= L) break;  // a leaf as large as this one covers the whole face
-                t += 1 << (2 * m);
+            if (WRITE) out[k] = oct_entry(g, m, q[0], q[1], q[2]);
+            k++;
+            t += 1 << (2 * m);  // m < L here: the rest of that leaf's run on this face
+        }
+    }
+    return k;
+}
+
+__device__ __forceinline__ void oct_cell(const OG& g, int L, uint32_t c, int& x, int& y, int& z) {
+    const uint32_t wl = (uint32_t)ow(g, L), hl = (uint32_t)oh(g, L);
+    x = (int)(c % wl);
+    y = (int)((c / wl) % hl);
+    z = (int)(c / (wl * hl));
+}
+
+// entries per listed cell; cnt[n] = 0 closes the array for the exclusive scan
+__global__ __launch_bounds__(256) void k_oct_count(OG g, const uint8_t* __restrict__ lev, int L,
+                                                   const uint32_t* __restrict__ list, uint32_t n,
+                                                   uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        cnt[n] = 0;
+        return;
+    }
+    int x, y, z;
+    oct_cell(g, L, list[i], x, y, z);
+    cnt[i] = (uint32_t)oct_walk<false>(g, lev, L, x, y, z, nullptr);
+}
+
+__global__ __launch_bounds__(256) void k_oct_fill(OG g, const uint8_t* __restrict__ lev, int L,
+                                                  const uint32_t* __restrict__ list, uint32_t n,
+                                                  const uint32_t* __restrict__ off, uint32_t* __restrict__ ent) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int x, y, z;
+    oct_cell(g, L, list[i], x, y, z);
+    oct_walk<true>(g, lev, L, x, y, z, ent + off[i]);
+}
+
+struct TotPtrs {
+    const uint32_t* off[6];
+    uint32_t n[6];
+};
+
+__global__ void k_oct_totals(TotPtrs t, uint32_t* out) {
+    const int k = threadIdx.x;
+    if (k < 6) out[k] = t.off[k][t.n[k]];
+}
+
+// level L >= 1 cells of one colour: a gather over each cell's entries in order
+__global__ __launch_bounds__(256) void k_oct_gsL(OG g, OctW wt, LevelPtrs P, int L, const float* __restrict__ dl,
+                                                 float* __restrict__ pl, const uint32_t* __restrict__ list, uint32_t n,
+                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ ent,
+                                                 const float* __restrict__ mean) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = list[i];
+    const uint32_t e0 = off[i], e1 = off[i + 1];
+    const float dd = dl[c];
+    float coeff = 0.0f, sigma = 0.0f;
+    // eight entries at a time: their loads, then their pressures' loads, all in flight together;
+    // the sums then run in entry order
+    for (uint32_t k0 = e0; k0 < e1; k0 += 8) {
+        uint32_t ev[8];
+        float pv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) ev[j] = k0 + j < e1 ? ent[k0 + j] : OCT_OPEN;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            pv[j] = (k0 + j < e1 && ev[j] != OCT_OPEN) ? P.p[ev[j] >> 30][ev[j] & 0x3FFFFFFFu] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (k0 + j >= e1) break;
+            if (ev[j] == OCT_OPEN) {
+                coeff += wt.w[L][3];  // open domain face
+            } else {
+                const float wg = wt.w[L][ev[j] >> 30];
+                coeff += wg;
+                sigma += wg * pv[j];
             }
         }
     }
-    const size_t c = oat(g, L, x, y, z);
-    const float dd = dl[c];
     sigma -= mean ? dd - *mean * (float)(1 << (3 * L)) : dd;
     pl[c] = sigma / coeff;
 }
@@ -324,13 +464,22 @@ dim3 fine_grid(const Dims& g) { return dim3((unsigned)((g.W + 63) / 64), (unsign
 size_t level_cells(const Dims& g, int L) { return (size_t)(g.W >> L) * (g.H >> L) * (g.D >> L); }
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+size_t scan_temp_bytes(const Dims& g) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (int)(level_cells(g, 1) / 2 + 1));
+    return bytes;
+}
+
 }  // namespace
 
 size_t octree_bytes(const Dims& g) {
     size_t b = up256(level_cells(g, 0)) + up256(4 * level_cells(g, 0));  // lev, dist
     for (int L = 1; L <= 3; L++) b += up256(level_cells(g, L)) + up256(4 * level_cells(g, L));  // leaf, mn
     for (int L = 0; L <= 3; L++) b += 2 * up256(4 * level_cells(g, L));  // p, d
-    return b;
+    for (int L = 0; L <= 3; L++) b += 2 * up256(4 * (level_cells(g, L) / 2 + 1));  // lists
+    for (int L = 1; L <= 3; L++) b += 4 * up256(4 * (level_cells(g, L) / 2 + 2));  // cnt, off
+    return b + 256 + up256(scan_temp_bytes(g));  // counts, scan scratch
 }
 
 void octree_layout(const Dims& g, char* mem, OctreeBufs* B) {
@@ -352,6 +501,17 @@ void octree_layout(const Dims& g, char* mem, OctreeBufs* B) {
         B->p[L] = (float*)take(4 * level_cells(g, L));
         B->d[L] = (float*)take(4 * level_cells(g, L));
     }
+    for (int L = 0; L <= 3; L++)
+        for (int c = 0; c < 2; c++) B->list[L][c] = (uint32_t*)take(4 * (level_cells(g, L) / 2 + 1));
+    B->cnt[0][0] = B->cnt[0][1] = B->off[0][0] = B->off[0][1] = nullptr;
+    for (int L = 1; L <= 3; L++)
+        for (int c = 0; c < 2; c++) {
+            B->cnt[L][c] = (uint32_t*)take(4 * (level_cells(g, L) / 2 + 2));
+            B->off[L][c] = (uint32_t*)take(4 * (level_cells(g, L) / 2 + 2));
+        }
+    B->count = (uint32_t*)take(64);
+    B->scan_tmp = take(scan_temp_bytes(g));
+    B->scan_bytes = scan_temp_bytes(g);
 }
 
 void launch_octree_build(const Dims& d, const uint8_t* occ, int t1, int t2, int t3, const OctreeBufs& B,
@@ -383,22 +543,87 @@ void launch_octree_prepare(const Dims& d, const float* u, const float* v, const 
                            (const float*)B.d[L - 1], B.d[L], B.p[L]);
 }
 
-void launch_octree_sweeps(const Dims& d, const OctreeBufs& B, int iters, const float* mean, hipStream_t s) {
+hipError_t octree_build_lists(const Dims& d, const OctreeBufs& B, OctreeLists* Ls, hipStream_t s) {
+    const OG g{d.W, d.H, d.D};
+    hipLaunchKernelGGL(k_oct_zero, dim3(1), dim3(64), 0, s, B.count);
+    hipLaunchKernelGGL(k_oct_lists, dim3(blocks_of(level_cells(d, 0), 256)), dim3(256), 0, s, g,
+                       (const uint8_t*)B.lev, B.list[0][0], B.list[0][1], B.list[1][0], B.list[1][1], B.list[2][0],
+                       B.list[2][1], B.list[3][0], B.list[3][1], B.count);
+    hipError_t e = hipMemcpyAsync(Ls->counts, B.count, 32, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    // neighbour entries of the level 1-3 cells: count, scan, fill
+    TotPtrs tp{};
+    for (int L = 1; L <= 3; L++)
+        for (int c = 0; c < 2; c++) {
+            const uint32_t n = Ls->counts[2 * L + c];
+            hipLaunchKernelGGL(k_oct_count, dim3(blocks_of((size_t)n + 1, 256)), dim3(256), 0, s, g,
+                               (const uint8_t*)B.lev, L, (const uint32_t*)B.list[L][c], n, B.cnt[L][c]);
+            size_t bytes = B.scan_bytes;
+            e = hipcub::DeviceScan::ExclusiveSum(B.scan_tmp, bytes, (const uint32_t*)B.cnt[L][c], B.off[L][c],
+                                                 (int)(n + 1), s);
+            if (e != hipSuccess) return e;
+            tp.off[2 * (L - 1) + c] = B.off[L][c];
+            tp.n[2 * (L - 1) + c] = n;
+        }
+    hipLaunchKernelGGL(k_oct_totals, dim3(1), dim3(64), 0, s, tp, B.count + 8);
+    uint32_t tot[6];
+    e = hipMemcpyAsync(tot, B.count + 8, 24, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    size_t need = 0;
+    for (int k = 0; k < 6; k++) {
+        Ls->base[k] = need;
+        need += tot[k];
+    }
+    if (need > Ls->cap) {
+        if (Ls->ent) (void)hipFree(Ls->ent);
+        Ls->ent = nullptr;
+        Ls->cap = 0;
+        const size_t cap = need + need / 2 + 1024;
+        e = hipMalloc(&Ls->ent, cap * 4);
+        if (e != hipSuccess) return e;
+        Ls->cap = cap;
+    }
+    for (int L = 1; L <= 3; L++)
+        for (int c = 0; c < 2; c++) {
+            const uint32_t n = Ls->counts[2 * L + c];
+            if (!n) continue;
+            hipLaunchKernelGGL(k_oct_fill, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, (const uint8_t*)B.lev, L,
+                               (const uint32_t*)B.list[L][c], n, (const uint32_t*)B.off[L][c],
+                               Ls->ent + Ls->base[2 * (L - 1) + c]);
+        }
+    return hipGetLastError();
+}
+
+void octree_free_lists(OctreeLists* Ls) {
+    if (Ls && Ls->ent) (void)hipFree(Ls->ent);
+    if (Ls) {
+        Ls->ent = nullptr;
+        Ls->cap = 0;
+    }
+}
+
+void launch_octree_sweeps(const Dims& d, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
+                          hipStream_t s) {
     const OG g{d.W, d.H, d.D};
     const OctW wt = make_weights();
     const LevelPtrs P{{B.p[0], B.p[1], B.p[2], B.p[3]}};
-    const dim3 g0((unsigned)((d.W / 2 + 63) / 64), (unsigned)((d.H + 3) / 4), (unsigned)d.D);
-    for (int it = 0; it < iters; it++) {
-        for (int c = 0; c < 2; c++)
-            hipLaunchKernelGGL(k_oct_gs0, g0, dim3(64, 4), 0, s, g, wt, P, (const uint8_t*)B.lev,
-                               (const float*)B.d[0], B.p[0], c, mean);
-        for (int L = 1; L <= 3; L++) {
-            const dim3 gl((unsigned)(((d.W >> L) / 2 + 63) / 64), (unsigned)(d.H >> L), (unsigned)(d.D >> L));
-            for (int c = 0; c < 2; c++)
-                hipLaunchKernelGGL(k_oct_gsL, gl, dim3(64), 0, s, g, wt, P, (const uint8_t*)B.lev, L,
-                                   (const float*)B.d[L], B.p[L], c, mean);
-        }
-    }
+    for (int it = 0; it < iters; it++)
+        for (int L = 0; L <= 3; L++)
+            for (int c = 0; c < 2; c++) {
+                const uint32_t n = Ls.counts[2 * L + c];
+                if (!n) continue;
+                if (L == 0)
+                    hipLaunchKernelGGL(k_oct_gs0, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, wt, P,
+                                       (const uint8_t*)B.lev, (const float*)B.d[0], B.p[0],
+                                       (const uint32_t*)B.list[0][c], n, mean);
+                else
+                    hipLaunchKernelGGL(k_oct_gsL, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, wt, P, L,
+                                       (const float*)B.d[L], B.p[L], (const uint32_t*)B.list[L][c], n,
+                                       (const uint32_t*)B.off[L][c],
+                                       (const uint32_t*)(Ls.ent + Ls.base[2 * (L - 1) + c]), mean);
+            }
 }
 
 void launch_octree_adjust(const Dims& d, const OctreeBufs& B, float* u, float* v, float* w, hipStream_t s) {

@@ -727,6 +727,10 @@ int check_config(const zq3d_config& c, const char* where) {
             set_err(where, "octree: width, height and depth must be multiples of 8");
             return ZQ3D_ERR_ARG;
         }
+        if ((long long)c.width * c.height * c.depth >= (1LL << 30)) {
+            set_err(where, "octree: at most 2^30 cells (neighbour entries hold 30-bit indices)");
+            return ZQ3D_ERR_ARG;
+        }
         if (c.sub_size || c.sor.mixed_precision) {
             set_err(where, "octree runs its own FP32 solve: sub_size and sor.mixed_precision must be 0");
             return ZQ3D_ERR_ARG;

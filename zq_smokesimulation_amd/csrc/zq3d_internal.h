@@ -125,6 +125,7 @@ struct OctreeState {
     Dims d{};
     int thresh[3] = {2, 4, 16};  // octree_thresh1..3
     OctreeBufs B{};
+    OctreeLists lists;
     void* mem = nullptr;
 };
 int octree_create(const Dims& d, const int thresh[3], OctreeState* out, const char* where);
@@ -132,7 +133,8 @@ void octree_destroy(OctreeState* o);
 // Octree projection of the faces in place: the leaves from occ, o.iters red-black passes over the
 // levels, the face correction, the leaf sub-projection; o.closed_mean subtracts the mean divergence
 // per fine cell times each leaf's volume.  B provides the mean and stats scratch; stats may be null.
-int project_octree(const OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+// Synchronises the stream twice (the cell lists' lengths and entry totals size the sweep launches).
+int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                    const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf);
 
 }  // namespace detail

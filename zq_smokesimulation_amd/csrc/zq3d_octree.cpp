@@ -18,10 +18,13 @@ int octree_create(const Dims& d, const int thresh[3], OctreeState* out, const ch
 
 void octree_destroy(OctreeState* o) {
     if (o && o->mem) (void)hipFree(o->mem);
-    if (o) o->mem = nullptr;
+    if (o) {
+        o->mem = nullptr;
+        octree_free_lists(&o->lists);
+    }
 }
 
-int project_octree(const OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
+int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                    const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf) {
     const char* where = "project_octree";
     // the leaves follow the occupancy, which callers may change between projections
@@ -34,16 +37,19 @@ int project_octree(const OctreeState& t, float* u, float* v, float* w, const uin
         launch_mean_finalize(t.d, B.part, &B.sc->mean32, &B.sc->mean64, 1, s);
         mean = &B.sc->mean32;
     }
+    ZQ_HIP(octree_build_lists(t.d, t.B, &t.lists, s), where);
     launch_octree_prepare(t.d, u, v, w, t.B, s);
     if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
-    launch_octree_sweeps(t.d, t.B, o.iters, mean, s);
+    launch_octree_sweeps(t.d, t.B, t.lists, o.iters, mean, s);
     if (pf) {
         ZQ_HIP(hipEventRecord(pf->b, s), where);
         ZQ_HIP(hipEventSynchronize(pf->b), where);
         float ms = 0.0f;
         ZQ_HIP(hipEventElapsedTime(&ms, pf->a, pf->b), where);
         pf->ms += ms;
-        pf->launches += 8LL * o.iters;
+        int nl = 0;
+        for (int i = 0; i < 8; i++) nl += t.lists.counts[i] ? 1 : 0;
+        pf->launches += (long long)nl * o.iters;
     }
     launch_octree_adjust(t.d, t.B, u, v, w, s);
     launch_octree_sub_project(t.d, t.B, u, v, w, s);
@@ -95,6 +101,11 @@ OctArena& oct_arena() {
     return a;
 }
 
+OctreeLists& oct_arena_lists() {  // the entry buffer of the host-pointer calls (guarded by the arena's mutex)
+    static OctreeLists l;
+    return l;
+}
+
 // One octree solve on host arrays: faces and leaf flags in, corrected faces out.  The index and
 // neighbour lists the reference passes are a function of the leaf flags alone (its
 // _selectInfos_for_CUDAOpenPoissonRedBlack3 builds them from nothing else); the kernels derive the
@@ -105,6 +116,10 @@ float octree_call(const char* where, float* mac_u, float* mac_v, float* mac_w, c
     if (check_dims(width, height, depth, where)) return -1.0f;
     if (width % 8 || height % 8 || depth % 8) {
         set_err(where, "width, height and depth must be multiples of 8 (the octree's 8^3 blocks)");
+        return -1.0f;
+    }
+    if ((long long)width * height * depth >= (1LL << 30)) {
+        set_err(where, "at most 2^30 cells (neighbour entries hold 30-bit indices)");
         return -1.0f;
     }
     if (!mac_u || !mac_v || !mac_w || !leaf0 || !leaf1 || !leaf2 || !leaf3) {
@@ -144,8 +159,10 @@ float octree_call(const char* where, float* mac_u, float* mac_v, float* mac_w, c
     if (div_per_volume) E(hipMemcpy(mean, div_per_volume, 4, hipMemcpyHostToDevice));
     if (E.e == hipSuccess) {
         launch_octree_from_leaves(d, l0, l1, l2, l3, B, 0);
+        OctreeLists& Ls = oct_arena_lists();
+        E(octree_build_lists(d, B, &Ls, 0));
         launch_octree_prepare(d, du, dv, dw, B, 0);
-        launch_octree_sweeps(d, B, maxIter, div_per_volume ? mean : nullptr, 0);
+        if (E.e == hipSuccess) launch_octree_sweeps(d, B, Ls, maxIter, div_per_volume ? mean : nullptr, 0);
         launch_octree_adjust(d, B, du, dv, dw, 0);
         E(hipGetLastError());
     }

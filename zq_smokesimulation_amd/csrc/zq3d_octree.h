@@ -26,6 +26,22 @@ struct OctreeBufs {
     int32_t* mn[4];             // per level L >= 1: the smallest distance in each block
     float* p[4];                // pressure per level
     float* d[4];                // divergence per level
+    uint32_t* list[4][2];       // per level and colour: the linear level index of every cell solved
+    uint32_t* cnt[4][2];        // levels 1-3: neighbour entries per listed cell (scan input)
+    uint32_t* off[4][2];        // levels 1-3: exclusive sums of cnt (n + 1 values)
+    uint32_t* count;            // 8 list lengths (level-major, colour-minor), then 6 entry totals
+    void* scan_tmp;             // scan scratch
+    size_t scan_bytes;
+};
+
+// The per-level, per-colour cell lists and the neighbour entries of the level 1-3 cells (the
+// reference's _selectInfos_for_CUDAOpenPoissonRedBlack3 lists, built on the device), rebuilt with the
+// leaves.  ent is a grow-only device buffer owned here.
+struct OctreeLists {
+    uint32_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // cells per level and colour
+    size_t base[6] = {0, 0, 0, 0, 0, 0};            // start of each level 1-3 list's entries in ent
+    uint32_t* ent = nullptr;
+    size_t cap = 0;
 };
 
 // bytes of one OctreeBufs allocation for the grid, and the carving of `mem` into it
@@ -38,12 +54,17 @@ void launch_octree_build(const Dims& g, const uint8_t* occ, int t1, int t2, int 
 // level map and leaf flags from caller-given leaf flags of the four levels (0/1 bytes)
 void launch_octree_from_leaves(const Dims& g, const uint8_t* l0, const uint8_t* l1, const uint8_t* l2,
                                const uint8_t* l3, const OctreeBufs& B, hipStream_t s);
+// the cell lists and neighbour entries from the level map (synchronises the stream twice: the list
+// lengths and entry totals size the launches and the entry buffer)
+hipError_t octree_build_lists(const Dims& g, const OctreeBufs& B, OctreeLists* Ls, hipStream_t s);
+void octree_free_lists(OctreeLists* Ls);
 // divergence of every level, zero pressures
 void launch_octree_prepare(const Dims& g, const float* u, const float* v, const float* w, const OctreeBufs& B,
                            hipStream_t s);
 // `iters` passes of level 0 red, level 0 black, ..., level 3 black; mean != nullptr: the closed
 // right-hand side div - mean * 8^L
-void launch_octree_sweeps(const Dims& g, const OctreeBufs& B, int iters, const float* mean, hipStream_t s);
+void launch_octree_sweeps(const Dims& g, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
+                          hipStream_t s);
 // face correction from the level pressures
 void launch_octree_adjust(const Dims& g, const OctreeBufs& B, float* u, float* v, float* w, hipStream_t s);
 // every level-L leaf's interior faces (L = 1, 2, 3) through the regular-block projection
