@@ -169,3 +169,32 @@ def test_converged_octree_projection_is_divergence_free_outside_level1_leaves():
     up1 = np.repeat(np.repeat(np.repeat(leaves[1], 2, 0), 2, 1), 2, 2) == 1
     scale = max(np.abs(a).max() for a in (u, v, w))
     assert np.abs(d[(occ == 0) & ~up1]).max() < 2e-3 * scale
+
+
+def test_repeated_projections_are_bit_identical():
+    """The leaves, lists and neighbour entries are rebuilt on every projection (atomically appended
+    lists come out in varying order): every projection of the same input gives the first one's bits."""
+    from zq_smokesimulation_amd.sim import SmokeSim3D
+    import zq_smokesimulation_amd._lib as L
+    W, H, D = 40, 32, 24
+    rng = np.random.default_rng(21)
+    u, v, w = faces(rng, W, H, D)
+    occ = scene(W, H, D, 21, [(14.0, 16.0, 12.0)], 0.002)
+    sim = SmokeSim3D(W, H, D, octree=1, octree_thresh=(1, 2, 4), sor_iters=25)
+    try:
+        outs = []
+        for r in range(4):
+            print("octree repeat", r, flush=True)
+            sim.set_state(u=u, v=v, w=w, occ=occ)
+            sim.project()
+            outs.append([sim.download(f) for f in (L.FIELD_U, L.FIELD_V, L.FIELD_W)])
+            sim.synchronize()
+    finally:
+        sim.close()
+    for r in range(1, 4):
+        for a, b, n in zip(outs[r], outs[0], "uvw"):
+            assert_same(a, b, "%s projection %d" % (n, r))
+    ru, rv, rw = u.copy(), v.copy(), w.copy()
+    leaves, _ = O.octree_projection(ru, rv, rw, occ, 25, (1, 2, 4))
+    mu, mv, mw = exterior_masks(leaves)
+    assert_same(outs[3][0], ru, "u vs oracle", mu)

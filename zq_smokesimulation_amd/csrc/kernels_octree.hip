@@ -18,6 +18,8 @@
 // per-level, per-colour cell lists and, for the 2^3 to 8^3 leaves, their neighbour entries (counted,
 // scanned, written), so that each of the iters x 8 sweep launches is a gather over a short list
 // (a scene far from its obstacles is mostly 8^3 leaves, and the launches touch only those).
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "zq3d_coarse.h"
@@ -229,66 +231,29 @@ __global__ void k_oct_zero(uint32_t* count) {
     if (threadIdx.x < 8) count[threadIdx.x] = 0u;
 }
 
-// level-0 leaves of one colour: the six face neighbours read straight off the level map
-__global__ __launch_bounds__(256) void k_oct_gs0(OG g, OctW wt, LevelPtrs P, const uint8_t* __restrict__ lev,
-                                                 const float* __restrict__ d0, float* __restrict__ p0,
-                                                 const uint32_t* __restrict__ list, uint32_t n,
-                                                 const float* __restrict__ mean) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = list[i];
-    const int x = (int)(c % (uint32_t)g.W), y = (int)((c / (uint32_t)g.W) % (uint32_t)g.H);
-    const int z = (int)(c / ((uint32_t)g.W * (uint32_t)g.H));
-    const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
-    int m[6];
-#pragma unroll
-    for (int f = 0; f < 6; f++) {
-        const bool out = nx[f] < 0 || ny[f] < 0 || nz[f] < 0 || nx[f] >= g.W || ny[f] >= g.H || nz[f] >= g.D;
-        m[f] = out ? -1 : lev[oat(g, 0, nx[f], ny[f], nz[f])];
-    }
-    float pv[6];
-#pragma unroll
-    for (int f = 0; f < 6; f++) {
-        const int mm = m[f];
-        pv[f] = (mm >= 0 && mm != OCT_SOLID) ? P.p[mm][oat(g, mm, nx[f] >> mm, ny[f] >> mm, nz[f] >> mm)] : 0.0f;
-    }
-    float coeff = 0.0f, sigma = 0.0f;
-#pragma unroll
-    for (int f = 0; f < 6; f++) {
-        if (m[f] < 0) {
-            coeff += wt.w[0][3];  // open domain face
-        } else if (m[f] != OCT_SOLID) {
-            const float wg = wt.w[0][m[f]];
-            coeff += wg;
-            sigma += wg * pv[f];
-        }
-    }
-    const float dd = d0[c];
-    sigma -= mean ? dd - *mean : dd;
-    p0[c] = sigma / coeff;
-}
-
 // A neighbour entry: level in bits 30-31, the linear index at that level below; OCT_OPEN marks an
 // open domain face (its weight enters the diagonal only).
 constexpr uint32_t OCT_OPEN = 0xFFFFFFFFu;
+constexpr int OCT_HEAD = 6;  // entries per cell kept in the transposed head (one per face)
+
 __device__ __forceinline__ uint32_t oct_entry(const OG& g, int m, int x, int y, int z) {
     return ((uint32_t)m << 30) | (uint32_t)oat(g, m, x >> m, y >> m, z >> m);
 }
 
-// The neighbour entries of the level-L cell (x, y, z), in the reference's order (WRITE: stored to
-// out); returns their number.  Per face: an open domain face; else the leaf behind the face's first
-// fine cell when it is as large as this one; else the face's fine cells in interleaved order (slower
-// axis high), one entry per leaf met (its run of 4^m positions skipped), solid cells skipped.
-template <bool WRITE>
-__device__ int oct_walk(const OG& g, const uint8_t* __restrict__ lev, int L, int x, int y, int z, uint32_t* out) {
+// The neighbour entries of the level-L cell (x, y, z), in the reference's order, handed to put(k, e);
+// returns their number.  Per face: an open domain face; else the leaf behind the face's first fine
+// cell when it is as large as this one (at level 0: the fine neighbour unless solid); else the face's
+// fine cells in interleaved order (slower axis high), one entry per leaf met (its run of 4^m
+// positions skipped), solid cells skipped.
+template <class Put>
+__device__ int oct_walk(const OG& g, const uint8_t* __restrict__ lev, int L, int x, int y, int z, Put put) {
     const int S = 1 << L;
     const int cl[3] = {x, y, z}, dims[3] = {ow(g, L), oh(g, L), od(g, L)}, base[3] = {x * S, y * S, z * S};
     int k = 0;
     for (int f = 0; f < 6; f++) {
         const int ax = f >> 1, sd = (f & 1) ? 1 : -1;
         if ((sd < 0 && cl[ax] == 0) || (sd > 0 && cl[ax] == dims[ax] - 1)) {
-            if (WRITE) out[k] = OCT_OPEN;
-            k++;
+            put(k++, OCT_OPEN);
             continue;
         }
         const int slow = ax == 2 ? 1 : 2, fast = ax == 0 ? 1 : 0;
@@ -296,8 +261,7 @@ __device__ int oct_walk(const OG& g, const uint8_t* __restrict__ lev, int L, int
         q0[ax] = sd < 0 ? base[ax] - 1 : base[ax] + S;
         const int m0 = lev[oat(g, 0, q0[0], q0[1], q0[2])];
         if (m0 != OCT_SOLID && m0 >= L) {  // one leaf as large as this one covers the face
-            if (WRITE) out[k] = oct_entry(g, m0, q0[0], q0[1], q0[2]);
-            k++;
+            put(k++, oct_entry(g, m0, q0[0], q0[1], q0[2]));
             continue;
         }
         for (int t = 0; t < S * S;) {
@@ -314,8 +278,7 @@ __device__ int oct_walk(const OG& g, const uint8_t* __restrict__ lev, int L, int
                 t += 1;
                 continue;
             }
-            if (WRITE) out[k] = oct_entry(g, m, q[0], q[1], q[2]);
-            k++;
+            put(k++, oct_entry(g, m, q[0], q[1], q[2]));
             t += 1 << (2 * m);  // m < L here: the rest of that leaf's run on this face
         }
     }
@@ -329,75 +292,118 @@ __device__ __forceinline__ void oct_cell(const OG& g, int L, uint32_t c, int& x,
     z = (int)(c / (wl * hl));
 }
 
-// entries per listed cell; cnt[n] = 0 closes the array for the exclusive scan
+// entries per listed cell (tot) and beyond the head (ovc, the scan input; ovc[n] = 0 closes it)
 __global__ __launch_bounds__(256) void k_oct_count(OG g, const uint8_t* __restrict__ lev, int L,
                                                    const uint32_t* __restrict__ list, uint32_t n,
-                                                   uint32_t* __restrict__ cnt) {
+                                                   uint32_t* __restrict__ tot, uint32_t* __restrict__ ovc) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i > n) return;
     if (i == n) {
-        cnt[n] = 0;
+        ovc[n] = 0;
         return;
     }
     int x, y, z;
     oct_cell(g, L, list[i], x, y, z);
-    cnt[i] = (uint32_t)oct_walk<false>(g, lev, L, x, y, z, nullptr);
+    const int k = oct_walk(g, lev, L, x, y, z, [](int, uint32_t) {});
+    tot[i] = (uint32_t)k;
+    ovc[i] = k > OCT_HEAD ? (uint32_t)(k - OCT_HEAD) : 0u;
 }
 
+// entries: the first OCT_HEAD transposed (entry j of cell i at j * n + i, so a wave's loads of one
+// j are contiguous), the rest at the cell's scanned offset in ovf
 __global__ __launch_bounds__(256) void k_oct_fill(OG g, const uint8_t* __restrict__ lev, int L,
                                                   const uint32_t* __restrict__ list, uint32_t n,
-                                                  const uint32_t* __restrict__ off, uint32_t* __restrict__ ent) {
+                                                  uint32_t* __restrict__ head, const uint32_t* __restrict__ off,
+                                                  uint32_t* __restrict__ ovf) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     int x, y, z;
     oct_cell(g, L, list[i], x, y, z);
-    oct_walk<true>(g, lev, L, x, y, z, ent + off[i]);
+    uint32_t* ov = ovf + off[i];
+    oct_walk(g, lev, L, x, y, z, [&](int k, uint32_t e) {
+        if (k < OCT_HEAD) head[(size_t)k * n + i] = e;
+        else ov[k - OCT_HEAD] = e;
+    });
 }
 
 struct TotPtrs {
-    const uint32_t* off[6];
-    uint32_t n[6];
+    const uint32_t* off[8];
+    uint32_t n[8];
 };
 
 __global__ void k_oct_totals(TotPtrs t, uint32_t* out) {
     const int k = threadIdx.x;
-    if (k < 6) out[k] = t.off[k][t.n[k]];
+    if (k < 8) out[k] = t.n[k] ? t.off[k][t.n[k]] : 0u;
 }
 
-// level L >= 1 cells of one colour: a gather over each cell's entries in order
-__global__ __launch_bounds__(256) void k_oct_gsL(OG g, OctW wt, LevelPtrs P, int L, const float* __restrict__ dl,
-                                                 float* __restrict__ pl, const uint32_t* __restrict__ list, uint32_t n,
-                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ ent,
-                                                 const float* __restrict__ mean) {
+// right-hand side per listed cell in list order: the level's divergence, with `mean` the closed form
+// div - mean * 8^L
+__global__ __launch_bounds__(256) void k_oct_rhs(OG g, int L, const uint32_t* __restrict__ list, uint32_t n,
+                                                 const float* __restrict__ dl, const float* __restrict__ mean,
+                                                 float* __restrict__ rhs) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const uint32_t c = list[i];
-    const uint32_t e0 = off[i], e1 = off[i + 1];
-    const float dd = dl[c];
+    const float dd = dl[list[i]];
+    rhs[i] = mean ? dd - *mean * (float)(1 << (3 * L)) : dd;
+}
+
+// One colour of one level: per listed cell coeff += w, sigma += w * p over its entries in order, then
+// p = (sigma - rhs) / coeff (CPU_PoissonSolver3D_OpenOctreePoisson.cpp:404-425).  The head entries
+// and the right-hand side load without any dependence, then all the pressures they name.
+__global__ __launch_bounds__(256) void k_oct_gs(OctW wt, LevelPtrs P, int L, float* __restrict__ pl,
+                                                const uint32_t* __restrict__ list, uint32_t n,
+                                                const uint32_t* __restrict__ head, const uint32_t* __restrict__ tot,
+                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ ovf,
+                                                const float* __restrict__ rhs) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = list[i], nt = tot[i];
+    const float r = rhs[i];
+    uint32_t ev[OCT_HEAD];
+    float pv[OCT_HEAD];
+#pragma unroll
+    for (int j = 0; j < OCT_HEAD; j++) ev[j] = (uint32_t)j < nt ? head[(size_t)j * n + i] : OCT_OPEN;
+#pragma unroll
+    for (int j = 0; j < OCT_HEAD; j++)
+        pv[j] = ((uint32_t)j < nt && ev[j] != OCT_OPEN) ? P.p[ev[j] >> 30][ev[j] & 0x3FFFFFFFu] : 0.0f;
     float coeff = 0.0f, sigma = 0.0f;
-    // eight entries at a time: their loads, then their pressures' loads, all in flight together;
-    // the sums then run in entry order
-    for (uint32_t k0 = e0; k0 < e1; k0 += 8) {
-        uint32_t ev[8];
-        float pv[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) ev[j] = k0 + j < e1 ? ent[k0 + j] : OCT_OPEN;
+    for (int j = 0; j < OCT_HEAD; j++) {
+        if ((uint32_t)j >= nt) break;
+        if (ev[j] == OCT_OPEN) {
+            coeff += wt.w[L][3];  // open domain face
+        } else {
+            const float wg = wt.w[L][ev[j] >> 30];
+            coeff += wg;
+            sigma += wg * pv[j];
+        }
+    }
+    if (nt > OCT_HEAD) {
+        // a face with finer leaves behind it: the rest, eight at a time, in order
+        const uint32_t* ov = ovf + off[i];
+        const uint32_t no = nt - OCT_HEAD;
+        for (uint32_t k0 = 0; k0 < no; k0 += 8) {
+            uint32_t e8[8];
+            float p8[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-            pv[j] = (k0 + j < e1 && ev[j] != OCT_OPEN) ? P.p[ev[j] >> 30][ev[j] & 0x3FFFFFFFu] : 0.0f;
+            for (int j = 0; j < 8; j++) e8[j] = k0 + j < no ? ov[k0 + j] : OCT_OPEN;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (k0 + j >= e1) break;
-            if (ev[j] == OCT_OPEN) {
-                coeff += wt.w[L][3];  // open domain face
-            } else {
-                const float wg = wt.w[L][ev[j] >> 30];
-                coeff += wg;
-                sigma += wg * pv[j];
+            for (int j = 0; j < 8; j++)
+                p8[j] = (k0 + j < no && e8[j] != OCT_OPEN) ? P.p[e8[j] >> 30][e8[j] & 0x3FFFFFFFu] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (k0 + j >= no) break;
+                if (e8[j] == OCT_OPEN) {
+                    coeff += wt.w[L][3];
+                } else {
+                    const float wg = wt.w[L][e8[j] >> 30];
+                    coeff += wg;
+                    sigma += wg * p8[j];
+                }
             }
         }
     }
-    sigma -= mean ? dd - *mean * (float)(1 << (3 * L)) : dd;
+    sigma -= r;
     pl[c] = sigma / coeff;
 }
 
@@ -464,11 +470,15 @@ dim3 fine_grid(const Dims& g) { return dim3((unsigned)((g.W + 63) / 64), (unsign
 size_t level_cells(const Dims& g, int L) { return (size_t)(g.W >> L) * (g.H >> L) * (g.D >> L); }
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-size_t scan_temp_bytes(const Dims& g) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (int)(level_cells(g, 1) / 2 + 1));
-    return bytes;
+hipError_t grow(void** p, size_t* cap, size_t need) {
+    if (need <= *cap && *p) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t c = need + need / 2 + 4096;
+    hipError_t e = hipMalloc(p, c);
+    if (e == hipSuccess) *cap = c;
+    return e;
 }
 
 }  // namespace
@@ -478,8 +488,7 @@ size_t octree_bytes(const Dims& g) {
     for (int L = 1; L <= 3; L++) b += up256(level_cells(g, L)) + up256(4 * level_cells(g, L));  // leaf, mn
     for (int L = 0; L <= 3; L++) b += 2 * up256(4 * level_cells(g, L));  // p, d
     for (int L = 0; L <= 3; L++) b += 2 * up256(4 * (level_cells(g, L) / 2 + 1));  // lists
-    for (int L = 1; L <= 3; L++) b += 4 * up256(4 * (level_cells(g, L) / 2 + 2));  // cnt, off
-    return b + 256 + up256(scan_temp_bytes(g));  // counts, scan scratch
+    return b + 256;  // counts
 }
 
 void octree_layout(const Dims& g, char* mem, OctreeBufs* B) {
@@ -503,15 +512,7 @@ void octree_layout(const Dims& g, char* mem, OctreeBufs* B) {
     }
     for (int L = 0; L <= 3; L++)
         for (int c = 0; c < 2; c++) B->list[L][c] = (uint32_t*)take(4 * (level_cells(g, L) / 2 + 1));
-    B->cnt[0][0] = B->cnt[0][1] = B->off[0][0] = B->off[0][1] = nullptr;
-    for (int L = 1; L <= 3; L++)
-        for (int c = 0; c < 2; c++) {
-            B->cnt[L][c] = (uint32_t*)take(4 * (level_cells(g, L) / 2 + 2));
-            B->off[L][c] = (uint32_t*)take(4 * (level_cells(g, L) / 2 + 2));
-        }
     B->count = (uint32_t*)take(64);
-    B->scan_tmp = take(scan_temp_bytes(g));
-    B->scan_bytes = scan_temp_bytes(g);
 }
 
 void launch_octree_build(const Dims& d, const uint8_t* occ, int t1, int t2, int t3, const OctreeBufs& B,
@@ -552,56 +553,72 @@ hipError_t octree_build_lists(const Dims& d, const OctreeBufs& B, OctreeLists* L
     hipError_t e = hipMemcpyAsync(Ls->counts, B.count, 32, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    // neighbour entries of the level 1-3 cells: count, scan, fill
+    // per list: head (OCT_HEAD n), tot (n), ovc (n + 1), off (n + 1), rhs (n) in one buffer
+    size_t bytes = 0, nmax = 1;
+    for (int k = 0; k < 8; k++) {
+        const size_t n = Ls->counts[k];
+        bytes += up256(4 * OCT_HEAD * n) + up256(4 * n) + 2 * up256(4 * (n + 1)) + up256(4 * n);
+        nmax = std::max(nmax, n + 1);
+    }
+    if ((e = grow(&Ls->dyn, &Ls->dyn_cap, bytes)) != hipSuccess) return e;
+    size_t scan = 0;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nmax,
+                                              s)) != hipSuccess)
+        return e;
+    if ((e = grow(&Ls->scan, &Ls->scan_cap, scan)) != hipSuccess) return e;
+    char* p = (char*)Ls->dyn;
     TotPtrs tp{};
-    for (int L = 1; L <= 3; L++)
-        for (int c = 0; c < 2; c++) {
-            const uint32_t n = Ls->counts[2 * L + c];
-            hipLaunchKernelGGL(k_oct_count, dim3(blocks_of((size_t)n + 1, 256)), dim3(256), 0, s, g,
-                               (const uint8_t*)B.lev, L, (const uint32_t*)B.list[L][c], n, B.cnt[L][c]);
-            size_t bytes = B.scan_bytes;
-            e = hipcub::DeviceScan::ExclusiveSum(B.scan_tmp, bytes, (const uint32_t*)B.cnt[L][c], B.off[L][c],
-                                                 (int)(n + 1), s);
-            if (e != hipSuccess) return e;
-            tp.off[2 * (L - 1) + c] = B.off[L][c];
-            tp.n[2 * (L - 1) + c] = n;
-        }
+    for (int k = 0; k < 8; k++) {
+        const uint32_t n = Ls->counts[k];
+        Ls->head[k] = (uint32_t*)p;
+        p += up256(4 * OCT_HEAD * (size_t)n);
+        Ls->tot[k] = (uint32_t*)p;
+        p += up256(4 * (size_t)n);
+        Ls->ovc[k] = (uint32_t*)p;
+        p += up256(4 * ((size_t)n + 1));
+        Ls->off[k] = (uint32_t*)p;
+        p += up256(4 * ((size_t)n + 1));
+        Ls->rhs[k] = (float*)p;
+        p += up256(4 * (size_t)n);
+        tp.off[k] = Ls->off[k];
+        tp.n[k] = n;
+        if (!n) continue;
+        hipLaunchKernelGGL(k_oct_count, dim3(blocks_of((size_t)n + 1, 256)), dim3(256), 0, s, g,
+                           (const uint8_t*)B.lev, k / 2, (const uint32_t*)B.list[k / 2][k % 2], n, Ls->tot[k],
+                           Ls->ovc[k]);
+        size_t sb = Ls->scan_cap;
+        if ((e = hipcub::DeviceScan::ExclusiveSum(Ls->scan, sb, (const uint32_t*)Ls->ovc[k], Ls->off[k], (int)(n + 1),
+                                                  s)) != hipSuccess)
+            return e;
+    }
     hipLaunchKernelGGL(k_oct_totals, dim3(1), dim3(64), 0, s, tp, B.count + 8);
-    uint32_t tot[6];
-    e = hipMemcpyAsync(tot, B.count + 8, 24, hipMemcpyDeviceToHost, s);
+    uint32_t tot[8];
+    e = hipMemcpyAsync(tot, B.count + 8, 32, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     size_t need = 0;
-    for (int k = 0; k < 6; k++) {
-        Ls->base[k] = need;
+    for (int k = 0; k < 8; k++) {
+        Ls->ovf_base[k] = need;
         need += tot[k];
     }
-    if (need > Ls->cap) {
-        if (Ls->ent) (void)hipFree(Ls->ent);
-        Ls->ent = nullptr;
-        Ls->cap = 0;
-        const size_t cap = need + need / 2 + 1024;
-        e = hipMalloc(&Ls->ent, cap * 4);
-        if (e != hipSuccess) return e;
-        Ls->cap = cap;
+    if ((e = grow((void**)&Ls->ovf, &Ls->ovf_cap, 4 * need + 4)) != hipSuccess) return e;
+    for (int k = 0; k < 8; k++) {
+        const uint32_t n = Ls->counts[k];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_oct_fill, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, (const uint8_t*)B.lev, k / 2,
+                           (const uint32_t*)B.list[k / 2][k % 2], n, Ls->head[k], (const uint32_t*)Ls->off[k],
+                           Ls->ovf + Ls->ovf_base[k]);
     }
-    for (int L = 1; L <= 3; L++)
-        for (int c = 0; c < 2; c++) {
-            const uint32_t n = Ls->counts[2 * L + c];
-            if (!n) continue;
-            hipLaunchKernelGGL(k_oct_fill, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, (const uint8_t*)B.lev, L,
-                               (const uint32_t*)B.list[L][c], n, (const uint32_t*)B.off[L][c],
-                               Ls->ent + Ls->base[2 * (L - 1) + c]);
-        }
     return hipGetLastError();
 }
 
 void octree_free_lists(OctreeLists* Ls) {
-    if (Ls && Ls->ent) (void)hipFree(Ls->ent);
-    if (Ls) {
-        Ls->ent = nullptr;
-        Ls->cap = 0;
-    }
+    if (!Ls) return;
+    for (void** p : {&Ls->dyn, (void**)&Ls->ovf, &Ls->scan})
+        if (*p) (void)hipFree(*p);
+    Ls->dyn = Ls->scan = nullptr;
+    Ls->ovf = nullptr;
+    Ls->dyn_cap = Ls->ovf_cap = Ls->scan_cap = 0;
 }
 
 void launch_octree_sweeps(const Dims& d, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
@@ -609,21 +626,21 @@ void launch_octree_sweeps(const Dims& d, const OctreeBufs& B, const OctreeLists&
     const OG g{d.W, d.H, d.D};
     const OctW wt = make_weights();
     const LevelPtrs P{{B.p[0], B.p[1], B.p[2], B.p[3]}};
+    for (int k = 0; k < 8; k++) {
+        const uint32_t n = Ls.counts[k];
+        if (n)
+            hipLaunchKernelGGL(k_oct_rhs, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, k / 2,
+                               (const uint32_t*)B.list[k / 2][k % 2], n, (const float*)B.d[k / 2], mean, Ls.rhs[k]);
+    }
     for (int it = 0; it < iters; it++)
-        for (int L = 0; L <= 3; L++)
-            for (int c = 0; c < 2; c++) {
-                const uint32_t n = Ls.counts[2 * L + c];
-                if (!n) continue;
-                if (L == 0)
-                    hipLaunchKernelGGL(k_oct_gs0, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, wt, P,
-                                       (const uint8_t*)B.lev, (const float*)B.d[0], B.p[0],
-                                       (const uint32_t*)B.list[0][c], n, mean);
-                else
-                    hipLaunchKernelGGL(k_oct_gsL, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, wt, P, L,
-                                       (const float*)B.d[L], B.p[L], (const uint32_t*)B.list[L][c], n,
-                                       (const uint32_t*)B.off[L][c],
-                                       (const uint32_t*)(Ls.ent + Ls.base[2 * (L - 1) + c]), mean);
-            }
+        for (int k = 0; k < 8; k++) {
+            const uint32_t n = Ls.counts[k];
+            if (!n) continue;
+            hipLaunchKernelGGL(k_oct_gs, dim3(blocks_of(n, 256)), dim3(256), 0, s, wt, P, k / 2, B.p[k / 2],
+                               (const uint32_t*)B.list[k / 2][k % 2], n, (const uint32_t*)Ls.head[k],
+                               (const uint32_t*)Ls.tot[k], (const uint32_t*)Ls.off[k],
+                               (const uint32_t*)(Ls.ovf + Ls.ovf_base[k]), (const float*)Ls.rhs[k]);
+        }
 }
 
 void launch_octree_adjust(const Dims& d, const OctreeBufs& B, float* u, float* v, float* w, hipStream_t s) {

@@ -17,11 +17,10 @@ int octree_create(const Dims& d, const int thresh[3], OctreeState* out, const ch
 }
 
 void octree_destroy(OctreeState* o) {
-    if (o && o->mem) (void)hipFree(o->mem);
-    if (o) {
-        o->mem = nullptr;
-        octree_free_lists(&o->lists);
-    }
+    if (!o) return;
+    if (o->mem) (void)hipFree(o->mem);
+    o->mem = nullptr;
+    octree_free_lists(&o->lists);
 }
 
 int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,

@@ -27,21 +27,28 @@ struct OctreeBufs {
     float* p[4];                // pressure per level
     float* d[4];                // divergence per level
     uint32_t* list[4][2];       // per level and colour: the linear level index of every cell solved
-    uint32_t* cnt[4][2];        // levels 1-3: neighbour entries per listed cell (scan input)
-    uint32_t* off[4][2];        // levels 1-3: exclusive sums of cnt (n + 1 values)
-    uint32_t* count;            // 8 list lengths (level-major, colour-minor), then 6 entry totals
-    void* scan_tmp;             // scan scratch
-    size_t scan_bytes;
+    uint32_t* count;            // 8 list lengths (level-major, colour-minor), then 8 overflow totals
 };
 
-// The per-level, per-colour cell lists and the neighbour entries of the level 1-3 cells (the
-// reference's _selectInfos_for_CUDAOpenPoissonRedBlack3 lists, built on the device), rebuilt with the
-// leaves.  ent is a grow-only device buffer owned here.
+// Per list k = 2 L + colour: the neighbour entries of its cells (the reference's
+// _selectInfos_for_CUDAOpenPoissonRedBlack3 lists, built on the device) and their right-hand sides,
+// rebuilt with the leaves in grow-only device buffers owned here.  head holds each cell's first six
+// entries transposed (entry j of cell i at j * n + i); the rest of a cell's entries sit in ovf at
+// ovf_base[k] + off[k][i].
 struct OctreeLists {
     uint32_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // cells per level and colour
-    size_t base[6] = {0, 0, 0, 0, 0, 0};            // start of each level 1-3 list's entries in ent
-    uint32_t* ent = nullptr;
-    size_t cap = 0;
+    uint32_t* head[8] = {};
+    uint32_t* tot[8] = {};    // entries per cell
+    uint32_t* ovc[8] = {};    // entries beyond the head per cell (n + 1 values, last 0)
+    uint32_t* off[8] = {};    // exclusive sums of ovc
+    float* rhs[8] = {};       // right-hand side per cell, in list order
+    size_t ovf_base[8] = {};
+    void* dyn = nullptr;
+    size_t dyn_cap = 0;
+    uint32_t* ovf = nullptr;
+    size_t ovf_cap = 0;
+    void* scan = nullptr;
+    size_t scan_cap = 0;
 };
 
 // bytes of one OctreeBufs allocation for the grid, and the carving of `mem` into it
@@ -61,8 +68,8 @@ void octree_free_lists(OctreeLists* Ls);
 // divergence of every level, zero pressures
 void launch_octree_prepare(const Dims& g, const float* u, const float* v, const float* w, const OctreeBufs& B,
                            hipStream_t s);
-// `iters` passes of level 0 red, level 0 black, ..., level 3 black; mean != nullptr: the closed
-// right-hand side div - mean * 8^L
+// the right-hand sides (mean != nullptr: the closed form div - mean * 8^L), then `iters` passes of
+// level 0 red, level 0 black, ..., level 3 black
 void launch_octree_sweeps(const Dims& g, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
                           hipStream_t s);
 // face correction from the level pressures
