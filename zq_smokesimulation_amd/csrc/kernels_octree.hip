@@ -347,16 +347,14 @@ __global__ __launch_bounds__(256) void k_oct_rhs(OG g, int L, const uint32_t* __
     rhs[i] = mean ? dd - *mean * (float)(1 << (3 * L)) : dd;
 }
 
-// One colour of one level: per listed cell coeff += w, sigma += w * p over its entries in order, then
-// p = (sigma - rhs) / coeff (CPU_PoissonSolver3D_OpenOctreePoisson.cpp:404-425).  The head entries
-// and the right-hand side load without any dependence, then all the pressures they name.
-__global__ __launch_bounds__(256) void k_oct_gs(OctW wt, LevelPtrs P, int L, float* __restrict__ pl,
-                                                const uint32_t* __restrict__ list, uint32_t n,
-                                                const uint32_t* __restrict__ head, const uint32_t* __restrict__ tot,
-                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ ovf,
-                                                const float* __restrict__ rhs) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+// One listed cell of one colour of one level: coeff += w, sigma += w * p over its entries in order,
+// then p = (sigma - rhs) / coeff (CPU_PoissonSolver3D_OpenOctreePoisson.cpp:404-425).  The head
+// entries and the right-hand side load without any dependence, then all the pressures they name.
+__device__ __forceinline__ void oct_gs_cell(const OctW& wt, const LevelPtrs& P, int L, float* pl,
+                                            const uint32_t* __restrict__ list, uint32_t n,
+                                            const uint32_t* __restrict__ head, const uint32_t* __restrict__ tot,
+                                            const uint32_t* __restrict__ off, const uint32_t* __restrict__ ovf,
+                                            const float* __restrict__ rhs, uint32_t i) {
     const uint32_t c = list[i], nt = tot[i];
     const float r = rhs[i];
     uint32_t ev[OCT_HEAD];
@@ -405,6 +403,16 @@ __global__ __launch_bounds__(256) void k_oct_gs(OctW wt, LevelPtrs P, int L, flo
     }
     sigma -= r;
     pl[c] = sigma / coeff;
+}
+
+// One colour of one level, one launch.
+__global__ __launch_bounds__(256) void k_oct_gs(OctW wt, LevelPtrs P, int L, float* __restrict__ pl,
+                                                const uint32_t* __restrict__ list, uint32_t n,
+                                                const uint32_t* __restrict__ head, const uint32_t* __restrict__ tot,
+                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ ovf,
+                                                const float* __restrict__ rhs) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) oct_gs_cell(wt, P, L, pl, list, n, head, tot, off, ovf, rhs, i);
 }
 
 // ---------------------------------------------------------------------------- face correction
@@ -621,17 +629,21 @@ void octree_free_lists(OctreeLists* Ls) {
     Ls->dyn_cap = Ls->ovf_cap = Ls->scan_cap = 0;
 }
 
-void launch_octree_sweeps(const Dims& d, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
-                          hipStream_t s) {
+int launch_octree_sweeps(const Dims& d, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
+                         hipStream_t s) {
     const OG g{d.W, d.H, d.D};
     const OctW wt = make_weights();
-    const LevelPtrs P{{B.p[0], B.p[1], B.p[2], B.p[3]}};
+    uint32_t nmax = 0;
     for (int k = 0; k < 8; k++) {
         const uint32_t n = Ls.counts[k];
+        nmax = std::max(nmax, n);
         if (n)
             hipLaunchKernelGGL(k_oct_rhs, dim3(blocks_of(n, 256)), dim3(256), 0, s, g, k / 2,
                                (const uint32_t*)B.list[k / 2][k % 2], n, (const float*)B.d[k / 2], mean, Ls.rhs[k]);
     }
+    if (iters <= 0 || nmax == 0) return 0;
+    const LevelPtrs P{{B.p[0], B.p[1], B.p[2], B.p[3]}};
+    int launches = 0;
     for (int it = 0; it < iters; it++)
         for (int k = 0; k < 8; k++) {
             const uint32_t n = Ls.counts[k];
@@ -640,7 +652,9 @@ void launch_octree_sweeps(const Dims& d, const OctreeBufs& B, const OctreeLists&
                                (const uint32_t*)B.list[k / 2][k % 2], n, (const uint32_t*)Ls.head[k],
                                (const uint32_t*)Ls.tot[k], (const uint32_t*)Ls.off[k],
                                (const uint32_t*)(Ls.ovf + Ls.ovf_base[k]), (const float*)Ls.rhs[k]);
+            launches++;
         }
+    return launches;
 }
 
 void launch_octree_adjust(const Dims& d, const OctreeBufs& B, float* u, float* v, float* w, hipStream_t s) {

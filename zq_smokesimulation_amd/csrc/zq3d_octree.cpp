@@ -39,16 +39,14 @@ int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* 
     ZQ_HIP(octree_build_lists(t.d, t.B, &t.lists, s), where);
     launch_octree_prepare(t.d, u, v, w, t.B, s);
     if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
-    launch_octree_sweeps(t.d, t.B, t.lists, o.iters, mean, s);
+    const int launches = launch_octree_sweeps(t.d, t.B, t.lists, o.iters, mean, s);
     if (pf) {
         ZQ_HIP(hipEventRecord(pf->b, s), where);
         ZQ_HIP(hipEventSynchronize(pf->b), where);
         float ms = 0.0f;
         ZQ_HIP(hipEventElapsedTime(&ms, pf->a, pf->b), where);
         pf->ms += ms;
-        int nl = 0;
-        for (int i = 0; i < 8; i++) nl += t.lists.counts[i] ? 1 : 0;
-        pf->launches += (long long)nl * o.iters;
+        pf->launches += launches;
     }
     launch_octree_adjust(t.d, t.B, u, v, w, s);
     launch_octree_sub_project(t.d, t.B, u, v, w, s);

@@ -69,9 +69,9 @@ void octree_free_lists(OctreeLists* Ls);
 void launch_octree_prepare(const Dims& g, const float* u, const float* v, const float* w, const OctreeBufs& B,
                            hipStream_t s);
 // the right-hand sides (mean != nullptr: the closed form div - mean * 8^L), then `iters` passes of
-// level 0 red, level 0 black, ..., level 3 black
-void launch_octree_sweeps(const Dims& g, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
-                          hipStream_t s);
+// level 0 red, level 0 black, ..., level 3 black, one launch per phase.  Returns the launches made.
+int launch_octree_sweeps(const Dims& g, const OctreeBufs& B, const OctreeLists& Ls, int iters, const float* mean,
+                         hipStream_t s);
 // face correction from the level pressures
 void launch_octree_adjust(const Dims& g, const OctreeBufs& B, float* u, float* v, float* w, hipStream_t s);
 // every level-L leaf's interior faces (L = 1, 2, 3) through the regular-block projection
