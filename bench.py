@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the 3D smoke forward step on one MI355X.
 
-Workload (BASELINE config 4): 256^3 MAC grid, FP32; velocity, density and temperature
+Workload: 256^3 MAC grid, FP32; velocity, density and temperature
 advection; buoyancy (alpha 0.05, beta 0.5) and vorticity confinement (eps 0.3); attenuation
 0.99 per step on velocity, temperature and density; mixed-precision red-black SOR projection
 (omega 1.8, FP32 sweeps, FP64 residual refinement) to max|div| <= 1e-6 with a warm start;
@@ -429,7 +429,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "3D smoke forward step, %d^3 MAC grid, closed box, mixed-precision red-black SOR to "
-                        "max|div|<=%g (config 4)" % (W, args.tol),
+                        "max|div|<=%g" % (W, args.tol),
             "grid": [W, H, D],
             "advection_substeps": 20,
             "sor_omega": 1.8,

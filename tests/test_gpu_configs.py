@@ -1,12 +1,12 @@
 """GPU: the plain-forward workloads at their full sizes against the oracle.
 
-Config 0 -- 64^3, closed walls, no obstacle, zero initial velocity, sphere source (r 0.08 at
+Closed box at 64^3 -- closed walls, no obstacle, zero initial velocity, sphere source (r 0.08 at
 (0.5, 0.1, 0.5), density 1, temperature 1), seeded +-0.01 temperature noise, alpha 0.05, beta 0.5,
 eps 0.3, dt 0.02, red-black SOR with omega 1.8 for 200 iterations, 100 steps.  The oracle's
 closed-domain mean folds the same per-tile FP64 partials in the same order as the device, so all
 fields agree bit for bit after 100 steps.
 
-Config 1 -- 128^3, a solid sphere (radius 0.1) moving on a seeded straight path, open top face,
+Moving sphere at 128^3 -- a solid sphere (radius 0.1) moving on a seeded straight path, open top face,
 walls elsewhere, mixed-precision projection to max|div| <= 1e-6.  Every step: the device reaches
 the tolerance, the obstacle's faces carry its velocity; the first steps agree with the oracle's to
 1e-5 of the field scale.
@@ -44,7 +44,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
 
 
-def test_config0_100_steps():
+def test_closed_box_64_100_steps():
     W = 64
     p = params(W)
     occ, u, v, w = O.init_fine_grid(W, W, W, O.ALL_FACES, 0)
@@ -57,12 +57,12 @@ def test_config0_100_steps():
         O.step(p, u, v, w, T, R, occ, spheres=SRC)
         sim.step(stats=False)
         if s % 20 == 19:
-            print("config 0 step", s + 1, flush=True)
+            print("closed box 64 step", s + 1, flush=True)
     g = sim.get_state()
     sim.close()
     err_rho = rel_l2(g["rho"], R)
     err_vel = max(rel_l2(g[k], a) for k, a in (("u", u), ("v", v), ("w", w)))
-    print("config 0 after 100 steps: density rel L2 %.3g, velocity rel L2 %.3g, mass %.6g vs %.6g"
+    print("closed box 64 after 100 steps: density rel L2 %.3g, velocity rel L2 %.3g, mass %.6g vs %.6g"
           % (err_rho, err_vel, g["rho"].sum(), R.sum()), flush=True)
     assert R.max() > 0.5 and v.max() > 0.0  # the plume rose
     assert err_rho == 0.0 and err_vel == 0.0
@@ -70,7 +70,7 @@ def test_config0_100_steps():
         assert np.array_equal(g[k].view(np.uint32), a.view(np.uint32)), k
 
 
-def test_config1_moving_sphere_open_top():
+def test_moving_sphere_128_open_top():
     W = 128
     shell = O.ALL_FACES & ~O.OPEN_YHI
     p = params(W, open_mask=O.OPEN_YHI, closed_mean=0, mixed=1, sor_iters=40, max_outer=2000, tol=1e-6)
@@ -98,7 +98,7 @@ def test_config1_moving_sphere_open_top():
                 scale = max(float(np.abs(a).max()), 1e-6)
                 err = float(np.abs(g[name] - a).max())
                 assert err <= 1e-5 * scale, "step %d %s: max err %g (scale %g)" % (s, name, err, scale)
-        print("config 1 step", s, st["outer"], st["max_div"], flush=True)
+        print("moving sphere 128 step", s, st["outer"], st["max_div"], flush=True)
     g = sim.get_state()
     sim.close()
     # every face of an obstacle voxel carries the obstacle's velocity (ZQ_ApplyMovingObject3D)

@@ -81,8 +81,8 @@ def test_step_open_bfecc_bit_exact(mode):
         assert bad.size == 0, "mode %d %s: %d cells differ (max |d| %g)" % (mode, name, bad.size, np.abs(a - b).max())
 
 
-def test_step_closed_config0_like():
-    """Closed shell, alpha/beta buoyancy, confinement, 200 SOR iterations (config 0 at 32^3)."""
+def test_step_closed_box_32():
+    """Closed shell, alpha/beta buoyancy, confinement, 200 SOR iterations, at 32^3."""
     W = H = D = 32
     p = oracle_params(W)
     ref, g, _, stats_g = run_both(W, H, D, p, 5, O.ALL_FACES, [[0.5, 0.1, 0.5, 0.08, 1.0, 1.0]])
