@@ -142,6 +142,18 @@ def test_cli_flux_solvers(tmp_path, resident, solver):
     assert want[2].max() > 0.1
 
 
+def test_cli_random_reinjection_takes_the_staged_path(tmp_path):
+    """randDensity scales each source cell's density by rand() % 256 / 255 (ZQ_SmokeSimulationUtils3D.cpp:3842-3882):
+    with deviceResident true the driver still runs it on the host grid, so the source holds
+    differing densities in [0, 0.8] rather than one value."""
+    text = BASE + "frameCount 1\nintObj sphere\nrandDensity true\nglobalGridSolver TOTAL_OPEN_POISSON_SOR\n" \
+        "deviceResident true\n"
+    got = run_cli(tmp_path, text, 1)[0]
+    src = got[got > 0]
+    assert src.size > 100
+    assert src.max() <= 0.8 + 1e-6 and len(np.unique(src)) > 20
+
+
 @pytest.mark.parametrize("resident", ["false", "true"])
 @pytest.mark.parametrize("solver", ["OPEN_OCTREE_POISSON_SOR", "CLOSED_OCTREE_POISSON_SOR"])
 def test_cli_octree_solvers(tmp_path, resident, solver):

@@ -100,7 +100,9 @@ bool ZQ_SmokeSimulation3D::_initRegular() {
     if (para.tempNoise != 0.0f)
         add_noise(fineGrid->GetTemperaturePtr(), (size_t)para.width * para.height * para.depth, para.noiseSeed,
                   para.tempNoise);
-    if (para.deviceResident && !para.useMovingObject) {
+    // moving-object files and randomised reinjection (std::rand per cell, in the reference's cell
+    // order) run on the host grid: those configs take the staged path
+    if (para.deviceResident && !para.useMovingObject && !para.randDensity) {
         zq3d_config c;
         zq3d_config_default(&c, para.width, para.height, para.depth);
         c.voxel_size = para.voxelSize;
