@@ -563,6 +563,47 @@ __global__ __launch_bounds__(BX* BY) void k_residual64(const uint8_t* __restrict
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
 }
 
+// k_residual64<true> marching z: each thread owns one (i, j) column of a chunk of RZ planes and keeps
+// the pressures of planes k-1, k, k+1 in registers, so each padded FP64 pressure is read once (the
+// per-plane kernel reads every plane for three neighbouring planes' blocks).  The same operations
+// per cell in the same order; the block maxima are over the chunk (a maximum is order-free).
+constexpr int RES_RZ = 16;
+__global__ __launch_bounds__(BX* BY) void k_residual64_z(const uint8_t* __restrict__ nc, const double* __restrict__ b64,
+                                                         const double* __restrict__ mean64,
+                                                         const double* __restrict__ p, float* __restrict__ r0,
+                                                         float* __restrict__ r1, double* __restrict__ part, Dims d,
+                                                         int Wh) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k0 = blockIdx.z * RES_RZ, k1 = min(k0 + RES_RZ, d.D);
+    double m = 0.0;
+    if (i < d.W && j < d.H) {
+        const int sy = d.W + 2, sz = (d.W + 2) * (d.H + 2);
+        const double mean = *mean64;
+        int q = pidx(d, i, j, k0);
+        double pm = p[q - sz], pc = p[q];
+        for (int k = k0; k < k1; k++, q += sz) {
+            const double pp = p[q + sz];
+            const int o = cidx(d, i, j, k);
+            const int n = nc[o];
+            float r = 0.0f;
+            if (n) {
+                const double sum = p[q - 1] + p[q + 1] + p[q - sy] + p[q + sy] + pm + pp;
+                const double bb = b64[o] - mean;
+                const double rr = bb - (sum - (double)n * pc);
+                r = (float)rr;
+                m = fmax(m, fabs(rr));
+            }
+            (((i + j + k) & 1) ? r1 : r0)[(k * d.H + j) * Wh + (i >> 1)] = r;
+            pm = pc;
+            pc = pp;
+        }
+    }
+    m = block_reduce<true>(m, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
+}
+
 // p64 += e, e = 0 on counted cells; SPLIT: e lives in the padded colour-split arrays e0/e1.
 template <bool SPLIT>
 __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict__ nc, float* __restrict__ e0,
@@ -862,9 +903,10 @@ void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, cons
 void launch_residual64_split(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
                              const double* p64_pad, const SplitBufs& sp, double* part, unsigned long long* rmax,
                              hipStream_t s) {
-    hipLaunchKernelGGL(k_residual64<true>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad,
-                       sp.b[0], sp.b[1], part, d, (d.W + 1) / 2);
-    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), rmax);
+    const dim3 g = grid_for(d.W, d.H, (d.D + RES_RZ - 1) / RES_RZ);
+    hipLaunchKernelGGL(k_residual64_z, g, dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad, sp.b[0], sp.b[1], part, d,
+                       (d.W + 1) / 2);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, (int)(g.x * g.y * g.z), rmax);
 }
 
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s) {
