@@ -198,3 +198,10 @@ def test_repeated_projections_are_bit_identical():
     leaves, _ = O.octree_projection(ru, rv, rw, occ, 25, (1, 2, 4))
     mu, mv, mw = exterior_masks(leaves)
     assert_same(outs[3][0], ru, "u vs oracle", mu)
+
+
+def test_large_level_lists_take_their_own_launches():
+    """Scattered solids make the level-0 lists tens of thousands of cells while the coarser levels stay
+    small: lists of very different lengths give the oracle's bits outside leaf interiors."""
+    leaves, _ = check_projection(64, 64, 48, 15, [(30.0, 30.0, 24.0)], (1, 2, 4), iters=12, sprinkle=0.01)
+    assert leaves[0].sum() > 2 * 8192 and 0 < leaves[1].sum() < 2 * 8192
