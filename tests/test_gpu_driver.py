@@ -231,6 +231,16 @@ def test_cli_moving_object(tmp_path):
         np.testing.assert_array_equal(got[f], want[f], err_msg="frame %d" % f)
 
 
+def test_cli_rejects_guiding_with_the_octree(tmp_path):
+    cfg = tmp_path / "scene.txt"
+    cfg.write_text(BASE + "useGuidingControl true\nguidingFold %s\nglobalGridSolver OPEN_OCTREE_POISSON_SOR\n"
+                   % tmp_path)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([CLI, str(cfg), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "octree" in r.stdout and "not built" in r.stdout
+
+
 def test_cli_rejects_unbuilt_solver(tmp_path):
     cfg = tmp_path / "scene.txt"
     cfg.write_text(BASE + "globalGridSolver OPEN_OCTREE_POISSON\n")
