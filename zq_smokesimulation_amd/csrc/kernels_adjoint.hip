@@ -365,12 +365,12 @@ __device__ __forceinline__ int trace_box(const Vel& V, const uint8_t* __restrict
     return n;
 }
 
-// inRegular_outRegular velocity advection adjoint, tiled (see k_adj_advect_cells).
+// inRegular_outRegular velocity advection adjoint, tiled (see k_adj_advect_cells).  The FP32 build
+// is held to 80 VGPRs (one spilled), which lets six waves per SIMD run instead of five.
 template <class R, int CAP>
-__global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_cells_tile(RegularVel<R> V, R* __restrict__ vb,
-                                                                      const uint8_t* __restrict__ occ,
-                                                                      const R* __restrict__ advb, R h, int steps,
-                                                                      R dtt) {
+__global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1))) void
+k_adj_advect_cells_tile(RegularVel<R> V, R* __restrict__ vb, const uint8_t* __restrict__ occ,
+                        const R* __restrict__ advb, R h, int steps, R dtt) {
     __shared__ R swin[3 * CAP];
     __shared__ int sbox[6];
     const Dims d = V.d;
@@ -789,11 +789,17 @@ __global__ __launch_bounds__(BX* BY) void k_adj_force_faces(R* __restrict__ face
 
 inline dim3 tile_grid(const Dims& d) { return dim3((d.W + TX - 1) / TX, (d.H + TY - 1) / TY, (d.D + TZ - 1) / TZ); }
 
-// LDS window capacity in cells: 40 KB for the five-array window
-template <class R>
+// LDS window capacity in cells of a tile kernel with NA window arrays: at most 2048 (FP32) / 1024
+// (FP64) cells, and few enough that four 256-thread blocks (four waves per SIMD) fit the CU's
+// 160 KiB of LDS with their 24-byte box.  For the five-array scalar window that is 2044 / 1022
+// cells: 2048 would need 24 bytes more than a quarter and leave only three blocks on a CU.
+template <class R, int NA>
 constexpr int tile_cap() {
-    return sizeof(R) == 4 ? 2048 : 1024;
+    return (40960 - 64) / (NA * (int)sizeof(R)) < 8192 / (int)sizeof(R) ? (40960 - 64) / (NA * (int)sizeof(R))
+                                                                          : 8192 / (int)sizeof(R);
 }
+static_assert(tile_cap<float, 5>() == 2044 && tile_cap<float, 3>() == 2048, "FP32 window capacities");
+static_assert(tile_cap<double, 5>() == 1022 && tile_cap<double, 3>() == 1024, "FP64 window capacities");
 
 }  // namespace
 
@@ -827,7 +833,7 @@ void launch_adj_advect_velocity(const Dims& d, const R* u, const R* v, const R* 
                            is_open);
         launch_fill_adj<R>(cellb, 3 * N, s);
         const RegularVel<R> V{vel4, d};
-        hipLaunchKernelGGL((k_adj_advect_cells_tile<R, tile_cap<R>()>), tile_grid(d), dim3(TX * TY * TZ), 0, s, V,
+        hipLaunchKernelGGL((k_adj_advect_cells_tile<R, tile_cap<R, 3>()>), tile_grid(d), dim3(TX * TY * TZ), 0, s, V,
                            cellb, occ, advb, h, ap.steps, dtt);
         hipLaunchKernelGGL((k_adj_vel_to_faces<R, 0, true>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, ub, (R*)nullptr,
                            cellb, occ, d, is_open);
@@ -891,7 +897,7 @@ void launch_adj_advect_scalar(const Dims& d, const R* u, const R* v, const R* w,
     }
     launch_fill_adj<R>(cellb, 3 * N, s);
     const RegularVel<R> V{vel4, d};
-    hipLaunchKernelGGL((k_adj_advect_scalar_tile<R, tile_cap<R>()>), tile_grid(d), dim3(TX * TY * TZ), 0, s, V, cellb,
+    hipLaunchKernelGGL((k_adj_advect_scalar_tile<R, tile_cap<R, 5>()>), tile_grid(d), dim3(TX * TY * TZ), 0, s, V, cellb,
                        occ, Tin, Rin, Tb, Rb, Tb_in, Rb_in, h, ap.steps, dtt);
     launch_add_cells_to_faces<R>(d, cellb, ub, vb, wb, s);
 }
