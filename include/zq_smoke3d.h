@@ -419,6 +419,13 @@ void zq3d_adj_destroy(zq3d_adj* a);
 int zq3d_adj_set_stream(zq3d_adj* a, void* stream);
 int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8, int n_sphere, const float* spheres6);
 int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol);
+/* on != 0: keep every forward step's final FP64 pressure (nsteps x (W+2)(H+2)(D+2) doubles,
+ * allocated on the first call) so that the backward's recomputed steps apply the forward's own
+ * pressure instead of solving their projections again: the same correction, so the same states and
+ * gradient, for O(nsteps) pressure memory beside the O(nsteps / checkpoint_every) checkpoints.  The
+ * store is filled by the next zq3d_adj_forward and used by the zq3d_adj_backward that follows it
+ * (whose controls are those of that forward); until then recomputed steps solve as without it. */
+int zq3d_adj_set_pressure_store(zq3d_adj* a, int on);
 /* sweep algorithm / tile shape of every projection of the rollout, one of zq3d_autotune_sor's
  * candidates for this grid (default (-1, 0)); all give the same bits */
 int zq3d_adj_set_sor_shape(zq3d_adj* a, int ty, int zc);

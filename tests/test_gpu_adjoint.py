@@ -216,6 +216,40 @@ def test_sweep_shape_does_not_change_rollout_or_gradient(dtype):
         np.testing.assert_allclose(g, out[0][1], rtol=0, atol=tol * gmax)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_pressure_store_gives_the_same_rollout_and_gradient(dtype):
+    """With the forward's pressures stored, the backward's recomputed steps apply them instead of
+    solving their projections again: no recomputed step runs a sweep, the forward is unchanged bit for
+    bit, and the gradient matches the solving backward's (to the atomic accumulation order of the
+    advection adjoint, as between sweep shapes).  A store switched on after the forward holds nothing
+    yet: that backward solves."""
+    W, H, D, nsteps, K = 40, 36, 28, 6, 2
+    out = []
+    for mode in ("solve", "store", "late"):
+        eng, _ = make_engine(W, H, D, nsteps, K, dtype, closed=True, obstacle=True)
+        if mode == "store":
+            eng.set_pressure_store(True)
+        gen = torch.Generator(device="cpu").manual_seed(11)
+        ctrl = (0.5 * torch.randn(eng.controls_shape(), generator=gen, dtype=dtype)).to(DEV)
+        target = torch.rand((D, H, W), generator=gen, dtype=dtype).to(DEV)
+        rho = eng.forward(ctrl)
+        if mode == "late":
+            eng.set_pressure_store(True)
+        g = eng.backward(ctrl, rho - target)
+        st = eng.last_backward
+        print(mode, eng.last_forward, st, flush=True)
+        assert st["recomputed_steps"] == (nsteps - 1) // K * K
+        assert (st["forward_sweeps"] == 0) == (mode == "store"), (mode, st)
+        out.append((rho.cpu().numpy(), g.cpu().numpy()))
+        eng.close()
+    gmax = np.abs(out[0][1]).max()
+    assert gmax > 0
+    tol = 1e-5 if dtype == torch.float32 else 1e-12
+    for rho, g in out[1:]:
+        np.testing.assert_array_equal(rho, out[0][0])
+        np.testing.assert_allclose(g, out[0][1], rtol=0, atol=tol * gmax)
+
+
 def test_fp32_rollout_is_the_product_step():
     import zq_smokesimulation_amd as z
     W, H, D, nsteps = 24, 28, 20, 5

@@ -272,6 +272,7 @@ def _bind_adjoint(L):
     L.zq3d_adj_set_stream.argtypes = [vp, vp]
     L.zq3d_adj_set_sources.argtypes = [vp, C.c_int, vp, C.c_int, vp]
     L.zq3d_adj_set_adjoint_tol.argtypes = [vp, C.c_double]
+    L.zq3d_adj_set_pressure_store.argtypes = [vp, C.c_int]
     L.zq3d_adj_set_sor_shape.argtypes = [vp, C.c_int, C.c_int]
     L.zq3d_adj_set_initial.argtypes = [vp, C.c_int, vp]
     L.zq3d_adj_forward.argtypes = [vp, vp, C.POINTER(AdjStats), vp]

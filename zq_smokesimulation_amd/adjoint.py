@@ -100,6 +100,11 @@ class SmokeRollout:
         """Adjoint projection tolerance relative to max|rhs|; 0 matches each step's forward projection."""
         check(_lib.lib().zq3d_adj_set_adjoint_tol(self._h, float(rtol)), "zq3d_adj_set_adjoint_tol")
 
+    def set_pressure_store(self, on=True):
+        """Keep each forward step's final FP64 pressure (nsteps padded grids) so that the backward's
+        recomputed steps apply it instead of solving again; same states and gradient."""
+        check(_lib.lib().zq3d_adj_set_pressure_store(self._h, int(bool(on))), "zq3d_adj_set_pressure_store")
+
     def set_sor_shape(self, ty, zc):
         """Sweep algorithm of every projection (a SmokeSim3D.autotune_sor candidate); same bits."""
         check(_lib.lib().zq3d_adj_set_sor_shape(self._h, int(ty), int(zc)), "zq3d_adj_set_sor_shape")

@@ -106,6 +106,10 @@ struct zq3d_adj {
     // checkpoints: ncp x (u, v, w, T, rho values + p64)
     char* cp;
     size_t cp_bytes;
+    // optional store of every forward step's final FP64 pressure (nsteps x Np): a recomputed step
+    // then applies the forward's own pressure instead of solving again (same correction, same bits)
+    double* pstore = nullptr;
+    bool pstore_on = false, pstore_valid = false;
     // tape of one segment: K x (us, uf, up faces; Tf, Rf cells)
     char* tape;
     size_t tape_bytes;
@@ -195,8 +199,12 @@ hipError_t checkpoint(zq3d_adj* a, int c, bool save) {
 
 AdvParams adv_params(const zq3d_config& c) { return AdvParams{c.voxel_size, c.step_time / c.steps, c.steps}; }
 
+// forward: the rollout's forward (stores the pressure when the store is on); solve: a recomputed
+// step solved again; replay: a recomputed step that applies the stored pressure
+enum class StepMode { forward, solve, replay };
+
 template <class R>
-int forward_step(zq3d_adj* a, int s, const R* controls, Slot<R>* sl, zq3d_adj_stats* st) {
+int forward_step(zq3d_adj* a, int s, const R* controls, Slot<R>* sl, zq3d_adj_stats* st, StepMode mode) {
     const char* where = "zq3d_adj_forward";
     const zq3d_config& c = a->cfg;
     const Dims& d = a->d;
@@ -216,10 +224,19 @@ int forward_step(zq3d_adj* a, int s, const R* controls, Slot<R>* sl, zq3d_adj_st
     launch_advect_velocity<R>(d, u, v, w, a->occ, 1, c.adv_vel_type, ap, (vec4_t<R>*)a->vel4, (R*)a->scratch,
                               a->stream);
     zq3d_proj_stats ps{};
-    double bmax = 0.0;
-    if (int rc = project_mp<R>(d, u, v, w, a->occ, c.sor, proj_bufs(a, a->p64), false, 0.0, &ps, a->stream, &bmax))
-        return rc;
-    a->fwd_rel[s] = bmax > 0.0 ? c.sor.tol / bmax : 1.0;
+    if (mode == StepMode::replay) {
+        // the correction the forward applied, from the forward's own final pressure of this step
+        launch_correct64<R>(d, a->occ, c.sor.open_mask, a->pstore + (size_t)s * z.Np, u, v, w, a->stream);
+    } else {
+        double bmax = 0.0;
+        if (int rc = project_mp<R>(d, u, v, w, a->occ, c.sor, proj_bufs(a, a->p64), false, 0.0, &ps, a->stream,
+                                   &bmax))
+            return rc;
+        a->fwd_rel[s] = bmax > 0.0 ? c.sor.tol / bmax : 1.0;
+        if (mode == StepMode::forward && a->pstore_on)
+            ZQ_HIP(hipMemcpyAsync(a->pstore + (size_t)s * z.Np, a->p64, z.Np * 8, hipMemcpyDeviceToDevice, a->stream),
+                   where);
+    }
     if (sl) ZQ_HIP(copy_faces<R>(a, sl->up, sl->vp, sl->wp, u, v, w), where);
     launch_advect_scalar<R>(d, u, v, w, a->occ, T, Rr, (R*)a->T2, (R*)a->R2, c.adv_sca_type == 0 ? 1 : 0, ap,
                             (vec4_t<R>*)a->vel4, a->stream);
@@ -289,15 +306,17 @@ int run_forward(zq3d_adj* a, const R* controls, R* rho_out, zq3d_adj_stats* st) 
     ZQ_HIP(hipMemcpyAsync(a->R, a->R0, z.N * sizeof(R), hipMemcpyDeviceToDevice, a->stream), where);
     ZQ_HIP(hipMemcpyAsync(a->p64, a->p640, z.Np * 8, hipMemcpyDeviceToDevice, a->stream), where);
     const int last_seg = a->ncp - 1;
+    a->pstore_valid = false;
     for (int s = 0; s < a->nsteps; s++) {
         if (s % a->K == 0) ZQ_HIP(checkpoint<R>(a, s / a->K, true), where);
         // the last segment's tape is recorded now, so the backward need not recompute it
         Slot<R> sl;
         const bool rec = s / a->K == last_seg;
         if (rec) sl = tape_slot<R>(a, s - last_seg * a->K);
-        if (int rc = forward_step<R>(a, s, controls, rec ? &sl : nullptr, st)) return rc;
+        if (int rc = forward_step<R>(a, s, controls, rec ? &sl : nullptr, st, StepMode::forward)) return rc;
     }
     a->tape_seg = last_seg;
+    a->pstore_valid = a->pstore_on;
     if (rho_out) ZQ_HIP(hipMemcpyAsync(rho_out, a->R, z.N * sizeof(R), hipMemcpyDeviceToDevice, a->stream), where);
     return ZQ3D_OK;
 }
@@ -316,10 +335,11 @@ int run_backward(zq3d_adj* a, const R* controls, const R* rho_bar, R* controls_b
         const int s0 = seg * a->K, s1 = std::min(a->nsteps, s0 + a->K);
         if (a->tape_seg != seg) {
             ZQ_HIP(checkpoint<R>(a, seg, false), where);
+            const StepMode mode = a->pstore_on && a->pstore_valid ? StepMode::replay : StepMode::solve;
             for (int s = s0; s < s1; s++) {
                 Slot<R> sl = tape_slot<R>(a, s - s0);
                 zq3d_adj_stats tmp{};
-                if (int rc = forward_step<R>(a, s, controls, &sl, &tmp)) return rc;
+                if (int rc = forward_step<R>(a, s, controls, &sl, &tmp, mode)) return rc;
                 if (st) {
                     st->forward_sweeps += tmp.forward_sweeps;
                     st->recomputed_steps++;
@@ -454,6 +474,7 @@ extern "C" int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8,
         return ZQ3D_ERR_ARG;
     }
     a->boxes.assign(boxes8, boxes8 + 8 * n_box);
+    a->pstore_valid = false;
     a->spheres.assign(sph6, sph6 + 6 * n_sphere);
     a->boxes0 = a->boxes;
     a->spheres0 = a->spheres;
@@ -465,6 +486,21 @@ extern "C" int zq3d_adj_set_sources(zq3d_adj* a, int n_box, const float* boxes8,
 extern "C" int zq3d_adj_set_adjoint_tol(zq3d_adj* a, double rtol) {
     if (!a || !(rtol >= 0.0)) { set_err("zq3d_adj_set_adjoint_tol", "bad argument"); return ZQ3D_ERR_ARG; }
     a->rtol = rtol;
+    return ZQ3D_OK;
+}
+
+extern "C" int zq3d_adj_set_pressure_store(zq3d_adj* a, int on) {
+    const char* where = "zq3d_adj_set_pressure_store";
+    if (!a) { set_err(where, "null handle"); return ZQ3D_ERR_ARG; }
+    if (on && !a->pstore) {
+        const Sizes z(a->d);
+        if (adj_alloc(a, (void**)&a->pstore, (size_t)a->nsteps * z.Np * 8)) {
+            set_err(where, "device allocation failed (nsteps x the padded FP64 pressure)");
+            return ZQ3D_ERR_HIP;
+        }
+    }
+    a->pstore_on = on != 0;
+    a->pstore_valid = false;  // filled by the next forward
     return ZQ3D_OK;
 }
 
@@ -507,6 +543,7 @@ extern "C" int zq3d_adj_set_initial(zq3d_adj* a, int field, const void* dev) {
     if (field_ptr(a, field, true, false, &p, &n)) { set_err(where, "unknown field"); return ZQ3D_ERR_ARG; }
     ZQ_HIP(hipMemcpyAsync(p, dev, n, hipMemcpyDeviceToDevice, a->stream), where);
     a->forward_done = a->backward_done = false;
+    a->pstore_valid = false;
     a->tape_seg = -1;
     return ZQ3D_OK;
 }
