@@ -58,65 +58,71 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        i, f, d, u32 = C.c_int, C.c_float, C.c_double, C.c_uint32
-        vp = C.c_void_p
-        L.orc_mac_to_cell4.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p]
-        L.orc_cell4_to_mac.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p]
-        L.orc_sample3.argtypes = [_f32p, i, i, i, f, f, f]
-        L.orc_sample3.restype = f
-        L.orc_add_force.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, f, f, f, f, f, f,
-                                    vp, vp, vp, vp]
-        L.orc_attenuation.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, f, f, f]
-        L.orc_advect_velocity.argtypes = [i, i, i, f, i, f, _f32p, _f32p, _f32p, _u8p, i, i]
-        L.orc_advect_velocity.restype = i
-        L.orc_advect_scalar.argtypes = [i, i, i, f, i, f, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p, _f32p, i]
-        L.orc_advect_scalar.restype = i
-        L.orc_backtrace.argtypes = [i, i, i, f, i, f, _f32p, _f32p, _f32p, _u8p, i, f, f, f, _f32p]
-        L.orc_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p]
-        L.orc_sor_count.argtypes = [i, i, i, _u8p, i, _u8p]
-        L.orc_sor.argtypes = [i, i, i, _u8p, _f32p, _f32p, f, i]
-        L.orc_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, f, f, i, vp]
-        L.orc_project_mp.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, f, i, i, d, _f64p, _f64p]
-        L.orc_project_mp.restype = i
-        L.orc_project_cg64.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, d, i, vp]
-        L.orc_project_cg64.restype = i
-        L.orc_max_abs_div.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, d]
-        L.orc_max_abs_div.restype = d
-        L.orc_closed_mean_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, vp]
-        L.orc_closed_mean_divergence.restype = f
-        L.orc_mean_div32_f64.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p]
-        L.orc_mean_div32_f64.restype = f
-        L.orc_init_fine_grid.argtypes = [i, i, i, i, i, _u8p, _f32p, _f32p, _f32p]
-        L.orc_reinject_box.argtypes = [i, i, i, _f32p, _f32p, _f32p]
-        L.orc_reinject_sphere.argtypes = [i, i, i, f, f, f, f, f, f, _f32p, _f32p]
-        L.orc_stamp_sphere.argtypes = [i, i, i, f, f, f, f, f, f, f, i, _u8p, _f32p, _f32p, _f32p]
-        L.orc_noise.argtypes = [i, i, i, u32, f, _f32p]
-        L.orc_num_threads.argtypes = []
-        L.orc_num_threads.restype = i
-        L.orc_step.argtypes = [C.POINTER(StepParams), _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, vp, vp, vp]
-        L.orc_step.restype = i
-        L.orc_coarse_setup.argtypes = [i, i, i, i, _u8p, _f32p, _f32p, _f32p, _u8p, _u8p]
-        L.orc_coarse_inv_diag.argtypes = [i, i, i, i, i, i, _f32p, _f32p, _f32p, i, i]
-        L.orc_coarse_inv_diag.restype = f
-        L.orc_fine_to_coarse.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
-        L.orc_coarse_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p, i, f, i,
-                                         _f32p, _f32p, _f32p, _f32p, vp, i]
-        L.orc_coarse_to_fine.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p]
-        L.orc_sub_project_regular.argtypes = [i, i, i, i, _f32p, _u8p, _f32p, _f32p, _f32p]
-        L.orc_oct_distance.argtypes = [i, i, i, _u8p, _i32p]
-        L.orc_oct_leaves.argtypes = [i, i, i, _u8p, _i32p, i, i, i, _u8p, _u8p, _u8p, _u8p]
-        L.orc_oct_neighbors.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, i, i, i, i, _i32p]
-        L.orc_oct_neighbors.restype = i
-        L.orc_oct_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
-        L.orc_oct_closed_mean.argtypes = [i, i, i, _u8p, _f32p, _f32p, _f32p]
-        L.orc_oct_closed_mean.restype = f
-        L.orc_oct_solve.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, i, i, f,
-                                    _f32p, _f32p, _f32p, _f32p]
-        L.orc_oct_adjust.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
-                                     _f32p]
-        _lib = L
+        _lib = load(_LIB_PATH)
     return _lib
+
+
+def load(path):
+    """Open an oracle build at `path` and declare its signatures (the tests also load builds of
+    deliberately altered sources this way, to show that their pins reject them)."""
+    L = C.CDLL(path)
+    i, f, d, u32 = C.c_int, C.c_float, C.c_double, C.c_uint32
+    vp = C.c_void_p
+    L.orc_mac_to_cell4.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p]
+    L.orc_cell4_to_mac.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p]
+    L.orc_sample3.argtypes = [_f32p, i, i, i, f, f, f]
+    L.orc_sample3.restype = f
+    L.orc_add_force.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, f, f, f, f, f, f,
+                                vp, vp, vp, vp]
+    L.orc_attenuation.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, f, f, f]
+    L.orc_advect_velocity.argtypes = [i, i, i, f, i, f, _f32p, _f32p, _f32p, _u8p, i, i]
+    L.orc_advect_velocity.restype = i
+    L.orc_advect_scalar.argtypes = [i, i, i, f, i, f, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p, _f32p, i]
+    L.orc_advect_scalar.restype = i
+    L.orc_backtrace.argtypes = [i, i, i, f, i, f, _f32p, _f32p, _f32p, _u8p, i, f, f, f, _f32p]
+    L.orc_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p]
+    L.orc_sor_count.argtypes = [i, i, i, _u8p, i, _u8p]
+    L.orc_sor.argtypes = [i, i, i, _u8p, _f32p, _f32p, f, i]
+    L.orc_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, f, f, i, vp]
+    L.orc_project_mp.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, f, i, i, d, _f64p, _f64p]
+    L.orc_project_mp.restype = i
+    L.orc_project_cg64.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, d, i, vp]
+    L.orc_project_cg64.restype = i
+    L.orc_max_abs_div.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, d]
+    L.orc_max_abs_div.restype = d
+    L.orc_closed_mean_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, vp]
+    L.orc_closed_mean_divergence.restype = f
+    L.orc_mean_div32_f64.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p]
+    L.orc_mean_div32_f64.restype = f
+    L.orc_init_fine_grid.argtypes = [i, i, i, i, i, _u8p, _f32p, _f32p, _f32p]
+    L.orc_reinject_box.argtypes = [i, i, i, _f32p, _f32p, _f32p]
+    L.orc_reinject_sphere.argtypes = [i, i, i, f, f, f, f, f, f, _f32p, _f32p]
+    L.orc_stamp_sphere.argtypes = [i, i, i, f, f, f, f, f, f, f, i, _u8p, _f32p, _f32p, _f32p]
+    L.orc_noise.argtypes = [i, i, i, u32, f, _f32p]
+    L.orc_num_threads.argtypes = []
+    L.orc_num_threads.restype = i
+    L.orc_step.argtypes = [C.POINTER(StepParams), _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, vp, vp, vp]
+    L.orc_step.restype = i
+    L.orc_coarse_setup.argtypes = [i, i, i, i, _u8p, _f32p, _f32p, _f32p, _u8p, _u8p]
+    L.orc_coarse_inv_diag.argtypes = [i, i, i, i, i, i, _f32p, _f32p, _f32p, i, i]
+    L.orc_coarse_inv_diag.restype = f
+    L.orc_fine_to_coarse.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_coarse_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p, i, f, i,
+                                     _f32p, _f32p, _f32p, _f32p, vp, i]
+    L.orc_coarse_to_fine.argtypes = [i, i, i, i, _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, _f32p]
+    L.orc_sub_project_regular.argtypes = [i, i, i, i, _f32p, _u8p, _f32p, _f32p, _f32p]
+    L.orc_oct_distance.argtypes = [i, i, i, _u8p, _i32p]
+    L.orc_oct_leaves.argtypes = [i, i, i, _u8p, _i32p, i, i, i, _u8p, _u8p, _u8p, _u8p]
+    L.orc_oct_neighbors.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, i, i, i, i, _i32p]
+    L.orc_oct_neighbors.restype = i
+    L.orc_oct_divergence.argtypes = [i, i, i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_oct_closed_mean.argtypes = [i, i, i, _u8p, _f32p, _f32p, _f32p]
+    L.orc_oct_closed_mean.restype = f
+    L.orc_oct_solve.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, i, i, f,
+                                _f32p, _f32p, _f32p, _f32p]
+    L.orc_oct_adjust.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                 _f32p]
+    return L
 
 
 def _dims(occ):

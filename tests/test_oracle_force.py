@@ -2,6 +2,7 @@
 with closed forms and invariants: buoyancy only at rest, the curl of known fields, confinement
 perpendicular to the vorticity, unit gradient directions, and the face masks."""
 import numpy as np
+import pytest
 
 import zq_oracle as O
 from util import cells, faces
@@ -137,3 +138,87 @@ def test_attenuation_closed_form(rng):
     O.attenuation(u, v, w, T, R, occ, 0.9, 0.8, 0.7)
     for x, x0, f in zip((u, v, w, T, R), a, (0.9, 0.9, 0.9, 0.8, 0.7)):
         assert np.array_equal(x, x0 * np.float32(f))
+
+
+def _quadratic_field(W, H, D, comp, axis, a):
+    """Velocity component `comp` (0 u, 1 v, 2 w) = a * (c + 1/2)^2 on every face, c the cell index along
+    `axis` (0 x, 1 y, 2 z; axis != comp), so the averaged cell velocity is exactly a*(c + 1/2)^2."""
+    u, v, w = O.zeros_faces(W, H, D)
+    f = (u, v, w)[comp]
+    c = np.arange(f.shape[2 - axis], dtype=np.float64) + 0.5
+    shape = [1, 1, 1]
+    shape[2 - axis] = -1
+    f[...] = (a * c * c).reshape(shape).astype(np.float32)
+    return u, v, w
+
+
+_CASES = [(comp, axis) for comp in range(3) for axis in range(3) if comp != axis]
+
+
+@pytest.mark.parametrize("comp,axis", _CASES)
+def test_confinement_closed_form(comp, axis):
+    """V = a (x_A + 1/2)^2 e_B (A = axis, B = comp) has curl w = a (2 x_A + 1)/h * eps_ABC e_C on interior
+    cells (central differences over 2h are exact on a quadratic), so |w| grows along e_A and
+    N = grad|w| / |grad|w|| = e_A two or more cells from the border.  Then f = eps h N x w
+    = -eps a (2 x_A + 1) e_B: the confinement force opposes the velocity component.  The order
+    w x N, or a central difference of the wrong sign, flips f.  On the border the one-sided
+    differences (ZQ_CUDA_AddForce3D.cu:64-111) give N = +e_A at x_A = 0 and -e_A at the far end;
+    one cell in, at x_A = 1 and along the other axes, N mixes e_A with the border's drop to |w| = 0
+    and f = eps h (N x w) takes both of its terms (:161-184)."""
+    W, H, D = 11, 10, 9
+    dims = (W, H, D)
+    a, eps = 0.25, 0.75
+    h = 1.0 / 16.0
+    u, v, w = _quadratic_field(W, H, D, comp, axis, a)
+    occ = np.zeros((D, H, W), np.uint8)
+    T = np.zeros((D, H, W), np.float32)
+    vort, gradn, force = O.add_force(u, v, w, occ, T, T.copy(), h, 0.01, 0.0, eps, 0.0, debug=True)
+    C = 3 - comp - axis
+    lc = np.zeros((3, 3, 3))
+    lc[0, 1, 2] = lc[1, 2, 0] = lc[2, 0, 1] = 1
+    lc[0, 2, 1] = lc[2, 1, 0] = lc[1, 0, 2] = -1
+    sgn = lc[axis, comp, C]
+
+    def idx(fixed_axis_index, lo=2):
+        """Index (k, j, i) tuple: `axis` coordinate as given, the other two in [lo, dim-1-lo]."""
+        sl = [slice(lo, dims[ax] - lo) for ax in range(3)]
+        sl[axis] = fixed_axis_index
+        return tuple(sl[::-1])
+
+    for c in range(1, dims[axis] - 1):
+        wmag = a * (2 * c + 1) / h
+        got = vort[idx(c, 1)]
+        np.testing.assert_allclose(got[..., C], sgn * wmag, rtol=1e-6)
+        assert np.all(got[..., comp] == 0) and np.all(got[..., axis] == 0)
+        np.testing.assert_allclose(got[..., 3], wmag, rtol=1e-6)
+    # deep interior: N = e_A, f = -eps a (2c+1) e_B; at c = dims - 2 the next cell is the border
+    # (|w| = 0), so the central difference turns negative: N = -e_A, f = +eps a (2c+1) e_B
+    for c in range(1, dims[axis] - 1):
+        s = 1.0 if c < dims[axis] - 2 else -1.0
+        n = gradn[idx(c)]
+        f = force[idx(c)]
+        np.testing.assert_allclose(n[..., axis], s, rtol=1e-6)
+        assert np.all(n[..., comp] == 0) and np.all(n[..., C] == 0)
+        np.testing.assert_allclose(f[..., comp], -s * eps * a * (2 * c + 1), rtol=1e-5)
+        assert np.all(f[..., axis] == 0) and np.all(f[..., C] == 0)
+    # one-sided ends along A: |w| = 0 on the border cells, so N = +e_A at c = 0, -e_A at the far end
+    np.testing.assert_allclose(gradn[idx(0)][..., axis], 1.0, rtol=1e-6)
+    np.testing.assert_allclose(gradn[idx(dims[axis] - 1)][..., axis], -1.0, rtol=1e-6)
+    assert np.all(force[idx(0)] == 0) and np.all(force[idx(dims[axis] - 1)] == 0)
+    # one cell in from a border across B (the velocity's own axis): |w| drops to 0 on the border
+    # plane, so the central difference along B is +/-|w|/2 while along A it is 2a/h
+    for side, cb in ((+1, 1), (-1, dims[comp] - 2)):
+        for c in range(2, dims[axis] - 2):
+            sl = [slice(2, dims[ax] - 2) for ax in range(3)]
+            sl[axis] = c
+            sl[comp] = cb
+            key = tuple(sl[::-1])
+            wmag = a * (2 * c + 1) / h
+            gA, gB = 2 * a / h, side * 0.5 * wmag
+            nA, nB = gA / np.hypot(gA, gB), gB / np.hypot(gA, gB)
+            np.testing.assert_allclose(gradn[key][..., axis], nA, rtol=1e-5)
+            np.testing.assert_allclose(gradn[key][..., comp], nB, rtol=1e-5)
+            # f = eps h (N x w e_C * sgn): (e_A x e_C) = -sgn e_B, (e_B x e_C) = sgn e_A
+            wC = sgn * wmag
+            np.testing.assert_allclose(force[key][..., comp], eps * h * nA * wC * (-sgn), rtol=1e-5)
+            np.testing.assert_allclose(force[key][..., axis], eps * h * nB * wC * sgn, rtol=1e-5)
