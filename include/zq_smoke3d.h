@@ -172,8 +172,9 @@ typedef struct zq3d_sor_opts {
     float omega;          /* SOR relaxation, 1 = Gauss-Seidel */
     int iters;            /* plain: iterations; mixed: FP32 sweeps per refinement step */
     int mixed_precision;  /* 0: FP32 SOR from p = 0.  1: FP64 residual + pressure, FP32 sweeps */
-    int max_outer;        /* mixed: refinement steps at most */
-    double tol;           /* mixed: stop when max|r| <= tol */
+    int max_outer;        /* mixed: refinement steps at most (>= 1) */
+    double tol;           /* mixed: stop once max|r| <= tol and the corrected faces (each rounded to FP32
+                             once) have max|div - mean| <= tol */
 } zq3d_sor_opts;
 
 typedef struct zq3d_proj_stats {
@@ -182,6 +183,8 @@ typedef struct zq3d_proj_stats {
     double max_residual;  /* mixed: final max|r| (FP64) */
     double max_div;       /* max |div - mean| over counted cells after the correction, FP64 */
     double mean;          /* subtracted mean divergence */
+    int converged;        /* mixed: 1 when max_div <= tol, 0 when max_outer passes ended first;
+                             -1 for the fixed-count solvers (no tolerance) */
 } zq3d_proj_stats;
 
 /* No reference counterpart (extends the two calls above): host pointers, omega, per-face open
@@ -189,6 +192,7 @@ typedef struct zq3d_proj_stats {
 float ZQ3D_SolvePoissonRedBlack_ex(float* mac_u, float* mac_v, float* mac_w, const bool* occupy, const int width,
                                    const int height, const int depth, const zq3d_sor_opts* opts,
                                    zq3d_proj_stats* stats);
+
 
 /* No reference counterpart (extends AddForce3D): also applies the density term
  * f_y -= gravityCoeff*rho (the reference's gravityCoeff, declared unused there) and adds an
@@ -298,8 +302,10 @@ int zq3d_step(zq3d_sim* sim, const float* extra_force_dev, zq3d_proj_stats* stat
 int zq3d_mac_to_regular(zq3d_sim* sim);
 
 /* Stage timing with HIP events on the simulation's stream (the reference prints per-stage
- * QueryPerformanceCounter / cudaEvent times, ZQ_SmokeSimulation3D.cpp:158-226).  While enabled,
- * each stage records an event pair and waits for it, so the numbers are exact kernel time. */
+ * QueryPerformanceCounter / cudaEvent times, ZQ_SmokeSimulation3D.cpp:158-226).  on == 1: each stage
+ * records an event pair and waits for it, so the stage numbers are exact kernel time, and every
+ * red-black sweep loop is timed as well.  on == 2: only the sweep loops are timed, by event pairs
+ * that nothing waits for until zq3d_get_timing, so the timing adds no synchronisation to a step. */
 typedef struct zq3d_timing {
     double ms_force;            /* force + attenuation */
     double ms_advect_velocity;
@@ -310,7 +316,7 @@ typedef struct zq3d_timing {
     long long sor_launches;     /* single-colour sweep kernels timed in ms_sor */
     long long steps;
 } zq3d_timing;
-/* on != 0 enables and resets the counters */
+/* on = 0 (off), 1 (stages and sweeps) or 2 (sweeps only); enables and resets the counters */
 int zq3d_set_profiling(zq3d_sim* sim, int on);
 int zq3d_get_timing(zq3d_sim* sim, zq3d_timing* out);
 
@@ -388,6 +394,12 @@ int zq3d_scene_init_occupancy(const char* path, uint8_t* occ, size_t bytes);
 /* ZQ_ApplyMovingObject3D applied after `frames` object updates of the config's moving-object file,
  * into caller arrays (occ W*H*D bytes, faces in the reference layout, updated in place) */
 int zq3d_scene_apply_moving_objects(const char* path, int frames, uint8_t* occ, float* u, float* v, float* w);
+/* reads a .di3 density file (the layout ZQ_SmokeGrid3D::ExportDensity writes, ZQ_SmokeGrid3D.cpp:55-81:
+ * 16-byte NUL-terminated type tag "float", int32 width, height, depth, channels, a flag byte, FP32 data
+ * x fastest): dims4 receives width, height, depth, channels; data (capacity floats) receives the
+ * samples, or with data == NULL only the header is read and the file's length is checked against it.
+ * 0 on success; nonzero for a missing, malformed or short file or too small a buffer. */
+int zq3d_di3_read(const char* path, int* dims4, float* data, size_t capacity);
 
 /* ====================== Part 5: adjoint rollout (gradient-based control) ====================== */
 
@@ -439,6 +451,22 @@ int zq3d_adj_forward(zq3d_adj* a, const void* controls, zq3d_adj_stats* st, void
  * be those of the preceding forward */
 int zq3d_adj_backward(zq3d_adj* a, const void* controls, const void* rho_bar, void* controls_bar,
                       zq3d_adj_stats* st);
+/* The final density (rho_out, W*H*D) for `controls` without disturbing the latest forward: the
+ * rollout restarts from that forward's checkpoint at or before from_step and runs to the end,
+ * saving no checkpoint, tape or pressure.  The controls of the steps before that checkpoint must be
+ * the forward's own (its saved states embody them).  For finite-difference checks and line
+ * searches: a perturbation of the controls of step s needs only from_step = s. */
+int zq3d_adj_evaluate(zq3d_adj* a, int from_step, const void* controls, zq3d_adj_stats* st, void* rho_out);
+/* number of zq3d_adj_forward calls so far (-1 for a NULL rollout).  A backward always differentiates
+ * the latest forward (its checkpoints, tape and pressure store): a caller holding an earlier forward's
+ * result compares the count it saw with the current one before asking for that result's gradient. */
+long long zq3d_adj_generation(zq3d_adj* a);
+/* the recorded states of step `step` (absolute index) of the segment the rollout holds on tape (the
+ * last segment after a forward; after a backward, the first): ZQ3D_FIELD_U/V/W the projected faces of
+ * that step, ZQ3D_FIELD_T/RHO the temperature and density after its force stage; device memory of
+ * the chosen precision, sizes as for zq3d_adj_set_initial.  For checking a backward's recomputation
+ * against the forward. */
+int zq3d_adj_get_tape(zq3d_adj* a, int step, int field, void* dev);
 /* after a forward: the final state field; after a backward: with grad != 0, dL/d(initial field) */
 int zq3d_adj_get(zq3d_adj* a, int field, int grad, void* dev);
 

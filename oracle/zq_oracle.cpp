@@ -768,9 +768,28 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
             }
     std::vector<float> r32(N), e(d.Npad());
     long long sweeps = 0;
-    double rmax = residual64(d, nc.data(), b.data(), p64, r32.data());
     int outer = 0;
-    while (rmax > tol && outer < max_outer) {
+    double rmax = 0.0;
+    /* The residual is the divergence of the faces corrected in FP64; the stored faces are rounded to
+     * FP32 once more.  So a residual within tol ends the passes only when the faces the correction
+     * will store also have max|div - mean| <= tol (evaluated on a corrected copy). */
+    std::vector<float> cu(u, u + d.Nu()), cv(v, v + d.Nv()), cw(w, w + d.Nw());
+    double dprev = -1.0;
+    for (;;) {
+        rmax = residual64(d, nc.data(), b.data(), p64, r32.data());
+        if (!(rmax > tol)) {
+            std::copy(u, u + d.Nu(), cu.begin());
+            std::copy(v, v + d.Nv(), cv.begin());
+            std::copy(w, w + d.Nw(), cw.begin());
+            correct_faces<double>(d, occ, open_mask, p64, cu.data(), cv.data(), cw.data());
+            const double dmax = orc_max_abs_div(W, H, D, cu.data(), cv.data(), cw.data(), occ, open_mask, mean);
+            if (!(dmax > tol)) break;
+            /* a pass that no longer shrinks it below 0.9 of the previous one meets the faces' FP32
+             * rounding floor: the bound is out of reach, stop */
+            if (dprev >= 0.0 && !(dmax < 0.9 * dprev)) break;
+            dprev = dmax;
+        }
+        if (outer >= max_outer) break;
         std::fill(e.begin(), e.end(), 0.0f);
         orc_sor(W, H, D, nc.data(), r32.data(), e.data(), omega, inner_iters);
         sweeps += inner_iters;
@@ -781,7 +800,6 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
                     p64[q] += (double)e[q];
                 }
         outer++;
-        rmax = residual64(d, nc.data(), b.data(), p64, r32.data());
     }
     correct_faces<double>(d, occ, open_mask, p64, u, v, w);
     if (stats) {
@@ -803,6 +821,7 @@ int orc_project_cg64(int W, int H, int D, float* u, float* v, float* w, const ui
     const size_t sx = 1, sy = (size_t)(W + 2), sz = (size_t)(W + 2) * (H + 2);
     /* A = -L (SPD on counted cells): (A x)_c = n x_c - sum_nb x_nb ; solve A p = -b */
     auto applyA = [&](const std::vector<double>& x, std::vector<double>& y) {
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < D; k++)
             for (int j = 0; j < H; j++)
                 for (int i = 0; i < W; i++) {
@@ -822,24 +841,33 @@ int orc_project_cg64(int W, int H, int D, float* u, float* v, float* w, const ui
                 r[q] = bb; bnorm = std::max(bnorm, std::fabs(bb));
             }
     pdir = r;
+    const long long NPl = (long long)NP;
     double rr = 0.0;
-    for (size_t q = 0; q < NP; q++) rr += r[q] * r[q];
+#pragma omp parallel for schedule(static) reduction(+ : rr)
+    for (long long q = 0; q < NPl; q++) rr += r[q] * r[q];
     int it = 0;
     for (; it < max_iter; it++) {
         double rmax = 0.0;
-        for (size_t q = 0; q < NP; q++) rmax = std::max(rmax, std::fabs(r[q]));
+#pragma omp parallel for schedule(static) reduction(max : rmax)
+        for (long long q = 0; q < NPl; q++) rmax = std::max(rmax, std::fabs(r[q]));
         if (rmax <= tol) break;
         applyA(pdir, Ap);
         double pAp = 0.0;
-        for (size_t q = 0; q < NP; q++) pAp += pdir[q] * Ap[q];
+#pragma omp parallel for schedule(static) reduction(+ : pAp)
+        for (long long q = 0; q < NPl; q++) pAp += pdir[q] * Ap[q];
         if (pAp <= 0.0) break;
         double alpha = rr / pAp;
-        for (size_t q = 0; q < NP; q++) { x[q] += alpha * pdir[q]; r[q] -= alpha * Ap[q]; }
         double rr2 = 0.0;
-        for (size_t q = 0; q < NP; q++) rr2 += r[q] * r[q];
+#pragma omp parallel for schedule(static) reduction(+ : rr2)
+        for (long long q = 0; q < NPl; q++) {
+            x[q] += alpha * pdir[q];
+            r[q] -= alpha * Ap[q];
+            rr2 += r[q] * r[q];
+        }
         double beta = rr2 / rr;
         rr = rr2;
-        for (size_t q = 0; q < NP; q++) pdir[q] = r[q] + beta * pdir[q];
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < NPl; q++) pdir[q] = r[q] + beta * pdir[q];
     }
     correct_faces<double>(d, occ, open_mask, x.data(), u, v, w);
     if (p64) std::memcpy(p64, x.data(), sizeof(double) * NP);

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_zq3d.so")
+# ZQ3D_ORACLE_LIB may name another build of the same sources (the sanitizer build)
+_LIB_PATH = os.environ.get("ZQ3D_ORACLE_LIB") or os.path.join(_HERE, "liboracle_zq3d.so")
 
 ADV_VEL_INREG_OUTREG, ADV_VEL_INREG_OUTREG_BFECC, ADV_VEL_INMAC_OUTMAC, \
     ADV_VEL_INMAC_OUTMAC_BFECC, ADV_VEL_INREG_OUTMAC, ADV_VEL_INREG_OUTMAC_BFECC = range(6)

@@ -102,3 +102,18 @@ def test_bad_arguments_fail_without_gpu():
 def test_oracle_loads():
     import zq_oracle
     assert zq_oracle.lib() is not None
+
+
+def test_mixed_precision_needs_refinement_passes_without_gpu():
+    """The default config allows refinement passes (maxOuter's default, 1000); a mixed-precision config
+    with max_outer = 0, which would never project, is refused before any device call."""
+    import zq_smokesimulation_amd._lib as L
+    cfg = L.Config()
+    L.lib().zq3d_config_default(C.byref(cfg), 8, 8, 8)
+    assert cfg.sor.max_outer == 1000
+    cfg.sor.mixed_precision = 1
+    cfg.sor.max_outer = 0
+    h = C.c_void_p()
+    assert L.lib().zq3d_create(C.byref(cfg), C.byref(h)) == L.ERR_ARG
+    assert "max_outer" in L.last_error()
+    assert L.lib().zq3d_adj_create(C.byref(cfg), 4, 4, 2, C.byref(h)) == L.ERR_ARG

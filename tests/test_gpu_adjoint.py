@@ -299,3 +299,93 @@ def test_backward_before_forward_is_an_error():
     ctrl = torch.zeros(eng.controls_shape(), device=DEV)
     with pytest.raises(ZQ3DError):
         eng.backward(ctrl, torch.zeros((8, 8, 8), device=DEV))
+
+
+def test_stale_rollout_backward_is_refused():
+    """Two rollouts of one engine in one graph: the engine keeps only the latest forward's checkpoints,
+    tape and pressure store, so the earlier rollout's backward raises instead of returning the gradient
+    of the later one."""
+    from zq_smokesimulation_amd.adjoint import rollout
+    W, H, D = 12, 12, 12
+    eng, _ = make_engine(W, H, D, 2, 1, torch.float32)
+    ctrl = torch.zeros(eng.controls_shape(), device=DEV, requires_grad=True)
+    g0 = eng.generation()
+    rho1 = rollout(eng, ctrl)
+    rho2 = rollout(eng, ctrl * 2.0)
+    assert eng.generation() == g0 + 2
+    with pytest.raises(RuntimeError, match="another forward"):
+        (rho1.sum() + rho2.sum()).backward()
+    # the latest rollout alone still differentiates
+    ctrl.grad = None
+    rho3 = rollout(eng, ctrl)
+    rho3.sum().backward()
+    assert ctrl.grad is not None and torch.isfinite(ctrl.grad).all()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_recomputed_segment_replays_the_forward_bit_for_bit(dtype):
+    """After a backward the tape holds the first segment as the backward recomputed it.  Whether the
+    recomputed steps solve their projections again or apply the forward's stored pressures, every taped
+    state equals the one the forward produced: the projected faces of step s are the state a plain
+    device simulation reaches after s + 1 steps (FP32), and the two modes agree bit for bit (both
+    precisions).  A misindexed or stale pressure slot changes these faces."""
+    import zq_smokesimulation_amd as z
+    W, H, D, nsteps, K = 24, 28, 20, 6, 3
+    tapes = {}
+    for mode in ("solve", "store"):
+        eng, f = make_engine(W, H, D, nsteps, K, dtype, closed=True, obstacle=True)
+        if mode == "store":
+            eng.set_pressure_store(True)
+        gen = torch.Generator(device="cpu").manual_seed(12)
+        ctrl = (0.5 * torch.randn(eng.controls_shape(), generator=gen, dtype=torch.float64)).to(DEV, dtype)
+        target = torch.rand((D, H, W), generator=gen, dtype=torch.float64).to(DEV, dtype)
+        rho = eng.forward(ctrl)
+        eng.backward(ctrl, rho - target)
+        st = eng.last_backward
+        assert st["recomputed_steps"] == K and (st["forward_sweeps"] == 0) == (mode == "store"), st
+        tapes[mode] = [{n: eng.tape(s, n).cpu().numpy() for n in ("u", "v", "w", "T", "rho")} for s in range(K)]
+        if mode == "solve" and dtype == torch.float32:
+            c = eng.cfg
+            sim = z.SmokeSim3D(W, H, D, voxel_size=c.voxel_size, step_time=c.step_time, steps=c.steps, Tamb=c.Tamb,
+                               buoyancy=c.buoyancy, gravity=c.gravity, confine=c.confine, vel_atten=c.vel_atten,
+                               temp_atten=c.temp_atten, dens_atten=c.dens_atten, adv_vel_type=c.adv_vel_type,
+                               adv_sca_type=c.adv_sca_type, sor_open_mask=c.sor.open_mask,
+                               sor_closed_mean=c.sor.closed_mean, sor_omega=c.sor.omega, sor_iters=c.sor.iters,
+                               sor_mixed_precision=1, sor_max_outer=c.sor.max_outer, sor_tol=c.sor.tol, warm_start=1)
+            sim.set_state(**f)
+            sim.set_sphere_sources(SOURCES)
+            torch.cuda.synchronize()
+            for s in range(K):
+                sim.step(ctrl[s].data_ptr())
+                g = sim.get_state()
+                for n in ("u", "v", "w"):
+                    np.testing.assert_array_equal(tapes[mode][s][n], g[n], err_msg="step %d %s" % (s, n))
+            sim.close()
+        eng.close()
+    for s in range(K):
+        for n in ("u", "v", "w", "T", "rho"):
+            np.testing.assert_array_equal(tapes["store"][s][n], tapes["solve"][s][n], err_msg="step %d %s" % (s, n))
+
+
+def test_evaluate_reruns_from_a_checkpoint():
+    """zq3d_adj_evaluate: the same controls from any step reproduce the forward's density bit for bit;
+    a perturbation of step s's controls evaluated from step s equals a whole forward of the perturbed
+    controls; and the forward's gradient is unaffected by evaluations in between."""
+    W, H, D, nsteps = 20, 22, 16, 6
+    eng, _ = make_engine(W, H, D, nsteps, 2, torch.float64, closed=True, obstacle=True)
+    gen = torch.Generator(device="cpu").manual_seed(13)
+    ctrl = (0.5 * torch.randn(eng.controls_shape(), generator=gen, dtype=torch.float64)).to(DEV)
+    target = torch.rand((D, H, W), generator=gen, dtype=torch.float64).to(DEV)
+    rho = eng.forward(ctrl)
+    g = eng.backward(ctrl, rho - target).clone()
+    for s in (0, 3, 5):
+        assert torch.equal(eng.evaluate(ctrl, from_step=s), rho), s
+    c2 = ctrl.clone()
+    c2[4] += 0.01
+    r_eval = eng.evaluate(c2, from_step=4)
+    g_again = eng.backward(ctrl, rho - target)
+    np.testing.assert_allclose(g_again.cpu().numpy(), g.cpu().numpy(), rtol=0,
+                               atol=1e-12 * float(g.abs().max()))
+    r_full = eng.forward(c2)
+    assert torch.equal(r_eval, r_full)
+    assert not torch.equal(r_full, rho)

@@ -1,0 +1,281 @@
+"""GPU: the workloads at their full sizes, each against the oracle.
+
+* 256^3 full pipeline: closed box, sphere source, alpha 0.05, beta 0.5, eps 0.3, dt 0.02, 20
+  advection substeps, attenuation 0.99 per step, mixed-precision SOR (omega at the grid's optimum
+  2/(1+sin(pi/256)), 40 sweeps per pass, FP64 residual) to max|div| <= 1e-6, warm start,
+  MAC->regular output every 10 steps.  The first two steps agree with the oracle's to 1e-5 of each
+  field's scale (the closed-domain FP64 mean is summed in another order on the device); then 100
+  steps each end with max|div - mean| <= 1e-6 after the FP32 correction, and every output
+  conversion equals the oracle's conversion of the same faces bit for bit.
+* 128^3 moving obstacle, open top, mixed precision to 1e-6, 200 steps, against a recorded oracle
+  run (tests/golden/c1_128_200.json, made by tests/golden/make_c1_golden.py): every step meets the
+  bound, the refinement takes the oracle's number of passes, the final density's 8^3 block sums
+  agree to a relative L2 of 1e-4, and the final u, v, w, T, rho are the oracle's bit for bit (their
+  SHA-256 digests).
+* 64-member ensemble (per-axis sizes from {32..96}, alpha and beta per member): three members
+  against the oracle's steps for 3 steps bit for bit, then 100 graph-replayed steps with bounded,
+  finite fields and a member equal bit for bit to its own single-grid simulation.
+"""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import zq_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = [[0.5, 0.1, 0.5, 0.08, 1.0, 1.0]]
+
+
+def make_sim(W, H, D, p):
+    import zq_smokesimulation_amd as z
+    return z.SmokeSim3D(W, H, D, voxel_size=p["h"], step_time=p["dt"], steps=p["steps"], Tamb=p["Tamb"],
+                        buoyancy=p["beta"], gravity=p["alpha"], confine=p["eps"], vel_atten=p["aV"],
+                        temp_atten=p["aT"], dens_atten=p["aR"], adv_vel_type=p["adv_vel_type"],
+                        adv_sca_type=p["adv_sca_type"], sor_open_mask=p["open_mask"],
+                        sor_closed_mean=p["closed_mean"], sor_omega=p["omega"], sor_iters=p["sor_iters"],
+                        sor_mixed_precision=p["mixed"], sor_max_outer=p["max_outer"], sor_tol=p["tol"],
+                        warm_start=1)
+
+
+def close_to(g, ref, what, rel=1e-5):
+    for name, a in ref.items():
+        scale = max(float(np.abs(a).max()), 1e-6)
+        err = float(np.abs(g[name] - a).max())
+        assert err <= rel * scale, "%s %s: max err %g (scale %g)" % (what, name, err, scale)
+
+
+def test_full_pipeline_256():
+    import zq_smokesimulation_amd as z
+    W = 256
+    p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=0.99, aT=0.99, aR=0.99,
+             adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1,
+             omega=2.0 / (1.0 + math.sin(math.pi / W)), sor_iters=40, mixed=1, max_outer=1000, tol=1e-6)
+    occ, u, v, w = O.init_fine_grid(W, W, W, O.ALL_FACES, 0)
+    T = O.noise(W, W, W, 0, 0.01)
+    R = np.zeros_like(T)
+    p64 = np.zeros((W + 2, W + 2, W + 2))
+    sim = make_sim(W, W, W, p)
+    sim.set_sphere_sources(SRC)
+    sim.set_state(u=u, v=v, w=w, T=T, rho=R, occ=occ)
+    for s in range(2):
+        print("256^3 step %d against the oracle" % s, flush=True)
+        st_o = O.step(p, u, v, w, T, R, occ, p64=p64, spheres=SRC)
+        st = sim.step()
+        assert st["converged"] == 1 and st["max_div"] <= 1e-6, st
+        assert st_o[2] <= 1e-6
+        close_to(sim.get_state(), dict(u=u, v=v, w=w, T=T, rho=R), "step %d" % s)
+    worst = 0.0
+    for s in range(2, 102):
+        st = sim.step()
+        worst = max(worst, st["max_div"])
+        assert st["converged"] == 1 and st["max_div"] <= 1e-6, (s, st)
+        if s % 10 == 0:
+            sim.mac_to_regular()
+            vel4 = sim.download(z._lib.FIELD_VEL4)
+            g = sim.get_state()
+            np.testing.assert_array_equal(vel4, O.mac_to_cell4(g["u"], g["v"], g["w"]), err_msg="step %d" % s)
+            print("256^3 step %d: %d sweeps, max|div| %.3g" % (s, st["sweeps"], st["max_div"]), flush=True)
+    g = sim.get_state()
+    sim.close()
+    for k in ("u", "v", "w", "T", "rho"):
+        assert np.isfinite(g[k]).all(), k
+    assert 0.0 <= g["rho"].min() and g["rho"].max() <= 1.0 and g["rho"].max() > 0.5
+    print("256^3: 102 steps, worst max|div| %.4g" % worst, flush=True)
+
+
+def test_moving_obstacle_128_200_steps():
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_c1_golden as G
+    with open(os.path.join(HERE, "golden", "c1_128_200.json")) as f:
+        gold = json.load(f)
+    W, p = G.W, G.params()
+    centres, vel = G.path()
+    occ, u, v, w, T, R = G.initial_state()
+    sim = make_sim(W, W, W, p)
+    sim.set_sphere_sources(G.SRC)
+    sim.set_state(u=u, v=v, w=w, T=T, rho=R, occ=occ)
+    passes = []
+    for s in range(gold["steps"]):
+        c = centres[s]
+        sim.stamp_sphere_obstacle(float(c[0]), float(c[1]), float(c[2]), 0.1, vel, G.SHELL)
+        st = sim.step()
+        assert st["converged"] == 1 and st["max_div"] <= 1e-6, (s, st)
+        passes.append(st["outer"])
+        if s % 50 == 49:
+            print("128^3 moving obstacle step %d: %d passes (oracle %d), max|div| %.3g"
+                  % (s + 1, passes[-1], gold["passes"][s], st["max_div"]), flush=True)
+    g = sim.get_state()
+    sim.close()
+    bs = G.block_sums(g["rho"])
+    ref = np.array(gold["rho_block_sums_8"])
+    err = float(np.linalg.norm(bs - ref) / np.linalg.norm(ref))
+    same = {k: G.digest(g[k]) == gold["sha256"][k] for k in ("u", "v", "w", "T", "rho")}
+    print("128^3 moving obstacle after %d steps: density block sums rel L2 %.3g, mass %.6g vs %.6g, "
+          "passes equal on %d/%d steps, bit-identical fields %s"
+          % (gold["steps"], err, g["rho"].astype(np.float64).sum(), gold["rho_total"],
+             sum(a == b for a, b in zip(passes, gold["passes"])), len(passes), same), flush=True)
+    assert err <= 1e-4
+    assert passes == gold["passes"]
+    # open top, no closed-domain mean: nothing is summed in a different order, so the whole run is
+    # the oracle's bit for bit
+    assert all(same.values()), same
+
+
+def ensemble_members(count=64, seed=0):
+    """The seeded 64-member workload: per-axis sizes from {32..96}, alpha in [0.02, 0.1], beta in
+    [0.2, 1.0], closed boxes, plain red-black SOR (omega 1.8, 200 iterations)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(count):
+        W, H, D = (int(x) for x in rng.integers(32, 97, size=3))
+        alpha, beta = float(rng.uniform(0.02, 0.1)), float(rng.uniform(0.2, 1.0))
+        out.append(((W, H, D), alpha, beta))
+    return out
+
+
+def ensemble_params(W, alpha, beta):
+    return dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=beta, alpha=alpha, eps=0.3, aV=1.0, aT=1.0, aR=1.0,
+                adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1, omega=1.8, sor_iters=200, mixed=0,
+                max_outer=0, tol=1e-6)
+
+
+def test_ensemble_64_members():
+    from zq_smokesimulation_amd.ensemble import SmokeEnsemble, member_config
+    members = ensemble_members()
+    cfgs, states = [], []
+    for m, ((W, H, D), a, b) in enumerate(members):
+        p = ensemble_params(W, a, b)
+        cfgs.append(member_config(W, H, D, voxel_size=p["h"], step_time=p["dt"], steps=p["steps"], Tamb=p["Tamb"],
+                                  buoyancy=b, gravity=a, confine=p["eps"], sor_open_mask=0, sor_closed_mean=1,
+                                  sor_omega=p["omega"], sor_iters=p["sor_iters"], sor_mixed_precision=0))
+        occ, u, v, w = O.init_fine_grid(W, H, D, O.ALL_FACES, 0)
+        states.append(dict(u=u, v=v, w=w, T=O.noise(W, H, D, m, 0.01), rho=np.zeros((D, H, W), np.float32),
+                           occ=occ))
+    ens = SmokeEnsemble(cfgs)
+    for m, st in enumerate(states):
+        ens.set_state(m, **st)
+        ens.set_sources(m, spheres=SRC)
+    sizes = [W * H * D for (W, H, D), _, _ in members]
+    aniso = [max(d) / min(d) for d, _, _ in members]
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), int(np.argmax(aniso))})
+    print("ensemble: %d members, %d cells; oracle members %s" % (len(members), ens.cells(),
+                                                                  [members[i][0] for i in picks]), flush=True)
+    ref = {i: {k: a.copy() for k, a in states[i].items()} for i in picks}
+    for s in range(3):
+        ens.step(1, graph=True)
+        ens.synchronize()
+        for i in picks:
+            (W, H, D), a, b = members[i]
+            r = ref[i]
+            O.step(ensemble_params(W, a, b), r["u"], r["v"], r["w"], r["T"], r["rho"], r["occ"], spheres=SRC)
+            g = ens.get_state(i)
+            for k in ("u", "v", "w", "T", "rho"):
+                assert np.array_equal(g[k].view(np.uint32), r[k].view(np.uint32)), "step %d member %d %s" % (s, i, k)
+        print("ensemble step %d: members %s bit-exact against the oracle" % (s, picks), flush=True)
+    ens.step(97, graph=True)
+    ens.synchronize()
+    for m, ((W, H, D), _, _) in enumerate(members):
+        g = ens.get_state(m)
+        for k in ("u", "v", "w", "T", "rho"):
+            assert np.isfinite(g[k]).all(), (m, k)
+        # trilinear samples stay within their taps: density in [0, source density], temperature within
+        # [noise minimum, source temperature]
+        assert g["rho"].min() >= 0.0 and g["rho"].max() <= 1.0, m
+        assert g["T"].min() >= -0.01 and g["T"].max() <= 1.0, m
+        # the wall shell's faces are never written
+        assert not g["u"][:, :, :2].any() and not g["u"][:, :, -2:].any(), m
+        assert not g["v"][:, :2].any() and not g["v"][:, -2:].any(), m
+        assert not g["w"][:2].any() and not g["w"][-2:].any(), m
+    # the largest member, stepped alone, reaches the same bits after the same 100 steps
+    i = int(np.argmax(sizes))
+    (W, H, D), a, b = members[i]
+    p = ensemble_params(W, a, b)
+    sim = make_sim(W, H, D, p)
+    sim.set_sphere_sources(SRC)
+    sim.set_state(**states[i])
+    for _ in range(100):
+        sim.step(stats=False)
+    g1, g2 = sim.get_state(), ens.get_state(i)
+    sim.close()
+    ens.close()
+    for k in ("u", "v", "w", "T", "rho"):
+        np.testing.assert_array_equal(g1[k], g2[k], err_msg=k)
+    assert g1["rho"].max() > 0.5
+
+
+def test_control_gradient_192_fd():
+    """Differentiable control at 192^3: 16 steps, checkpoints every 4, the 8-blob target, central
+    differences at eps 1e-3 in the FP64 check mode at 32 seeded entries chosen by the protocol of
+    zq_smokesimulation_amd.gradcheck (entries with a kink inside +-eps are rejected by their one-sided
+    differences before the adjoint is consulted); every probe agrees to 1e-4 relative."""
+    from zq_smokesimulation_amd.gradcheck import run_check
+    summary, rows = run_check(n=192, steps=16, every=4, voxels=32, eps=1e-3, max_visits=160,
+                              log=lambda s: print(s, flush=True))
+    print(json.dumps(summary), flush=True)
+    assert summary["probes"] == 32
+    assert summary["within_1e-4"] == 32, [r for r in rows if r["probe"] and r["rel_err"] > 1e-4]
+
+
+def field_errors(got, ref, mask=None):
+    d = (got.astype(np.float64) - ref.astype(np.float64))
+    r = ref.astype(np.float64)
+    if mask is not None:
+        d, r = d[mask], r[mask]
+    return {"max_abs": float(np.abs(d).max()), "rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(r), 1e-300)),
+            "scale": float(np.abs(r).max())}
+
+
+@pytest.mark.parametrize("case", ["closed_box", "open_top_obstacle"])
+def test_mixed_precision_error_against_fp64_projection(case):
+    """The mixed-precision projection (FP32 sweeps, FP64 residual and pressure, to max|div| <= 1e-6)
+    against the oracle's FP64 conjugate-gradient projection of the same input to max|r| <= 1e-12, at
+    128^3 after 20 steps of the closed-box plume (omega optimal for the grid, attenuation 0.99) or of
+    the moving-obstacle workload (open top, omega 1.8).  Per field: max|delta| and relative L2 of u,
+    v, w and of the pressure (its fluid-cell mean removed from both on a closed domain)."""
+    import zq_smokesimulation_amd as z
+    W = 128
+    closed = case == "closed_box"
+    p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3,
+             aV=0.99 if closed else 1.0, aT=0.99 if closed else 1.0, aR=0.99 if closed else 1.0,
+             adv_vel_type=0, adv_sca_type=2, open_mask=0 if closed else O.OPEN_YHI, closed_mean=1 if closed else 0,
+             omega=2.0 / (1.0 + math.sin(math.pi / W)) if closed else 1.8, sor_iters=40, mixed=1, max_outer=2000,
+             tol=1e-6)
+    shell = O.ALL_FACES if closed else O.ALL_FACES & ~O.OPEN_YHI
+    occ, u, v, w = O.init_fine_grid(W, W, W, shell, 0)
+    sim = make_sim(W, W, W, p)
+    sim.set_sphere_sources(SRC)
+    sim.set_state(u=u, v=v, w=w, T=O.noise(W, W, W, 0, 0.01), rho=np.zeros((W, W, W), np.float32), occ=occ)
+    for s in range(20):
+        if not closed:
+            sim.stamp_sphere_obstacle(0.45 + 0.002 * s, 0.55, 0.5, 0.1, (0.1, 0.0, 0.0), shell)
+        sim.step()
+    if not closed:
+        sim.stamp_sphere_obstacle(0.49, 0.55, 0.5, 0.1, (0.1, 0.0, 0.0), shell)
+    sim.add_force_attenuate()
+    sim.advect_velocity()
+    pre = sim.get_state()
+    st = sim.project()
+    post = sim.get_state()
+    p_dev = sim.download(z._lib.FIELD_P64)[1:-1, 1:-1, 1:-1]
+    sim.close()
+    assert st["converged"] == 1 and st["max_div"] <= 1e-6
+    ref = [pre[k].copy() for k in ("u", "v", "w")]
+    it, p_ref = O.project_cg64(*ref, pre["occ"], p["open_mask"], subtract_mean=closed, tol=1e-12, want_p=True)
+    p_ref = p_ref[1:-1, 1:-1, 1:-1]
+    fluid = pre["occ"] == 0
+    if closed:
+        p_dev = p_dev - p_dev[fluid].mean()
+        p_ref = p_ref - p_ref[fluid].mean()
+    errs = {k: field_errors(post[k], r) for k, r in zip(("u", "v", "w"), ref)}
+    errs["p"] = field_errors(p_dev, p_ref, fluid)
+    print(json.dumps({"case": case, "grid": W, "cg_iterations": it, "sweeps": st["sweeps"],
+                      "max_div": st["max_div"], "errors": errs}), flush=True)
+    for k in ("u", "v", "w"):
+        assert errs[k]["rel_l2"] <= 1e-4 and errs[k]["max_abs"] <= 1e-4 * max(errs[k]["scale"], 1.0), (k, errs[k])
+    assert errs["p"]["rel_l2"] <= 1e-3, errs["p"]

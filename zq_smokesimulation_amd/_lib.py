@@ -9,7 +9,8 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libzq_smoke3d.so")
+# ZQ3D_LIB may name another build of the same sources (the sanitizer build of the host mirror)
+LIB_PATH = os.environ.get("ZQ3D_LIB") or os.path.join(LIB_DIR, "libzq_smoke3d.so")
 
 # enum AdvectVelocityType / AdvectScalarType (ZQ_CUDA_Advection3D.h:7-21)
 ADV_VEL_INREG_OUTREG, ADV_VEL_INREG_OUTREG_BFECC, ADV_VEL_INMAC_OUTMAC, \
@@ -42,6 +43,7 @@ class ProjStats(C.Structure):
         ("max_residual", C.c_double),
         ("max_div", C.c_double),
         ("mean", C.c_double),
+        ("converged", C.c_int),
     ]
 
     def as_dict(self):
@@ -245,6 +247,7 @@ def _bind_scene(L):
     L.zq3d_scene_load.argtypes = [C.c_char_p, C.POINTER(Scene)]
     L.zq3d_scene_init_occupancy.argtypes = [C.c_char_p, vp, C.c_size_t]
     L.zq3d_scene_apply_moving_objects.argtypes = [C.c_char_p, C.c_int, vp, vp, vp, vp]
+    L.zq3d_di3_read.argtypes = [C.c_char_p, C.POINTER(C.c_int * 4), vp, C.c_size_t]
     L.ZQ3D_AddForce3D_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
     L.ZQ3D_AddForce3D_ex.restype = C.c_float
@@ -278,6 +281,10 @@ def _bind_adjoint(L):
     L.zq3d_adj_forward.argtypes = [vp, vp, C.POINTER(AdjStats), vp]
     L.zq3d_adj_backward.argtypes = [vp, vp, vp, vp, C.POINTER(AdjStats)]
     L.zq3d_adj_get.argtypes = [vp, C.c_int, C.c_int, vp]
+    L.zq3d_adj_generation.argtypes = [vp]
+    L.zq3d_adj_generation.restype = C.c_longlong
+    L.zq3d_adj_get_tape.argtypes = [vp, C.c_int, C.c_int, vp]
+    L.zq3d_adj_evaluate.argtypes = [vp, C.c_int, vp, C.POINTER(AdjStats), vp]
     return L
 
 

@@ -142,6 +142,30 @@ class SmokeRollout:
         self.last_backward = st.as_dict()
         return g
 
+    def evaluate(self, controls, from_step=0):
+        """Final density for `controls`, rerun from the latest forward's checkpoint at or before
+        `from_step` (the controls before that checkpoint must be the forward's); the forward's
+        checkpoints, tape and pressure store are kept, so its backward is unaffected."""
+        controls = self._check_controls(controls)
+        self._stream()
+        rho = torch.empty((self.D, self.H, self.W), dtype=self.dtype, device=self.device)
+        st = _lib.AdjStats()
+        check(_lib.lib().zq3d_adj_evaluate(self._h, int(from_step), _ptr(controls), C.byref(st), _ptr(rho)),
+              "zq3d_adj_evaluate")
+        return rho
+
+    def generation(self):
+        """Number of forwards run so far; a backward differentiates the latest one."""
+        return int(_lib.lib().zq3d_adj_generation(self._h))
+
+    def tape(self, step, name):
+        """Recorded state of `step` in the segment on tape: u/v/w the projected faces of that step,
+        T/rho the fields after its force stage."""
+        out = torch.empty(self.shape(name), dtype=self.dtype, device=self.device)
+        self._stream()
+        check(_lib.lib().zq3d_adj_get_tape(self._h, int(step), _FIELDS[name], _ptr(out)), "zq3d_adj_get_tape")
+        return out
+
     def get(self, name, grad=False):
         """Final state field after a forward, or dL/d(initial field) after a backward (grad=True)."""
         want = torch.uint8 if name == "occ" else self.dtype
@@ -156,12 +180,17 @@ class _Rollout(torch.autograd.Function):
     def forward(ctx, controls, engine):
         rho = engine.forward(controls)
         ctx.engine = engine
+        ctx.generation = engine.generation()
         ctx.save_for_backward(controls)
         return rho
 
     @staticmethod
     def backward(ctx, grad_rho):
         (controls,) = ctx.saved_tensors
+        if ctx.engine.generation() != ctx.generation:
+            # the engine's checkpoints, tape and pressure store now belong to a later forward
+            raise RuntimeError("rollout: the engine ran another forward after this one; its gradient can no "
+                               "longer be computed (use one engine per rollout in a graph, or backward first)")
         return ctx.engine.backward(controls, grad_rho), None
 
 

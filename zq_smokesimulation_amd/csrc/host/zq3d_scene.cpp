@@ -1,6 +1,7 @@
 // Host-only C ABI (Part 4 of zq_smoke3d.h): the reference's config parser and grid set-up,
 // usable without a GPU.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -112,5 +113,34 @@ extern "C" int zq3d_scene_apply_moving_objects(const char* path, int frames, uin
         std::memcpy(w, g.GetVelocityWptr(), sizeof(float) * p.width * p.height * (p.depth + 1));
     }
     for (ZQ_SmokeMovingObject3D* o : objs) delete o;
+    return rc;
+}
+
+// .di3 (the density container ZQ_SmokeGrid3D::ExportDensity writes, ZQ_SmokeGrid3D.cpp:55-81): a
+// 16-byte type tag holding a NUL-terminated name ("float"; bytes after the NUL are not part of
+// it), int32 width, height, depth, channels, one flag byte, then width*height*depth*channels FP32
+// samples, x fastest.
+extern "C" int zq3d_di3_read(const char* path, int* dims4, float* data, size_t capacity) {
+    if (!path || !dims4) return ZQ3D_ERR_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return ZQ3D_ERR_ARG;
+    char tag[16];
+    int hdr[4];
+    unsigned char flag = 0;
+    int rc = ZQ3D_OK;
+    if (std::fread(tag, 1, 16, f) != 16 || std::fread(hdr, 4, 4, f) != 4 || std::fread(&flag, 1, 1, f) != 1 ||
+        std::strncmp(tag, "float", 6) != 0 || hdr[0] < 1 || hdr[1] < 1 || hdr[2] < 1 || hdr[3] < 1) {
+        rc = ZQ3D_ERR_ARG;
+    } else {
+        std::memcpy(dims4, hdr, sizeof(hdr));
+        const size_t n = (size_t)hdr[0] * hdr[1] * hdr[2] * hdr[3];
+        if (data) {
+            if (capacity < n || std::fread(data, 4, n, f) != n) rc = ZQ3D_ERR_ARG;
+        } else {
+            // header only: the payload must still be whole
+            if (std::fseek(f, 0, SEEK_END) != 0 || std::ftell(f) != (long)(33 + 4 * n)) rc = ZQ3D_ERR_ARG;
+        }
+    }
+    std::fclose(f);
     return rc;
 }

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(BX* BY) void k_mac_to_cell4(const R* __restrict__ u
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_advect_cells(const vec4_t<R>* __restrict__ vel4,
                                                          const uint8_t* __restrict__ occ, R* __restrict__ adv,
-                                                         Dims d, R h, int steps, R dtt) {
+                                                         Dims d, StepScale<R> h, int steps, R dtt) {
     st_advect_cell<R>(vel4, occ, adv, d, h, steps, dtt, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
                       blockIdx.z);
 }
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const R* __rest
 // face centre; the face's own component sampled at the end point goes to a scratch face grid.
 template <class R, int AXIS, class Vel>
 __global__ __launch_bounds__(BX* BY) void k_advect_faces(Vel V, const uint8_t* __restrict__ occ, R* __restrict__ out,
-                                                         Dims d, R h, int steps, R dtt) {
+                                                         Dims d, StepScale<R> h, int steps, R dtt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(BX* BY) void k_advect_faces(Vel V, const uint8_t* _
 // SoA scratch the face apply reads.
 template <class R, int AXIS, class Vel, class In>
 __global__ __launch_bounds__(BX* BY) void k_advect_faces_in(Vel V, In I, const uint8_t* __restrict__ occ,
-                                                            R* __restrict__ out, Dims d, R h, int steps, R dtt) {
+                                                            R* __restrict__ out, Dims d, StepScale<R> h, int steps, R dtt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BX* BY) void k_advect_faces_in(Vel V, In I, const u
 
 template <class R, class Vel, class In, bool SOA>
 __global__ __launch_bounds__(BX* BY) void k_advect_cells_in(Vel V, In I, const uint8_t* __restrict__ occ,
-                                                            void* __restrict__ out, Dims d, R h, int steps, R dtt) {
+                                                            void* __restrict__ out, Dims d, StepScale<R> h, int steps, R dtt) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(BX* BY) void k_apply_faces(const R* __restrict__ sr
 template <class R, class Vel>
 __global__ __launch_bounds__(BX* BY) void k_advect_scalar(Vel V, const uint8_t* __restrict__ occ,
                                                           const R* __restrict__ Tin, const R* __restrict__ Rin,
-                                                          R* __restrict__ Tout, R* __restrict__ Rout, Dims d, R h,
+                                                          R* __restrict__ Tout, R* __restrict__ Rout, Dims d, StepScale<R> h,
                                                           int steps, R dtt) {
     st_advect_scalar<R, Vel>(V, occ, Tin, Rin, Tout, Rout, d, h, steps, dtt, blockIdx.x * BX + threadIdx.x,
                              blockIdx.y * BY + threadIdx.y, blockIdx.z);
@@ -167,7 +167,8 @@ template <class R, class Vel, class In>
 void faces_in(const Dims& d, const Vel& V, const In& I, const uint8_t* occ, R* ou, R* ov, R* ow, const AdvParams& ap,
               hipStream_t s) {
     const dim3 blk(BX, BY);
-    const R h = (R)ap.h, dtt = (R)ap.dtt;
+    const StepScale<R> h = step_scale<R>(ap.h);
+    const R dtt = (R)ap.dtt;
     hipLaunchKernelGGL((k_advect_faces_in<R, 0, Vel, In>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, I, occ, ou, d, h,
                        ap.steps, dtt);
     hipLaunchKernelGGL((k_advect_faces_in<R, 1, Vel, In>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, V, I, occ, ov, d, h,
@@ -180,7 +181,7 @@ template <class R, bool SOA, class Vel, class In>
 void cells_in(const Dims& d, const Vel& V, const In& I, const uint8_t* occ, void* out, const AdvParams& ap,
               hipStream_t s) {
     hipLaunchKernelGGL((k_advect_cells_in<R, Vel, In, SOA>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, V, I, occ,
-                       out, d, (R)ap.h, ap.steps, (R)ap.dtt);
+                       out, d, step_scale<R>(ap.h), ap.steps, (R)ap.dtt);
 }
 
 template <class R>
@@ -261,7 +262,8 @@ void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ,
     R* ou = scratch;
     R* ov = scratch + Nu;
     R* ow = scratch + Nu + Nv;
-    const R h = (R)ap.h, dtt = (R)ap.dtt;
+    const StepScale<R> h = step_scale<R>(ap.h);
+    const R dtt = (R)ap.dtt;
     if (mode == 0) {
         launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
         R* adv = scratch;  // 3N values <= Nu+Nv+Nw
@@ -302,7 +304,8 @@ void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, con
                           const R* Rin, R* Tout, R* Rout, int mac, const AdvParams& ap, vec4_t<R>* vel4,
                           hipStream_t s) {
     const dim3 blk(BX, BY);
-    const R h = (R)ap.h, dtt = (R)ap.dtt;
+    const StepScale<R> h = step_scale<R>(ap.h);
+    const R dtt = (R)ap.dtt;
     if (mac) {
         const MacVel<R> V{u, v, w, d};
         hipLaunchKernelGGL((k_advect_scalar<R, MacVel<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout,

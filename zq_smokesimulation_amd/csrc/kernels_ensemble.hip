@@ -78,7 +78,7 @@ __global__ __launch_bounds__(NT) void ke_mac_to_cell4(const EnsMember* __restric
 __global__ __launch_bounds__(NT) void ke_advect_cells(const EnsMember* __restrict__ M, EnsTable T, int B) {
     for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
         const EnsMember& m = M[s];
-        st_advect_cell<float>(m.vel4, m.occ, m.scratch, m.d, m.h, m.steps, m.dtt, i, j, k);
+        st_advect_cell<float>(m.vel4, m.occ, m.scratch, m.d, StepScale<float>{m.h, m.inv_h}, m.steps, m.dtt, i, j, k);
     });
 }
 
@@ -96,8 +96,8 @@ __global__ __launch_bounds__(NT) void ke_advect_scalar(const EnsMember* __restri
     for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
         const EnsMember& m = M[s];
         const RegularVelCached<float> V{m.vel4, m.d, {}};
-        st_advect_scalar<float, RegularVelCached<float>>(V, m.occ, m.T, m.R, m.T2, m.R2, m.d, m.h, m.steps, m.dtt, i,
-                                                        j, k);
+        st_advect_scalar<float, RegularVelCached<float>>(V, m.occ, m.T, m.R, m.T2, m.R2, m.d,
+                                                         StepScale<float>{m.h, m.inv_h}, m.steps, m.dtt, i, j, k);
     });
 }
 

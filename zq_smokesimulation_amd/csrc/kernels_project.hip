@@ -640,6 +640,38 @@ __global__ __launch_bounds__(BX* BY) void k_max_div64(const F* __restrict__ u, c
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
 }
 
+// max |div - mean| over counted cells of the field the FP64 correction WOULD produce: each of a
+// cell's six faces is evaluated as st_correct stores it (rounded once to F), then summed in FP64
+// as k_max_div64 sums stored faces, so the value equals k_max_div64 after launch_correct64 bit for
+// bit.  Reads the faces and the pressure only; nothing is written but the per-block maxima.
+template <class F>
+__global__ __launch_bounds__(BX* BY) void k_max_div64_pred(const F* __restrict__ u, const F* __restrict__ v,
+                                                           const F* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                           int open_mask, const uint8_t* __restrict__ nc,
+                                                           const double* __restrict__ p64,
+                                                           const double* __restrict__ mean64,
+                                                           double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double m = 0.0;
+    if (i < d.W && j < d.H && nc[cidx(d, i, j, k)]) {
+        const F u0 = st_corrected_face<0, double, F>(occ, open_mask, p64, u, d, i, j, k);
+        const F u1 = st_corrected_face<0, double, F>(occ, open_mask, p64, u, d, i + 1, j, k);
+        const F v0 = st_corrected_face<1, double, F>(occ, open_mask, p64, v, d, i, j, k);
+        const F v1 = st_corrected_face<1, double, F>(occ, open_mask, p64, v, d, i, j + 1, k);
+        const F w0 = st_corrected_face<2, double, F>(occ, open_mask, p64, w, d, i, j, k);
+        const F w1 = st_corrected_face<2, double, F>(occ, open_mask, p64, w, d, i, j, k + 1);
+        const double du = (double)u1 - (double)u0;
+        const double dv = (double)v1 - (double)v0;
+        const double dw = (double)w1 - (double)w0;
+        m = fabs(((du + dv) + dw) - *mean64);
+    }
+    m = block_reduce<true>(m, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
+}
+
 int cell_blocks(const Dims& d) {
     const dim3 g = grid_for(d.W, d.H, d.D);
     return (int)(g.x * g.y * g.z);
@@ -926,11 +958,23 @@ void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const u
     hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), dmax);
 }
 
+template <class F>
+void launch_max_div64_pred(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* occ, int open_mask,
+                           const uint8_t* nc, const double* p64_pad, const double* mean64, double* part,
+                           unsigned long long* dmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_div64_pred<F>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ, open_mask, nc,
+                       p64_pad, mean64, part, d);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), dmax);
+}
+
 #define ZQ3D_PROJ_INSTANTIATE(F)                                                                                       \
     template void launch_correct64<F>(const Dims&, const uint8_t*, int, const double*, F*, F*, F*, hipStream_t);       \
     template void launch_correct_all64<F>(const Dims&, const double*, F*, F*, F*, hipStream_t);                        \
     template void launch_max_div64<F>(const Dims&, const F*, const F*, const F*, const uint8_t*, const double*,        \
-                                      double*, unsigned long long*, hipStream_t);
+                                      double*, unsigned long long*, hipStream_t);                                      \
+    template void launch_max_div64_pred<F>(const Dims&, const F*, const F*, const F*, const uint8_t*, int,             \
+                                           const uint8_t*, const double*, const double*, double*,                      \
+                                           unsigned long long*, hipStream_t);
 ZQ3D_PROJ_INSTANTIATE(float)
 ZQ3D_PROJ_INSTANTIATE(double)
 #undef ZQ3D_PROJ_INSTANTIATE

@@ -57,16 +57,22 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
         if (bmax_out) *bmax_out = bmax;
     }
     int outer = 0;
-    double rmax = 0.0;
-    if (int rc = refine_mixed(d, o, tol, B, s, nullptr, where, &outer, &rmax)) return rc;
-    double md = -1.0;
+    double rmax = 0.0, md = -1.0;
+    // the forward checks the divergence its stored faces will have; the adjoint's faces are not
+    // projected fields (its correction goes on every face)
+    const DivCheck chk{u, v, w, occ, o.open_mask, sizeof(F) == 8};
+    if (int rc = refine_mixed(d, o, tol, B, s, nullptr, where, adjoint ? nullptr : &chk, &outer, &rmax, &md))
+        return rc;
     if (adjoint) {
         if (o.closed_mean) launch_remove_mean_p64(d, occ, B.p64, B.part, &sc->aux64, s);
         launch_correct_all64<F>(d, B.p64, u, v, w, s);
+        md = -1.0;
     } else {
         launch_correct64<F>(d, occ, o.open_mask, B.p64, u, v, w, s);
-        launch_max_div64<F>(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
-        if (int rc = read_double_bits(&sc->dmax, &md, s, where)) return rc;
+        if (md < 0.0) {  // the passes ran out before the residual met tol: measure the stored faces
+            launch_max_div64<F>(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+            if (int rc = read_double_bits(&sc->dmax, &md, s, where)) return rc;
+        }
     }
     ZQ_HIP(hipGetLastError(), where);
     if (st) {
@@ -75,6 +81,7 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
         st->max_residual = rmax;
         st->max_div = md;
         st->mean = 0.0;
+        st->converged = adjoint ? -1 : (md <= o.tol ? 1 : 0);
     }
     return ZQ3D_OK;
 }
@@ -117,6 +124,7 @@ struct zq3d_adj {
     // adjoint buffers
     void *ub, *vb, *wb, *Tb, *Rb, *Tb2, *Rb2, *cellb, *advb, *gb, *wbar, *cb, *outb, *vort;
     bool forward_done = false, backward_done = false;
+    long long generation = 0;  // forwards started; a backward differentiates the latest one
     bool state_final = false;  // the working state holds the last forward's final state
     double last_max_div = 0.0;
 };
@@ -200,8 +208,9 @@ hipError_t checkpoint(zq3d_adj* a, int c, bool save) {
 AdvParams adv_params(const zq3d_config& c) { return AdvParams{c.voxel_size, c.step_time / c.steps, c.steps}; }
 
 // forward: the rollout's forward (stores the pressure when the store is on); solve: a recomputed
-// step solved again; replay: a recomputed step that applies the stored pressure
-enum class StepMode { forward, solve, replay };
+// step solved again; replay: a recomputed step that applies the stored pressure; evaluate: a step of
+// a side evaluation (solved, recording nothing the backward reads)
+enum class StepMode { forward, solve, replay, evaluate };
 
 template <class R>
 int forward_step(zq3d_adj* a, int s, const R* controls, Slot<R>* sl, zq3d_adj_stats* st, StepMode mode) {
@@ -232,7 +241,7 @@ int forward_step(zq3d_adj* a, int s, const R* controls, Slot<R>* sl, zq3d_adj_st
         if (int rc = project_mp<R>(d, u, v, w, a->occ, c.sor, proj_bufs(a, a->p64), false, 0.0, &ps, a->stream,
                                    &bmax))
             return rc;
-        a->fwd_rel[s] = bmax > 0.0 ? c.sor.tol / bmax : 1.0;
+        if (mode != StepMode::evaluate) a->fwd_rel[s] = bmax > 0.0 ? c.sor.tol / bmax : 1.0;
         if (mode == StepMode::forward && a->pstore_on)
             ZQ_HIP(hipMemcpyAsync(a->pstore + (size_t)s * z.Np, a->p64, z.Np * 8, hipMemcpyDeviceToDevice, a->stream),
                    where);
@@ -350,6 +359,20 @@ int run_backward(zq3d_adj* a, const R* controls, const R* rho_bar, R* controls_b
         for (int s = s1 - 1; s >= s0; s--)
             if (int rc = adjoint_step<R>(a, s, tape_slot<R>(a, s - s0), controls_bar, st)) return rc;
     }
+    return ZQ3D_OK;
+}
+
+// The final density for `controls` from the latest forward's checkpoint at or before step `from`
+// (the steps before it are that forward's); the checkpoints, tape and pressure store are kept.
+template <class R>
+int run_evaluate(zq3d_adj* a, int from, const R* controls, R* rho_out, zq3d_adj_stats* st) {
+    const char* where = "zq3d_adj_evaluate";
+    const Sizes z(a->d);
+    const int c = from / a->K;
+    ZQ_HIP(checkpoint<R>(a, c, false), where);
+    for (int s = c * a->K; s < a->nsteps; s++)
+        if (int rc = forward_step<R>(a, s, controls, nullptr, st, StepMode::evaluate)) return rc;
+    if (rho_out) ZQ_HIP(hipMemcpyAsync(rho_out, a->R, z.N * sizeof(R), hipMemcpyDeviceToDevice, a->stream), where);
     return ZQ3D_OK;
 }
 
@@ -552,6 +575,7 @@ extern "C" int zq3d_adj_forward(zq3d_adj* a, const void* controls, zq3d_adj_stat
     const char* where = "zq3d_adj_forward";
     if (!a) { set_err(where, "null rollout"); return ZQ3D_ERR_ARG; }
     zq3d_adj_stats local{};
+    a->generation++;
     CallTimer t(a->stream);
     int rc = a->prec == 4 ? run_forward<float>(a, (const float*)controls, (float*)rho_out, &local)
                           : run_forward<double>(a, (const double*)controls, (double*)rho_out, &local);
@@ -579,6 +603,47 @@ extern "C" int zq3d_adj_backward(zq3d_adj* a, const void* controls, const void* 
     if (local.recomputed_steps) a->state_final = false;
     if (st) *st = local;
     return rc;
+}
+
+extern "C" int zq3d_adj_evaluate(zq3d_adj* a, int from_step, const void* controls, zq3d_adj_stats* st,
+                                 void* rho_out) {
+    const char* where = "zq3d_adj_evaluate";
+    if (!a || !rho_out) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    if (!a->forward_done) { set_err(where, "no forward whose checkpoints to start from"); return ZQ3D_ERR_STATE; }
+    if (from_step < 0 || from_step >= a->nsteps) { set_err(where, "from_step out of range"); return ZQ3D_ERR_ARG; }
+    zq3d_adj_stats local{};
+    CallTimer t(a->stream);
+    int rc = a->prec == 4 ? run_evaluate<float>(a, from_step, (const float*)controls, (float*)rho_out, &local)
+                          : run_evaluate<double>(a, from_step, (const double*)controls, (double*)rho_out, &local);
+    local.ms = t.stop();
+    a->state_final = false;  // the working state is this evaluation's
+    if (st) *st = local;
+    return rc;
+}
+
+extern "C" long long zq3d_adj_generation(zq3d_adj* a) { return a ? a->generation : -1; }
+
+extern "C" int zq3d_adj_get_tape(zq3d_adj* a, int step, int field, void* dev) {
+    const char* where = "zq3d_adj_get_tape";
+    if (!a || !dev) { set_err(where, "null argument"); return ZQ3D_ERR_ARG; }
+    if (a->tape_seg < 0) { set_err(where, "nothing taped yet"); return ZQ3D_ERR_STATE; }
+    const int s0 = a->tape_seg * a->K, s1 = std::min(s0 + a->K, a->nsteps);
+    if (step < s0 || step >= s1) { set_err(where, "step is not in the taped segment"); return ZQ3D_ERR_ARG; }
+    const Sizes z(a->d);
+    const size_t e = (size_t)a->prec;
+    const size_t faces = z.Nu + z.Nv + z.Nw;
+    const char* base = a->tape + (size_t)(step - s0) * (3 * faces + 2 * z.N) * e;
+    size_t off, n;
+    switch (field) {
+    case ZQ3D_FIELD_U: off = 2 * faces; n = z.Nu; break;
+    case ZQ3D_FIELD_V: off = 2 * faces + z.Nu; n = z.Nv; break;
+    case ZQ3D_FIELD_W: off = 2 * faces + z.Nu + z.Nv; n = z.Nw; break;
+    case ZQ3D_FIELD_T: off = 3 * faces; n = z.N; break;
+    case ZQ3D_FIELD_RHO: off = 3 * faces + z.N; n = z.N; break;
+    default: set_err(where, "unknown field"); return ZQ3D_ERR_ARG;
+    }
+    ZQ_HIP(hipMemcpyAsync(dev, base + off * e, n * e, hipMemcpyDeviceToDevice, a->stream), where);
+    return ZQ3D_OK;
 }
 
 extern "C" int zq3d_adj_get(zq3d_adj* a, int field, int grad, void* dev) {

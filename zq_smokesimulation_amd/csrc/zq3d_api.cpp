@@ -115,14 +115,8 @@ namespace detail {
 
 namespace {
 hipError_t account_sweeps(const Dims& d, SorShape shape, int iters, SorProf* pf) {
-    Errs E;
-    E(hipGetLastError());
-    E(hipEventSynchronize(pf->b));
-    float ms = 0.0f;
-    E(hipEventElapsedTime(&ms, pf->a, pf->b));
-    pf->ms += ms;
     pf->launches += sor_launches(d, shape, iters);
-    return E.e;
+    return hipGetLastError();
 }
 }  // namespace
 
@@ -135,12 +129,14 @@ hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const flo
         return hipGetLastError();
     }
     // the events bracket the sweep launches only (not the split-layout conversions)
-    *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s, shape, split, pf->a, pf->b);
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (hipError_t e = pf->next(&ea, &eb)) return e;
+    *result = launch_sor(d, nc, b, mean32, p, p_alt, omega, iters, s, shape, split, ea, eb);
     return account_sweeps(d, shape, iters, pf);
 }
 
 int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
-                 const char* where, int* outer_out, double* rmax_out) {
+                 const char* where, const DivCheck* chk, int* outer_out, double* rmax_out, double* dmax_out) {
     const Sizes z(d);
     Scalars* sc = B.sc;
     const bool split = is_split(B.shape);
@@ -155,17 +151,36 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
         launch_fill_f32(e, z.Np, 0.0f, s);
     }
     int outer = 0;
-    double rmax = 0.0;
+    double rmax = 0.0, dmax = -1.0, dprev = -1.0;
     for (;;) {
         if (split)
             launch_residual64_split(d, B.nc, B.b64, &sc->mean64, B.p64, B.split, B.part, &sc->rmax, s);
         else
             launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, &sc->rmax, s);
         if (int rc = read_double_bits(&sc->rmax, &rmax, s, where)) return rc;
-        if (!(rmax > tol) || outer >= o.max_outer) break;
+        if (!(rmax > tol)) {
+            if (!chk) break;
+            // the residual is the divergence of the FP64-corrected faces; the correction rounds each
+            // face to FP32 once more, so check the divergence the stored faces will have
+            if (chk->fp64)
+                launch_max_div64_pred(d, (const double*)chk->u, (const double*)chk->v, (const double*)chk->w,
+                                      chk->occ, chk->open_mask, B.nc, B.p64, &sc->mean64, B.part, &sc->dmax, s);
+            else
+                launch_max_div64_pred(d, (const float*)chk->u, (const float*)chk->v, (const float*)chk->w,
+                                      chk->occ, chk->open_mask, B.nc, B.p64, &sc->mean64, B.part, &sc->dmax, s);
+            if (int rc = read_double_bits(&sc->dmax, &dmax, s, where)) return rc;
+            if (!(dmax > tol)) break;
+            // what is left is the faces' FP32 rounding once a pass no longer shrinks it by a tenth:
+            // the bound is out of reach at this velocity scale (reported, not looped on)
+            if (dprev >= 0.0 && !(dmax < 0.9 * dprev)) break;
+            dprev = dmax;
+        }
+        if (outer >= o.max_outer) break;
+        dmax = -1.0;
         if (split) {
-            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, o.iters, B.shape, s,
-                                       pf ? pf->a : nullptr, pf ? pf->b : nullptr);
+            hipEvent_t ea = nullptr, eb = nullptr;
+            if (pf) ZQ_HIP(pf->next(&ea, &eb), where);
+            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, o.iters, B.shape, s, ea, eb);
             ZQ_HIP(pf ? account_sweeps(d, B.shape, o.iters, pf) : hipGetLastError(), where);
             launch_update_p64_split(d, B.nc, pair ? B.split.alt : B.split.p, B.p64, s);  // also zeroes e
         } else {
@@ -179,6 +194,7 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
     }
     *outer_out = outer;
     *rmax_out = rmax;
+    if (dmax_out) *dmax_out = dmax;
     return ZQ3D_OK;
 }
 
@@ -221,41 +237,50 @@ int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
         st->max_residual = -1.0;
         st->max_div = md;
         st->mean = mean64;
+        st->converged = -1;
     }
     return ZQ3D_OK;
 }
 
 // Mixed-precision projection: FP64 b, p and residual; FP32 inner SOR on the correction e.
-// p64 holds the warm start (zeroed by the caller for a cold start).
+// p64 holds the warm start (zeroed by the caller for a cold start).  Passes run until the faces the
+// correction stores have max|div - mean| <= tol (or o.max_outer passes).
 int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                   const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf) {
-    const Sizes z(d);
     Scalars* sc = B.sc;
     launch_fill_f32(&sc->zero32, 1, 0.0f, s);
     launch_proj_setup(d, u, v, w, occ, o.open_mask, B.nc, nullptr, B.b64, nullptr, B.p64, B.part, s);
     launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
     int outer = 0;
-    double rmax = 0.0;
-    if (int rc = refine_mixed(d, o, o.tol, B, s, pf, "project_mixed", &outer, &rmax)) return rc;
+    double rmax = 0.0, md = -1.0;
+    const DivCheck chk{u, v, w, occ, o.open_mask, false};
+    if (int rc = refine_mixed(d, o, o.tol, B, s, pf, "project_mixed", &chk, &outer, &rmax, &md)) return rc;
     launch_correct64(d, occ, o.open_mask, B.p64, u, v, w, s);
-    launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
-    double md = 0.0, mean64 = 0.0;
-    if (int rc = read_double_bits(&sc->dmax, &md, s, "project_mixed")) return rc;
+    if (!st) return ZQ3D_OK;
+    if (md < 0.0) {  // the passes ran out before the residual met tol: measure the stored faces
+        launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+        if (int rc = read_double_bits(&sc->dmax, &md, s, "project_mixed")) return rc;
+    }
+    double mean64 = 0.0;
     ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project_mixed");
     ZQ_HIP(hipStreamSynchronize(s), "project_mixed");
-    if (st) {
-        st->outer = outer;
-        st->sweeps = outer * o.iters;
-        st->max_residual = rmax;
-        st->max_div = md;
-        st->mean = mean64;
-    }
+    st->outer = outer;
+    st->sweeps = outer * o.iters;
+    st->max_residual = rmax;
+    st->max_div = md;
+    st->mean = mean64;
+    st->converged = md <= o.tol ? 1 : 0;
     return ZQ3D_OK;
 }
 
 int check_sor_opts(const zq3d_sor_opts& o, const char* where) {
     if (o.iters < 0 || o.max_outer < 0 || !(o.omega > 0.0f && o.omega < 2.0f) || (o.open_mask & ~63)) {
         set_err(where, "invalid SOR options (iters >= 0, 0 < omega < 2, open_mask in [0,63])");
+        return ZQ3D_ERR_ARG;
+    }
+    if (o.mixed_precision && (o.max_outer < 1 || o.iters < 1 || !(o.tol >= 0.0))) {
+        set_err(where, "mixed_precision needs max_outer >= 1, iters >= 1 and tol >= 0 (with max_outer = 0 the "
+                       "velocity would never be projected)");
         return ZQ3D_ERR_ARG;
     }
     return ZQ3D_OK;
@@ -642,7 +667,8 @@ struct zq3d_sim {
     Scalars* sc = nullptr;
     std::vector<float> boxes, spheres;
     bool p64_valid = false;
-    bool prof = false;
+    bool prof = false;      // stage timing (an event pair per stage, waited for)
+    bool sor_prof = false;  // sweep timing (pooled event pairs, not waited for until read)
     hipEvent_t ev[2] = {nullptr, nullptr};
     SorProf sorprof;
     zq3d_timing tm{};
@@ -771,7 +797,7 @@ extern "C" int zq3d_config_default(zq3d_config* c, int width, int height, int de
     c->sor.omega = 1.0f;
     c->sor.iters = 100;
     c->sor.mixed_precision = 0;
-    c->sor.max_outer = 0;
+    c->sor.max_outer = 1000;  // maxOuter's default in the config reader
     c->sor.tol = 1e-6;
     c->warm_start = 1;
     c->octree_thresh[0] = 2;  // octree_thresh1..3 (ZQ_SmokeSimulationUtils3D.cpp:52-54)
@@ -862,8 +888,9 @@ extern "C" void zq3d_destroy(zq3d_sim* s) {
                     s->b64, s->p64, s->sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    for (hipEvent_t e : {s->ev[0], s->ev[1], s->sorprof.a, s->sorprof.b})
+    for (hipEvent_t e : {s->ev[0], s->ev[1]})
         if (e) (void)hipEventDestroy(e);
+    s->sorprof.release();
     coarse_destroy(&s->coarse);
     octree_destroy(&s->octree);
     delete s;
@@ -962,7 +989,7 @@ extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
     if (!s) { set_err("zq3d_project", "null sim"); return ZQ3D_ERR_ARG; }
     const zq3d_config& c = s->cfg;
     StageTimer t(s, &s->tm.ms_project);
-    SorProf* pf = s->prof ? &s->sorprof : nullptr;
+    SorProf* pf = s->sor_prof ? &s->sorprof : nullptr;
     int rc;
     if (c.sub_size > 0) {
         rc = project_coarse(s->coarse, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
@@ -1055,11 +1082,14 @@ extern "C" int zq3d_step(zq3d_sim* s, const float* extra, zq3d_proj_stats* st) {
 extern "C" int zq3d_set_profiling(zq3d_sim* s, int on) {
     const char* where = "zq3d_set_profiling";
     if (!s) { set_err(where, "null sim"); return ZQ3D_ERR_ARG; }
-    if (on) {
-        for (hipEvent_t* e : {&s->ev[0], &s->ev[1], &s->sorprof.a, &s->sorprof.b})
+    if (on < 0 || on > 2) { set_err(where, "on must be 0, 1 or 2"); return ZQ3D_ERR_ARG; }
+    if (on == 1) {
+        for (hipEvent_t* e : {&s->ev[0], &s->ev[1]})
             if (!*e) ZQ_HIP(hipEventCreate(e), where);
     }
-    s->prof = on != 0;
+    ZQ_HIP(s->sorprof.resolve(), where);  // pairs still pending belong to the previous period
+    s->prof = on == 1;
+    s->sor_prof = on != 0;
     s->tm = zq3d_timing{};
     s->sorprof.ms = 0.0;
     s->sorprof.launches = 0;
@@ -1141,6 +1171,7 @@ extern "C" const char* zq3d_sor_kernel(zq3d_sim* s) {
 
 extern "C" int zq3d_get_timing(zq3d_sim* s, zq3d_timing* out) {
     if (!s || !out) { set_err("zq3d_get_timing", "null argument"); return ZQ3D_ERR_ARG; }
+    ZQ_HIP(s->sorprof.resolve(), "zq3d_get_timing");
     *out = s->tm;
     out->ms_sor = s->sorprof.ms;
     out->sor_launches = s->sorprof.launches;

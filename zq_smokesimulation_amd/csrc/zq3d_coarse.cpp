@@ -141,14 +141,14 @@ int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uin
     launch_coarse_setup(c.cd, occ, o.open_mask, c.B, s, c.flux);
     launch_coarse_restrict(c.cd, u, v, w, c.B, s);
     if (o.closed_mean) launch_coarse_remove_mean(c.cd, c.B, s);
-    if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (pf) {
+        ZQ_HIP(pf->next(&ea, &eb), where);
+        ZQ_HIP(hipEventRecord(ea, s), where);
+    }
     const float* p = launch_coarse_sor(c.cd, c.B, o.omega, o.iters, c.sweep, s);
     if (pf) {
-        ZQ_HIP(hipEventRecord(pf->b, s), where);
-        ZQ_HIP(hipEventSynchronize(pf->b), where);
-        float ms = 0.0f;
-        ZQ_HIP(hipEventElapsedTime(&ms, pf->a, pf->b), where);
-        pf->ms += ms;
+        ZQ_HIP(hipEventRecord(eb, s), where);
         pf->launches += c.sweep.k > 0 ? (o.iters + c.sweep.k - 1) / c.sweep.k : 2LL * o.iters;
     }
     launch_coarse_prolong(c.cd, c.B, p, o.open_mask, occ, u, v, w, s, c.guide, c.guide_coeff);
@@ -165,6 +165,7 @@ int project_coarse(const CoarseState& c, float* u, float* v, float* w, const uin
         st->sweeps = o.iters;
         st->max_residual = -1.0;
         st->max_div = md;
+        st->converged = -1;
         st->mean = 0.0;
     }
     return ZQ3D_OK;

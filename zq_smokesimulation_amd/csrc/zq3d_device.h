@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+#include <cmath>
+
 namespace zq3d {
 
 struct Dims {
@@ -161,33 +164,73 @@ struct Cell4Taps {
     }
 };
 
-// RegularVel with a per-thread tap cache (one instance per thread; not shared).
+// One axis of RegularVelCached's filter: the coordinate limited as in axis_setup, its floor k and
+// the weight a (the texels k and k+1 are clamped into [0, n) only when the cell's taps are loaded).
+template <class R>
+__device__ __forceinline__ int axis_floor(R x, int n, R& a) {
+    x = zmin(zmax(x, R(-1)), (R)n);
+    const R fx = zfloor(x);
+    a = x - fx;
+    return (int)fx;
+}
+
+// The 8 taps of one trilinear cell of a 4-wide grid in the form the x-lerps use: for each x pair
+// the low tap lo and the difference hi - lo, so that lerp(lo, hi, ax) = lo + ax*(hi - lo) takes its
+// subtraction once per cell instead of once per sample (the same operations on the same values:
+// identical results).  The cell is keyed by the floors (kx, ky, kz) of the filter coordinates.
+template <class R>
+struct Cell4Lerp {
+    int kx = INT_MIN, ky = INT_MIN, kz = INT_MIN;
+    R3<R> a00, d00, a10, d10, a01, d01, a11, d11;  // pairs (i0,j0,k0)-(i1,..), (i0,j1,k0), (i0,j0,k1), (i0,j1,k1)
+    __device__ __forceinline__ void fetch(const vec4_t<R>* __restrict__ g, const Dims& d, int x, int y, int z) {
+        if (x == kx && y == ky && z == kz) return;
+        kx = x, ky = y, kz = z;
+        const int i0 = clampi(x, 0, d.W - 1), i1 = clampi(x + 1, 0, d.W - 1);
+        const int j0 = clampi(y, 0, d.H - 1), j1 = clampi(y + 1, 0, d.H - 1);
+        const int k0 = clampi(z, 0, d.D - 1), k1 = clampi(z + 1, 0, d.D - 1);
+        const int r00 = (k0 * d.H + j0) * d.W, r10 = (k0 * d.H + j1) * d.W;
+        const int r01 = (k1 * d.H + j0) * d.W, r11 = (k1 * d.H + j1) * d.W;
+        const vec4_t<R> t000 = g[r00 + i0], t100 = g[r00 + i1], t010 = g[r10 + i0], t110 = g[r10 + i1];
+        const vec4_t<R> t001 = g[r01 + i0], t101 = g[r01 + i1], t011 = g[r11 + i0], t111 = g[r11 + i1];
+        auto pair = [](const vec4_t<R>& lo, const vec4_t<R>& hi, R3<R>& a, R3<R>& dd) {
+            a = R3<R>{lo.x, lo.y, lo.z};
+            dd = R3<R>{hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+        };
+        pair(t000, t100, a00, d00);
+        pair(t010, t110, a10, d10);
+        pair(t001, t101, a01, d01);
+        pair(t011, t111, a11, d11);
+    }
+};
+
+// RegularVel with a per-thread tap cache (one instance per thread; not shared): consecutive
+// backtrace positions move a small fraction of a cell, so most samples reuse the taps of the
+// previous one and load nothing.  The arithmetic is sample4's, so results are identical.
 template <class R>
 struct RegularVelCached {
     const vec4_t<R>* vel4;
     Dims d;
-    mutable Cell4Taps<R> c;
+    mutable Cell4Lerp<R> c;
     __device__ __forceinline__ R3<R> operator()(R px, R py, R pz) const {
-        int i0, i1, j0, j1, k0, k1;
         R ax, ay, az;
-        axis_setup(px - R(0.5), d.W, i0, i1, ax);
-        axis_setup(py - R(0.5), d.H, j0, j1, ay);
-        axis_setup(pz - R(0.5), d.D, k0, k1, az);
-        c.fetch(vel4, d.W, d.H, i0, i1, j0, j1, k0, k1);
+        const int x = axis_floor(px - R(0.5), d.W, ax);
+        const int y = axis_floor(py - R(0.5), d.H, ay);
+        const int z = axis_floor(pz - R(0.5), d.D, az);
+        c.fetch(vel4, d, x, y, z);
         R3<R> out;
         {
-            const R c00 = lerp(c.t000.x, c.t100.x, ax), c10 = lerp(c.t010.x, c.t110.x, ax);
-            const R c01 = lerp(c.t001.x, c.t101.x, ax), c11 = lerp(c.t011.x, c.t111.x, ax);
+            const R c00 = c.a00.x + ax * c.d00.x, c10 = c.a10.x + ax * c.d10.x;
+            const R c01 = c.a01.x + ax * c.d01.x, c11 = c.a11.x + ax * c.d11.x;
             out.x = lerp(lerp(c00, c10, ay), lerp(c01, c11, ay), az);
         }
         {
-            const R c00 = lerp(c.t000.y, c.t100.y, ax), c10 = lerp(c.t010.y, c.t110.y, ax);
-            const R c01 = lerp(c.t001.y, c.t101.y, ax), c11 = lerp(c.t011.y, c.t111.y, ax);
+            const R c00 = c.a00.y + ax * c.d00.y, c10 = c.a10.y + ax * c.d10.y;
+            const R c01 = c.a01.y + ax * c.d01.y, c11 = c.a11.y + ax * c.d11.y;
             out.y = lerp(lerp(c00, c10, ay), lerp(c01, c11, ay), az);
         }
         {
-            const R c00 = lerp(c.t000.z, c.t100.z, ax), c10 = lerp(c.t010.z, c.t110.z, ax);
-            const R c01 = lerp(c.t001.z, c.t101.z, ax), c11 = lerp(c.t011.z, c.t111.z, ax);
+            const R c00 = c.a00.z + ax * c.d00.z, c10 = c.a10.z + ax * c.d10.z;
+            const R c01 = c.a01.z + ax * c.d01.z, c11 = c.a11.z + ax * c.d11.z;
             out.z = lerp(lerp(c00, c10, ay), lerp(c01, c11, ay), az);
         }
         return out;
@@ -244,29 +287,48 @@ __device__ __forceinline__ bool trace_open_cached(const uint8_t* __restrict__ oc
     return c.open;
 }
 
+// Scaling of a backtrace displacement v*dtt into cells: (v*dtt)/h, the reference's division.  When h
+// is a power of two (voxelSize = 1/W for W = 64, 128, 256, ...) the division is exact scaling by
+// 2^k, so multiplying by inv = 1/h gives the same bits; inv = 0 selects the division.
+template <class R>
+struct StepScale {
+    R h, inv;
+    __device__ __forceinline__ R operator()(R x) const { return inv != R(0) ? x * inv : x / h; }
+};
+
+// The scale for voxel size h (host side): inv = 1/h when h is an exact power of two whose inverse
+// is a normal number of the type (then x / h == x * inv for every x), else 0.
+template <class R>
+inline StepScale<R> step_scale(float h) {
+    int e = 0;
+    const double m = std::frexp((double)h, &e);  // h = m * 2^e, m in [0.5, 1)
+    R inv = R(0);
+    if (m == 0.5 && e > -100 && e < 100) inv = (R)std::ldexp(1.0, 1 - e);
+    return StepScale<R>{(R)h, inv};
+}
+
 // Semi-Lagrangian backtrace with the reference's loop (ZQ_CUDA_Advection3D.cu:87-109): stop when
 // the point leaves [0,W]x[0,H]x[0,D] or lands in a solid cell; otherwise remember it and step
-// back by vel*dtt/h.  The end point is the last accepted position.  The test of the next point is
-// issued before its velocity sample (both may load when the point enters another cell), which the
-// reference's order of decisions allows: the test is only read at the top of the next iteration.
+// back by vel*dtt/h.  The end point is the last accepted position.  The reference samples the
+// velocity at every new point and takes `steps` steps, but the point reached by the last step is
+// never accepted (the loop ends first), so neither that step nor the velocity it uses is computed
+// here: the accepted points, hence the result, are the same.
 template <class R, class Vel>
-__device__ __forceinline__ R3<R> backtrace(const Vel& V, const uint8_t* __restrict__ occ, const Dims& d, R h,
-                                           int steps, R dtt, R px, R py, R pz) {
+__device__ __forceinline__ R3<R> backtrace(const Vel& V, const uint8_t* __restrict__ occ, const Dims& d,
+                                           const StepScale<R>& sc, int steps, R dtt, R px, R py, R pz) {
     R3<R> last{px, py, pz};
     OccCache oc;
     bool open = trace_open_cached(occ, d, px, py, pz, oc);
-    R3<R> vel = V(px, py, pz);
-    int istep = 0;
-    do {
+    for (int istep = 0; istep < steps; istep++) {
         if (!open) break;
         last = R3<R>{px, py, pz};
-        px = px - (vel.x * dtt) / h;
-        py = py - (vel.y * dtt) / h;
-        pz = pz - (vel.z * dtt) / h;
+        if (istep + 1 == steps) break;
+        const R3<R> vel = V(px, py, pz);
+        px = px - sc(vel.x * dtt);
+        py = py - sc(vel.y * dtt);
+        pz = pz - sc(vel.z * dtt);
         open = trace_open_cached(occ, d, px, py, pz, oc);
-        vel = V(px, py, pz);
-        istep++;
-    } while (istep < steps);
+    }
     return last;
 }
 

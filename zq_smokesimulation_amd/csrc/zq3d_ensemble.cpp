@@ -224,6 +224,7 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
         for (int q = 0; q < 4; q++) m.sp[q] = e->split_p + e->off_sp[s] + q * sp;
         // the single-grid launchers' host arithmetic (launch_force_atten, launch_advect_*, launch_sor)
         m.h = c.voxel_size;
+        m.inv_h = step_scale<float>(c.voxel_size).inv;
         m.dt = c.step_time;
         m.dth = c.step_time * 0.5f;
         m.two_h = 2.0f * c.voxel_size;

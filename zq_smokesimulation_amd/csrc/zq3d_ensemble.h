@@ -29,6 +29,7 @@ struct EnsMember {
     double* part;    // (sum, count) per set-up tile
     float* mean32;
     float h, dt, dth, two_h, ch, dtt;
+    float inv_h;     // 1/h when h is a power of two (backtrace steps multiply), else 0 (they divide)
     int steps;
     float beta, alpha, Tamb, aV, aT, aR;
     int open_mask, closed_mean, iters;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "zq3d_coarse.h"
 #include "zq3d_kernels.h"
@@ -57,11 +58,48 @@ struct Errs {
     }
 };
 
-// Event timing of the red-black sweep loops (null when profiling is off).
+// Event timing of the red-black sweep loops (null when profiling is off).  Every sweep loop is
+// bracketed by its own event pair from a pool; nothing waits for the events until resolve(), so
+// timing adds no synchronisation to the stream.
 struct SorProf {
-    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<hipEvent_t> ev;  // pairs (2i, 2i + 1)
+    int used = 0;                // pairs recorded since the last resolve
     double ms = 0.0;
     long long launches = 0;
+    static constexpr int kPool = 4096;
+    // the next pair (created on first use); a full pool is resolved first
+    hipError_t next(hipEvent_t* a, hipEvent_t* b) {
+        if (used == kPool)
+            if (hipError_t e = resolve()) return e;
+        if ((int)ev.size() < 2 * (used + 1)) {
+            for (int k = 0; k < 2; k++) {
+                hipEvent_t x = nullptr;
+                if (hipError_t e = hipEventCreate(&x)) return e;
+                ev.push_back(x);
+            }
+        }
+        *a = ev[2 * used];
+        *b = ev[2 * used + 1];
+        used++;
+        return hipSuccess;
+    }
+    // adds the recorded pairs' elapsed times to ms (waits for the last pair)
+    hipError_t resolve() {
+        if (used == 0) return hipSuccess;
+        if (hipError_t e = hipEventSynchronize(ev[2 * used - 1])) return e;
+        for (int i = 0; i < used; i++) {
+            float t = 0.0f;
+            if (hipError_t e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1])) return e;
+            ms += t;
+        }
+        used = 0;
+        return hipSuccess;
+    }
+    void release() {
+        for (hipEvent_t x : ev) (void)hipEventDestroy(x);
+        ev.clear();
+        used = 0;
+    }
 };
 
 // Device buffers of a projection.  p/p_alt: padded FP32 pressure (plain) or correction e (mixed),
@@ -84,11 +122,25 @@ hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const flo
                      float omega, int iters, hipStream_t s, SorProf* pf, float** result, SorShape shape,
                      const SplitBufs* split);
 int read_double_bits(const unsigned long long* dev, double* out, hipStream_t s, const char* where);
+// Divergence check of a forward projection's refinement: the faces (float, or double when fp64) the
+// correction will update in place, with its occupancy and open faces.
+struct DivCheck {
+    const void* u;
+    const void* v;
+    const void* w;
+    const uint8_t* occ;
+    int open_mask;
+    bool fp64;
+};
 // Mixed-precision refinement of B.p64: FP64 residual, FP32 SOR on the correction e (zero start),
-// p64 += e, until max|r| <= tol or o.max_outer passes.  With the colour-split sweep algorithm the
-// residual and e stay in split storage across passes (counts split once per call).
+// p64 += e, until max|r| <= tol or o.max_outer passes.  With chk, a residual within tol is not yet
+// the end: the corrected faces' max|div - mean| is predicted (the correction rounds every face to
+// the face type once more) and the passes go on while it exceeds tol and each pass still shrinks it
+// below 0.9 of the previous prediction (past that, the faces' rounding dominates); *dmax receives
+// that value, or -1 when the last pass left it unevaluated.  With the colour-split sweep algorithm the residual
+// and e stay in split storage across passes (counts split once per call).
 int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
-                 const char* where, int* outer, double* rmax);
+                 const char* where, const DivCheck* chk, int* outer, double* rmax, double* dmax);
 
 // Plain FP32 projection on device pointers: p = 0, setup, SOR, correct.  mean_mode: 0 = value
 // `mean` given, 1 = FP64 device mean over fluid cells.

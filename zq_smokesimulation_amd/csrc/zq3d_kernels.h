@@ -134,6 +134,12 @@ void launch_update_p64_split(const Dims& d, const uint8_t* nc, float* const* e, 
 template <class F>
 void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* nc, const double* mean64,
                       double* part, unsigned long long* dmax, hipStream_t s);
+// max |div64 - mean64| over counted cells of the faces launch_correct64 would produce from p64_pad
+// (equal bit for bit to launch_max_div64 after launch_correct64), faces left unchanged
+template <class F>
+void launch_max_div64_pred(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* occ, int open_mask,
+                           const uint8_t* nc, const double* p64_pad, const double* mean64, double* part,
+                           unsigned long long* dmax, hipStream_t s);
 // max |b64| over counted cells into *out (bits of a non-negative double)
 void launch_max_abs64(const Dims& d, const uint8_t* nc, const double* b64, double* part, unsigned long long* out,
                       hipStream_t s);

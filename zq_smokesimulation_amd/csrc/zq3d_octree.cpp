@@ -38,14 +38,14 @@ int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* 
     }
     ZQ_HIP(octree_build_lists(t.d, t.B, &t.lists, s), where);
     launch_octree_prepare(t.d, u, v, w, t.B, s);
-    if (pf) ZQ_HIP(hipEventRecord(pf->a, s), where);
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (pf) {
+        ZQ_HIP(pf->next(&ea, &eb), where);
+        ZQ_HIP(hipEventRecord(ea, s), where);
+    }
     const int launches = launch_octree_sweeps(t.d, t.B, t.lists, o.iters, mean, s);
     if (pf) {
-        ZQ_HIP(hipEventRecord(pf->b, s), where);
-        ZQ_HIP(hipEventSynchronize(pf->b), where);
-        float ms = 0.0f;
-        ZQ_HIP(hipEventElapsedTime(&ms, pf->a, pf->b), where);
-        pf->ms += ms;
+        ZQ_HIP(hipEventRecord(eb, s), where);
         pf->launches += launches;
     }
     launch_octree_adjust(t.d, t.B, u, v, w, s);
@@ -62,6 +62,7 @@ int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* 
         st->sweeps = o.iters;
         st->max_residual = -1.0;
         st->max_div = md;
+        st->converged = -1;
         st->mean = 0.0;
     }
     return ZQ3D_OK;

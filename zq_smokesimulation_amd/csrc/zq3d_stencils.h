@@ -172,7 +172,8 @@ __device__ __forceinline__ void st_mac_to_cell4(const R* __restrict__ u, const R
 // (adv[0..N) x, [N..2N) y, [2N..3N) z) so each face pass reads one component plane.
 template <class R>
 __device__ __forceinline__ void st_advect_cell(const vec4_t<R>* __restrict__ vel4, const uint8_t* __restrict__ occ,
-                                               R* __restrict__ adv, const Dims& d, R h, int steps, R dtt, int i, int j,
+                                               R* __restrict__ adv, const Dims& d, const StepScale<R>& h, int steps,
+                                               R dtt, int i, int j,
                                                int k) {
     if (i >= d.W || j >= d.H || k >= d.D) return;
     const RegularVelCached<R> V{vel4, d, {}};
@@ -216,7 +217,8 @@ __device__ __forceinline__ void st_cells_to_face(const R* __restrict__ adv, cons
 template <class R, class Vel>
 __device__ __forceinline__ void st_advect_scalar(const Vel& V, const uint8_t* __restrict__ occ,
                                                  const R* __restrict__ Tin, const R* __restrict__ Rin,
-                                                 R* __restrict__ Tout, R* __restrict__ Rout, const Dims& d, R h,
+                                                 R* __restrict__ Tout, R* __restrict__ Rout, const Dims& d,
+                                                 const StepScale<R>& h,
                                                  int steps, R dtt, int i, int j, int k) {
     if (i >= d.W || j >= d.H || k >= d.D) return;
     const R3<R> last = backtrace<R>(V, occ, d, h, steps, dtt, i + R(0.5), j + R(0.5), k + R(0.5));
@@ -337,10 +339,12 @@ __device__ __forceinline__ void st_sor_half(const uint8_t* __restrict__ nc, cons
 
 // Face correction u -= p_r - p_l (ZQ_SmokeSimulationUtils3D.cpp:2815-2861 open form, :2888-2922
 // closed form).  P = float (plain) or double (mixed precision; the face is rounded once).
+// st_corrected_face returns the value face (AXIS, i, j, k) holds after the correction (its own value
+// when the face is not corrected): st_correct stores it, and the mixed projection's divergence check
+// predicts the corrected field's divergence from it before anything is stored.
 template <int AXIS, class P, class F>
-__device__ __forceinline__ void st_correct(const uint8_t* __restrict__ occ, int open_mask, const P* __restrict__ p,
-                                           F* __restrict__ face, const Dims& d, int i, int j, int k) {
-    if (!face_in<AXIS>(d, i, j, k)) return;
+__device__ __forceinline__ F st_corrected_face(const uint8_t* __restrict__ occ, int open_mask, const P* __restrict__ p,
+                                               const F* __restrict__ face, const Dims& d, int i, int j, int k) {
     const int pos = AXIS == 0 ? i : (AXIS == 1 ? j : k);
     const int lim = AXIS == 0 ? d.W : (AXIS == 1 ? d.H : d.D);
     const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
@@ -351,11 +355,18 @@ __device__ __forceinline__ void st_correct(const uint8_t* __restrict__ occ, int 
     if (pos == 0) ok = (open_mask & lo_bit) && !occ[ci];
     else if (pos == lim) ok = (open_mask & (lo_bit << 1)) && !occ[ci];
     else ok = !occ[ci - step] && !occ[ci];
-    if (!ok) return;
+    const F f = face[face_index<AXIS>(d, i, j, k)];
+    if (!ok) return f;
     // padded index of the cell on the high side of the face (ghost when pos == lim)
     const int qh = pidx(d, i, j, k);
-    const int fi = face_index<AXIS>(d, i, j, k);
-    face[fi] = (F)((P)face[fi] - (p[qh] - p[qh - pstep]));
+    return (F)((P)f - (p[qh] - p[qh - pstep]));
+}
+
+template <int AXIS, class P, class F>
+__device__ __forceinline__ void st_correct(const uint8_t* __restrict__ occ, int open_mask, const P* __restrict__ p,
+                                           F* __restrict__ face, const Dims& d, int i, int j, int k) {
+    if (!face_in<AXIS>(d, i, j, k)) return;
+    face[face_index<AXIS>(d, i, j, k)] = st_corrected_face<AXIS, P, F>(occ, open_mask, p, face, d, i, j, k);
 }
 
 // ---- sources ----

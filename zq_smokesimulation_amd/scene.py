@@ -45,3 +45,21 @@ def apply_moving_objects(path, frames, u, v, w):
     if rc != 0:
         raise ValueError("moving objects of %s could not be applied" % path)
     return occ
+
+
+def read_di3(path, header_only=False):
+    """A .di3 density file (ZQ_SmokeGrid3D::ExportDensity's layout): returns (width, height, depth,
+    channels) and, unless header_only, the samples as a float32 array (depth, height, width[, channels])."""
+    dims = (C.c_int * 4)()
+    rc = _lib.lib().zq3d_di3_read(str(path).encode(), C.byref(dims), None, 0)
+    if rc != 0:
+        raise ValueError("%s is not a whole .di3 file" % path)
+    W, H, D, nch = (int(x) for x in dims)
+    if header_only:
+        return (W, H, D, nch), None
+    data = np.empty(W * H * D * nch, np.float32)
+    rc = _lib.lib().zq3d_di3_read(str(path).encode(), C.byref(dims), data.ctypes.data_as(C.c_void_p), data.size)
+    if rc != 0:
+        raise ValueError("%s could not be read" % path)
+    shape = (D, H, W) if nch == 1 else (D, H, W, nch)
+    return (W, H, D, nch), data.reshape(shape)
