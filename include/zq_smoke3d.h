@@ -193,6 +193,23 @@ float ZQ3D_SolvePoissonRedBlack_ex(float* mac_u, float* mac_v, float* mac_w, con
                                    const int height, const int depth, const zq3d_sor_opts* opts,
                                    zq3d_proj_stats* stats);
 
+/* No reference counterpart.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): ZQ3D_SolvePoissonRedBlack_ex on
+ * DEVICE pointers (faces in the reference layout, occupancy one byte per cell, nonzero = solid),
+ * stream-ordered on `stream` (a hipStream_t, NULL = the default stream), with the same kernels and
+ * results bit for bit.  It calls no hipMalloc, hipFree or allocating library routine: its scratch
+ * (counts, right-hand side, two sweep pressures, reduction partials; in the mixed mode also the FP64
+ * pressure and right-hand side) lives in one static device array of ZQ3D_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends.  That bounds the grid: about 80^3 cells in the mixed mode and
+ * 100^3 in FP32; larger grids are rejected with ZQ3D_ERR_ARG and a message giving the bytes needed.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller.  With stats == NULL and mixed_precision == 0
+ * the call only enqueues work; stats, or the mixed mode's convergence test, read results back and
+ * synchronize the stream.  Returns ZQ3D_OK or a ZQ3D_ERR_* code (message in zq3d_last_error). */
+#define ZQ3D_NOALLOC_SCRATCH_BYTES (16u << 20)
+int ZQ3D_SolvePoissonRedBlack_noalloc(float* mac_u, float* mac_v, float* mac_w, const bool* occupy,
+                                      const int width, const int height, const int depth,
+                                      const zq3d_sor_opts* opts, zq3d_proj_stats* stats, void* stream);
 
 /* No reference counterpart (extends AddForce3D): also applies the density term
  * f_y -= gravityCoeff*rho (the reference's gravityCoeff, declared unused there) and adds an

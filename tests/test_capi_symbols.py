@@ -104,6 +104,25 @@ def test_oracle_loads():
     assert zq_oracle.lib() is not None
 
 
+def test_noalloc_rejects_grids_beyond_its_scratch_without_gpu():
+    """The size check runs before any device call: a grid whose scratch exceeds the static array is
+    refused with the bytes it would need."""
+    import numpy as np
+    import zq_smokesimulation_amd._lib as L
+    host = np.zeros(8, np.float32)
+    p = host.ctypes.data_as(C.c_void_p)
+    opts = L.SorOpts(open_mask=0, closed_mean=1, omega=1.8, iters=40, mixed_precision=1, max_outer=100, tol=1e-6)
+    rc = L.lib().ZQ3D_SolvePoissonRedBlack_noalloc(p, p, p, p, 128, 128, 128, C.byref(opts), None, None)
+    assert rc == L.ERR_ARG
+    msg = L.last_error()
+    assert "bytes of scratch" in msg and str(16 << 20) in msg
+    opts.mixed_precision = 0
+    rc = L.lib().ZQ3D_SolvePoissonRedBlack_noalloc(p, p, p, p, 128, 100, 100, C.byref(opts), None, None)
+    assert rc == L.ERR_ARG
+    rc = L.lib().ZQ3D_SolvePoissonRedBlack_noalloc(p, p, p, p, 64, 64, 64, None, None, None)
+    assert rc == L.ERR_ARG and "null options" in L.last_error()
+
+
 def test_mixed_precision_needs_refinement_passes_without_gpu():
     """The default config allows refinement passes (maxOuter's default, 1000); a mixed-precision config
     with max_outer = 0, which would never project, is refused before any device call."""

@@ -145,6 +145,9 @@ def _load():
     L.SolveClosedPoissonRedBlackwithOccupy3D_MAC.restype = f
     L.ZQ3D_SolvePoissonRedBlack_ex.argtypes = [vp, vp, vp, vp, i, i, i, C.POINTER(SorOpts), C.POINTER(ProjStats)]
     L.ZQ3D_SolvePoissonRedBlack_ex.restype = f
+    L.ZQ3D_SolvePoissonRedBlack_noalloc.argtypes = [vp, vp, vp, vp, i, i, i, C.POINTER(SorOpts), C.POINTER(ProjStats),
+                                                    vp]
+    L.ZQ3D_SolvePoissonRedBlack_noalloc.restype = i
     L.ZQ3D_MACtoRegular4.argtypes = [vp, vp, vp, vp, i, i, i]
     L.ZQ3D_MACtoRegular4.restype = f
     L.SolveOpenPoissonRedBlackwithFaceRatio3D_MAC.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, i, i]
