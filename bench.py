@@ -82,6 +82,9 @@ def parse():
     ap.add_argument("--traffic", default=None,
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the sweep kernel (default: "
                          "profiles/<kernel>_<W>_traffic.txt for the kernel the autotuner picked)")
+    ap.add_argument("--cpu-c0", action="store_true",
+                    help="also time the oracle on the whole 64^3 closed-box workload (100 steps, plain SOR omega "
+                         "1.8 x 200) on one host thread and on all of them (about a minute)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region run a plain 512 MiB device copy (3 launches) whose counter "
                          "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
@@ -155,6 +158,36 @@ def cpu_baseline(state, W, H, D, inner, outer_mean, omega):
                    "(%.2f s) + %d SOR iterations (%.3f s); step time = %.2f + %.1f passes (GPU mean per step) x %.3f = "
                    "%.2f s" % (W, H, D, t_fixed, inner, t_pass, t_fixed, outer_mean, t_pass, step_s)),
     }
+
+
+def cpu_c0():
+    """The oracle on the whole 64^3 closed-box workload: closed walls, zero initial velocity, sphere
+    source (r 0.08 at (0.5, 0.1, 0.5), density 1, temperature 1), seeded +-0.01 temperature noise,
+    alpha 0.05, beta 0.5, eps 0.3, dt 0.02, 20 substeps, red-black SOR omega 1.8 x 200 with the
+    closed-domain mean, 100 steps; timed on one thread and on every host thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import zq_oracle as O
+    W = 64
+    p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=1.0, aT=1.0, aR=1.0,
+             adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1, omega=1.8, sor_iters=200, mixed=0,
+             max_outer=0, tol=1e-6)
+    out = {}
+    for label, threads in (("one_thread", 1), ("all_threads", 0)):
+        O.lib().orc_set_num_threads(threads)
+        occ, u, v, w = O.init_fine_grid(W, W, W, O.ALL_FACES, 0)
+        T = O.noise(W, W, W, 0, 0.01)
+        R = np.zeros_like(T)
+        t0 = time.perf_counter()
+        for _ in range(100):
+            O.step(p, u, v, w, T, R, occ, spheres=[SOURCE])
+        wall = time.perf_counter() - t0
+        out[label] = {"value": round(W ** 3 * 100 / wall / 1e6, 4), "unit": "Mvoxel-steps/s",
+                      "cores": int(O.lib().orc_num_threads()), "seconds": round(wall, 2),
+                      "density_total": float(R.astype(np.float64).sum())}
+    O.lib().orc_set_num_threads(0)
+    out["kind"] = "port"
+    out["sample"] = "the whole 64^3 closed-box workload, 100 steps, plain SOR omega 1.8 x 200"
+    return out
 
 
 def ensemble_leg(args):
@@ -584,6 +617,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean, omega)
     else:
         out["cpu_baseline"] = None
+    if args.cpu_c0:
+        print("bench: cpu baseline, 64^3 workload", file=sys.stderr, flush=True)
+        out["cpu_baseline_64"] = cpu_c0()
     print(json.dumps(out))
 
 

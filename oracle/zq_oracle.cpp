@@ -1041,6 +1041,15 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* threads of the following calls (n <= 0: the OpenMP default) */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+#else
+    (void)n;
+#endif
+}
+
 void orc_noise(int W, int H, int D, uint32_t seed, float amp, float* out) {
     const size_t N = (size_t)W * H * D;
     for (size_t o = 0; o < N; o++) {

@@ -130,6 +130,7 @@ void orc_stamp_sphere(int W, int H, int D, float cx, float cy, float cz, float r
                       int shell_mask, uint8_t* occ, float* u, float* v, float* w);
 /* OpenMP threads the oracle's parallel loops use */
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 /* counter-based hash noise in [-amp, amp] per cell (identical in the product) */
 void orc_noise(int W, int H, int D, uint32_t seed, float amp, float* out);
 

@@ -102,6 +102,8 @@ def load(path):
     L.orc_noise.argtypes = [i, i, i, u32, f, _f32p]
     L.orc_num_threads.argtypes = []
     L.orc_num_threads.restype = i
+    L.orc_set_num_threads.argtypes = [i]
+    L.orc_set_num_threads.restype = None
     L.orc_step.argtypes = [C.POINTER(StepParams), _f32p, _f32p, _f32p, _f32p, _f32p, _u8p, vp, vp, vp]
     L.orc_step.restype = i
     L.orc_coarse_setup.argtypes = [i, i, i, i, _u8p, _f32p, _f32p, _f32p, _u8p, _u8p]

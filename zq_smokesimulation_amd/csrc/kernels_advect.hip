@@ -24,8 +24,9 @@ inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (
 template <class R>
 __global__ __launch_bounds__(BX* BY) void k_mac_to_cell4(const R* __restrict__ u, const R* __restrict__ v,
                                                          const R* __restrict__ w, vec4_t<R>* __restrict__ vel4,
-                                                         Dims d) {
-    st_mac_to_cell4<R>(u, v, w, vel4, d, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
+                                                         Dims d, const uint8_t* __restrict__ occ) {
+    st_mac_to_cell4<R>(u, v, w, vel4, d, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z,
+                       occ);
 }
 
 template <class R>
@@ -151,8 +152,9 @@ __global__ __launch_bounds__(BX* BY) void k_advect_scalar(Vel V, const uint8_t* 
 }  // namespace
 
 template <class R>
-void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vel4, hipStream_t s) {
-    hipLaunchKernelGGL(k_mac_to_cell4<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, vel4, d);
+void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vel4, hipStream_t s,
+                         const uint8_t* occ) {
+    hipLaunchKernelGGL(k_mac_to_cell4<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, vel4, d, occ);
 }
 
 size_t bfecc_scratch_values(const Dims& d) {
@@ -206,7 +208,7 @@ void advect_velocity_bfecc(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, 
     if (mode == 1) {  // inReg_outReg: vel4 = phi; A = adv(phi; v); B = adv(A; -v); B = phi~; adv(phi~; v) -> SoA
         V4* A = (V4*)bf;
         V4* B = (V4*)bf + N;
-        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s, occ);
         const RegularVelCached<R> Vf{vel4, d, {}};
         const NegVel<R, RegularVelCached<R>> Vb{Vf};
         cells_in<R, false>(d, Vf, RegularVel<R>{vel4, d}, occ, A, ap, s);
@@ -234,7 +236,7 @@ void advect_velocity_bfecc(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, 
         faces_in<R>(d, MacVel<R>{K, K + Nu, K + Nu + Nv, d}, MacVel<R>{u, v, w, d}, occ, ou, ov, ow, ap, s);
     } else {  // mode 5, inReg_outMAC: F = adv(phi; v); C = cells(F); F = adv(C; -v); C = cells(F); C = phi~; F = adv(C; v)
         V4* C = (V4*)bf;
-        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s, occ);
         const RegularVelCached<R> Vf{vel4, d, {}};
         faces_in<R>(d, Vf, RegularVel<R>{vel4, d}, occ, ou, ov, ow, ap, s);
         launch_mac_to_cell4<R>(d, ou, ov, ow, C, s);
@@ -265,7 +267,7 @@ void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ,
     const StepScale<R> h = step_scale<R>(ap.h);
     const R dtt = (R)ap.dtt;
     if (mode == 0) {
-        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s, occ);
         R* adv = scratch;  // 3N values <= Nu+Nv+Nw
         hipLaunchKernelGGL(k_advect_cells<R>, grid_for(d.W, d.H, d.D), blk, 0, s, vel4, occ, adv, d, h, ap.steps, dtt);
         hipLaunchKernelGGL((k_cells_to_faces_apply<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
@@ -277,7 +279,7 @@ void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ,
         return;
     }
     if (mode == 4) {
-        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s, occ);
         const RegularVelCached<R> V{vel4, d, {}};
         hipLaunchKernelGGL((k_advect_faces<R, 0, RegularVelCached<R>>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, V, occ,
                            ou, d, h, ap.steps, dtt);
@@ -311,7 +313,7 @@ void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, con
         hipLaunchKernelGGL((k_advect_scalar<R, MacVel<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin, Rin, Tout,
                            Rout, d, h, ap.steps, dtt);
     } else {
-        launch_mac_to_cell4<R>(d, u, v, w, vel4, s);
+        launch_mac_to_cell4<R>(d, u, v, w, vel4, s, occ);
         const RegularVelCached<R> V{vel4, d, {}};
         hipLaunchKernelGGL((k_advect_scalar<R, RegularVelCached<R>>), grid_for(d.W, d.H, d.D), blk, 0, s, V, occ, Tin,
                            Rin, Tout, Rout, d, h, ap.steps, dtt);
@@ -319,7 +321,8 @@ void launch_advect_scalar(const Dims& d, const R* u, const R* v, const R* w, con
 }
 
 #define ZQ3D_ADVECT_INSTANTIATE(R)                                                                                     \
-    template void launch_mac_to_cell4<R>(const Dims&, const R*, const R*, const R*, vec4_t<R>*, hipStream_t);          \
+    template void launch_mac_to_cell4<R>(const Dims&, const R*, const R*, const R*, vec4_t<R>*, hipStream_t,           \
+                                         const uint8_t*);                                                              \
     template void launch_advect_velocity<R>(const Dims&, R*, R*, R*, const uint8_t*, int, int, const AdvParams&,       \
                                             vec4_t<R>*, R*, hipStream_t, R*);                                          \
     template void launch_advect_scalar<R>(const Dims&, const R*, const R*, const R*, const uint8_t*, const R*,         \

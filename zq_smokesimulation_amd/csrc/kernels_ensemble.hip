@@ -71,7 +71,7 @@ __global__ __launch_bounds__(NT) void ke_force_faces(const EnsMember* __restrict
 __global__ __launch_bounds__(NT) void ke_mac_to_cell4(const EnsMember* __restrict__ M, EnsTable T, int B) {
     for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
         const EnsMember& m = M[s];
-        st_mac_to_cell4<float>(m.u, m.v, m.w, m.vel4, m.d, i, j, k);
+        st_mac_to_cell4<float>(m.u, m.v, m.w, m.vel4, m.d, i, j, k, m.occ);
     });
 }
 

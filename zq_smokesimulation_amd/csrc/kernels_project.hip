@@ -401,12 +401,25 @@ __global__ __launch_bounds__(64 * NW) void k_sor_split_rb2(const uint8_t* __rest
 }
 
 // Face correction u -= p_r - p_l (ZQ_SmokeSimulationUtils3D.cpp:2815-2861 open form, :2888-2922
-// closed form).  P = float (plain) or double (mixed precision; the face is rounded once).
-template <int AXIS, class P, class F>
-__global__ __launch_bounds__(BX* BY) void k_correct(const uint8_t* __restrict__ occ, int open_mask,
-                                                    const P* __restrict__ p, F* __restrict__ face, Dims d) {
-    st_correct<AXIS, P, F>(occ, open_mask, p, face, d, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
-                           blockIdx.z);
+// closed form); P = float (plain) or double (mixed precision; the face is rounded once).
+// All three axes in one pass over the cells: cell (i, j, k) corrects its
+// low u, v and w faces, and the cells of the last column / row / plane also the domain's high face.
+// Every face is written once, with st_correct's value, so the result equals the three per-axis
+// launches bit for bit while each pressure value is loaded once instead of three times.
+template <class P, class F>
+__global__ __launch_bounds__(BX* BY) void k_correct_cells(const uint8_t* __restrict__ occ, int open_mask,
+                                                          const P* __restrict__ p, F* __restrict__ u,
+                                                          F* __restrict__ v, F* __restrict__ w, Dims d) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H) return;
+    st_correct<0, P, F>(occ, open_mask, p, u, d, i, j, k);
+    if (i == d.W - 1) st_correct<0, P, F>(occ, open_mask, p, u, d, i + 1, j, k);
+    st_correct<1, P, F>(occ, open_mask, p, v, d, i, j, k);
+    if (j == d.H - 1) st_correct<1, P, F>(occ, open_mask, p, v, d, i, j + 1, k);
+    st_correct<2, P, F>(occ, open_mask, p, w, d, i, j, k);
+    if (k == d.D - 1) st_correct<2, P, F>(occ, open_mask, p, w, d, i, j, k + 1);
 }
 
 // Transposed correction: every face, solid and ghost pressures being 0 (the adjoint of the
@@ -869,23 +882,15 @@ int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
 
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s) {
-    hipLaunchKernelGGL((k_correct<0, float, float>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
-                       p_pad, u, d);
-    hipLaunchKernelGGL((k_correct<1, float, float>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, occ, open_mask,
-                       p_pad, v, d);
-    hipLaunchKernelGGL((k_correct<2, float, float>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, occ, open_mask,
-                       p_pad, w, d);
+    hipLaunchKernelGGL((k_correct_cells<float, float>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
+                       p_pad, u, v, w, d);
 }
 
 template <class F>
 void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, F* u, F* v, F* w,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_correct<0, double, F>), grid_for(d.W + 1, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
-                       p64_pad, u, d);
-    hipLaunchKernelGGL((k_correct<1, double, F>), grid_for(d.W, d.H + 1, d.D), dim3(BX, BY), 0, s, occ, open_mask,
-                       p64_pad, v, d);
-    hipLaunchKernelGGL((k_correct<2, double, F>), grid_for(d.W, d.H, d.D + 1), dim3(BX, BY), 0, s, occ, open_mask,
-                       p64_pad, w, d);
+    hipLaunchKernelGGL((k_correct_cells<double, F>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
+                       p64_pad, u, v, w, d);
 }
 
 template <class F>

@@ -51,8 +51,8 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
     double tol = o.tol;
     if (adjoint || bmax_out) {
         double bmax = 0.0;
-        launch_max_abs64(d, B.nc, B.b64, B.part, &sc->dmax, s);
-        if (int rc = read_double_bits(&sc->dmax, &bmax, s, where)) return rc;
+        launch_max_abs64(d, B.nc, B.b64, B.part, B.target(1, &sc->dmax), s);
+        if (int rc = read_result(B, 1, &sc->dmax, &bmax, s, where)) return rc;
         if (adjoint) tol = rtol * bmax;
         if (bmax_out) *bmax_out = bmax;
     }
@@ -70,8 +70,8 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
     } else {
         launch_correct64<F>(d, occ, o.open_mask, B.p64, u, v, w, s);
         if (md < 0.0) {  // the passes ran out before the residual met tol: measure the stored faces
-            launch_max_div64<F>(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
-            if (int rc = read_double_bits(&sc->dmax, &md, s, where)) return rc;
+            launch_max_div64<F>(d, u, v, w, B.nc, &sc->mean64, B.part, B.target(1, &sc->dmax), s);
+            if (int rc = read_result(B, 1, &sc->dmax, &md, s, where)) return rc;
         }
     }
     ZQ_HIP(hipGetLastError(), where);
@@ -110,6 +110,7 @@ struct zq3d_adj {
     uint8_t* split_n;
     double *b64, *p64, *p64_adj, *part;
     Scalars* sc;
+    Readback rb;  // pinned read-back slots of the projections' reductions
     // checkpoints: ncp x (u, v, w, T, rho values + p64)
     char* cp;
     size_t cp_bytes;
@@ -146,7 +147,7 @@ ProjBufs proj_bufs(zq3d_adj* a, double* p64) {
                           {a->split_b, a->split_b + sc},
                           {a->split_n, a->split_n + sc},
                           {a->split_p + 2 * sp, a->split_p + 3 * sp}};
-    return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, a->shape, split};
+    return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, a->shape, split, a->rb};
 }
 
 // pointers into one tape slot
@@ -443,6 +444,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     A((void**)&a->p640, z.Np * 8);
     A((void**)&a->part, (size_t)reduction_partials(a->d) * 8);
     A((void**)&a->sc, sizeof(Scalars));
+    bad |= a->rb.alloc() != hipSuccess;
     a->cp_bytes = (faces + 2 * z.N) * e + z.Np * 8;
     a->cp_bytes = (a->cp_bytes + 255) & ~(size_t)255;
     A((void**)&a->cp, a->cp_bytes * a->ncp);
@@ -482,6 +484,7 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
 extern "C" void zq3d_adj_destroy(zq3d_adj* a) {
     if (!a) return;
     for (void* p : a->allocs) (void)hipFree(p);
+    a->rb.release();
     delete a;
 }
 

@@ -154,21 +154,23 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
     double rmax = 0.0, dmax = -1.0, dprev = -1.0;
     for (;;) {
         if (split)
-            launch_residual64_split(d, B.nc, B.b64, &sc->mean64, B.p64, B.split, B.part, &sc->rmax, s);
+            launch_residual64_split(d, B.nc, B.b64, &sc->mean64, B.p64, B.split, B.part, B.target(0, &sc->rmax), s);
         else
-            launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, &sc->rmax, s);
-        if (int rc = read_double_bits(&sc->rmax, &rmax, s, where)) return rc;
+            launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, B.target(0, &sc->rmax), s);
+        if (int rc = read_result(B, 0, &sc->rmax, &rmax, s, where)) return rc;
         if (!(rmax > tol)) {
             if (!chk) break;
             // the residual is the divergence of the FP64-corrected faces; the correction rounds each
             // face to FP32 once more, so check the divergence the stored faces will have
             if (chk->fp64)
                 launch_max_div64_pred(d, (const double*)chk->u, (const double*)chk->v, (const double*)chk->w,
-                                      chk->occ, chk->open_mask, B.nc, B.p64, &sc->mean64, B.part, &sc->dmax, s);
+                                      chk->occ, chk->open_mask, B.nc, B.p64, &sc->mean64, B.part,
+                                      B.target(1, &sc->dmax), s);
             else
                 launch_max_div64_pred(d, (const float*)chk->u, (const float*)chk->v, (const float*)chk->w,
-                                      chk->occ, chk->open_mask, B.nc, B.p64, &sc->mean64, B.part, &sc->dmax, s);
-            if (int rc = read_double_bits(&sc->dmax, &dmax, s, where)) return rc;
+                                      chk->occ, chk->open_mask, B.nc, B.p64, &sc->mean64, B.part,
+                                      B.target(1, &sc->dmax), s);
+            if (int rc = read_result(B, 1, &sc->dmax, &dmax, s, where)) return rc;
             if (!(dmax > tol)) break;
             // what is left is the faces' FP32 rounding once a pass no longer shrinks it by a tenth:
             // the bound is out of reach at this velocity scale (reported, not looped on)
@@ -195,6 +197,15 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
     *outer_out = outer;
     *rmax_out = rmax;
     if (dmax_out) *dmax_out = dmax;
+    return ZQ3D_OK;
+}
+
+int read_result(const ProjBufs& B, int slot, const unsigned long long* dev_scalar, double* out, hipStream_t s,
+                const char* where) {
+    if (!B.rb.host) return read_double_bits(dev_scalar, out, s, where);
+    ZQ_HIP(hipStreamSynchronize(s), where);
+    const unsigned long long bits = ((volatile unsigned long long*)B.rb.host)[slot];
+    std::memcpy(out, &bits, sizeof(*out));
     return ZQ3D_OK;
 }
 
@@ -227,9 +238,9 @@ int project_plain(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
            "project");
     launch_correct(d, occ, o.open_mask, res, u, v, w, s);
     if (st) {
-        launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+        launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, B.target(1, &sc->dmax), s);
         double md = 0.0, mean64 = 0.0;
-        if (int rc = read_double_bits(&sc->dmax, &md, s, "project")) return rc;
+        if (int rc = read_result(B, 1, &sc->dmax, &md, s, "project")) return rc;
         ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project");
         ZQ_HIP(hipStreamSynchronize(s), "project");
         st->outer = 0;
@@ -258,8 +269,8 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
     launch_correct64(d, occ, o.open_mask, B.p64, u, v, w, s);
     if (!st) return ZQ3D_OK;
     if (md < 0.0) {  // the passes ran out before the residual met tol: measure the stored faces
-        launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
-        if (int rc = read_double_bits(&sc->dmax, &md, s, "project_mixed")) return rc;
+        launch_max_div64(d, u, v, w, B.nc, &sc->mean64, B.part, B.target(1, &sc->dmax), s);
+        if (int rc = read_result(B, 1, &sc->dmax, &md, s, "project_mixed")) return rc;
     }
     double mean64 = 0.0;
     ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project_mixed");
@@ -673,6 +684,7 @@ struct zq3d_sim {
     SorProf sorprof;
     zq3d_timing tm{};
     SorShape shape{0, 0};  // fused sweep tile shape chosen by zq3d_autotune_sor / zq3d_set_sor_shape
+    Readback rb;           // pinned read-back slots of the projection's reductions
     CoarseState coarse;    // sub_size > 0: the coarse grid of the coarse + sub-grid projection
     OctreeState octree;    // octree != 0: the leaves and level arrays of the octree projection
 };
@@ -705,7 +717,7 @@ ProjBufs proj_bufs(zq3d_sim* s) {
                           {s->split_b, s->split_b + sc},
                           {s->split_n, s->split_n + sc},
                           {s->split_p + 2 * sp, s->split_p + 3 * sp}};
-    return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc, s->shape, split};
+    return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc, s->shape, split, s->rb};
 }
 
 // Times one stage of a simulation with an event pair when profiling is on.
@@ -836,6 +848,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     A(dalloc(&s->split_b, 2 * split_cell_size(s->d) * 4));
     A(dalloc(&s->split_n, 2 * split_cell_size(s->d)));
     A(dalloc(&s->sc, sizeof(Scalars)));
+    A(s->rb.alloc());
     if (cfg->sor.mixed_precision) {
         A(dalloc(&s->b64, z.N * 8));
         A(dalloc(&s->p64, z.Np * 8));
@@ -891,6 +904,7 @@ extern "C" void zq3d_destroy(zq3d_sim* s) {
     for (hipEvent_t e : {s->ev[0], s->ev[1]})
         if (e) (void)hipEventDestroy(e);
     s->sorprof.release();
+    s->rb.release();
     coarse_destroy(&s->coarse);
     octree_destroy(&s->octree);
     delete s;

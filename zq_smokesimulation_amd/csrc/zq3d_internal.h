@@ -102,6 +102,25 @@ struct SorProf {
     }
 };
 
+// Pinned, device-visible host slots the reduction kernels write the scalars the host reads back
+// (slot 0: max|r| of a refinement pass, slot 1: a max|div| or max|rhs|), read after a stream
+// synchronisation with no copy; null when the owner has none (then the device scalars in Scalars
+// are copied back).
+struct Readback {
+    unsigned long long* host = nullptr;
+    unsigned long long* dev = nullptr;
+    hipError_t alloc() {
+        if (hipError_t e = hipHostMalloc((void**)&host, 4 * sizeof(unsigned long long), hipHostMallocMapped |
+                                         hipHostMallocCoherent))
+            return e;
+        return hipHostGetDevicePointer((void**)&dev, host, 0);
+    }
+    void release() {
+        if (host) (void)hipHostFree(host);
+        host = dev = nullptr;
+    }
+};
+
 // Device buffers of a projection.  p/p_alt: padded FP32 pressure (plain) or correction e (mixed),
 // ping-ponged by the fused sweep kernel; both keep a zero ghost layer.  part: reduction partials.
 struct ProjBufs {
@@ -115,7 +134,16 @@ struct ProjBufs {
     Scalars* sc;
     SorShape shape;  // sweep algorithm / tile shape ({0, 0}: default)
     SplitBufs split; // colour-split scratch (used when shape.ty == SOR_SPLIT)
+    Readback rb{};   // host read-back slots (optional)
+    // where a reduction result of `slot` goes: the pinned slot, or the device scalar
+    unsigned long long* target(int slot, unsigned long long* dev_scalar) const {
+        return rb.dev ? rb.dev + slot : dev_scalar;
+    }
 };
+
+// the value a reduction left in `slot` (pinned) or in dev_scalar (copied back); waits for the stream
+int read_result(const ProjBufs& B, int slot, const unsigned long long* dev_scalar, double* out, hipStream_t s,
+                const char* where);
 
 // launch_sor, timed when pf is set (the sweeps are the only work between the two events)
 hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const float* mean32, float* p, float* p_alt,

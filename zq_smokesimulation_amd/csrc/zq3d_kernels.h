@@ -30,8 +30,10 @@ void launch_attenuation(const Dims& d, float* u, float* v, float* w, float* T, f
                         float aV, float aT, float aR, hipStream_t s);
 
 // ---- MAC <-> cell (ZQ_CUDA_MACtoRegular.cu) ----
+// with occ, the fourth component holds each cell's occupancy (the grids the forward backtrace samples)
 template <class R>
-void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vel4, hipStream_t s);
+void launch_mac_to_cell4(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vel4, hipStream_t s,
+                         const uint8_t* occ = nullptr);
 
 // ---- advection (ZQ_CUDA_Advection3D.cu) ----
 struct AdvParams {

@@ -55,9 +55,9 @@ int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* 
         Scalars* sc = B.sc;
         launch_fill_f64(&sc->mean64, 1, 0.0, s);
         launch_proj_setup(t.d, u, v, w, occ, ZQ3D_OPEN_ALL, B.nc, B.b, nullptr, B.p, nullptr, nullptr, s);
-        launch_max_div64(t.d, u, v, w, B.nc, &sc->mean64, B.part, &sc->dmax, s);
+        launch_max_div64(t.d, u, v, w, B.nc, &sc->mean64, B.part, B.target(1, &sc->dmax), s);
         double md = 0.0;
-        if (int rc = read_double_bits(&sc->dmax, &md, s, where)) return rc;
+        if (int rc = read_result(B, 1, &sc->dmax, &md, s, where)) return rc;
         st->outer = 0;
         st->sweeps = o.iters;
         st->max_residual = -1.0;

@@ -154,16 +154,18 @@ __device__ __forceinline__ void st_force_face(R* __restrict__ face, const uint8_
 // ---- advection (ZQ_CUDA_Advection3D.cu, ZQ_CUDA_MACtoRegular.cu) ----
 
 // MAC_to_Regular4_Kernel (ZQ_CUDA_MACtoRegular.cu:9-30)
+// (the fourth component is 0, or with occ the cell's occupancy, which the forward backtrace reads
+// from the taps it samples)
 template <class R>
 __device__ __forceinline__ void st_mac_to_cell4(const R* __restrict__ u, const R* __restrict__ v,
                                                 const R* __restrict__ w, vec4_t<R>* __restrict__ vel4, const Dims& d,
-                                                int i, int j, int k) {
+                                                int i, int j, int k, const uint8_t* __restrict__ occ = nullptr) {
     if (i >= d.W || j >= d.H || k >= d.D) return;
     vec4_t<R> c;
     c.x = R(0.5) * (u[uidx(d, i, j, k)] + u[uidx(d, i + 1, j, k)]);
     c.y = R(0.5) * (v[vidx(d, i, j, k)] + v[vidx(d, i, j + 1, k)]);
     c.z = R(0.5) * (w[widx(d, i, j, k)] + w[widx(d, i, j, k + 1)]);
-    c.w = R(0);
+    c.w = (occ && occ[cidx(d, i, j, k)]) ? R(1) : R(0);
     vel4[cidx(d, i, j, k)] = c;
 }
 
