@@ -62,12 +62,21 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5,
                     help="untimed steps first (the state is then reset to the initial one)")
     ap.add_argument("--size", type=int, default=256, help="grid edge (cube)")
-    ap.add_argument("--inner", type=int, default=40, help="FP32 SOR sweeps per refinement pass")
+    ap.add_argument("--inner", type=int, default=None,
+                    help="FP32 SOR sweeps per refinement pass (at most, with --pass-rule 1 or 2); default 80, or "
+                         "1000 with --pass-rule 2")
     ap.add_argument("--omega", type=float, default=None,
                     help="SOR relaxation (default: the Poisson-optimal 2/(1+sin(pi/n)) for the grid edge n)")
+    ap.add_argument("--full-run", type=int, default=FULL_RUN,
+                    help="steps of the whole run: after the timed steps the run continues to this step on one clock "
+                         "(reported as full_run; a value <= --steps adds nothing)")
     ap.add_argument("--profile-steps", type=int, default=10,
                     help="steps after the timed region run with per-stage event timing (the stage breakdown)")
     ap.add_argument("--max-outer", type=int, default=1000, help="refinement passes at most per step")
+    ap.add_argument("--pass-rule", type=int, default=1, choices=(0, 1, 2),
+                    help="refinement pass length (zq3d_sor_opts.pass_rule): 0 every pass runs --inner sweeps; "
+                         "1 (default) fitted from the last pass's residual decay; 2 gated: a pass sweeps until an "
+                         "iteration's largest update meets the tolerance (at most --inner sweeps)")
     ap.add_argument("--tol", type=float, default=1e-6, help="max|div| target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
@@ -88,7 +97,10 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region run a plain 512 MiB device copy (3 launches) whose counter "
                          "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.inner is None:
+        args.inner = 1000 if args.pass_rule == 2 else 80
+    return args
 
 
 def make_state(W, H, D):
@@ -127,12 +139,12 @@ def read_traffic(path):
     return vals
 
 
-def cpu_baseline(state, W, H, D, inner, outer_mean, omega):
+def cpu_baseline(state, W, H, D, inner, sweeps_mean, omega):
     """The CPU oracle on a bounded sample of one step, on the GPU's own state:
     (1) a whole step whose projection stops after its first FP64 residual (no refinement pass),
-    (2) `inner` red-black SOR iterations of the oracle on the same grid (one refinement pass's sweeps);
-    step time = (1) + (GPU's mean passes per step) x (2).  The passes' residual/update work is left out
-    of (2), which favours the CPU slightly."""
+    (2) `inner` red-black SOR iterations of the oracle on the same grid;
+    step time = (1) + (GPU's mean sweeps per step) x (2) / inner.  The passes' residual/update work is
+    left out, which favours the CPU slightly."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import zq_oracle as O
     p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=0.99, aT=0.99, aR=0.99,
@@ -148,15 +160,16 @@ def cpu_baseline(state, W, H, D, inner, outer_mean, omega):
     t0 = time.perf_counter()
     O.sor(nc, rhs, e, omega, inner)
     t_pass = time.perf_counter() - t0
-    step_s = t_fixed + outer_mean * t_pass
+    step_s = t_fixed + sweeps_mean * t_pass / inner
     return {
         "value": (W * H * D) / step_s / 1e6,
         "unit": "Mvoxel-steps/s",
         "cores": int(O.lib().orc_num_threads()),
         "kind": "port",
         "sample": ("oracle at %dx%dx%d on the GPU's state after the timed steps: one step with no refinement pass "
-                   "(%.2f s) + %d SOR iterations (%.3f s); step time = %.2f + %.1f passes (GPU mean per step) x %.3f = "
-                   "%.2f s" % (W, H, D, t_fixed, inner, t_pass, t_fixed, outer_mean, t_pass, step_s)),
+                   "(%.2f s) + %d SOR iterations (%.3f s); step time = %.2f + %.1f sweeps (GPU mean per step) x "
+                   "%.3f / %d = %.2f s" % (W, H, D, t_fixed, inner, t_pass, t_fixed, sweeps_mean, t_pass, inner,
+                                           step_s)),
     }
 
 
@@ -340,7 +353,8 @@ def adjoint_window(state, W, H, D, args, fwd_ms_per_step, shape, omega):
     eng = SmokeRollout(W, H, D, n, K, dtype=torch.float32, voxel_size=1.0 / W, step_time=0.02, steps=20, Tamb=0.0,
                        buoyancy=0.5, gravity=0.05, confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99,
                        sor_open_mask=0, sor_closed_mean=1, sor_omega=omega, sor_iters=args.inner,
-                       sor_max_outer=args.max_outer, sor_tol=args.tol, warm_start=1)
+                       sor_max_outer=args.max_outer, sor_tol=args.tol, sor_pass_rule=args.pass_rule,
+                       warm_start=1)
     if shape is not None:
         eng.set_sor_shape(shape["ty"], shape["zc"])
     eng.set_initial(**{k: torch.from_numpy(state[k]) for k in ("u", "v", "w", "T", "rho", "occ", "p64")})
@@ -436,7 +450,8 @@ def main():
     sim = z.SmokeSim3D(W, H, D, voxel_size=1.0 / W, step_time=0.02, steps=20, Tamb=0.0, buoyancy=0.5, gravity=0.05,
                        confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99, sor_open_mask=0,
                        sor_closed_mean=1, sor_omega=omega, sor_iters=args.inner, sor_mixed_precision=1,
-                       sor_max_outer=args.max_outer, sor_tol=args.tol, warm_start=1)
+                       sor_max_outer=args.max_outer, sor_tol=args.tol, sor_pass_rule=args.pass_rule,
+                       warm_start=1)
     sim.set_sphere_sources([SOURCE])
     st0 = make_state(W, H, D)
     p64_zero = np.zeros((D + 2, H + 2, W + 2), np.float64)
@@ -485,7 +500,7 @@ def main():
     # same clock, so the whole run's rate is reported beside the timed window's
     full_stats = list(stats)
     sim.set_profiling(0)
-    for i in range(args.steps, max(args.steps, FULL_RUN)):
+    for i in range(args.steps, max(args.steps, args.full_run)):
         full_stats.append(sim.step())
         if i % 10 == 0:
             sim.mac_to_regular()
@@ -567,6 +582,7 @@ def main():
             "sor_omega": round(omega, 6),
             "sor_omega_rule": "given" if args.omega is not None else "2/(1+sin(pi/n)), n = %d" % max(W, H, D),
             "sor_sweeps_per_pass": args.inner,
+            "sor_pass_rule": args.pass_rule,
             "sor_tile_autotuned": best,
             "parallelism": "single GPU",
         },
@@ -614,7 +630,7 @@ def main():
         out["octree_projection"] = octree_leg(args)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(g, W, H, D, args.inner, outer_mean, omega)
+        out["cpu_baseline"] = cpu_baseline(g, W, H, D, 40, sweeps_mean, omega)
     else:
         out["cpu_baseline"] = None
     if args.cpu_c0:

@@ -175,11 +175,18 @@ typedef struct zq3d_sor_opts {
     int max_outer;        /* mixed: refinement steps at most (>= 1) */
     double tol;           /* mixed: stop once max|r| <= tol and the corrected faces (each rounded to FP32
                              once) have max|div - mean| <= tol */
+    int pass_rule;        /* mixed: how long a refinement pass runs.  0: `iters` sweeps.  1: fitted, once a
+                             pass has lowered max|r| (still above tol) the next runs only the sweeps that
+                             pass's per-sweep decay needs to reach tol, doubled (1 .. iters).  2: gated,
+                             each pass sweeps until an iteration's largest update |new - old| is at most
+                             0.75 tol omega / 6, or is no smaller than 16 iterations earlier (at most
+                             `iters` <= 8192 sweeps); the iterations run back to back on the device,
+                             which skips the rest of the pass */
 } zq3d_sor_opts;
 
 typedef struct zq3d_proj_stats {
     int outer;            /* refinement steps run (mixed) */
-    int sweeps;           /* red-black iterations run in total */
+    int sweeps;           /* red-black iterations run in total (over all passes) */
     double max_residual;  /* mixed: final max|r| (FP64) */
     double max_div;       /* max |div - mean| over counted cells after the correction, FP64 */
     double mean;          /* subtracted mean divergence */

@@ -609,16 +609,39 @@ void orc_sor_count(int W, int H, int D, const uint8_t* occ, int open_mask, uint8
  * cells hold 0 so they add nothing), pg = (s - b) * (1/n), p = (1-omega)*p + omega*pg.
  * With omega = 1 this is the Gauss-Seidel update p = (sum_nb p - div)/#nb of the in-tree analog
  * CPU_PoissonSolver3D_OpenOctreePoisson.cpp:416-438. */
-void orc_sor(int W, int H, int D, const uint8_t* ncount, const float* b, float* p, float omega, int iters) {
+}  // extern "C"
+
+namespace {
+
+uint32_t f32_bits(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    return u;
+}
+
+/* Red-black SOR iterations (the one loop both entry points run).  gated: before iteration it >= 1,
+ * stop when the largest |new - old| of iteration it-1 (FP32 bits) is <= thresh_bits, or, past
+ * iteration 2K, when it is no smaller than that of iteration it-1-K (K = 16: the updates have
+ * stopped shrinking -- FP32 rounding of the iterate bounds them from below). */
+constexpr int GATE_K = 16;
+int sor_iterations(int W, int H, int D, const uint8_t* ncount, const float* b, float* p, float omega, int iters,
+                   bool gated, uint32_t thresh_bits) {
     Dims d{W, H, D};
     const size_t sx = 1, sy = (size_t)(W + 2), sz = (size_t)(W + 2) * (H + 2);
     float rcp[7];
     rcp[0] = 0.0f;
     for (int n = 1; n <= 6; n++) rcp[n] = 1.0f / (float)n;
     const float omw = 1.0f - omega;
-    for (int it = 0; it < iters; it++)
+    std::vector<uint32_t> hist;  // largest update of each iteration run (gated)
+    int it = 0;
+    for (; it < iters; it++) {
+        if (gated && it > 0) {
+            const uint32_t g1 = hist[it - 1];
+            if (g1 <= thresh_bits || (it > 2 * GATE_K && g1 >= hist[it - 1 - GATE_K])) break;
+        }
+        uint32_t dm = 0;
         for (int color = 0; color < 2; color++) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) reduction(max : dm)
             for (int k = 0; k < D; k++)
                 for (int j = 0; j < H; j++)
                     for (int i = (color + j + k) & 1; i < W; i += 2) {
@@ -626,12 +649,35 @@ void orc_sor(int W, int H, int D, const uint8_t* ncount, const float* b, float* 
                         int n = ncount[o];
                         if (!n) continue;
                         size_t q = d.pp(i, j, k);
+                        const float old = p[q];
                         float s = p[q - sx] + p[q + sx] + p[q - sy] + p[q + sy] + p[q - sz] + p[q + sz];
                         float pg = (s - b[o]) * rcp[n];
                         p[q] = omw * p[q] + omega * pg;
+                        if (gated) {
+                            const uint32_t db = f32_bits(std::fabs(p[q] - old));
+                            dm = db > dm ? db : dm;
+                        }
                     }
         }
+        if (gated) hist.push_back(dm);
+    }
+    return it;
 }
+
+}  // namespace
+
+extern "C" {
+
+void orc_sor(int W, int H, int D, const uint8_t* ncount, const float* b, float* p, float omega, int iters) {
+    sor_iterations(W, H, D, ncount, b, p, omega, iters, false, 0u);
+}
+
+int orc_sor_gated(int W, int H, int D, const uint8_t* ncount, const float* b, float* p, float omega, int iters,
+                  float thresh) {
+    return sor_iterations(W, H, D, ncount, b, p, omega, iters, true, f32_bits(thresh));
+}
+
+float orc_gate_threshold(double tol, float omega) { return (float)(0.75 * tol * (double)omega / 6.0); }
 
 }  // extern "C"
 
@@ -749,9 +795,22 @@ void orc_project(int W, int H, int D, float* u, float* v, float* w, const uint8_
     if (p_pad_out) std::memcpy(p_pad_out, p.data(), sizeof(float) * d.Npad());
 }
 
+/* The pass-length rule of the mixed-precision projection with fitted passes.  Over a pass of nprev
+ * sweeps max|r| fell from rprev to r, a per-sweep factor rho = (r/rprev)^(1/nprev); reaching tol at
+ * that rate takes log(tol/r)/log(rho) more sweeps.  The count is doubled because the decay slows as
+ * the smooth error modes take over (a stationary iteration's early sweeps remove the rough modes
+ * fastest).  Evaluated as 2 * nprev * (log(tol/r) / log(r/rprev)). */
+int orc_fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol) {
+    if (!fit || !(rprev > 0.0) || !(r < rprev) || !(r > tol) || nprev < 1) return iters;
+    const double need = 2.0 * (double)nprev * (std::log(tol / r) / std::log(r / rprev));
+    if (!(need < (double)iters)) return iters;
+    const int n = (int)std::ceil(need);
+    return n < 1 ? 1 : n;
+}
+
 int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint8_t* occ, int open_mask,
                    int subtract_mean, float omega, int inner_iters, int max_outer, double tol,
-                   double* p64, double* stats) {
+                   double* p64, double* stats, int pass_rule) {
     Dims d{W, H, D};
     const size_t N = d.N();
     std::vector<uint8_t> nc(N);
@@ -774,7 +833,8 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
      * FP32 once more.  So a residual within tol ends the passes only when the faces the correction
      * will store also have max|div - mean| <= tol (evaluated on a corrected copy). */
     std::vector<float> cu(u, u + d.Nu()), cv(v, v + d.Nv()), cw(w, w + d.Nw());
-    double dprev = -1.0;
+    double dprev = -1.0, rprev = -1.0;
+    int nprev = 0;
     for (;;) {
         rmax = residual64(d, nc.data(), b.data(), p64, r32.data());
         if (!(rmax > tol)) {
@@ -791,14 +851,23 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
         }
         if (outer >= max_outer) break;
         std::fill(e.begin(), e.end(), 0.0f);
-        orc_sor(W, H, D, nc.data(), r32.data(), e.data(), omega, inner_iters);
-        sweeps += inner_iters;
+        int n;
+        if (pass_rule == 2) {
+            n = orc_sor_gated(W, H, D, nc.data(), r32.data(), e.data(), omega, inner_iters,
+                              orc_gate_threshold(tol, omega));
+        } else {
+            n = orc_fitted_pass_length(inner_iters, pass_rule == 1, rmax, rprev, nprev, tol);
+            orc_sor(W, H, D, nc.data(), r32.data(), e.data(), omega, n);
+        }
+        sweeps += n;
         for (int k = 0; k < D; k++)
             for (int j = 0; j < H; j++)
                 for (int i = 0; i < W; i++) {
                     size_t q = d.pp(i, j, k);
                     p64[q] += (double)e[q];
                 }
+        rprev = rmax;
+        nprev = n;
         outer++;
     }
     correct_faces<double>(d, occ, open_mask, p64, u, v, w);
@@ -1072,7 +1141,7 @@ int orc_step(const orc_step_params* P, float* u, float* v, float* w, float* T, f
     if (orc_advect_velocity(W, H, D, P->h, P->steps, dtt, u, v, w, occ, 1, P->adv_vel_type)) return -1;
     if (P->mixed) {
         orc_project_mp(W, H, D, u, v, w, occ, P->open_mask, P->closed_mean, P->omega, P->sor_iters, P->max_outer,
-                       P->tol, p64_pad, stats);
+                       P->tol, p64_pad, stats, P->pass_rule);
     } else {
         float mean = 0.0f;
         if (P->closed_mean) mean = orc_mean_div32_f64(W, H, D, u, v, w, occ);

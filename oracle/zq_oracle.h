@@ -91,16 +91,30 @@ void orc_divergence(int W, int H, int D, const float* u, const float* v, const f
 void orc_sor_count(int W, int H, int D, const uint8_t* occ, int open_mask, uint8_t* ncount);
 /* K red-black SOR iterations on padded p ((W+2)(H+2)(D+2)), rhs b (unpadded). */
 void orc_sor(int W, int H, int D, const uint8_t* ncount, const float* b, float* p_pad, float omega, int iters);
+/* orc_sor that stops early: before iteration it >= 1 it returns when the largest update |new - old|
+ * of iteration it-1 is <= thresh, or (it > 32) no smaller than that of iteration it-17 (the updates
+ * no longer shrink: the FP32 iterate's rounding floor).  Sizes compare as FP32 bit patterns, so a NaN
+ * update never stops it.  Returns the iterations run. */
+int orc_sor_gated(int W, int H, int D, const uint8_t* ncount, const float* b, float* p_pad, float omega, int iters,
+                  float thresh);
+/* the gated passes' threshold: (float)(0.75 * tol * omega / 6) */
+float orc_gate_threshold(double tol, float omega);
 /* full FP32 projection: div -> b = div - mean -> SOR(iters) from p=0 -> correct faces.
  * p_pad_out (optional) receives the final padded pressure. */
 void orc_project(int W, int H, int D, float* u, float* v, float* w, const uint8_t* occ, int open_mask,
                  float mean, float omega, int iters, float* p_pad_out);
 /* mixed precision: FP32 SOR inner sweeps, FP64 residual and pressure, iterative refinement.
  * p64_pad (in/out, padded, FP64) is the warm start.  Returns the number of outer iterations;
- * stats[0] = total inner sweeps, stats[1] = final max|r|, stats[2] = max|div| after correction. */
+ * stats[0] = total inner sweeps, stats[1] = final max|r|, stats[2] = max|div| after correction.
+ * pass_rule != 0: pass lengths from orc_fitted_pass_length (else inner_iters each). */
 int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint8_t* occ, int open_mask,
                    int subtract_mean, float omega, int inner_iters, int max_outer, double tol,
-                   double* p64_pad, double* stats);
+                   double* p64_pad, double* stats, int pass_rule);
+/* Sweeps of the next refinement pass when the pass lengths are fitted: r is the max|r| the last pass
+ * left, rprev the one before it, nprev that pass's sweeps.  Once a pass has lowered max|r| (still
+ * above tol), the next pass runs the sweeps that pass's per-sweep decay (r/rprev)^(1/nprev) needs to
+ * reach tol, doubled, at least 1 and at most iters; otherwise iters. */
+int orc_fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol);
 /* FP64 conjugate gradient on the same operator (pins the converged projection).
  * Writes corrected faces into u,v,w (FP32) and returns iterations. */
 int orc_project_cg64(int W, int H, int D, float* u, float* v, float* w, const uint8_t* occ, int open_mask,
@@ -175,6 +189,7 @@ typedef struct orc_step_params {
     double tol;
     int n_box; const float* boxes;              /* 8 floats each */
     int n_sphere; const float* spheres;         /* cx,cy,cz,r,rho,T each */
+    int pass_rule;        /* mixed: pass lengths, as orc_project_mp */
 } orc_step_params;
 /* p64_pad may be NULL in plain mode; in mixed mode it carries the warm start */
 int orc_step(const orc_step_params* P, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,

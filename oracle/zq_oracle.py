@@ -42,6 +42,7 @@ class StepParams(C.Structure):
         ("tol", C.c_double),
         ("n_box", C.c_int), ("boxes", C.c_void_p),
         ("n_sphere", C.c_int), ("spheres", C.c_void_p),
+        ("pass_rule", C.c_int),
     ]
 
 
@@ -85,8 +86,14 @@ def load(path):
     L.orc_sor_count.argtypes = [i, i, i, _u8p, i, _u8p]
     L.orc_sor.argtypes = [i, i, i, _u8p, _f32p, _f32p, f, i]
     L.orc_project.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, f, f, i, vp]
-    L.orc_project_mp.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, f, i, i, d, _f64p, _f64p]
+    L.orc_project_mp.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, f, i, i, d, _f64p, _f64p, i]
     L.orc_project_mp.restype = i
+    L.orc_fitted_pass_length.argtypes = [i, i, d, d, i, d]
+    L.orc_fitted_pass_length.restype = i
+    L.orc_sor_gated.argtypes = [i, i, i, _u8p, _f32p, _f32p, f, i, f]
+    L.orc_sor_gated.restype = i
+    L.orc_gate_threshold.argtypes = [d, f]
+    L.orc_gate_threshold.restype = f
     L.orc_project_cg64.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, i, d, i, vp]
     L.orc_project_cg64.restype = i
     L.orc_max_abs_div.argtypes = [i, i, i, _f32p, _f32p, _f32p, _u8p, i, d]
@@ -225,6 +232,15 @@ def sor(ncount, b, p_pad, omega, iters):
     lib().orc_sor(W, H, D, ncount, b, p_pad, omega, iters)
 
 
+def sor_gated(ncount, b, p_pad, omega, iters, thresh):
+    D, H, W = ncount.shape
+    return int(lib().orc_sor_gated(W, H, D, ncount, b, p_pad, omega, iters, thresh))
+
+
+def gate_threshold(tol, omega):
+    return float(lib().orc_gate_threshold(tol, omega))
+
+
 def project(u, v, w, occ, open_mask, mean=0.0, omega=1.8, iters=200, want_p=False):
     W, H, D = _dims(occ)
     p = np.zeros((D + 2, H + 2, W + 2), np.float32) if want_p else None
@@ -233,14 +249,18 @@ def project(u, v, w, occ, open_mask, mean=0.0, omega=1.8, iters=200, want_p=Fals
 
 
 def project_mp(u, v, w, occ, open_mask, subtract_mean=False, omega=1.8, inner=50, max_outer=100,
-               tol=1e-6, p64=None):
+               tol=1e-6, p64=None, pass_rule=0):
     W, H, D = _dims(occ)
     if p64 is None:
         p64 = np.zeros((D + 2, H + 2, W + 2), np.float64)
     stats = np.zeros(4, np.float64)
     outer = lib().orc_project_mp(W, H, D, u, v, w, occ, open_mask, int(subtract_mean), omega, inner,
-                                 max_outer, tol, p64, stats)
+                                 max_outer, tol, p64, stats, int(pass_rule))
     return outer, stats, p64
+
+
+def fitted_pass_length(iters, fit, r, rprev, nprev, tol):
+    return int(lib().orc_fitted_pass_length(iters, int(fit), r, rprev, nprev, tol))
 
 
 def project_cg64(u, v, w, occ, open_mask, subtract_mean=False, tol=1e-12, max_iter=100000, want_p=False):

@@ -200,6 +200,34 @@ def test_sor_mixed_precision(zs, rng):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-6)
 
 
+@pytest.mark.parametrize("rule", [1, 2])
+@pytest.mark.parametrize("open_mask", [O.OPEN_YHI, 63])
+def test_sor_mixed_precision_pass_rules(zs, rng, open_mask, rule):
+    """Fitted (pass_rule 1) and gated (2) refinement passes: the device chooses the oracle's pass
+    lengths from the same values, so passes, total sweeps and the corrected faces agree bit for bit
+    (open domains: no FP64 mean summed in another order), and the bound is met."""
+    W, H, D = 36, 28, 24
+    print("mixed precision, pass rule", rule, "open mask", open_mask, flush=True)
+    u, v, w = faces(rng, W, H, D, scale=0.05)
+    occ = occupancy(rng, W, H, D, 0.1, shell=False)
+    tol = 1e-7
+    iters = 40 if rule == 1 else 2000
+    sweeps = {}
+    for r in (0, rule):
+        n = 40 if r == 0 else iters
+        ref, got = copies(u, v, w), copies(u, v, w)
+        outer, st_o, _ = O.project_mp(*ref, occ, open_mask, False, 1.9, n, 400, tol, pass_rule=r)
+        _, st = zs.solve_ex(*got, occ, open_mask=open_mask, closed_mean=False, omega=1.9, iters=n, mixed=True,
+                            max_outer=400, tol=tol, pass_rule=r)
+        assert st["outer"] == outer and st["sweeps"] == int(st_o[0]), (r, st, outer, st_o)
+        assert st["converged"] == 1 and st["max_div"] <= tol
+        for a, b, name in zip(got, ref, "uvw"):
+            assert_same(a, b, "pass rule %d %s" % (r, name))
+        sweeps[r] = st["sweeps"]
+    assert sweeps[0] % 40 == 0
+    print("sweeps:", sweeps, flush=True)
+
+
 def test_sor_mixed_precision_stops_at_the_rounding_floor(zs, rng):
     """Velocities of order 1 stored in FP32 carry rounding of a few 1e-7 per face, so a bound of 1e-7
     on the stored faces' divergence is out of reach: the passes stop once they no longer shrink it

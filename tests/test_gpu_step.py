@@ -28,7 +28,7 @@ def make_sim(W, H, D, p, sphere_sources):
                        adv_sca_type=p["adv_sca_type"], sor_open_mask=p["open_mask"],
                        sor_closed_mean=p["closed_mean"], sor_omega=p["omega"], sor_iters=p["sor_iters"],
                        sor_mixed_precision=p["mixed"], sor_max_outer=p["max_outer"], sor_tol=p["tol"],
-                       warm_start=1)
+                       sor_pass_rule=p.get("pass_rule", 0), warm_start=1)
     sim.set_sphere_sources(sphere_sources)
     return sim
 
@@ -209,3 +209,34 @@ def test_one_launch_iteration_mixed_precision_same_bits():
         for name in ("u", "v", "w", "T", "rho"):
             assert np.array_equal(g0[name].view(np.uint32), g1[name].view(np.uint32)), name
         assert np.array_equal(g0["p64"].view(np.uint64), g1["p64"].view(np.uint64)), "p64"
+
+
+@pytest.mark.parametrize("shape", [(0, 0), (-4, 8), (-16, 16)])
+def test_gated_passes_match_the_oracle(shape):
+    """Gated refinement passes (pass_rule 2): every pass sweeps until an iteration's largest update
+    meets the threshold; the device runs the oracle's iterations, so passes, sweep counts and every
+    field agree bit for bit on an open-top box, whatever sweep shape the handle carries (the gated
+    iteration uses the one-launch kernel at that shape's rows and planes, or 32 x 16)."""
+    import zq_smokesimulation_amd._lib as L
+    W, H, D = 96, 64, 48
+    shell = O.ALL_FACES & ~O.OPEN_YHI
+    p = oracle_params(W, mixed=1, sor_iters=1000, max_outer=400, tol=1e-6, open_mask=O.OPEN_YHI, closed_mean=0,
+                      pass_rule=2)
+    occ, u, v, w, T, R = initial_state(W, H, D, shell)
+    sim = make_sim(W, H, D, p, SRC)
+    sim.set_sor_shape(*shape)
+    sim.set_state(u=u, v=v, w=w, T=T, rho=R, occ=occ)
+    p64 = np.zeros((D + 2, H + 2, W + 2))
+    for s in range(4):
+        print("gated, shape", shape, "step", s, flush=True)
+        st_o = O.step(p, u, v, w, T, R, occ, p64=p64, spheres=SRC)
+        st_g = sim.step()
+        assert st_g["sweeps"] == int(st_o[0]), (st_g, st_o)
+        assert st_g["max_div"] <= 1e-6 and st_g["converged"] == 1, st_g
+        assert st_g["sweeps"] % 1000 != 0  # the gate ended every pass
+    g = sim.get_state()
+    g["p64"] = sim.download(L.FIELD_P64)
+    sim.close()
+    for name, a in zip(("u", "v", "w", "T", "rho"), (u, v, w, T, R)):
+        assert np.array_equal(g[name].view(np.uint32), a.view(np.uint32)), name
+    assert np.array_equal(g["p64"].view(np.uint64), p64.view(np.uint64)), "p64"

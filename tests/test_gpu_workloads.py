@@ -40,7 +40,7 @@ def make_sim(W, H, D, p):
                         adv_sca_type=p["adv_sca_type"], sor_open_mask=p["open_mask"],
                         sor_closed_mean=p["closed_mean"], sor_omega=p["omega"], sor_iters=p["sor_iters"],
                         sor_mixed_precision=p["mixed"], sor_max_outer=p["max_outer"], sor_tol=p["tol"],
-                        warm_start=1)
+                        sor_pass_rule=p.get("pass_rule", 0), warm_start=1)
 
 
 def close_to(g, ref, what, rel=1e-5):
@@ -55,7 +55,8 @@ def test_full_pipeline_256():
     W = 256
     p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=0.99, aT=0.99, aR=0.99,
              adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1,
-             omega=2.0 / (1.0 + math.sin(math.pi / W)), sor_iters=40, mixed=1, max_outer=1000, tol=1e-6)
+             omega=2.0 / (1.0 + math.sin(math.pi / W)), sor_iters=80, mixed=1, max_outer=1000, tol=1e-6,
+             pass_rule=1)  # the bench's projection: fitted passes of at most 80 sweeps
     occ, u, v, w = O.init_fine_grid(W, W, W, O.ALL_FACES, 0)
     T = O.noise(W, W, W, 0, 0.01)
     R = np.zeros_like(T)

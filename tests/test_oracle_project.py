@@ -243,3 +243,66 @@ def test_sor_iteration_equals_dense_red_black_sor(rng, shape, open_mask, frac, o
     O.sor(nc, b, p, np.float32(omega), iters)
     scale = max(1.0, float(np.abs(ref).max()))
     np.testing.assert_allclose(p, ref, rtol=0, atol=2e-6 * scale * iters)
+
+
+def test_fitted_pass_length_rule():
+    """orc_fitted_pass_length: a whole pass unless fitting applies; otherwise the sweeps that the last
+    pass's per-sweep decay needs to reach tol, doubled, at least 1 and at most a whole pass."""
+    import math
+    f = O.fitted_pass_length
+    assert f(40, 0, 1e-5, 1e-4, 40, 1e-6) == 40    # fitting off
+    assert f(40, 1, 1e-5, -1.0, 0, 1e-6) == 40     # no earlier pass
+    assert f(40, 1, 1e-4, 1e-4, 40, 1e-6) == 40    # the last pass did not lower max|r|
+    assert f(40, 1, 2e-4, 1e-4, 40, 1e-6) == 40
+    assert f(40, 1, 5e-7, 1e-5, 40, 1e-6) == 40    # within tol already (the divergence check's extra pass)
+    # a geometric decay of rho per sweep over a pass of n sweeps
+    rho, n, rprev = 0.9, 30, 1e-3
+    r = rprev * rho ** n
+    for extra, want in ((4.3, 9), (2.2, 5), (0.01, 1), (19.6, 40), (200.0, 40)):
+        tol = r * rho ** extra      # reached after `extra` more sweeps at that rate
+        assert f(40, 1, r, rprev, n, tol) == want, (extra, f(40, 1, r, rprev, n, tol))
+        assert want == min(40, max(1, math.ceil(2 * extra)))
+
+
+def test_fitted_passes_reach_tolerance_in_fewer_sweeps():
+    """With fitted pass lengths the projection still meets tol, and on this seeded case it stops sooner
+    than with whole passes (which overshoot by up to a pass): 206 sweeps against 240."""
+    W, H, D = 24, 20, 16
+    u, v, w = faces(np.random.default_rng(1), W, H, D, scale=0.05)
+    occ = np.zeros((D, H, W), np.uint8)
+    res = {}
+    for fit in (False, True):
+        uu, vv, ww = u.copy(), v.copy(), w.copy()
+        outer, st, _ = O.project_mp(uu, vv, ww, occ, O.OPEN_YHI, False, 1.9, 40, 400, 1e-7, pass_rule=fit)
+        assert st[1] <= 1e-7 and st[2] <= 1e-7
+        res[fit] = int(st[0])
+    assert res[False] % 40 == 0 and res[True] < res[False], res
+
+
+def test_gated_sor_stops_after_the_first_small_iteration():
+    """orc_sor_gated runs orc_sor's iterations and stops after the first one whose largest update
+    |new - old| is at most the threshold (checked here by running the iterations one at a time and
+    differencing); with a threshold no update meets it runs them all."""
+    rng = np.random.default_rng(3)
+    W, H, D = 12, 10, 9
+    occ = (rng.random((D, H, W)) < 0.1).astype(np.uint8)
+    nc = O.sor_count(occ, O.OPEN_YHI)
+    b = np.where(nc > 0, rng.normal(size=(D, H, W)), 0).astype(np.float32)
+    p0 = np.zeros((D + 2, H + 2, W + 2), np.float32)
+    steps, sizes = [p0.copy()], []
+    for _ in range(200):
+        q = steps[-1].copy()
+        O.sor(nc, b, q, 1.7, 1)
+        sizes.append(float(np.abs(q - steps[-1]).max()))
+        steps.append(q)
+    thresh = np.float32(sizes[60] * 1.0001)
+    want = next(i for i, m in enumerate(sizes) if m <= thresh) + 1
+    q = p0.copy()
+    ran = O.sor_gated(nc, b, q, 1.7, 200, float(thresh))
+    assert ran == want and 1 < ran <= 61
+    assert np.array_equal(q.view(np.uint32), steps[ran].view(np.uint32))
+    q = p0.copy()
+    assert O.sor_gated(nc, b, q, 1.7, 37, 0.0) == 37
+    assert np.array_equal(q.view(np.uint32), steps[37].view(np.uint32))
+    # the threshold of the gated refinement passes
+    assert O.gate_threshold(1e-6, 1.5) == np.float32(0.75 * 1e-6 * np.float64(np.float32(1.5)) / 6.0)

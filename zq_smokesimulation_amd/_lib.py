@@ -33,6 +33,7 @@ class SorOpts(C.Structure):
         ("mixed_precision", C.c_int),
         ("max_outer", C.c_int),
         ("tol", C.c_double),
+        ("pass_rule", C.c_int),
     ]
 
 

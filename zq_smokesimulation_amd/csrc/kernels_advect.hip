@@ -37,12 +37,35 @@ __global__ __launch_bounds__(BX* BY) void k_advect_cells(const vec4_t<R>* __rest
                       blockIdx.z);
 }
 
-template <class R, int AXIS>
+// The three face grids from the advected cells in one launch: thread (i, j) walks a chunk of
+// APPLY_RZ planes and writes each cell's low u, v and w faces (the last column / row / plane also the
+// domain's high face), every face with st_cells_to_face's value.  The w face below plane k reads the
+// cell of plane k-1 that the same thread read one iteration earlier.
+constexpr int APPLY_RZ = 16;
+template <class R>
 __global__ __launch_bounds__(BX* BY) void k_cells_to_faces_apply(const R* __restrict__ adv,
-                                                                 const uint8_t* __restrict__ occ,
-                                                                 R* __restrict__ face, Dims d, int is_open) {
-    st_cells_to_face<R, AXIS>(adv, occ, face, d, is_open, blockIdx.x * BX + threadIdx.x,
-                              blockIdx.y * BY + threadIdx.y, blockIdx.z);
+                                                                 const uint8_t* __restrict__ occ, R* __restrict__ u,
+                                                                 R* __restrict__ v, R* __restrict__ w, Dims d,
+                                                                 int is_open) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= d.W || j >= d.H) return;
+    const int k1 = min((int)(blockIdx.z + 1) * APPLY_RZ, d.D);
+    for (int k = blockIdx.z * APPLY_RZ; k < k1; k++) {
+        st_cells_to_face<R, 0>(adv, occ, u, d, is_open, i, j, k);
+        if (i == d.W - 1) st_cells_to_face<R, 0>(adv, occ, u, d, is_open, i + 1, j, k);
+        st_cells_to_face<R, 1>(adv, occ, v, d, is_open, i, j, k);
+        if (j == d.H - 1) st_cells_to_face<R, 1>(adv, occ, v, d, is_open, i, j + 1, k);
+        st_cells_to_face<R, 2>(adv, occ, w, d, is_open, i, j, k);
+        if (k == d.D - 1) st_cells_to_face<R, 2>(adv, occ, w, d, is_open, i, j, k + 1);
+    }
+}
+
+template <class R>
+void launch_cells_to_faces_apply(const Dims& d, const R* adv, const uint8_t* occ, R* u, R* v, R* w, int is_open,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_cells_to_faces_apply<R>, grid_for(d.W, d.H, (d.D + APPLY_RZ - 1) / APPLY_RZ), dim3(BX, BY),
+                       0, s, adv, occ, u, v, w, d, is_open);
 }
 
 // Advect_Velocity_inRegular_outMAC_* (:115-230) and inMAC_outMAC_* (:232-377): backtrace from the
@@ -216,12 +239,7 @@ void advect_velocity_bfecc(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, 
         increment<R>((R*)vel4, (const R*)B, nullptr, (R*)B, 4 * N, s);
         R* adv = scratch;  // 3N values <= Nu+Nv+Nw
         cells_in<R, true>(d, Vf, RegularVel<R>{B, d}, occ, adv, ap, s);
-        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
-                           is_open);
-        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 1>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, adv, occ, v, d,
-                           is_open);
-        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 2>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, adv, occ, w, d,
-                           is_open);
+        launch_cells_to_faces_apply<R>(d, adv, occ, u, v, w, is_open, s);
         return;
     }
     if (mode == 3) {  // inMAC_outMAC: F1 = adv(u; u); F2 = adv(F1; -u); keep u, u = u~; adv(u~; kept) -> F1
@@ -270,12 +288,7 @@ void launch_advect_velocity(const Dims& d, R* u, R* v, R* w, const uint8_t* occ,
         launch_mac_to_cell4<R>(d, u, v, w, vel4, s, occ);
         R* adv = scratch;  // 3N values <= Nu+Nv+Nw
         hipLaunchKernelGGL(k_advect_cells<R>, grid_for(d.W, d.H, d.D), blk, 0, s, vel4, occ, adv, d, h, ap.steps, dtt);
-        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 0>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, adv, occ, u, d,
-                           is_open);
-        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 1>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, adv, occ, v, d,
-                           is_open);
-        hipLaunchKernelGGL((k_cells_to_faces_apply<R, 2>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, adv, occ, w, d,
-                           is_open);
+        launch_cells_to_faces_apply<R>(d, adv, occ, u, v, w, is_open, s);
         return;
     }
     if (mode == 4) {

@@ -5,7 +5,7 @@
 //   k_vorticity     faces -> (wx,wy,wz,|w|) per cell, cell velocities formed on the fly
 //   k_force_cells   |w| stencil -> normalized gradient -> f = eps*h*(N x w) + buoyancy,
 //                   T and rho attenuated in the same pass
-//   k_force_faces   face += dt*avg(f) and *= aV on one mask, one pass per face grid
+//   k_force_faces   face += dt*avg(f) and *= aV on one mask, the three face grids in one pass
 // Threads: 64 x 4 per block, x across the wave's lanes (coalesced rows), one cell per thread.
 #include "zq3d_device.h"
 #include "zq3d_kernels.h"
@@ -35,11 +35,27 @@ __global__ __launch_bounds__(BX* BY) void k_force_cells(const vec4_t<R>* __restr
                             blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
-template <class R, int AXIS, bool ATTEN>
-__global__ __launch_bounds__(BX* BY) void k_force_faces(R* __restrict__ face, const uint8_t* __restrict__ occ,
-                                                        const R* __restrict__ fc, Dims d, R dt, R dth, R aV) {
-    st_force_face<R, AXIS, ATTEN>(face, occ, fc, d, dt, dth, aV, blockIdx.x * BX + threadIdx.x,
-                                  blockIdx.y * BY + threadIdx.y, blockIdx.z);
+// All three face grids in one launch: thread (i, j) walks a chunk of FACE_RZ planes and updates each
+// cell's low u, v and w faces (the last column / row / plane also the domain's high face) with
+// st_force_face.  The w face below plane k reads the force and occupancy of plane k-1, which the
+// same thread read one iteration earlier.
+constexpr int FACE_RZ = 16;
+template <class R, bool ATTEN>
+__global__ __launch_bounds__(BX* BY) void k_force_faces(R* __restrict__ u, R* __restrict__ v, R* __restrict__ w,
+                                                        const uint8_t* __restrict__ occ, const R* __restrict__ fc,
+                                                        Dims d, R dt, R dth, R aV) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= d.W || j >= d.H) return;
+    const int k1 = min((int)(blockIdx.z + 1) * FACE_RZ, d.D);
+    for (int k = blockIdx.z * FACE_RZ; k < k1; k++) {
+        st_force_face<R, 0, ATTEN>(u, occ, fc, d, dt, dth, aV, i, j, k);
+        if (i == d.W - 1) st_force_face<R, 0, ATTEN>(u, occ, fc, d, dt, dth, aV, i + 1, j, k);
+        st_force_face<R, 1, ATTEN>(v, occ, fc, d, dt, dth, aV, i, j, k);
+        if (j == d.H - 1) st_force_face<R, 1, ATTEN>(v, occ, fc, d, dt, dth, aV, i, j + 1, k);
+        st_force_face<R, 2, ATTEN>(w, occ, fc, d, dt, dth, aV, i, j, k);
+        if (k == d.D - 1) st_force_face<R, 2, ATTEN>(w, occ, fc, d, dt, dth, aV, i, j, k + 1);
+    }
 }
 
 // Atten_{u,v,w}_Kernel without a force (:8-96)
@@ -75,13 +91,8 @@ inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (
 template <class R, bool ATTEN>
 void force_faces(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, const R* fcell, R dt, R dth, R aV,
                  hipStream_t s) {
-    const dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL((k_force_faces<R, 0, ATTEN>), grid_for(d.W + 1, d.H, d.D), blk, 0, s, u, occ, fcell, d, dt, dth,
-                       aV);
-    hipLaunchKernelGGL((k_force_faces<R, 1, ATTEN>), grid_for(d.W, d.H + 1, d.D), blk, 0, s, v, occ, fcell, d, dt, dth,
-                       aV);
-    hipLaunchKernelGGL((k_force_faces<R, 2, ATTEN>), grid_for(d.W, d.H, d.D + 1), blk, 0, s, w, occ, fcell, d, dt, dth,
-                       aV);
+    hipLaunchKernelGGL((k_force_faces<R, ATTEN>), grid_for(d.W, d.H, (d.D + FACE_RZ - 1) / FACE_RZ), dim3(BX, BY, 1),
+                       0, s, u, v, w, occ, fcell, d, dt, dth, aV);
 }
 
 }  // namespace

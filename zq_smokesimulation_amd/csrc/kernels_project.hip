@@ -22,53 +22,85 @@ namespace zq3d {
 namespace {
 
 constexpr int BX = 64, BY = 4;
+// planes per thread of the z-marching kernels (each thread owns one (i, j) column of a chunk)
+constexpr int RES_RZ = 16;
 
 inline dim3 grid_for(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
+inline dim3 grid_z(const Dims& d) { return grid_for(d.W, d.H, (d.D + RES_RZ - 1) / RES_RZ); }
 
 __device__ __forceinline__ int block_linear() { return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x; }
 
 // Neighbour counts, divergence (the reference's unscaled sum, ZQ_SmokeSimulationUtils3D.cpp:2403-2405,
 // evaluated left to right), zero pressure where the count is 0, and per-block FP64 sum/count of the
 // divergence over fluid cells (closed-domain mean).
-template <bool MP>
 __global__ __launch_bounds__(BX* BY) void k_proj_setup(const float* __restrict__ u, const float* __restrict__ v,
                                                        const float* __restrict__ w, const uint8_t* __restrict__ occ,
                                                        int open_mask, uint8_t* __restrict__ nc,
-                                                       float* __restrict__ b, double* __restrict__ b64,
-                                                       float* __restrict__ p, double* __restrict__ p64,
+                                                       float* __restrict__ b, float* __restrict__ p,
                                                        double* __restrict__ part, Dims d) {
     __shared__ double red[4];
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     double dsum = 0.0, dcnt = 0.0;
-    if (!MP) {
-        st_proj_setup_plain(u, v, w, occ, open_mask, nc, b, p, d, i, j, k, dsum, dcnt);
-    } else if (i < d.W && j < d.H) {
-        const int o = cidx(d, i, j, k);
-        const int HW = d.W * d.H;
-        int n = 0;
-        const bool fluid = !occ[o];
-        if (fluid) {
-            n += (i > 0) ? !occ[o - 1] : ((open_mask & 1) != 0);
-            n += (i < d.W - 1) ? !occ[o + 1] : ((open_mask & 2) != 0);
-            n += (j > 0) ? !occ[o - d.W] : ((open_mask & 4) != 0);
-            n += (j < d.H - 1) ? !occ[o + d.W] : ((open_mask & 8) != 0);
-            n += (k > 0) ? !occ[o - HW] : ((open_mask & 16) != 0);
-            n += (k < d.D - 1) ? !occ[o + HW] : ((open_mask & 32) != 0);
+    st_proj_setup_plain(u, v, w, occ, open_mask, nc, b, p, d, i, j, k, dsum, dcnt);
+    if (part) {
+        const double s = block_reduce<false>(dsum, red);
+        __syncthreads();
+        const double c = block_reduce<false>(dcnt, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            part[2 * block_linear()] = s;
+            part[2 * block_linear() + 1] = c;
         }
-        nc[o] = (uint8_t)n;
-        const float u0 = u[uidx(d, i, j, k)], u1 = u[uidx(d, i + 1, j, k)];
-        const float v0 = v[vidx(d, i, j, k)], v1 = v[vidx(d, i, j + 1, k)];
-        const float w0 = w[widx(d, i, j, k)], w1 = w[widx(d, i, j, k + 1)];
-        {
+    }
+}
+
+// The mixed-precision set-up: neighbour counts, FP64 divergence b64 (the same sum as the plain
+// set-up, in double), p64 = 0 where the count is 0, per-block FP64 sum/count of the divergence over
+// fluid cells.  One (i, j) column per thread up a chunk of RES_RZ planes: the occupancy of planes
+// k-1, k, k+1 and the w face between planes stay in registers, so every input is read once.
+__global__ __launch_bounds__(BX* BY) void k_proj_setup_z(const float* __restrict__ u, const float* __restrict__ v,
+                                                         const float* __restrict__ w, const uint8_t* __restrict__ occ,
+                                                         int open_mask, uint8_t* __restrict__ nc,
+                                                         double* __restrict__ b64, double* __restrict__ p64,
+                                                         double* __restrict__ part, Dims d) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    double dsum = 0.0, dcnt = 0.0;
+    if (i < d.W && j < d.H) {
+        const int HW = d.W * d.H;
+        const int k0 = blockIdx.z * RES_RZ, k1 = min(k0 + RES_RZ, d.D);
+        int o = cidx(d, i, j, k0);
+        int ob = k0 > 0 ? occ[o - HW] : 1;  // plane below (not read at k = 0)
+        int oc = occ[o];
+        float w0 = w[widx(d, i, j, k0)];
+        for (int k = k0; k < k1; k++, o += HW) {
+            const int oa = k < d.D - 1 ? occ[o + HW] : 1;  // plane above (not read at k = D-1)
+            const bool fluid = !oc;
+            int n = 0;
+            if (fluid) {
+                n += (i > 0) ? !occ[o - 1] : ((open_mask & 1) != 0);
+                n += (i < d.W - 1) ? !occ[o + 1] : ((open_mask & 2) != 0);
+                n += (j > 0) ? !occ[o - d.W] : ((open_mask & 4) != 0);
+                n += (j < d.H - 1) ? !occ[o + d.W] : ((open_mask & 8) != 0);
+                n += (k > 0) ? !ob : ((open_mask & 16) != 0);
+                n += (k < d.D - 1) ? !oa : ((open_mask & 32) != 0);
+            }
+            nc[o] = (uint8_t)n;
+            const float u0 = u[uidx(d, i, j, k)], u1 = u[uidx(d, i + 1, j, k)];
+            const float v0 = v[vidx(d, i, j, k)], v1 = v[vidx(d, i, j + 1, k)];
+            const float w1 = w[widx(d, i, j, k + 1)];
             const double dd = (((double)u1 - (double)u0) + ((double)v1 - (double)v0)) + ((double)w1 - (double)w0);
             b64[o] = n ? dd : 0.0;
             if (!n) p64[pidx(d, i, j, k)] = 0.0;
             if (fluid) {
-                dsum = dd;
-                dcnt = 1.0;
+                dsum += dd;
+                dcnt += 1.0;
             }
+            ob = oc;
+            oc = oa;
+            w0 = w1;
         }
     }
     if (part) {
@@ -381,6 +413,87 @@ __global__ __launch_bounds__(64 * NW) void k_sor_split_rb(const uint8_t* __restr
                               min(z0 + zc, d.D), omega, omw);
 }
 
+// Iteration `it` of a gated pass (k_sor_split_rb otherwise): it returns at once when the previous
+// iteration's largest update, gate[it - 1] (FP32 bits; 0 when that iteration did not run either),
+// is at most `thresh`, or, past iteration 2 GATE_K, no smaller than gate[it - 1 - GATE_K] (the
+// updates have stopped shrinking: the FP32 iterate's rounding floor; the refinement's next FP64
+// residual takes over); otherwise it runs the iteration, folds its largest update into gate[it]
+// and counts itself in *done, copied to the mapped host word *host_done for the host's progress
+// check.  gate[it] is GATE_LANES words (the maximum is their maximum): each workgroup reduces its
+// waves in LDS and adds one atomic to word blockIdx % GATE_LANES, so no address takes more than a
+// few dozen atomics per launch (thousands of atomics on one word serialise and doubled the
+// iteration's time).  All workgroups read the same words, written by the previous launch, so they
+// all run or all return.  ZS: the first iteration of a pass (zero source).
+template <int TY, int NW, bool ZS>
+__global__ __launch_bounds__(64 * NW) void k_sor_split_rb_gated(
+    const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1, const float* __restrict__ b0,
+    const float* __restrict__ b1, const float* __restrict__ mean32, const float* __restrict__ s0,
+    const float* __restrict__ s1, float* __restrict__ d0, float* __restrict__ d1, Dims d, int Wh, int gx, int gy,
+    int zc, float omega, float omw, unsigned* __restrict__ gate, int it, unsigned thresh, int* __restrict__ done,
+    unsigned long long* __restrict__ host_done) {
+    auto word = [&](int i) {
+        unsigned m = 0;
+#pragma unroll
+        for (int l = 0; l < GATE_LANES; l++) m = max(m, gate[i * GATE_LANES + l]);
+        return m;
+    };
+    if (it > 0) {
+        const unsigned g1 = word(it - 1);
+        if (g1 <= thresh || (it > 2 * GATE_K && g1 >= word(it - 1 - GATE_K))) return;
+    }
+    const int per = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int bx = tile % gx;
+    const int rest = tile / gx;
+    const int by = rest % gy;
+    const int z0 = (rest / gy) * zc;
+    if (z0 >= d.D) return;  // spare block (the grid is a multiple of 8): uniform exit before any barrier
+    unsigned dm = 0;
+    sor_split_rb_tile<SPLIT_TX, TY, NW, true, ZS>(n0, n1, b0, b1, *mean32, s0, s1, d0, d1, d, Wh, bx * SPLIT_TX,
+                                                  by * TY, z0, min(z0 + zc, d.D), omega, omw, &dm);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dm = max(dm, (unsigned)__shfl_xor((int)dm, o));
+    __shared__ unsigned wave_max[NW];
+    if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = dm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) m = max(m, wave_max[w]);
+        atomicMax(&gate[it * GATE_LANES + (blockIdx.x & (GATE_LANES - 1))], m);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int v = *done + 1;
+        *done = v;
+        *host_done = (unsigned long long)v;
+    }
+}
+
+// Zeroes the gate words of a pass and its iteration count.
+__global__ __launch_bounds__(256) void k_gate_reset(unsigned* __restrict__ gate, int n, int* __restrict__ done) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) gate[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *done = 0;
+}
+
+// p64 += e after a gated pass: e is in the pair the last executed iteration wrote (*done iterations
+// from pair p: odd -> alt).  e is not cleared (the next pass's first iteration does not read it).
+__global__ __launch_bounds__(BX* BY) void k_update_p64_gated(const uint8_t* __restrict__ nc,
+                                                             const float* __restrict__ p0, const float* __restrict__ p1,
+                                                             const float* __restrict__ a0, const float* __restrict__ a1,
+                                                             const int* __restrict__ done, double* __restrict__ p,
+                                                             Dims d, int Wh) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.W || j >= d.H) return;
+    if (!nc[cidx(d, i, j, k)]) return;
+    const bool odd = (*done) & 1;
+    const bool c1 = (i + j + k) & 1;
+    const float* e = odd ? (c1 ? a1 : a0) : (c1 ? p1 : p0);
+    const int q = pidx(d, i, j, k);
+    p[q] = p[q] + (double)e[split_q(Wh, d.H, i >> 1, j, k)];
+}
+
 // Two red+black iterations per launch on split storage (sor_split_rb2_tile); grid as k_sor_split_rb.
 template <int TY, int NW>
 __global__ __launch_bounds__(64 * NW) void k_sor_split_rb2(const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1,
@@ -412,14 +525,19 @@ __global__ __launch_bounds__(BX* BY) void k_correct_cells(const uint8_t* __restr
                                                           F* __restrict__ v, F* __restrict__ w, Dims d) {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    st_correct<0, P, F>(occ, open_mask, p, u, d, i, j, k);
-    if (i == d.W - 1) st_correct<0, P, F>(occ, open_mask, p, u, d, i + 1, j, k);
-    st_correct<1, P, F>(occ, open_mask, p, v, d, i, j, k);
-    if (j == d.H - 1) st_correct<1, P, F>(occ, open_mask, p, v, d, i, j + 1, k);
-    st_correct<2, P, F>(occ, open_mask, p, w, d, i, j, k);
-    if (k == d.D - 1) st_correct<2, P, F>(occ, open_mask, p, w, d, i, j, k + 1);
+    // the column marches up its chunk: the w face below plane k reads plane k-1's pressure and
+    // occupancy, which this thread loaded for its u face one iteration earlier (a cache hit, not a
+    // second trip to HBM as when neighbouring planes belong to other workgroups)
+    const int k1 = min((int)(blockIdx.z + 1) * RES_RZ, d.D);
+    for (int k = blockIdx.z * RES_RZ; k < k1; k++) {
+        st_correct<0, P, F>(occ, open_mask, p, u, d, i, j, k);
+        if (i == d.W - 1) st_correct<0, P, F>(occ, open_mask, p, u, d, i + 1, j, k);
+        st_correct<1, P, F>(occ, open_mask, p, v, d, i, j, k);
+        if (j == d.H - 1) st_correct<1, P, F>(occ, open_mask, p, v, d, i, j + 1, k);
+        st_correct<2, P, F>(occ, open_mask, p, w, d, i, j, k);
+        if (k == d.D - 1) st_correct<2, P, F>(occ, open_mask, p, w, d, i, j, k + 1);
+    }
 }
 
 // Transposed correction: every face, solid and ghost pressures being 0 (the adjoint of the
@@ -576,11 +694,10 @@ __global__ __launch_bounds__(BX* BY) void k_residual64(const uint8_t* __restrict
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
 }
 
-// k_residual64<true> marching z: each thread owns one (i, j) column of a chunk of RZ planes and keeps
-// the pressures of planes k-1, k, k+1 in registers, so each padded FP64 pressure is read once (the
-// per-plane kernel reads every plane for three neighbouring planes' blocks).  The same operations
+// k_residual64<true> marching z: each thread owns one (i, j) column of a chunk of RES_RZ planes and
+// keeps the pressures of planes k-1, k, k+1 in registers, so each padded FP64 pressure is read once
+// (the per-plane kernel reads every plane for three neighbouring planes' blocks).  The same operations
 // per cell in the same order; the block maxima are over the chunk (a maximum is order-free).
-constexpr int RES_RZ = 16;
 __global__ __launch_bounds__(BX* BY) void k_residual64_z(const uint8_t* __restrict__ nc, const double* __restrict__ b64,
                                                          const double* __restrict__ mean64,
                                                          const double* __restrict__ p, float* __restrict__ r0,
@@ -664,6 +781,9 @@ __global__ __launch_bounds__(BX* BY) void k_max_div64_pred(const F* __restrict__
                                                            const double* __restrict__ p64,
                                                            const double* __restrict__ mean64,
                                                            double* __restrict__ part, Dims d) {
+    // one cell per thread, one plane per grid row: with a column of planes per thread (as the
+    // residual) it measured slower (342 against 304 us at 256^3): six face evaluations per cell
+    // leave too few waves to hide their loads
     __shared__ double red[4];
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
@@ -731,15 +851,20 @@ void launch_proj_setup(const Dims& d, const float* u, const float* v, const floa
                        int open_mask, uint8_t* nc, float* b, double* b64, float* p_pad, double* p64_pad,
                        double* part, hipStream_t s) {
     if (b64)
-        hipLaunchKernelGGL(k_proj_setup<true>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ, open_mask,
-                           nc, b, b64, p_pad, p64_pad, part, d);
+        hipLaunchKernelGGL(k_proj_setup_z, grid_z(d), dim3(BX, BY), 0, s, u, v, w, occ, open_mask, nc, b64, p64_pad,
+                           part, d);
     else
-        hipLaunchKernelGGL(k_proj_setup<false>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ, open_mask,
-                           nc, b, b64, p_pad, p64_pad, part, d);
+        hipLaunchKernelGGL(k_proj_setup, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, u, v, w, occ, open_mask, nc, b,
+                           p_pad, part, d);
 }
 
 void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s) {
     hipLaunchKernelGGL(k_finish_mean, dim3(1), dim3(1024), 0, s, part, cell_blocks(d), mean32, mean64, want);
+}
+
+void launch_mean_finalize_mp(const Dims& d, const double* part, double* mean64, int want, hipStream_t s) {
+    const dim3 g = grid_z(d);
+    hipLaunchKernelGGL(k_finish_mean, dim3(1), dim3(1024), 0, s, part, (int)(g.x * g.y * g.z), nullptr, mean64, want);
 }
 
 size_t split_padded_size(const Dims& d) { return (size_t)split_row((d.W + 1) / 2) * (d.H + 2) * (d.D + 2); }
@@ -797,6 +922,52 @@ int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const floa
     }
     if (ev1) (void)hipEventRecord(ev1, s);
     return from;
+}
+
+template <int TY, int NW>
+void launch_gated_one(const Dims& d, const SplitBufs& sp, int it, const float* mean32, float omega, int zc,
+                      const GateBufs& g, unsigned thresh, hipStream_t s) {
+    const int Wh = (d.W + 1) / 2;
+    const int gx = (Wh + SPLIT_TX - 1) / SPLIT_TX, gy = (d.H + TY - 1) / TY, gz = (d.D + zc - 1) / zc;
+    const int nb = (gx * gy * gz + 7) & ~7;
+    const float omw = 1.0f - omega;
+    // iteration it reads pair (it & 1: alt) and writes the other
+    const float* const* src = (it & 1) ? sp.alt : sp.p;
+    float* const* dst = (it & 1) ? sp.p : sp.alt;
+    if (it == 0)
+        hipLaunchKernelGGL((k_sor_split_rb_gated<TY, NW, true>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1],
+                           sp.b[0], sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw,
+                           g.gate, it, thresh, g.done, g.host_done);
+    else
+        hipLaunchKernelGGL((k_sor_split_rb_gated<TY, NW, false>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1],
+                           sp.b[0], sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw,
+                           g.gate, it, thresh, g.done, g.host_done);
+}
+
+void launch_gate_reset(const GateBufs& g, int n, hipStream_t s) {
+    const int words = n * GATE_LANES;
+    hipLaunchKernelGGL(k_gate_reset, dim3(std::min(64, (words + 255) / 256 + 1)), dim3(256), 0, s, g.gate, words,
+                       g.done);
+}
+
+void launch_gated_sweeps(const Dims& d, const SplitBufs& sp, int it0, int count, const float* mean32, float omega,
+                         SorShape shape, const GateBufs& g, unsigned thresh, hipStream_t s) {
+    // one iteration per launch; the one-launch shapes use their rows and planes, others 32 x zc 16
+    const bool one = is_split(shape) && shape.ty != SOR_SPLIT && shape.ty > -SOR_TWO;
+    const int ty = one ? split_rows(shape) : 32;
+    const int zc = one && shape.zc > 0 ? shape.zc : 16;
+    for (int it = it0; it < it0 + count; it++) {
+        if (ty == 4) launch_gated_one<4, 4>(d, sp, it, mean32, omega, zc, g, thresh, s);
+        else if (ty == 8) launch_gated_one<8, 8>(d, sp, it, mean32, omega, zc, g, thresh, s);
+        else if (ty == 16) launch_gated_one<16, 8>(d, sp, it, mean32, omega, zc, g, thresh, s);
+        else launch_gated_one<32, 16>(d, sp, it, mean32, omega, zc, g, thresh, s);
+    }
+}
+
+void launch_update_p64_gated(const Dims& d, const uint8_t* nc, const SplitBufs& sp, const int* done, double* p64_pad,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_update_p64_gated, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, sp.p[0], sp.p[1],
+                       sp.alt[0], sp.alt[1], done, p64_pad, d, (d.W + 1) / 2);
 }
 
 void launch_split_counts(const Dims& d, const uint8_t* nc, const SplitBufs& sp, hipStream_t s) {
@@ -882,15 +1053,15 @@ int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
 
 void launch_correct(const Dims& d, const uint8_t* occ, int open_mask, const float* p_pad, float* u, float* v,
                     float* w, hipStream_t s) {
-    hipLaunchKernelGGL((k_correct_cells<float, float>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
-                       p_pad, u, v, w, d);
+    hipLaunchKernelGGL((k_correct_cells<float, float>), grid_z(d), dim3(BX, BY), 0, s, occ, open_mask, p_pad, u, v,
+                       w, d);
 }
 
 template <class F>
 void launch_correct64(const Dims& d, const uint8_t* occ, int open_mask, const double* p64_pad, F* u, F* v, F* w,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_correct_cells<double, F>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, occ, open_mask,
-                       p64_pad, u, v, w, d);
+    hipLaunchKernelGGL((k_correct_cells<double, F>), grid_z(d), dim3(BX, BY), 0, s, occ, open_mask, p64_pad, u, v,
+                       w, d);
 }
 
 template <class F>
@@ -940,7 +1111,7 @@ void launch_residual64(const Dims& d, const uint8_t* nc, const double* b64, cons
 void launch_residual64_split(const Dims& d, const uint8_t* nc, const double* b64, const double* mean64,
                              const double* p64_pad, const SplitBufs& sp, double* part, unsigned long long* rmax,
                              hipStream_t s) {
-    const dim3 g = grid_for(d.W, d.H, (d.D + RES_RZ - 1) / RES_RZ);
+    const dim3 g = grid_z(d);
     hipLaunchKernelGGL(k_residual64_z, g, dim3(BX, BY), 0, s, nc, b64, mean64, p64_pad, sp.b[0], sp.b[1], part, d,
                        (d.W + 1) / 2);
     hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, part, (int)(g.x * g.y * g.z), rmax);

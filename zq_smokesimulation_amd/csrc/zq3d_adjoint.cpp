@@ -47,7 +47,11 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
         else
             launch_proj_setup64(d, u, v, w, occ, o.open_mask, 0, B.nc, B.b64, B.p64, B.part, s);
     }
-    launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
+    // the FP32-face forward set-up marches z (its own partial layout); the others run per plane
+    if (sizeof(F) == 4 && !adjoint)
+        launch_mean_finalize_mp(d, B.part, &sc->mean64, o.closed_mean, s);
+    else
+        launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
     double tol = o.tol;
     if (adjoint || bmax_out) {
         double bmax = 0.0;
@@ -56,13 +60,12 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
         if (adjoint) tol = rtol * bmax;
         if (bmax_out) *bmax_out = bmax;
     }
-    int outer = 0;
-    double rmax = 0.0, md = -1.0;
     // the forward checks the divergence its stored faces will have; the adjoint's faces are not
     // projected fields (its correction goes on every face)
     const DivCheck chk{u, v, w, occ, o.open_mask, sizeof(F) == 8};
-    if (int rc = refine_mixed(d, o, tol, B, s, nullptr, where, adjoint ? nullptr : &chk, &outer, &rmax, &md))
-        return rc;
+    RefineOut ro;
+    if (int rc = refine_mixed(d, o, tol, B, s, nullptr, where, adjoint ? nullptr : &chk, &ro)) return rc;
+    double md = ro.dmax;
     if (adjoint) {
         if (o.closed_mean) launch_remove_mean_p64(d, occ, B.p64, B.part, &sc->aux64, s);
         launch_correct_all64<F>(d, B.p64, u, v, w, s);
@@ -76,9 +79,9 @@ int project_mp(const Dims& d, F* u, F* v, F* w, const uint8_t* occ, const zq3d_s
     }
     ZQ_HIP(hipGetLastError(), where);
     if (st) {
-        st->outer = outer;
-        st->sweeps = outer * o.iters;
-        st->max_residual = rmax;
+        st->outer = ro.outer;
+        st->sweeps = (int)ro.sweeps;
+        st->max_residual = ro.rmax;
         st->max_div = md;
         st->mean = 0.0;
         st->converged = adjoint ? -1 : (md <= o.tol ? 1 : 0);
@@ -111,6 +114,7 @@ struct zq3d_adj {
     double *b64, *p64, *p64_adj, *part;
     Scalars* sc;
     Readback rb;  // pinned read-back slots of the projections' reductions
+    hipEvent_t gate_ev[2] = {nullptr, nullptr};  // progress checks of gated passes
     // checkpoints: ncp x (u, v, w, T, rho values + p64)
     char* cp;
     size_t cp_bytes;
@@ -147,7 +151,8 @@ ProjBufs proj_bufs(zq3d_adj* a, double* p64) {
                           {a->split_b, a->split_b + sc},
                           {a->split_n, a->split_n + sc},
                           {a->split_p + 2 * sp, a->split_p + 3 * sp}};
-    return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, a->shape, split, a->rb};
+    return ProjBufs{a->nc, a->b, a->p, a->p_alt, a->b64, p64, a->part, a->sc, a->shape, split, a->rb,
+                    {a->gate_ev[0], a->gate_ev[1]}};
 }
 
 // pointers into one tape slot
@@ -445,6 +450,8 @@ extern "C" int zq3d_adj_create(const zq3d_config* cfg, int precision, int nsteps
     A((void**)&a->part, (size_t)reduction_partials(a->d) * 8);
     A((void**)&a->sc, sizeof(Scalars));
     bad |= a->rb.alloc() != hipSuccess;
+    bad |= hipEventCreateWithFlags(&a->gate_ev[0], hipEventDisableTiming) != hipSuccess;
+    bad |= hipEventCreateWithFlags(&a->gate_ev[1], hipEventDisableTiming) != hipSuccess;
     a->cp_bytes = (faces + 2 * z.N) * e + z.Np * 8;
     a->cp_bytes = (a->cp_bytes + 255) & ~(size_t)255;
     A((void**)&a->cp, a->cp_bytes * a->ncp);
@@ -485,6 +492,8 @@ extern "C" void zq3d_adj_destroy(zq3d_adj* a) {
     if (!a) return;
     for (void* p : a->allocs) (void)hipFree(p);
     a->rb.release();
+    for (hipEvent_t e : {a->gate_ev[0], a->gate_ev[1]})
+        if (e) (void)hipEventDestroy(e);
     delete a;
 }
 

@@ -51,7 +51,7 @@ namespace {
 
 // ---------------- grow-only workspace for the host-pointer entry points ----------------
 enum Slot { S_U, S_V, S_W, S_OCC, S_T, S_R, S_T2, S_R2, S_VEL4, S_SCR, S_NC, S_B, S_P, S_P2, S_B64, S_P64, S_PART, S_SCAL,
-            S_BFECC, S_COUNT };
+            S_BFECC, S_SPLIT, S_COUNT };
 
 struct Workspace {
     void* ptr[S_COUNT] = {};
@@ -135,11 +135,88 @@ hipError_t timed_sor(const Dims& d, const uint8_t* nc, const float* b, const flo
     return account_sweeps(d, shape, iters, pf);
 }
 
+float gate_threshold(double tol, float omega) { return (float)(GATE_THETA * tol * (double)omega / 6.0); }
+
+namespace {
+
+// launches per progress check of a gated pass: the iterations already queued when the check sees
+// the pass end return at once (a few microseconds each)
+constexpr int GATE_CHUNK = 8;
+
+unsigned float_bits(float x) {
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    return u;
+}
+
+// One gated pass on split storage: up to o.iters iterations from a zero correction, queued in
+// chunks; with pinned read-back and events the host checks after each chunk (one chunk behind, so
+// the device never waits) whether an iteration returned at once, and stops queueing.  Then p64 += e.
+// The executed count is in B.sc->done (and pinned slot 2) once the stream passes the update.
+int gated_pass(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
+               const char* where) {
+    const GateBufs g = B.gate();
+    const unsigned thresh = float_bits(gate_threshold(tol, o.omega));
+    const int n = o.iters;
+    launch_gate_reset(g, n, s);
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (pf) ZQ_HIP(pf->next(&ea, &eb), where);
+    if (ea) ZQ_HIP(hipEventRecord(ea, s), where);
+    const bool poll = B.rb.host && B.gate_ev[0] && B.gate_ev[1];
+    int launched = 0;
+    for (int k = 0; launched < n; k++) {
+        const int m = std::min(GATE_CHUNK, n - launched);
+        launch_gated_sweeps(d, B.split, launched, m, &B.sc->zero32, o.omega, B.shape, g, thresh, s);
+        launched += m;
+        if (!poll) continue;
+        ZQ_HIP(hipEventRecord(B.gate_ev[k & 1], s), where);
+        if (k >= 1) {
+            ZQ_HIP(hipEventSynchronize(B.gate_ev[(k - 1) & 1]), where);
+            const long long ran = (long long)((volatile unsigned long long*)B.rb.host)[2];
+            if (ran < launched - m) break;  // an iteration up to the previous chunk returned at once
+        }
+    }
+    if (eb) ZQ_HIP(hipEventRecord(eb, s), where);
+    ZQ_HIP(hipGetLastError(), where);
+    launch_update_p64_gated(d, B.nc, B.split, g.done, B.p64, s);
+    return ZQ3D_OK;
+}
+
+// iterations the last gated pass ran (the stream has passed it: called after a read_result)
+int gated_count(const ProjBufs& B, hipStream_t s, const char* where, long long* ran) {
+    if (B.rb.host) {
+        *ran = (long long)((volatile unsigned long long*)B.rb.host)[2];
+        return ZQ3D_OK;
+    }
+    int v = 0;
+    ZQ_HIP(hipMemcpyAsync(&v, &B.sc->done, sizeof(v), hipMemcpyDeviceToHost, s), where);
+    ZQ_HIP(hipStreamSynchronize(s), where);
+    *ran = v;
+    return ZQ3D_OK;
+}
+
+}  // namespace
+
+int fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol) {
+    if (!fit || !(rprev > 0.0) || !(r < rprev) || !(r > tol) || nprev < 1) return iters;
+    // per-sweep decay over the last pass, rho = (r / rprev)^(1/nprev); sweeps to tol at that rate,
+    // doubled because the decay slows as the smooth error modes take over
+    const double need = 2.0 * (double)nprev * (std::log(tol / r) / std::log(r / rprev));
+    if (!(need < (double)iters)) return iters;
+    const int n = (int)std::ceil(need);
+    return n < 1 ? 1 : n;
+}
+
 int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
-                 const char* where, const DivCheck* chk, int* outer_out, double* rmax_out, double* dmax_out) {
+                 const char* where, const DivCheck* chk, RefineOut* out) {
     const Sizes z(d);
     Scalars* sc = B.sc;
-    const bool split = is_split(B.shape);
+    const bool gated = o.pass_rule == 2;
+    if (gated && !B.has_split()) {
+        set_err(where, "pass_rule 2 needs the colour-split buffers");
+        return ZQ3D_ERR_ARG;
+    }
+    const bool split = gated || is_split(B.shape);
     int pair = 0;  // split pressure pair holding the correction
     float* e = B.p;
     float* e_alt = B.p_alt;
@@ -150,14 +227,23 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
     } else {
         launch_fill_f32(e, z.Np, 0.0f, s);
     }
-    int outer = 0;
-    double rmax = 0.0, dmax = -1.0, dprev = -1.0;
+    int outer = 0, nprev = 0;
+    long long sweeps = 0;
+    bool count_pending = false;  // a gated pass ran; its count is read after the next wait
+    double rmax = 0.0, rprev = -1.0, dmax = -1.0, dprev = -1.0;
     for (;;) {
         if (split)
             launch_residual64_split(d, B.nc, B.b64, &sc->mean64, B.p64, B.split, B.part, B.target(0, &sc->rmax), s);
         else
             launch_residual64(d, B.nc, B.b64, &sc->mean64, B.p64, B.b, B.part, B.target(0, &sc->rmax), s);
         if (int rc = read_result(B, 0, &sc->rmax, &rmax, s, where)) return rc;
+        if (count_pending) {
+            long long ran = 0;
+            if (int rc = gated_count(B, s, where, &ran)) return rc;
+            sweeps += ran;
+            if (pf) pf->launches += ran;  // one launch per gated iteration
+            count_pending = false;
+        }
         if (!(rmax > tol)) {
             if (!chk) break;
             // the residual is the divergence of the FP64-corrected faces; the correction rounds each
@@ -179,24 +265,34 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
         }
         if (outer >= o.max_outer) break;
         dmax = -1.0;
+        if (gated) {
+            if (int rc = gated_pass(d, o, tol, B, s, pf, where)) return rc;
+            count_pending = true;
+            outer++;
+            continue;
+        }
+        const int n = fitted_pass_length(o.iters, o.pass_rule == 1, rmax, rprev, nprev, tol);
         if (split) {
             hipEvent_t ea = nullptr, eb = nullptr;
             if (pf) ZQ_HIP(pf->next(&ea, &eb), where);
-            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, o.iters, B.shape, s, ea, eb);
-            ZQ_HIP(pf ? account_sweeps(d, B.shape, o.iters, pf) : hipGetLastError(), where);
+            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, n, B.shape, s, ea, eb);
+            ZQ_HIP(pf ? account_sweeps(d, B.shape, n, pf) : hipGetLastError(), where);
             launch_update_p64_split(d, B.nc, pair ? B.split.alt : B.split.p, B.p64, s);  // also zeroes e
         } else {
             float* res = nullptr;
-            ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, o.iters, s, pf, &res, B.shape, &B.split),
-                   where);
+            ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, n, s, pf, &res, B.shape, &B.split), where);
             if (res != e) std::swap(e, e_alt);
             launch_update_p64(d, B.nc, e, B.p64, s);  // also zeroes e for the next pass
         }
+        rprev = rmax;
+        nprev = n;
+        sweeps += n;
         outer++;
     }
-    *outer_out = outer;
-    *rmax_out = rmax;
-    if (dmax_out) *dmax_out = dmax;
+    out->outer = outer;
+    out->sweeps = sweeps;
+    out->rmax = rmax;
+    out->dmax = dmax;
     return ZQ3D_OK;
 }
 
@@ -261,11 +357,11 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
     Scalars* sc = B.sc;
     launch_fill_f32(&sc->zero32, 1, 0.0f, s);
     launch_proj_setup(d, u, v, w, occ, o.open_mask, B.nc, nullptr, B.b64, nullptr, B.p64, B.part, s);
-    launch_mean_finalize(d, B.part, nullptr, &sc->mean64, o.closed_mean, s);
-    int outer = 0;
-    double rmax = 0.0, md = -1.0;
+    launch_mean_finalize_mp(d, B.part, &sc->mean64, o.closed_mean, s);
     const DivCheck chk{u, v, w, occ, o.open_mask, false};
-    if (int rc = refine_mixed(d, o, o.tol, B, s, pf, "project_mixed", &chk, &outer, &rmax, &md)) return rc;
+    RefineOut ro;
+    if (int rc = refine_mixed(d, o, o.tol, B, s, pf, "project_mixed", &chk, &ro)) return rc;
+    double md = ro.dmax;
     launch_correct64(d, occ, o.open_mask, B.p64, u, v, w, s);
     if (!st) return ZQ3D_OK;
     if (md < 0.0) {  // the passes ran out before the residual met tol: measure the stored faces
@@ -275,9 +371,9 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
     double mean64 = 0.0;
     ZQ_HIP(hipMemcpyAsync(&mean64, &sc->mean64, sizeof(mean64), hipMemcpyDeviceToHost, s), "project_mixed");
     ZQ_HIP(hipStreamSynchronize(s), "project_mixed");
-    st->outer = outer;
-    st->sweeps = outer * o.iters;
-    st->max_residual = rmax;
+    st->outer = ro.outer;
+    st->sweeps = (int)ro.sweeps;
+    st->max_residual = ro.rmax;
     st->max_div = md;
     st->mean = mean64;
     st->converged = md <= o.tol ? 1 : 0;
@@ -287,6 +383,10 @@ int project_mixed(const Dims& d, float* u, float* v, float* w, const uint8_t* oc
 int check_sor_opts(const zq3d_sor_opts& o, const char* where) {
     if (o.iters < 0 || o.max_outer < 0 || !(o.omega > 0.0f && o.omega < 2.0f) || (o.open_mask & ~63)) {
         set_err(where, "invalid SOR options (iters >= 0, 0 < omega < 2, open_mask in [0,63])");
+        return ZQ3D_ERR_ARG;
+    }
+    if (o.pass_rule < 0 || o.pass_rule > 2 || (o.pass_rule == 2 && o.iters > GATE_SLOTS)) {
+        set_err(where, "pass_rule must be 0, 1 or 2 (with 2, iters <= 8192)");
         return ZQ3D_ERR_ARG;
     }
     if (o.mixed_precision && (o.max_outer < 1 || o.iters < 1 || !(o.tol >= 0.0))) {
@@ -569,9 +669,22 @@ float solve_host(float* u, float* v, float* w, const bool* occupy, int W_, int H
     E(hipMemcpy(dv, v, z.Nv * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(dw, w, z.Nw * 4, hipMemcpyHostToDevice));
     E(hipMemcpy(docc, occupy, z.N, hipMemcpyHostToDevice));
+    // gated passes run on colour-split storage: pressure pairs, right-hand sides, counts
+    SplitBufs split{};
+    if (o.mixed_precision && o.pass_rule == 2) {
+        const size_t sp = split_padded_size(d), sc = split_cell_size(d);
+        if (W.ensure(S_SPLIT, 4 * sp * 4 + 2 * sc * 4 + 2 * sc)) {
+            set_err(where, "device allocation failed");
+            return -1.0f;
+        }
+        float* f = (float*)W.ptr[S_SPLIT];
+        uint8_t* n8 = (uint8_t*)(f + 4 * sp + 2 * sc);
+        split = SplitBufs{{f, f + sp}, {f + 4 * sp, f + 4 * sp + sc}, {n8, n8 + sc}, {f + 2 * sp, f + 3 * sp}};
+        launch_fill_f32(f, 4 * sp, 0.0f, 0);  // padding slots hold 0
+    }
     const ProjBufs B{(uint8_t*)W.ptr[S_NC],  (float*)W.ptr[S_B],      (float*)W.ptr[S_P],
                      (float*)W.ptr[S_P2],    (double*)W.ptr[S_B64],   (double*)W.ptr[S_P64],
-                     (double*)W.ptr[S_PART], (Scalars*)W.ptr[S_SCAL], SorShape{0, 0}, SplitBufs{}};
+                     (double*)W.ptr[S_PART], (Scalars*)W.ptr[S_SCAL], SorShape{0, 0}, split};
     launch_fill_f32(B.p_alt, z.Np, 0.0f, 0);  // zero ghost layer of the second sweep buffer
     int rc;
     if (o.mixed_precision) {
@@ -685,6 +798,7 @@ struct zq3d_sim {
     zq3d_timing tm{};
     SorShape shape{0, 0};  // fused sweep tile shape chosen by zq3d_autotune_sor / zq3d_set_sor_shape
     Readback rb;           // pinned read-back slots of the projection's reductions
+    hipEvent_t gate_ev[2] = {nullptr, nullptr};  // progress checks of gated passes
     CoarseState coarse;    // sub_size > 0: the coarse grid of the coarse + sub-grid projection
     OctreeState octree;    // octree != 0: the leaves and level arrays of the octree projection
 };
@@ -717,7 +831,8 @@ ProjBufs proj_bufs(zq3d_sim* s) {
                           {s->split_b, s->split_b + sc},
                           {s->split_n, s->split_n + sc},
                           {s->split_p + 2 * sp, s->split_p + 3 * sp}};
-    return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc, s->shape, split, s->rb};
+    return ProjBufs{s->nc, s->b, s->p, s->p_alt, s->b64, s->p64, s->part, s->sc, s->shape, split, s->rb,
+                    {s->gate_ev[0], s->gate_ev[1]}};
 }
 
 // Times one stage of a simulation with an event pair when profiling is on.
@@ -849,6 +964,8 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     A(dalloc(&s->split_n, 2 * split_cell_size(s->d)));
     A(dalloc(&s->sc, sizeof(Scalars)));
     A(s->rb.alloc());
+    A(hipEventCreateWithFlags(&s->gate_ev[0], hipEventDisableTiming));
+    A(hipEventCreateWithFlags(&s->gate_ev[1], hipEventDisableTiming));
     if (cfg->sor.mixed_precision) {
         A(dalloc(&s->b64, z.N * 8));
         A(dalloc(&s->p64, z.Np * 8));
@@ -901,7 +1018,7 @@ extern "C" void zq3d_destroy(zq3d_sim* s) {
                     s->b64, s->p64, s->sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    for (hipEvent_t e : {s->ev[0], s->ev[1]})
+    for (hipEvent_t e : {s->ev[0], s->ev[1], s->gate_ev[0], s->gate_ev[1]})
         if (e) (void)hipEventDestroy(e);
     s->sorprof.release();
     s->rb.release();

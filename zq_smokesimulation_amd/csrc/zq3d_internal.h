@@ -46,8 +46,11 @@ struct Scalars {
     double aux64;   // the removed pressure mean of an adjoint projection
     unsigned long long rmax;
     unsigned long long dmax;
+    unsigned long long done_sink;  // stands in for the mapped host word when there is none
     float mean32;
     float zero32;
+    int done;                 // iterations a gated pass ran
+    unsigned gate[GATE_SLOTS * GATE_LANES];  // per iteration of a gated pass: largest update (FP32 bits)
 };
 
 // Collects the first HIP error of a sequence of calls.
@@ -135,10 +138,15 @@ struct ProjBufs {
     SorShape shape;  // sweep algorithm / tile shape ({0, 0}: default)
     SplitBufs split; // colour-split scratch (used when shape.ty == SOR_SPLIT)
     Readback rb{};   // host read-back slots (optional)
+    hipEvent_t gate_ev[2] = {nullptr, nullptr};  // progress checks of a gated pass (optional)
     // where a reduction result of `slot` goes: the pinned slot, or the device scalar
     unsigned long long* target(int slot, unsigned long long* dev_scalar) const {
         return rb.dev ? rb.dev + slot : dev_scalar;
     }
+    // the gate words of a gated pass; the executed count goes to pinned slot 2 when there is one
+    GateBufs gate() const { return GateBufs{sc->gate, &sc->done, rb.dev ? rb.dev + 2 : &sc->done_sink}; }
+    // split storage present (the gated passes need it)
+    bool has_split() const { return split.p[0] && split.p[1] && split.alt[0] && split.alt[1] && split.b[0]; }
 };
 
 // the value a reduction left in `slot` (pinned) or in dev_scalar (copied back); waits for the stream
@@ -161,14 +169,31 @@ struct DivCheck {
     bool fp64;
 };
 // Mixed-precision refinement of B.p64: FP64 residual, FP32 SOR on the correction e (zero start),
-// p64 += e, until max|r| <= tol or o.max_outer passes.  With chk, a residual within tol is not yet
-// the end: the corrected faces' max|div - mean| is predicted (the correction rounds every face to
-// the face type once more) and the passes go on while it exceeds tol and each pass still shrinks it
-// below 0.9 of the previous prediction (past that, the faces' rounding dominates); *dmax receives
-// that value, or -1 when the last pass left it unevaluated.  With the colour-split sweep algorithm the residual
-// and e stay in split storage across passes (counts split once per call).
+// p64 += e, until max|r| <= tol or o.max_outer passes (pass lengths: fitted_pass_length).  With chk,
+// a residual within tol is not yet the end: the corrected faces' max|div - mean| is predicted (the
+// correction rounds every face to the face type once more) and the passes go on while it exceeds
+// tol and each pass still shrinks it below 0.9 of the previous prediction (past that, the faces'
+// rounding dominates); out->dmax receives that value, or -1 when the last pass left it unevaluated.
+// With the colour-split sweep algorithm the residual and e stay in split storage across passes
+// (counts split once per call).
+// Largest-update threshold of a gated pass: an iteration whose updates all satisfy |new - old| <=
+// this bound ends the pass.  The update of a cell with n neighbours is omega * r / n, r its
+// residual at that moment, so the bound asks |r| <= GATE_THETA * tol at every cell; measured on the
+// plume at 128^3 the FP64 residual after such an iteration was 0.6-1.3 times 6 max|update| / omega.
+constexpr double GATE_THETA = 0.75;
+float gate_threshold(double tol, float omega);
+struct RefineOut {
+    int outer = 0;       // passes run
+    long long sweeps = 0;  // red-black iterations run over all passes
+    double rmax = 0.0;   // last max|r|
+    double dmax = -1.0;  // last predicted max|div - mean| (-1: not evaluated after the last pass)
+};
 int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBufs& B, hipStream_t s, SorProf* pf,
-                 const char* where, const DivCheck* chk, int* outer, double* rmax, double* dmax);
+                 const char* where, const DivCheck* chk, RefineOut* out);
+// Sweeps of the next refinement pass (zq3d_sor_opts.pass_rule): r is the max|r| the last pass left,
+// rprev the one before it and nprev that pass's sweeps.  The same function in the same operations
+// as the oracle's, so both choose the same lengths from the same residuals.
+int fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol);
 
 // Plain FP32 projection on device pointers: p = 0, setup, SOR, correct.  mean_mode: 0 = value
 // `mean` given, 1 = FP64 device mean over fluid cells.

@@ -72,6 +72,8 @@ void launch_proj_setup_free(const Dims& d, const float* u, const float* v, const
                             int open_mask, uint8_t* nc, double* b64, double* p64_pad, double* part, hipStream_t s);
 // mean32/mean64 = sum/count of the setup partials (0 when want == 0)
 void launch_mean_finalize(const Dims& d, const double* part, float* mean32, double* mean64, int want, hipStream_t s);
+// the same fold over the mixed-precision set-up's partials (launch_proj_setup with b64)
+void launch_mean_finalize_mp(const Dims& d, const double* part, double* mean64, int want, hipStream_t s);
 // Sweep algorithm / tile shape: ty rows per block (2, 4 or 8) and zc planes per chunk of the fused
 // one-pass kernel on interleaved storage; ty == SOR_SPLIT selects single-colour sweeps on
 // colour-split storage; ty == -r (r = 4, 8 or 16) selects one launch per red-black iteration on
@@ -91,6 +93,26 @@ struct SplitBufs {
     uint8_t* n[2];
     float* alt[2];
 };
+// Gated passes (zq3d_sor_opts.pass_rule 2): per-iteration largest-update words, the executed
+// iteration count, and the mapped host word the count is copied to.
+constexpr int GATE_SLOTS = 8192;  // iterations per gated pass at most
+constexpr int GATE_K = 16;        // a gated pass also ends when the updates stop shrinking over GATE_K iterations
+constexpr int GATE_LANES = 8;     // words per iteration (their maximum is the iteration's largest update)
+struct GateBufs {
+    unsigned* gate;               // GATE_SLOTS x GATE_LANES words
+    int* done;
+    unsigned long long* host_done;  // device address of a mapped host word
+};
+// zero gate[0..n) and *done
+void launch_gate_reset(const GateBufs& g, int n, hipStream_t s);
+// iterations it0 .. it0+count-1 of a gated pass on split storage: iteration it reads pair (it & 1 ?
+// alt : p), writes the other, and returns at once once an iteration's largest update was <= thresh
+// (FP32 bits); iteration 0 takes the source as zero
+void launch_gated_sweeps(const Dims& d, const SplitBufs& sp, int it0, int count, const float* mean32, float omega,
+                         SorShape shape, const GateBufs& g, unsigned thresh, hipStream_t s);
+// p64 += e from the pair the gated pass ended in
+void launch_update_p64_gated(const Dims& d, const uint8_t* nc, const SplitBufs& sp, const int* done, double* p64_pad,
+                             hipStream_t s);
 size_t split_padded_size(const Dims& d);
 size_t split_cell_size(const Dims& d);
 // iters red-black SOR iterations starting from p_pad.  With p_alt (same shape, ghost layer zero)

@@ -37,6 +37,10 @@ __device__ __forceinline__ int split_q(int Wh, int H, int xh, int y, int z) {
 // m is the mean subtracted from the right-hand side.  A caller that runs several tiles in one
 // workgroup must synchronise the workgroup between them (the LDS rings are reused).
 // (TX is the widest tile; a row's last tile may be narrower.)
+// GATE: dmax accumulates, as FP32 bits, max |new - old| over the cells this tile updates (red on its
+// own slots and planes, black), the size of the iteration's last updates that decides whether a
+// gated pass goes on.  ZS: the source pair is taken as all zeros and not read (the first iteration
+// of a pass that starts from a zero correction).
 constexpr int SPLIT_TX = 64;  // tile width of the single-grid kernel
 
 struct SplitCellIn {
@@ -55,12 +59,13 @@ struct SplitItem {
     int bits;  // 1: a cell when the colour's parity is 0, 2: when it is 1, 4: tile-owned, 8: ys odd
 };
 
-template <int TX, int TY, int NW>
+template <int TX, int TY, int NW, bool GATE = false, bool ZS = false>
 __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1,
                                                   const float* __restrict__ b0, const float* __restrict__ b1, float m,
                                                   const float* __restrict__ s0, const float* __restrict__ s1,
                                                   float* __restrict__ d0, float* __restrict__ d1, const Dims& d,
-                                                  int Wh, int x0, int y0, int z0, int z1, float omega, float omw) {
+                                                  int Wh, int x0, int y0, int z0, int z1, float omega, float omw,
+                                                  unsigned* dmax = nullptr) {
     constexpr int KX = TX + 4, KY = TY + 4, RX = TX + 2, RY = TY + 2;  // ring plane row strides, rows
     constexpr int KPL = KY * KX, RPL = RY * RX;                        // floats per ring plane
     constexpr int NT = 64 * NW;
@@ -127,7 +132,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const bool zin = z >= -1 && z <= d.D;
         const int zo = (z + 1) * SZ;
 #pragma unroll
-        for (int i = 0; i < NK; i++) o[i] = s1[(zin && ki[i].bits) ? zo + ki[i].off : 0];
+        for (int i = 0; i < NK; i++) o[i] = ZS ? 0.0f : s1[(zin && ki[i].bits) ? zo + ki[i].off : 0];
     };
     auto store_k = [&](int z, const float (&o)[NK]) {
         float* K = Ks + ((z + 10) % 5) * KPL;
@@ -147,7 +152,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
             const int q = ok ? zo + ri[i].off : 0;
             const int n = n0[oh];
             const float bv = b0[oh];
-            const float pv = s0[q];
+            const float pv = ZS ? 0.0f : s0[q];
             o[i] = SplitCellIn{pv, bv, ok ? n : -1};
         }
     };
@@ -177,6 +182,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const float upd = omw * c.p + omega * pg;
         const float pc = c.n > 0 ? upd : c.p;
         if (c.n >= 0 && own_plane && (it.bits & 4)) d0[(t + 1) * SZ + it.off] = pc;
+        if (GATE && c.n > 0 && own_plane && (it.bits & 4)) *dmax = max(*dmax, __float_as_uint(fabsf(pc - c.p)));
         Rs[((t + 8) & 3) * RPL + it.lds] = c.n >= 0 ? pc : 0.0f;
     };
     auto black = [&](const SplitItem& it, int t, const SplitCellIn& c) {
@@ -191,6 +197,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const float pg = (sum - bv) * rtab[c.n & 7];
         const float upd = omw * p0 + omega * pg;
         if (c.n >= 0) d1[(t + 1) * SZ + it.off] = c.n > 0 ? upd : p0;
+        if (GATE && c.n > 0) *dmax = max(*dmax, __float_as_uint(fabsf(upd - p0)));
     };
 
     float kc[NK];

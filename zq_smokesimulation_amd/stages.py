@@ -141,11 +141,12 @@ def matrix_mul(A, B):
 
 
 def solve_ex(u, v, w, occ, open_mask=_lib.OPEN_ALL, closed_mean=False, omega=1.0, iters=100, mixed=False,
-             max_outer=0, tol=1e-6):
+             max_outer=0, tol=1e-6, pass_rule=0):
     """ZQ3D_SolvePoissonRedBlack_ex: returns (ms, stats dict)."""
     W, H, D = _dims_from_occ(occ)
     _check_faces(u, v, w, W, H, D)
-    o = _lib.SorOpts(open_mask, int(closed_mean), omega, int(iters), int(mixed), int(max_outer), tol)
+    o = _lib.SorOpts(open_mask, int(closed_mean), omega, int(iters), int(mixed), int(max_outer), tol,
+                     int(pass_rule))
     st = _lib.ProjStats()
     ms = check(_lib.lib().ZQ3D_SolvePoissonRedBlack_ex(_p(u, np.float32), _p(v, np.float32), _p(w, np.float32),
                                                        _occ(occ), W, H, D, C.byref(o), C.byref(st)), "Solve_ex")
