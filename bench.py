@@ -78,6 +78,12 @@ def parse():
                          "1 (default) fitted from the last pass's residual decay; 2 gated: a pass sweeps until an "
                          "iteration's largest update meets the tolerance (at most --inner sweeps)")
     ap.add_argument("--tol", type=float, default=1e-6, help="max|div| target")
+    ap.add_argument("--warm-start", type=int, default=2, choices=(0, 1, 2),
+                    help="projection start (zq3d_config.warm_start): 0 zero, 1 the previous step's pressure, "
+                         "2 (default) extrapolated from the previous two steps' pressures")
+    ap.add_argument("--warm-extrapolate", type=float, default=1.0,
+                    help="with --warm-start 2: p1 + f (p1 - p2) (zq3d_config.warm_extrapolate; the library's "
+                         "default is 0.5, steadier on start-up transients; on this workload 1 measured best)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adj-window", type=int, default=64, help="steps of the forward+adjoint window (0: skip)")
     ap.add_argument("--ens-members", type=int, default=64, help="ensemble members (0: skip the ensemble leg)")
@@ -451,7 +457,7 @@ def main():
                        confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99, sor_open_mask=0,
                        sor_closed_mean=1, sor_omega=omega, sor_iters=args.inner, sor_mixed_precision=1,
                        sor_max_outer=args.max_outer, sor_tol=args.tol, sor_pass_rule=args.pass_rule,
-                       warm_start=1)
+                       warm_start=args.warm_start, warm_extrapolate=args.warm_extrapolate)
     sim.set_sphere_sources([SOURCE])
     st0 = make_state(W, H, D)
     p64_zero = np.zeros((D + 2, H + 2, W + 2), np.float64)
@@ -583,6 +589,8 @@ def main():
             "sor_omega_rule": "given" if args.omega is not None else "2/(1+sin(pi/n)), n = %d" % max(W, H, D),
             "sor_sweeps_per_pass": args.inner,
             "sor_pass_rule": args.pass_rule,
+            "warm_start": {0: "zero", 1: "previous pressure",
+                           2: "p1 + %g (p1 - p2)" % args.warm_extrapolate}[args.warm_start],
             "sor_tile_autotuned": best,
             "parallelism": "single GPU",
         },

@@ -267,7 +267,11 @@ typedef struct zq3d_config {
     int adv_vel_type;      /* enum AdvectVelocityType value */
     int adv_sca_type;      /* enum AdvectScalarType value */
     zq3d_sor_opts sor;     /* projection */
-    int warm_start;        /* mixed: keep the pressure between steps */
+    int warm_start;        /* mixed: the projection's starting pressure.  0: zero.  1: the previous step's.
+                              2: extrapolated in time from the previous two steps' pressures,
+                              p1 + warm_extrapolate (p1 - p2) (p1 alone on the first step after a reset or
+                              an upload of the pressure); the adjoint rollout starts each step from the
+                              previous step's pressure */
     /* 0: projection on the full grid (TOTAL_*_POISSON_SOR).  s in [1, 8] dividing width, height and
      * depth: the reference's coarse + sub-grid projection (COARSE_OPEN_POISSON_SOR, subSize = s,
      * ZQ_SmokeSimulationUtils3D.cpp:3688-3795): sor.iters face-fraction SOR iterations on the
@@ -290,6 +294,9 @@ typedef struct zq3d_config {
      * and sor.mixed_precision 0. */
     int octree;
     int octree_thresh[3];
+    /* warm_start 2: the extrapolation factor (default 0.5; 1 is linear extrapolation, which
+     * amplifies the previous solves' errors more and measured erratic on start-up transients) */
+    float warm_extrapolate;
 } zq3d_config;
 
 typedef struct zq3d_sim zq3d_sim;

@@ -880,6 +880,14 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
     return outer;
 }
 
+void orc_extrapolate_warm_start(size_t n, double* p, double* prev, int valid, double beta) {
+    for (size_t i = 0; i < n; i++) {
+        const double t = p[i];
+        if (valid) p[i] = t + beta * (t - prev[i]);
+        prev[i] = t;
+    }
+}
+
 int orc_project_cg64(int W, int H, int D, float* u, float* v, float* w, const uint8_t* occ, int open_mask,
                      int subtract_mean, double tol, int max_iter, double* p64) {
     Dims d{W, H, D};

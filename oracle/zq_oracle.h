@@ -20,6 +20,7 @@
 #ifndef ZQ_ORACLE_H
 #define ZQ_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -115,6 +116,9 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
  * above tol), the next pass runs the sweeps that pass's per-sweep decay (r/rprev)^(1/nprev) needs to
  * reach tol, doubled, at least 1 and at most iters; otherwise iters. */
 int orc_fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol);
+/* Starting pressure extrapolated in time (the device's warm_start 2): over n padded cells,
+ * p = p + beta (p - prev) and prev = the old p; valid == 0: prev = p only. */
+void orc_extrapolate_warm_start(size_t n, double* p64_pad, double* prev_pad, int valid, double beta);
 /* FP64 conjugate gradient on the same operator (pins the converged projection).
  * Writes corrected faces into u,v,w (FP32) and returns iterations. */
 int orc_project_cg64(int W, int H, int D, float* u, float* v, float* w, const uint8_t* occ, int open_mask,

@@ -90,6 +90,7 @@ def load(path):
     L.orc_project_mp.restype = i
     L.orc_fitted_pass_length.argtypes = [i, i, d, d, i, d]
     L.orc_fitted_pass_length.restype = i
+    L.orc_extrapolate_warm_start.argtypes = [C.c_size_t, _f64p, _f64p, i, d]
     L.orc_sor_gated.argtypes = [i, i, i, _u8p, _f32p, _f32p, f, i, f]
     L.orc_sor_gated.restype = i
     L.orc_gate_threshold.argtypes = [d, f]
@@ -235,6 +236,13 @@ def sor(ncount, b, p_pad, omega, iters):
 def sor_gated(ncount, b, p_pad, omega, iters, thresh):
     D, H, W = ncount.shape
     return int(lib().orc_sor_gated(W, H, D, ncount, b, p_pad, omega, iters, thresh))
+
+
+def extrapolate_warm_start(p64, prev, valid, beta=0.5):
+    """In place: p64 = p64 + beta (p64 - prev) (beta as float32, as the device config holds it),
+    prev = the old p64 (valid False: prev = p64 only)."""
+    assert p64.shape == prev.shape and p64.dtype == prev.dtype == np.float64
+    lib().orc_extrapolate_warm_start(p64.size, p64, prev, int(valid), float(np.float32(beta)))
 
 
 def gate_threshold(tol, omega):

@@ -306,3 +306,24 @@ def test_gated_sor_stops_after_the_first_small_iteration():
     assert np.array_equal(q.view(np.uint32), steps[37].view(np.uint32))
     # the threshold of the gated refinement passes
     assert O.gate_threshold(1e-6, 1.5) == np.float32(0.75 * 1e-6 * np.float64(np.float32(1.5)) / 6.0)
+
+
+def test_extrapolated_warm_start():
+    """orc_extrapolate_warm_start: p <- p + beta (p - prev) and prev <- the old p (beta 1: a straight
+    line through the last two pressures, evaluated one step on); without a valid prev only prev <- p."""
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=(4, 5, 6))
+    prev = rng.normal(size=(4, 5, 6))
+    p0, q0 = p.copy(), prev.copy()
+    O.extrapolate_warm_start(p, prev, True)  # default factor 0.5
+    assert np.array_equal(prev, p0) and np.array_equal(p, p0 + 0.5 * (p0 - q0))
+    # beta = 1 is exact on a pressure that changes linearly in time
+    a, b = rng.normal(size=(3, 3, 3)), rng.normal(size=(3, 3, 3))
+    p1, p2 = a + 1.0 * b, a + 2.0 * b
+    q2 = p2.copy()
+    O.extrapolate_warm_start(q2, p1.copy(), True, beta=1.0)
+    np.testing.assert_allclose(q2, a + 3.0 * b, rtol=0, atol=1e-12)
+    p5 = p0.copy()
+    q5 = np.zeros_like(p5)
+    O.extrapolate_warm_start(p5, q5, False)
+    assert np.array_equal(p5, p0) and np.array_equal(q5, p0)

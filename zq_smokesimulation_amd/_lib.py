@@ -71,6 +71,7 @@ class Config(C.Structure):
         ("coarse_flux", C.c_int),
         ("octree", C.c_int),
         ("octree_thresh", C.c_int * 3),
+        ("warm_extrapolate", C.c_float),
     ]
 
 

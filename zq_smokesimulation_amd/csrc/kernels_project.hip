@@ -469,6 +469,17 @@ __global__ __launch_bounds__(64 * NW) void k_sor_split_rb_gated(
     }
 }
 
+// Starting pressure extrapolated in time (zq3d_config.warm_start 2): p = p + beta (p - prev) and
+// prev = the old p, over the padded array (ghosts stay 0); with valid == 0 only prev = p.
+__global__ __launch_bounds__(256) void k_extrapolate_p64(double* __restrict__ p, double* __restrict__ prev, size_t n,
+                                                         int valid, double beta) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double t = p[i];
+        if (valid) p[i] = t + beta * (t - prev[i]);
+        prev[i] = t;
+    }
+}
+
 // Zeroes the gate words of a pass and its iteration count.
 __global__ __launch_bounds__(256) void k_gate_reset(unsigned* __restrict__ gate, int n, int* __restrict__ done) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) gate[i] = 0u;
@@ -942,6 +953,11 @@ void launch_gated_one(const Dims& d, const SplitBufs& sp, int it, const float* m
         hipLaunchKernelGGL((k_sor_split_rb_gated<TY, NW, false>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1],
                            sp.b[0], sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw,
                            g.gate, it, thresh, g.done, g.host_done);
+}
+
+void launch_extrapolate_p64(double* p64_pad, double* prev_pad, size_t n, int valid, float beta, hipStream_t s) {
+    hipLaunchKernelGGL(k_extrapolate_p64, dim3((unsigned)std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                       p64_pad, prev_pad, n, valid, (double)beta);
 }
 
 void launch_gate_reset(const GateBufs& g, int n, hipStream_t s) {

@@ -791,6 +791,8 @@ struct zq3d_sim {
     Scalars* sc = nullptr;
     std::vector<float> boxes, spheres;
     bool p64_valid = false;
+    double* p64_prev = nullptr;    // warm_start 2: the pressure of the step before p64's
+    bool p64_prev_valid = false;
     bool prof = false;      // stage timing (an event pair per stage, waited for)
     bool sor_prof = false;  // sweep timing (pooled event pairs, not waited for until read)
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -866,6 +868,11 @@ int check_config(const zq3d_config& c, const char* where) {
     }
     if (c.adv_vel_type < 0 || c.adv_vel_type > 5) { set_err(where, "unknown adv_vel_type"); return ZQ3D_ERR_ARG; }
     if (c.adv_sca_type < 0 || c.adv_sca_type > 3) { set_err(where, "unknown adv_sca_type"); return ZQ3D_ERR_ARG; }
+    if (c.warm_start < 0 || c.warm_start > 2) { set_err(where, "warm_start must be 0, 1 or 2"); return ZQ3D_ERR_ARG; }
+    if (c.warm_start == 2 && !(c.warm_extrapolate >= 0.0f && c.warm_extrapolate <= 2.0f)) {
+        set_err(where, "warm_extrapolate must be in [0, 2]");
+        return ZQ3D_ERR_ARG;
+    }
     if (c.sub_size < 0 || c.sub_size > COARSE_MAX_S ||
         (c.sub_size && (c.width % c.sub_size || c.height % c.sub_size || c.depth % c.sub_size))) {
         set_err(where, "sub_size must be 0 or in [1, 8] dividing width, height and depth");
@@ -927,6 +934,7 @@ extern "C" int zq3d_config_default(zq3d_config* c, int width, int height, int de
     c->sor.max_outer = 1000;  // maxOuter's default in the config reader
     c->sor.tol = 1e-6;
     c->warm_start = 1;
+    c->warm_extrapolate = 0.5f;
     c->octree_thresh[0] = 2;  // octree_thresh1..3 (ZQ_SmokeSimulationUtils3D.cpp:52-54)
     c->octree_thresh[1] = 4;
     c->octree_thresh[2] = 16;
@@ -969,6 +977,7 @@ extern "C" int zq3d_create(const zq3d_config* cfg, zq3d_sim** out) {
     if (cfg->sor.mixed_precision) {
         A(dalloc(&s->b64, z.N * 8));
         A(dalloc(&s->p64, z.Np * 8));
+        if (cfg->warm_start == 2) A(dalloc(&s->p64_prev, z.Np * 8));
     }
     if (cfg->adv_vel_type == 1 || cfg->adv_vel_type == 3 || cfg->adv_vel_type == 5)
         A(dalloc(&s->bfecc, bfecc_scratch_values(s->d) * 4));
@@ -1015,7 +1024,7 @@ extern "C" void zq3d_destroy(zq3d_sim* s) {
     if (!s) return;
     void* ptrs[] = {s->u, s->v, s->w, s->T, s->R, s->T2, s->R2, s->occ, s->vel4, s->scratch, s->bfecc, s->nc, s->b, s->p, s->p_alt, s->part,
                     s->split_p, s->split_b, s->split_n,
-                    s->b64, s->p64, s->sc};
+                    s->b64, s->p64, s->p64_prev, s->sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : {s->ev[0], s->ev[1], s->gate_ev[0], s->gate_ev[1]})
@@ -1048,7 +1057,10 @@ extern "C" int zq3d_upload(zq3d_sim* s, int field, const void* host, size_t byte
     if (bytes != n) { set_err(where, "size mismatch"); return ZQ3D_ERR_ARG; }
     ZQ_HIP(hipMemcpyAsync(p, host, n, hipMemcpyHostToDevice, s->stream), where);
     ZQ_HIP(hipStreamSynchronize(s->stream), where);
-    if (field == ZQ3D_FIELD_P64) s->p64_valid = true;
+    if (field == ZQ3D_FIELD_P64) {
+        s->p64_valid = true;
+        s->p64_prev_valid = false;  // the extrapolation starts again from the uploaded pressure
+    }
     return ZQ3D_OK;
 }
 
@@ -1127,9 +1139,17 @@ extern "C" int zq3d_project(zq3d_sim* s, zq3d_proj_stats* st) {
     } else if (c.octree) {
         rc = project_octree(s->octree, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
     } else if (c.sor.mixed_precision) {
-        if (!c.warm_start || !s->p64_valid) launch_fill_f64(s->p64, Sizes(s->d).Np, 0.0, s->stream);
+        if (!c.warm_start || !s->p64_valid) {
+            launch_fill_f64(s->p64, Sizes(s->d).Np, 0.0, s->stream);
+            s->p64_prev_valid = false;
+        } else if (c.warm_start == 2) {
+            launch_extrapolate_p64(s->p64, s->p64_prev, Sizes(s->d).Np, s->p64_prev_valid ? 1 : 0, c.warm_extrapolate,
+                                   s->stream);
+            s->p64_prev_valid = true;
+        }
         rc = project_mixed(s->d, s->u, s->v, s->w, s->occ, c.sor, proj_bufs(s), st, s->stream, pf);
         s->p64_valid = (rc == ZQ3D_OK);
+        if (!s->p64_valid) s->p64_prev_valid = false;
     } else {
         rc = project_plain(s->d, s->u, s->v, s->w, s->occ, c.sor, 0.0f, c.sor.closed_mean ? 1 : 0, proj_bufs(s), st,
                            s->stream, pf);

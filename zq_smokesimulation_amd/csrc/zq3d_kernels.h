@@ -103,6 +103,8 @@ struct GateBufs {
     int* done;
     unsigned long long* host_done;  // device address of a mapped host word
 };
+// warm_start 2: p64 += beta (p64 - prev), prev = the old p64 (valid == 0: prev = p64 only), n padded cells
+void launch_extrapolate_p64(double* p64_pad, double* prev_pad, size_t n, int valid, float beta, hipStream_t s);
 // zero gate[0..n) and *done
 void launch_gate_reset(const GateBufs& g, int n, hipStream_t s);
 // iterations it0 .. it0+count-1 of a gated pass on split storage: iteration it reads pair (it & 1 ?
