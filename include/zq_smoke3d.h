@@ -382,7 +382,13 @@ int zq3d_get_coarse_sweep(zq3d_sim* sim, int* k, int* zc);
  * correction, :3748-3755): while guide faces are set, every coarse face's delta gets
  * + coeff * (guide - averaged coarse face).  gu, gv, gw: host arrays of the coarse grid's faces
  * ((Wc+1)*Hc*Dc, Wc*(Hc+1)*Dc, Wc*Hc*(Dc+1) floats: the first three arrays of a .guidedat file at the
- * coarse resolution); gu == NULL turns guiding off.  Requires sub_size > 0. */
+ * coarse resolution); gu == NULL turns guiding off.  Requires sub_size > 0 or octree != 0.  With
+ * octree != 0, gu != NULL turns on the special-region projections of ZQ_SmokeOctreeGrid3D::
+ * ApplyGuidingVelocity (ZQ_SmokeOctreeGrid3D.cpp:546-720, the reference's CUDA branch, run between
+ * the octree's face correction and its sub-projection): each 6-connected region of fine cells in
+ * level-0 and level-1 leaves projected alone by the closed red-black solver on its bounding box,
+ * with min(sor.iters, region cells) iterations; that branch never adds the guide, so coeff and the
+ * arrays' values are not read.  Host work per projection: the level map and faces cross PCIe. */
 int zq3d_set_guide(zq3d_sim* sim, float coeff, const float* gu, const float* gv, const float* gw);
 /* name of the sweep kernel the current shape runs ("k_sor_split_rb", "k_sor_split", "k_sor_fused"
  * or "k_sor_half") */

@@ -219,6 +219,21 @@ void orc_oct_solve(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, co
                    float mean, float* p0, float* p1, float* p2, float* p3);
 void orc_oct_adjust(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, const uint8_t* l2, const uint8_t* l3,
                     const float* p0, const float* p1, const float* p2, const float* p3, float* u, float* v, float* w);
+/* The special-region projections of octree guiding (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity with
+ * guiding_specialRegion_useCUDA, the reference's build: ZQ_SmokeOctreeGrid3D.h:37, .cpp:546-720).
+ * Regions are the 6-connected components of the fine cells in level-0 or level-1 leaves
+ * (specialRegion_mask, :5563-5591), numbered in scan order (x fastest) of their first cell.  Each is
+ * projected alone, in order, by the closed red-black solver on its bounding box (orc_project, omega 1,
+ * open_mask 0): the box's cells outside the region occupied, the region's mean divergence (float,
+ * summed in scan order as u1 - u0 + v1 - v0 + w1 - w0) subtracted, min(iters, cells) iterations; the
+ * box's faces are written back except the u faces of its high x side (:700-712).  The guide's blended
+ * deltas (:265-545) are not applied on this path of the reference (only its PCG branch adds them), so
+ * no guide is read.  labels (optional, W*H*D): 0 outside the regions, else the region number.
+ * Returns the number of regions.  Parity unpinned against the reference's labelling, which seeds every
+ * plane in one pass and leaves cells reached only along z unlabelled (:5169-5320, see the design
+ * notes): a plain flood fill is used. */
+int orc_oct_special_regions(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, int iters, float* u, float* v,
+                            float* w, int32_t* labels);
 
 #ifdef __cplusplus
 }

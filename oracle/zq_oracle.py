@@ -133,6 +133,8 @@ def load(path):
                                 _f32p, _f32p, _f32p, _f32p]
     L.orc_oct_adjust.argtypes = [i, i, i, _u8p, _u8p, _u8p, _u8p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                  _f32p]
+    L.orc_oct_special_regions.argtypes = [i, i, i, _u8p, _u8p, i, _f32p, _f32p, _f32p, _i32p]
+    L.orc_oct_special_regions.restype = i
     return L
 
 
@@ -654,6 +656,15 @@ def octree_adjust(leaves, ps, u, v, w):
     lib().orc_oct_adjust(W, H, D, *leaves, *ps, u, v, w)
 
 
+def octree_special_regions(u, v, w, leaves, iters):
+    """The special-region projections of octree guiding, in place (orc_oct_special_regions).  Returns
+    (number of regions, labels)."""
+    D, H, W = leaves[0].shape
+    labels = np.zeros((D, H, W), np.int32)
+    n = lib().orc_oct_special_regions(W, H, D, leaves[0], leaves[1], int(iters), u, v, w, labels)
+    return n, labels
+
+
 def octree_sub_project(u, v, w, leaves, mats=None):
     """SubProjection (ZQ_SmokeOctreeGrid3D.cpp:840-1245): every level-L leaf's interior faces (L >= 1)
     through the regular-block operator of ZQ_InitSubMatPoisson3D with s = 2^L (:4994-5032).  At level 1
@@ -671,29 +682,33 @@ def octree_sub_project(u, v, w, leaves, mats=None):
                 w[z, y:y + 2, x:x + 2] = w[z, y, x]
 
 
-def octree_projection(u, v, w, occ, iters, thresh=OCTREE_THRESH, closed=False, mats=None):
-    """One octree projection in place (ZQ_SmokeSimulation3D.cpp:478-513 without guiding): build the
-    leaves, red-black solve over the levels (iters = maxIter), face correction, sub-projection of the
-    leaf blocks.  Returns the leaves and the pressures."""
+def octree_projection(u, v, w, occ, iters, thresh=OCTREE_THRESH, closed=False, mats=None, guiding=False):
+    """One octree projection in place (ZQ_SmokeSimulation3D.cpp:478-513): build the leaves, red-black
+    solve over the levels (iters = maxIter), face correction, with guiding the special-region
+    projections (ApplyGuidingVelocity), sub-projection of the leaf blocks.  Returns the leaves and the
+    pressures."""
     leaves = octree_leaves(occ, thresh)
     W, H, D = _dims(occ)
     mean = lib().orc_oct_closed_mean(W, H, D, np.ascontiguousarray(occ, np.uint8), u, v, w) if closed else 0.0
     ps = octree_solve(leaves, octree_divergence(u, v, w), iters, closed, mean)
     octree_adjust(leaves, ps, u, v, w)
+    if guiding:
+        octree_special_regions(u, v, w, leaves, iters)
     octree_sub_project(u, v, w, leaves, mats)
     return leaves, ps
 
 
-def octree_step(params, u, v, w, T, rho, occ, boxes=None, thresh=OCTREE_THRESH, closed=False, mats=None):
+def octree_step(params, u, v, w, T, rho, occ, boxes=None, thresh=OCTREE_THRESH, closed=False, mats=None,
+                guiding=False):
     """One step in the order of orc_step (ZQ_SmokeSimulation3D.cpp:156-243) with the octree projection
-    (OPEN_OCTREE_POISSON_SOR / CLOSED_OCTREE_POISSON_SOR, params' sor_iters passes) in place of the
-    full-grid solve."""
+    (OPEN_OCTREE_POISSON_SOR / CLOSED_OCTREE_POISSON_SOR, params' sor_iters passes; guiding: with the
+    special-region projections of useGuidingControl) in place of the full-grid solve."""
     P = dict(params)
     add_force(u, v, w, occ, T, rho, P["h"], P["dt"], P["beta"], P["eps"], P["Tamb"], alpha=P.get("alpha", 0.0))
     attenuation(u, v, w, T, rho, occ, P["aV"], P["aT"], P["aR"])
     dtt = float(np.float32(P["dt"]) / np.float32(P["steps"]))
     advect_velocity(u, v, w, occ, P["h"], P["steps"], dtt, P["adv_vel_type"], True)
-    octree_projection(u, v, w, occ, P["sor_iters"], thresh, closed, mats)
+    octree_projection(u, v, w, occ, P["sor_iters"], thresh, closed, mats, guiding)
     To, Ro = advect_scalar(u, v, w, occ, T, rho, P["h"], P["steps"], dtt, P["adv_sca_type"])
     T[...] = To
     rho[...] = Ro

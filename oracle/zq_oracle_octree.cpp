@@ -7,6 +7,7 @@
 // sweeps over the levels and the face correction.  The sub-projection of the leaf blocks reuses
 // orc_sub_project_regular (zq_oracle_coarse.cpp) from zq_oracle.py.
 // Only tests/ may load this; the product library never links or calls it.
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -354,6 +355,92 @@ void orc_oct_adjust(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, c
                     }
                 }
     }
+}
+
+int orc_oct_special_regions(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, int iters, float* u, float* v,
+                            float* w, int32_t* labels) {
+    const size_t N = (size_t)W * H * D;
+    const OG g{W, H, D};
+    auto cell = [&](int x, int y, int z) { return ((size_t)z * H + y) * W + x; };
+    std::vector<int32_t> lab(N, 0);
+    for (int z = 0; z < D; z++)
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++)
+                if (l0[g.at(0, x, y, z)] || l1[g.at(1, x >> 1, y >> 1, z >> 1)]) lab[cell(x, y, z)] = -1;
+    // flood fill in scan order of the seeds
+    int regions = 0;
+    std::vector<size_t> queue;
+    for (size_t o = 0; o < N; o++) {
+        if (lab[o] != -1) continue;
+        regions++;
+        lab[o] = regions;
+        queue.assign(1, o);
+        for (size_t h = 0; h < queue.size(); h++) {
+            const size_t c = queue[h];
+            const int x = (int)(c % W), y = (int)((c / W) % H), z = (int)(c / ((size_t)W * H));
+            const int nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y},
+                      nz[6] = {z, z, z, z, z + 1, z - 1};
+            for (int q = 0; q < 6; q++) {
+                if (nx[q] < 0 || nx[q] >= W || ny[q] < 0 || ny[q] >= H || nz[q] < 0 || nz[q] >= D) continue;
+                const size_t n = cell(nx[q], ny[q], nz[q]);
+                if (lab[n] == -1) {
+                    lab[n] = regions;
+                    queue.push_back(n);
+                }
+            }
+        }
+    }
+    const size_t Wu = (size_t)W + 1, Hv = (size_t)H + 1;
+    for (int r = 1; r <= regions; r++) {
+        int x0 = W, y0 = H, z0 = D, x1 = -1, y1 = -1, z1 = -1, count = 0;
+        float mean = 0.0f;
+        for (int z = 0; z < D; z++)
+            for (int y = 0; y < H; y++)
+                for (int x = 0; x < W; x++) {
+                    if (lab[cell(x, y, z)] != r) continue;
+                    count++;
+                    mean += u[((size_t)z * H + y) * Wu + x + 1] - u[((size_t)z * H + y) * Wu + x] +
+                            v[((size_t)z * Hv + y + 1) * W + x] - v[((size_t)z * Hv + y) * W + x] +
+                            w[((size_t)(z + 1) * H + y) * W + x] - w[((size_t)z * H + y) * W + x];
+                    x0 = std::min(x0, x), y0 = std::min(y0, y), z0 = std::min(z0, z);
+                    x1 = std::max(x1, x), y1 = std::max(y1, y), z1 = std::max(z1, z);
+                }
+        mean /= count;
+        const int sx = x1 - x0 + 1, sy = y1 - y0 + 1, sz = z1 - z0 + 1;
+        std::vector<uint8_t> bo((size_t)sx * sy * sz);
+        std::vector<float> bu((size_t)(sx + 1) * sy * sz), bv((size_t)sx * (sy + 1) * sz), bw((size_t)sx * sy * (sz + 1));
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++) bo[((size_t)k * sy + j) * sx + i] = lab[cell(x0 + i, y0 + j, z0 + k)] != r;
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i <= sx; i++)
+                    bu[((size_t)k * sy + j) * (sx + 1) + i] = u[((size_t)(z0 + k) * H + y0 + j) * Wu + x0 + i];
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j <= sy; j++)
+                for (int i = 0; i < sx; i++)
+                    bv[((size_t)k * (sy + 1) + j) * sx + i] = v[((size_t)(z0 + k) * Hv + y0 + j) * W + x0 + i];
+        for (int k = 0; k <= sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)
+                    bw[((size_t)k * sy + j) * sx + i] = w[((size_t)(z0 + k) * H + y0 + j) * W + x0 + i];
+        orc_project(sx, sy, sz, bu.data(), bv.data(), bw.data(), bo.data(), 0, mean, 1.0f, std::min(iters, count),
+                    nullptr);
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)  // the reference's copy-back stops short of the high x face
+                    u[((size_t)(z0 + k) * H + y0 + j) * Wu + x0 + i] = bu[((size_t)k * sy + j) * (sx + 1) + i];
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j <= sy; j++)
+                for (int i = 0; i < sx; i++)
+                    v[((size_t)(z0 + k) * Hv + y0 + j) * W + x0 + i] = bv[((size_t)k * (sy + 1) + j) * sx + i];
+        for (int k = 0; k <= sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)
+                    w[((size_t)(z0 + k) * H + y0 + j) * W + x0 + i] = bw[((size_t)k * sy + j) * sx + i];
+    }
+    if (labels) std::memcpy(labels, lab.data(), N * sizeof(int32_t));
+    return regions;
 }
 
 }  // extern "C"

@@ -62,7 +62,7 @@ def oracle_frames(path, frames, moving=False, sub_size=0, guides=None, guide_coe
             occ = scene.apply_moving_objects(path, f, u, v, w)
         if c.octree:
             O.octree_step(p, u, v, w, T, R, occ, boxes=boxes, thresh=tuple(c.octree_thresh),
-                          closed=bool(c.sor.closed_mean))
+                          closed=bool(c.sor.closed_mean), guiding=guides is not None)
         elif c.coarse_flux:
             # the *_FLUX_SOR solvers: the flux form on the coarse grid (s = 1 for TOTAL_*)
             O.coarse_step(p, u, v, w, T, R, occ, c.sub_size, boxes=boxes, open_mask=c.sor.open_mask,
@@ -231,14 +231,28 @@ def test_cli_moving_object(tmp_path):
         np.testing.assert_array_equal(got[f], want[f], err_msg="frame %d" % f)
 
 
-def test_cli_rejects_guiding_with_the_octree(tmp_path):
-    cfg = tmp_path / "scene.txt"
-    cfg.write_text(BASE + "useGuidingControl true\nguidingFold %s\nglobalGridSolver OPEN_OCTREE_POISSON_SOR\n"
-                   % tmp_path)
-    out = tmp_path / "out"
-    out.mkdir()
-    r = subprocess.run([CLI, str(cfg), str(out)], capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0 and "octree" in r.stdout and "not built" in r.stdout
+@pytest.mark.parametrize("resident", ["false", "true"])
+def test_cli_octree_guiding(tmp_path, resident):
+    """useGuidingControl with an octree solver: the guide files are loaded per frame as the reference
+    does, and the projection adds the special-region step (its CUDA branch reads no guide values)."""
+    s = 4
+    W, H, D = 32, 24, 16
+    gdir = tmp_path / "guide"
+    gdir.mkdir()
+    for f in range(3):
+        n = (W // s + 1) * (H // s) * (D // s) + (W // s) * (H // s + 1) * (D // s) + \
+            (W // s) * (H // s) * (D // s + 1) + 2 * (W // s) * (H // s) * (D // s)
+        (gdir / ("%d.guidedat" % f)).write_bytes(np.zeros(n, np.float32).tobytes())
+    text = BASE + ("frameCount 3\nintObj sphere\noctree_thresh1 1\noctree_thresh2 2\noctree_thresh3 4\n"
+                   "subSize %d\nglobalGridSolver OPEN_OCTREE_POISSON_SOR\nuseGuidingControl true\n"
+                   "guidingCoeff 0.3\nguidingFold %s\ndeviceResident %s\n" % (s, gdir, resident))
+    got = run_cli(tmp_path, text, 3)
+    want = oracle_frames(str(tmp_path / "scene.txt"), 3, guides=[None] * 3)
+    for f in range(3):
+        scale = max(1.0, float(np.abs(want[f]).max()))
+        np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-4 * scale, err_msg="frame %d" % f)
+    plain = oracle_frames(str(tmp_path / "scene.txt"), 3)
+    assert np.abs(plain[2] - want[2]).max() > 1e-4
 
 
 def test_cli_rejects_unbuilt_solver(tmp_path):

@@ -9,7 +9,9 @@ Bars:
 - leaf interiors: the reference applies the dense float32 block operator, the kernels the same
   projection matrix-free (DCT).  Tolerance: 2e-5 of the largest face magnitude;
 - closed domains through the simulation: the mean divergence is summed in FP64 on the device and in
-  float in scan order by the reference, so the fields agree to 2e-5 of the field scale.
+  float in scan order by the reference, so the fields agree to 2e-5 of the field scale;
+- guiding's special-region projections (closed red-black solves on each region's box): the same bars,
+  the regions' solves being bit-exact themselves.
 """
 import numpy as np
 import pytest
@@ -63,12 +65,14 @@ def exterior_masks(leaves):
     return mu, mv, mw
 
 
-def run_sim(W, H, D, u, v, w, occ, iters, thresh, closed=False, occ2=None):
+def run_sim(W, H, D, u, v, w, occ, iters, thresh, closed=False, occ2=None, guiding=False):
     from zq_smokesimulation_amd.sim import SmokeSim3D
     import zq_smokesimulation_amd._lib as L
     sim = SmokeSim3D(W, H, D, octree=1, octree_thresh=tuple(thresh), sor_iters=iters,
                      sor_closed_mean=1 if closed else 0)
     try:
+        if guiding:
+            sim.set_guide(0.5, gu=True)
         sim.set_state(u=u, v=v, w=w, occ=occ)
         st = sim.project()
         if occ2 is not None:
@@ -80,14 +84,15 @@ def run_sim(W, H, D, u, v, w, occ, iters, thresh, closed=False, occ2=None):
         sim.close()
 
 
-def check_projection(W, H, D, seed, centres, thresh, iters=30, sprinkle=0.0, closed=False, shell=False):
-    print("octree", (W, H, D), "thresh", thresh, "closed", closed, "seed", seed, flush=True)
+def check_projection(W, H, D, seed, centres, thresh, iters=30, sprinkle=0.0, closed=False, shell=False,
+                     guiding=False):
+    print("octree", (W, H, D), "thresh", thresh, "closed", closed, "seed", seed, "guiding", guiding, flush=True)
     rng = np.random.default_rng(seed)
     u, v, w = faces(rng, W, H, D)
     occ = scene(W, H, D, seed, centres, sprinkle, shell)
-    gu, gv, gw, st = run_sim(W, H, D, u, v, w, occ, iters, thresh, closed)
+    gu, gv, gw, st = run_sim(W, H, D, u, v, w, occ, iters, thresh, closed, guiding=guiding)
     ru, rv, rw = u.copy(), v.copy(), w.copy()
-    leaves, _ = O.octree_projection(ru, rv, rw, occ, iters, thresh, closed)
+    leaves, _ = O.octree_projection(ru, rv, rw, occ, iters, thresh, closed, guiding=guiding)
     scale = max(np.abs(ru).max(), np.abs(rv).max(), np.abs(rw).max())
     if not closed:
         mu, mv, mw = exterior_masks(leaves)
@@ -117,6 +122,25 @@ def test_non_cubic_grid_and_no_solids():
 
 def test_closed_domain_with_wall_shell():
     check_projection(40, 32, 32, 8, [(20.0, 16.0, 16.0)], (1, 2, 4), iters=40, closed=True, shell=True)
+
+
+@pytest.mark.parametrize("closed", [False, True])
+def test_guiding_special_regions_match_the_oracle(closed):
+    """useGuidingControl with the octree solvers: the special-region projections between the face
+    correction and the sub-projection (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity), to the same bars."""
+    W, H, D = 48, 40, 32
+    centres = [(12.0, 12.0, 10.0), (34.0, 27.0, 22.0)]
+    leaves, _ = check_projection(W, H, D, 13, centres, (1, 2, 4), iters=30, closed=closed, shell=closed,
+                                 guiding=True)
+    # the step did something: more than one region, and the fields differ from the unguided projection
+    n, _ = O.octree_special_regions(*O.zeros_faces(W, H, D), leaves, 0)
+    assert n >= (1 if closed else 2)  # the wall shell joins the closed scene's regions
+    rng = np.random.default_rng(13)
+    u, v, w = faces(rng, W, H, D)
+    occ = scene(W, H, D, 13, centres, 0.0, closed)
+    a = run_sim(W, H, D, u, v, w, occ, 30, (1, 2, 4), closed, guiding=True)
+    b = run_sim(W, H, D, u, v, w, occ, 30, (1, 2, 4), closed, guiding=False)
+    assert np.abs(a[0] - b[0]).max() > 1e-4
 
 
 def test_leaves_follow_a_changed_occupancy():

@@ -285,3 +285,40 @@ def test_closed_mean_is_the_float_mean_over_fluid_cells():
     got = O.lib().orc_oct_closed_mean(W, H, D, occ, u, v, w)
     d = fine_div(u.astype(np.float64), v.astype(np.float64), w.astype(np.float64))
     assert abs(got - d[occ == 0].mean()) < 1e-5
+
+
+def test_special_regions_are_scan_order_components_projected_alone():
+    """Octree guiding's special regions (ZQ_SmokeOctreeGrid3D.cpp:546-720): the 6-connected components
+    of the fine cells in level-0 and level-1 leaves, numbered by their first cell in scan order (as
+    scipy's labelling numbers them), each projected alone by the closed solver with the region's mean
+    divergence: converged, every region cell's divergence is that mean, and only faces between two
+    cells of one region change.  With zero iterations nothing changes."""
+    W, H, D = 32, 24, 16
+    occ = scene(W, H, D, 12, centres=[(7.0, 8.0, 8.0), (24.0, 15.0, 8.0)])
+    leaves = O.octree_leaves(occ, (1, 2, 4))
+    lev = owner_levels(leaves, occ)
+    want, n = ndi.label((lev == 0) | (lev == 1))
+    assert n >= 2
+    rng = np.random.default_rng(4)
+    u, v, w = faces(rng, W, H, D)
+    u0, v0, w0 = u.copy(), v.copy(), w.copy()
+    n0, _ = O.octree_special_regions(u, v, w, leaves, 0)
+    assert n0 == n
+    for a, b in ((u, u0), (v, v0), (w, w0)):
+        np.testing.assert_array_equal(a, b)
+    got, labels = O.octree_special_regions(u, v, w, leaves, 100000)
+    assert got == n
+    np.testing.assert_array_equal(labels, want)
+    d0 = fine_div(u0.astype(np.float64), v0.astype(np.float64), w0.astype(np.float64))
+    d1 = fine_div(u.astype(np.float64), v.astype(np.float64), w.astype(np.float64))
+    for r in range(1, n + 1):
+        m = want == r
+        np.testing.assert_allclose(d1[m], d0[m].mean(), rtol=0, atol=2e-3, err_msg="region %d" % r)
+    # a face may change only between two cells of one region
+    pad = np.pad(want, 1, constant_values=-1)
+    same_u = (pad[1:-1, 1:-1, :-1] == pad[1:-1, 1:-1, 1:]) & (pad[1:-1, 1:-1, :-1] > 0)
+    same_v = (pad[1:-1, :-1, 1:-1] == pad[1:-1, 1:, 1:-1]) & (pad[1:-1, :-1, 1:-1] > 0)
+    same_w = (pad[:-1, 1:-1, 1:-1] == pad[1:, 1:-1, 1:-1]) & (pad[:-1, 1:-1, 1:-1] > 0)
+    for a, b, s, name in ((u, u0, same_u, "u"), (v, v0, same_v, "v"), (w, w0, same_w, "w")):
+        assert not np.any((a != b) & ~s), name
+        assert np.any((a != b) & s), name

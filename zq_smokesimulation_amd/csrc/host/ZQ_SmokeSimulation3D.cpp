@@ -10,8 +10,8 @@
 // *_FLUX_SOR solvers run the flux form of that solve (TOTAL_* on the fine grid, s = 1), and
 // {OPEN,CLOSED}_OCTREE_POISSON_SOR the octree projection (zq3d_config.octree).  In the staged host
 // path those projections are one device call on a projection-only simulation (the fields cross
-// PCIe as in the other stages).  The CPU PCG solvers (plain and octree) and guiding with the octree
-// are not built and Init fails for them.
+// PCIe as in the other stages).  The CPU PCG solvers (plain and octree) are not built and Init
+// fails for them.
 #include "ZQ_SmokeSimulation3D.h"
 #include "ZQ_SmokeGuideGrid3D.h"
 
@@ -72,13 +72,6 @@ bool ZQ_SmokeSimulation3D::Init(const char* file) {
         std::printf("globalGridSolver %d is not built (the CPU PCG and octree PCG solvers): use a *_SOR solver\n",
                     (int)para.globalGridSolver);
         return false;
-    }
-    {
-        zq3d_config c{};
-        if (map_octree_solver(para, &c) && para.useGuidingControl) {
-            std::printf("guiding control with the octree solvers is not built\n");
-            return false;
-        }
     }
     if (para.useMovingObject) {
         // the reference resolves the file relative to the working directory
@@ -248,10 +241,19 @@ bool ZQ_SmokeSimulation3D::Run(const char* fold) {
 
 bool ZQ_SmokeSimulation3D::UpdateOneFrame(const ZQ_SmokeGuideGrid3D* guideGrid) {
     moving_frame += 1;
-    // guiding acts in the coarse projection only; the reference ignores it for the full-grid
-    // solvers (ZQ_SmokeSimulation3D.cpp:403-414)
+    // guiding acts in the coarse projection and, as the special-region step, in the octree
+    // projection (ZQ_SmokeSimulation3D.cpp:500-508); the reference ignores it for the full-grid
+    // solvers (:403-414)
     zq3d_sim* target = dev ? dev : proj;
-    if (target && is_coarse()) {
+    zq3d_config oc{};
+    if (target && map_octree_solver(para, &oc)) {
+        const bool on = guideGrid && para.useGuidingControl;
+        const float unread = 0.0f;  // any non-null pointer: the octree step reads no guide values
+        if (zq3d_set_guide(target, para.guidingCoeff, on ? &unread : nullptr, nullptr, nullptr) != ZQ3D_OK) {
+            std::printf("guide: %s\n", zq3d_last_error());
+            return false;
+        }
+    } else if (target && is_coarse()) {
         const bool on = guideGrid && para.useGuidingControl;
         if (on && ((int)guideGrid->GetWidth() != para.coarseWidth || (int)guideGrid->GetHeight() != para.coarseHeight ||
                    (int)guideGrid->GetDepth() != para.coarseDepth)) {

@@ -1175,7 +1175,15 @@ extern "C" int zq3d_set_coarse_sweep(zq3d_sim* s, int k, int zc) {
 extern "C" int zq3d_set_guide(zq3d_sim* s, float coeff, const float* gu, const float* gv, const float* gw) {
     const char* where = "zq3d_set_guide";
     if (!s) { set_err(where, "null sim"); return ZQ3D_ERR_ARG; }
-    if (s->cfg.sub_size <= 0) { set_err(where, "guiding needs the coarse grid (sub_size > 0)"); return ZQ3D_ERR_STATE; }
+    if (s->cfg.octree) {
+        // the octree's guiding is the special-region projection step; the guide values are not read
+        s->octree.guiding = gu != nullptr;
+        return ZQ3D_OK;
+    }
+    if (s->cfg.sub_size <= 0) {
+        set_err(where, "guiding needs the coarse grid (sub_size > 0) or the octree projection");
+        return ZQ3D_ERR_STATE;
+    }
     return coarse_set_guide(&s->coarse, coeff, gu, gv, gw, s->stream, where);
 }
 

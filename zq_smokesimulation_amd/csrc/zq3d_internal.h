@@ -232,13 +232,22 @@ struct OctreeState {
     OctreeBufs B{};
     OctreeLists lists;
     void* mem = nullptr;
+    bool guiding = false;  // octree guiding: the special-region projections after the face correction
 };
 int octree_create(const Dims& d, const int thresh[3], OctreeState* out, const char* where);
 void octree_destroy(OctreeState* o);
 // Octree projection of the faces in place: the leaves from occ, o.iters red-black passes over the
-// levels, the face correction, the leaf sub-projection; o.closed_mean subtracts the mean divergence
-// per fine cell times each leaf's volume.  B provides the mean and stats scratch; stats may be null.
-// Synchronises the stream twice (the cell lists' lengths and entry totals size the sweep launches).
+// levels, the face correction, with t.guiding the special-region projections (octree_special_regions),
+// the leaf sub-projection; o.closed_mean subtracts the mean divergence per fine cell times each
+// leaf's volume.  B provides the mean and stats scratch; stats may be null.  Synchronises the stream
+// twice (the cell lists' lengths and entry totals size the sweep launches), more with guiding.
+// Octree guiding (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity, the reference's build with
+// guiding_specialRegion_useCUDA, ZQ_SmokeOctreeGrid3D.h:37, .cpp:546-720): the 6-connected regions of
+// fine cells in level-0 and level-1 leaves, numbered in scan order, each projected alone by the closed
+// red-black solver (SolveClosedPoissonRedBlackwithOccupy3D_MAC) on its bounding box with the region's
+// mean divergence and min(iters, cells) iterations.  On that path the reference computes the guide's
+// blended deltas but never adds them (only its PCG branch does), so no guide values are read.  The
+// labelling and boxes are host work on the downloaded level map and faces, as in the reference.
 int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                    const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf);
 

@@ -1,7 +1,9 @@
 // zq3d_octree.cpp -- the octree projection on the device-resident simulation, and the two
 // ZQlibCuda octree solver calls of ZQ_SmokeOctreeGrid3D (host arrays).
+#include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "zq3d_internal.h"
 
@@ -22,6 +24,105 @@ void octree_destroy(OctreeState* o) {
     o->mem = nullptr;
     octree_free_lists(&o->lists);
 }
+
+namespace {
+
+// The special-region projections of octree guiding (see project_octree's declaration).
+int octree_special_regions(OctreeState& t, float* u, float* v, float* w, int iters, hipStream_t s,
+                           const char* where) {
+    const Dims& d = t.d;
+    const Sizes z(d);
+    const int W = d.W, H = d.H, D = d.D;
+    std::vector<uint8_t> lev(z.N);
+    std::vector<float> hu(z.Nu), hv(z.Nv), hw(z.Nw);
+    ZQ_HIP(hipMemcpyAsync(lev.data(), t.B.lev, z.N, hipMemcpyDeviceToHost, s), where);
+    ZQ_HIP(hipMemcpyAsync(hu.data(), u, z.Nu * 4, hipMemcpyDeviceToHost, s), where);
+    ZQ_HIP(hipMemcpyAsync(hv.data(), v, z.Nv * 4, hipMemcpyDeviceToHost, s), where);
+    ZQ_HIP(hipMemcpyAsync(hw.data(), w, z.Nw * 4, hipMemcpyDeviceToHost, s), where);
+    ZQ_HIP(hipStreamSynchronize(s), where);
+    // regions: flood fill of the cells in level-0 and level-1 leaves, seeds in scan order
+    std::vector<int32_t> lab(z.N, 0);
+    for (size_t o = 0; o < z.N; o++)
+        if (lev[o] == 0 || lev[o] == 1) lab[o] = -1;
+    int regions = 0;
+    std::vector<size_t> queue;
+    for (size_t o = 0; o < z.N; o++) {
+        if (lab[o] != -1) continue;
+        lab[o] = ++regions;
+        queue.assign(1, o);
+        for (size_t h = 0; h < queue.size(); h++) {
+            const size_t c = queue[h];
+            const int x = (int)(c % W), y = (int)((c / W) % H), k = (int)(c / ((size_t)W * H));
+            const size_t nb[6] = {c + 1, c - 1, c + W, c - W, c + (size_t)W * H, c - (size_t)W * H};
+            const bool in[6] = {x + 1 < W, x > 0, y + 1 < H, y > 0, k + 1 < D, k > 0};
+            for (int q = 0; q < 6; q++)
+                if (in[q] && lab[nb[q]] == -1) {
+                    lab[nb[q]] = regions;
+                    queue.push_back(nb[q]);
+                }
+        }
+    }
+    const size_t Wu = (size_t)W + 1, Hv = (size_t)H + 1;
+    for (int r = 1; r <= regions; r++) {
+        int x0 = W, y0 = H, z0 = D, x1 = -1, y1 = -1, z1 = -1, count = 0;
+        float mean = 0.0f;  // the reference's float sum in scan order (ZQ_SmokeOctreeGrid3D.cpp:575-580)
+        for (int k = 0; k < D; k++)
+            for (int j = 0; j < H; j++)
+                for (int i = 0; i < W; i++) {
+                    if (lab[((size_t)k * H + j) * W + i] != r) continue;
+                    count++;
+                    mean += hu[((size_t)k * H + j) * Wu + i + 1] - hu[((size_t)k * H + j) * Wu + i] +
+                            hv[((size_t)k * Hv + j + 1) * W + i] - hv[((size_t)k * Hv + j) * W + i] +
+                            hw[((size_t)(k + 1) * H + j) * W + i] - hw[((size_t)k * H + j) * W + i];
+                    x0 = std::min(x0, i), y0 = std::min(y0, j), z0 = std::min(z0, k);
+                    x1 = std::max(x1, i), y1 = std::max(y1, j), z1 = std::max(z1, k);
+                }
+        mean /= count;
+        const int sx = x1 - x0 + 1, sy = y1 - y0 + 1, sz = z1 - z0 + 1;
+        std::vector<uint8_t> bo((size_t)sx * sy * sz);
+        std::vector<float> bu((size_t)(sx + 1) * sy * sz), bv((size_t)sx * (sy + 1) * sz), bw((size_t)sx * sy * (sz + 1));
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)
+                    bo[((size_t)k * sy + j) * sx + i] = lab[((size_t)(z0 + k) * H + y0 + j) * W + x0 + i] != r;
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i <= sx; i++)
+                    bu[((size_t)k * sy + j) * (sx + 1) + i] = hu[((size_t)(z0 + k) * H + y0 + j) * Wu + x0 + i];
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j <= sy; j++)
+                for (int i = 0; i < sx; i++)
+                    bv[((size_t)k * (sy + 1) + j) * sx + i] = hv[((size_t)(z0 + k) * Hv + y0 + j) * W + x0 + i];
+        for (int k = 0; k <= sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)
+                    bw[((size_t)k * sy + j) * sx + i] = hw[((size_t)(z0 + k) * H + y0 + j) * W + x0 + i];
+        if (ZQ_CUDA_PoissonSolver3D::SolveClosedPoissonRedBlackwithOccupy3D_MAC(
+                bu.data(), bv.data(), bw.data(), reinterpret_cast<const bool*>(bo.data()), mean, sx, sy, sz,
+                std::min(iters, count)) < 0.0f)
+            return ZQ3D_ERR_HIP;
+        // written back as the reference does: the box's u faces except its high x side (:700-712)
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)
+                    hu[((size_t)(z0 + k) * H + y0 + j) * Wu + x0 + i] = bu[((size_t)k * sy + j) * (sx + 1) + i];
+        for (int k = 0; k < sz; k++)
+            for (int j = 0; j <= sy; j++)
+                for (int i = 0; i < sx; i++)
+                    hv[((size_t)(z0 + k) * Hv + y0 + j) * W + x0 + i] = bv[((size_t)k * (sy + 1) + j) * sx + i];
+        for (int k = 0; k <= sz; k++)
+            for (int j = 0; j < sy; j++)
+                for (int i = 0; i < sx; i++)
+                    hw[((size_t)(z0 + k) * H + y0 + j) * W + x0 + i] = bw[((size_t)k * sy + j) * sx + i];
+    }
+    ZQ_HIP(hipMemcpyAsync(u, hu.data(), z.Nu * 4, hipMemcpyHostToDevice, s), where);
+    ZQ_HIP(hipMemcpyAsync(v, hv.data(), z.Nv * 4, hipMemcpyHostToDevice, s), where);
+    ZQ_HIP(hipMemcpyAsync(w, hw.data(), z.Nw * 4, hipMemcpyHostToDevice, s), where);
+    ZQ_HIP(hipStreamSynchronize(s), where);  // the host vectors go out of scope
+    return ZQ3D_OK;
+}
+
+}  // namespace
 
 int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* occ, const zq3d_sor_opts& o,
                    const ProjBufs& B, zq3d_proj_stats* st, hipStream_t s, SorProf* pf) {
@@ -49,6 +150,8 @@ int project_octree(OctreeState& t, float* u, float* v, float* w, const uint8_t* 
         pf->launches += launches;
     }
     launch_octree_adjust(t.d, t.B, u, v, w, s);
+    if (t.guiding)
+        if (int rc = octree_special_regions(t, u, v, w, o.iters, s, where)) return rc;
     launch_octree_sub_project(t.d, t.B, u, v, w, s);
     ZQ_HIP(hipGetLastError(), where);
     if (st) {

@@ -176,9 +176,17 @@ class SmokeSim3D:
         check(_lib.lib().zq3d_set_sor_shape(self._h, int(ty), int(zc)), "set_sor_shape")
 
     def set_guide(self, coeff, gu=None, gv=None, gw=None):
-        """Guiding control of the coarse projection: coarse-grid guide faces (None: off)."""
+        """Guiding control of the coarse projection: coarse-grid guide faces (None: off).  On an octree
+        simulation any gu other than None turns on the special-region projections, which read no guide
+        values (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity's CUDA branch)."""
         if gu is None:
             check(_lib.lib().zq3d_set_guide(self._h, 0.0, None, None, None), "set_guide")
+            self._guide = None
+            return
+        if self.cfg.octree:
+            flag = np.zeros(1, np.float32)
+            check(_lib.lib().zq3d_set_guide(self._h, float(coeff), flag.ctypes.data_as(C.c_void_p), None, None),
+                  "set_guide")
             self._guide = None
             return
         arrs = [np.ascontiguousarray(a, np.float32) for a in (gu, gv, gw)]
