@@ -57,10 +57,10 @@ def test_full_pipeline_256():
     W = 256
     p = dict(h=1.0 / W, dt=0.02, steps=20, Tamb=0.0, beta=0.5, alpha=0.05, eps=0.3, aV=0.99, aT=0.99, aR=0.99,
              adv_vel_type=0, adv_sca_type=2, open_mask=0, closed_mean=1,
-             omega=2.0 / (1.0 + math.sin(math.pi / W)), sor_iters=80, mixed=1, max_outer=1000, tol=1e-6,
+             omega=1.968, sor_iters=80, mixed=1, max_outer=1000, tol=1e-6,
              pass_rule=1, warm_start=2, warm_extrapolate=1.0)
-    # the bench's projection: fitted passes of at most 80 sweeps, started from p1 + (p1 - p2) (steps 0
-    # and 1, compared with the oracle, start from 0 and from p1 alone)
+    # the bench's projection: omega 1.968, fitted passes of at most 80 sweeps, started from
+    # p1 + (p1 - p2) (steps 0 and 1, compared with the oracle, start from 0 and from p1 alone)
     occ, u, v, w = O.init_fine_grid(W, W, W, O.ALL_FACES, 0)
     T = O.noise(W, W, W, 0, 0.01)
     R = np.zeros_like(T)
