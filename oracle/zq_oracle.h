@@ -221,17 +221,19 @@ void orc_oct_adjust(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, c
                     const float* p0, const float* p1, const float* p2, const float* p3, float* u, float* v, float* w);
 /* The special-region projections of octree guiding (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity with
  * guiding_specialRegion_useCUDA, the reference's build: ZQ_SmokeOctreeGrid3D.h:37, .cpp:546-720).
- * Regions are the 6-connected components of the fine cells in level-0 or level-1 leaves
- * (specialRegion_mask, :5563-5591), numbered in scan order (x fastest) of their first cell.  Each is
+ * Regions are labelled over the fine cells in level-0 or level-1 leaves (specialRegion_mask,
+ * :5563-5591) exactly as _find_special_label labels them (:5169-5320): a pass seeds the first
+ * unlabelled cell of the lowest plane that has one and the first unlabelled cell of row 0 of every
+ * plane above, all with the pass's label, then fills from the last seed, labelling the cells it
+ * reaches along y and x but not those it reaches only along z, which seed later passes.  Each is
  * projected alone, in order, by the closed red-black solver on its bounding box (orc_project, omega 1,
  * open_mask 0): the box's cells outside the region occupied, the region's mean divergence (float,
  * summed in scan order as u1 - u0 + v1 - v0 + w1 - w0) subtracted, min(iters, cells) iterations; the
  * box's faces are written back except the u faces of its high x side (:700-712).  The guide's blended
  * deltas (:265-545) are not applied on this path of the reference (only its PCG branch adds them), so
  * no guide is read.  labels (optional, W*H*D): 0 outside the regions, else the region number.
- * Returns the number of regions.  Parity unpinned against the reference's labelling, which seeds every
- * plane in one pass and leaves cells reached only along z unlabelled (:5169-5320, see the design
- * notes): a plain flood fill is used. */
+ * Returns the number of regions, or -1 where the reference's fill does not end (two cells above each
+ * other reached only along z queue each other until its W*H*D queue overflows). */
 int orc_oct_special_regions(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, int iters, float* u, float* v,
                             float* w, int32_t* labels);
 

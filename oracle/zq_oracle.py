@@ -658,10 +658,12 @@ def octree_adjust(leaves, ps, u, v, w):
 
 def octree_special_regions(u, v, w, leaves, iters):
     """The special-region projections of octree guiding, in place (orc_oct_special_regions).  Returns
-    (number of regions, labels)."""
+    (number of regions, labels); raises where the reference's labelling does not end."""
     D, H, W = leaves[0].shape
     labels = np.zeros((D, H, W), np.int32)
     n = lib().orc_oct_special_regions(W, H, D, leaves[0], leaves[1], int(iters), u, v, w, labels)
+    if n < 0:
+        raise RuntimeError("the reference's special-region labelling does not end on these leaves")
     return n, labels
 
 

@@ -357,6 +357,58 @@ void orc_oct_adjust(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, c
     }
 }
 
+// Region labels of _find_special_label (ZQ_SmokeOctreeGrid3D.cpp:5169-5320); lab holds -1 on the
+// masked cells and 0 elsewhere.  A pass seeds the first unlabelled cell of the lowest plane that has
+// one and, in every plane above it, the first unlabelled cell of row 0 only (the plane loop is not
+// left after the first seed, the row loop is); the seeds take the pass's label and the fill starts
+// from the last one.  The fill queues the z neighbours without labelling them and labels the y and x
+// neighbours as it queues them (order z+1, z-1, y+1, y-1, x+1, x-1), so a cell reached only along z
+// stays unlabelled and seeds a later pass.  Two such cells above each other queue each other without
+// end; the reference's queue (W*H*D entries) then overflows: returns -1 there.
+static int reference_region_labels(int W, int H, int D, std::vector<int32_t>& lab) {
+    const size_t N = (size_t)W * H * D, plane = (size_t)W * H;
+    int regions = 0;
+    std::vector<size_t> queue;
+    for (;;) {
+        size_t seed = N;
+        bool seeded = false;
+        for (int z = 0; z < D; z++) {
+            const int rows = seeded ? 1 : H;
+            bool here = false;
+            for (int y = 0; y < rows && !here; y++)
+                for (int x = 0; x < W; x++) {
+                    const size_t c = (size_t)z * plane + (size_t)y * W + x;
+                    if (lab[c] == -1) {
+                        lab[c] = regions + 1;
+                        seed = c;
+                        here = true;
+                        break;
+                    }
+                }
+            seeded = seeded || here;
+        }
+        if (!seeded) return regions;
+        const int r = ++regions;
+        queue.assign(1, seed);
+        for (size_t h = 0; h < queue.size(); h++) {
+            const size_t c = queue[h];
+            const int x = (int)(c % W), y = (int)((c / W) % H), z = (int)(c / plane);
+            auto visit = [&](size_t n, bool label) {
+                if (lab[n] != -1) return;
+                if (label) lab[n] = r;
+                queue.push_back(n);
+            };
+            if (z + 1 < D) visit(c + plane, false);
+            if (z >= 1) visit(c - plane, false);
+            if (y + 1 < H) visit(c + W, true);
+            if (y >= 1) visit(c - W, true);
+            if (x + 1 < W) visit(c + 1, true);
+            if (x >= 1) visit(c - 1, true);
+            if (queue.size() > N) return -1;
+        }
+    }
+}
+
 int orc_oct_special_regions(int W, int H, int D, const uint8_t* l0, const uint8_t* l1, int iters, float* u, float* v,
                             float* w, int32_t* labels) {
     const size_t N = (size_t)W * H * D;
@@ -367,29 +419,8 @@ int orc_oct_special_regions(int W, int H, int D, const uint8_t* l0, const uint8_
         for (int y = 0; y < H; y++)
             for (int x = 0; x < W; x++)
                 if (l0[g.at(0, x, y, z)] || l1[g.at(1, x >> 1, y >> 1, z >> 1)]) lab[cell(x, y, z)] = -1;
-    // flood fill in scan order of the seeds
-    int regions = 0;
-    std::vector<size_t> queue;
-    for (size_t o = 0; o < N; o++) {
-        if (lab[o] != -1) continue;
-        regions++;
-        lab[o] = regions;
-        queue.assign(1, o);
-        for (size_t h = 0; h < queue.size(); h++) {
-            const size_t c = queue[h];
-            const int x = (int)(c % W), y = (int)((c / W) % H), z = (int)(c / ((size_t)W * H));
-            const int nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y},
-                      nz[6] = {z, z, z, z, z + 1, z - 1};
-            for (int q = 0; q < 6; q++) {
-                if (nx[q] < 0 || nx[q] >= W || ny[q] < 0 || ny[q] >= H || nz[q] < 0 || nz[q] >= D) continue;
-                const size_t n = cell(nx[q], ny[q], nz[q]);
-                if (lab[n] == -1) {
-                    lab[n] = regions;
-                    queue.push_back(n);
-                }
-            }
-        }
-    }
+    const int regions = reference_region_labels(W, H, D, lab);
+    if (regions < 0) return -1;
     const size_t Wu = (size_t)W + 1, Hv = (size_t)H + 1;
     for (int r = 1; r <= regions; r++) {
         int x0 = W, y0 = H, z0 = D, x1 = -1, y1 = -1, z1 = -1, count = 0;

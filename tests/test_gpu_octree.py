@@ -127,20 +127,35 @@ def test_closed_domain_with_wall_shell():
 @pytest.mark.parametrize("closed", [False, True])
 def test_guiding_special_regions_match_the_oracle(closed):
     """useGuidingControl with the octree solvers: the special-region projections between the face
-    correction and the sub-projection (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity), to the same bars."""
+    correction and the sub-projection (ZQ_SmokeOctreeGrid3D::ApplyGuidingVelocity), to the same bars.
+    A block against the y = 0 side gives the open scene level-0 cells in row 0 above the first
+    region, which the reference's labelling seeds with that region's label: its labels differ from a
+    flood fill's, so the device labelling is checked where the two differ."""
     W, H, D = 48, 40, 32
     centres = [(12.0, 12.0, 10.0), (34.0, 27.0, 22.0)]
-    leaves, _ = check_projection(W, H, D, 13, centres, (1, 2, 4), iters=30, closed=closed, shell=closed,
-                                 guiding=True)
-    # the step did something: more than one region, and the fields differ from the unguided projection
-    n, _ = O.octree_special_regions(*O.zeros_faces(W, H, D), leaves, 0)
-    assert n >= (1 if closed else 2)  # the wall shell joins the closed scene's regions
+    print("octree guiding", (W, H, D), "closed", closed, flush=True)
     rng = np.random.default_rng(13)
     u, v, w = faces(rng, W, H, D)
     occ = scene(W, H, D, 13, centres, 0.0, closed)
-    a = run_sim(W, H, D, u, v, w, occ, 30, (1, 2, 4), closed, guiding=True)
+    occ[18:23, 0:3, 30:35] = 1
+    g = run_sim(W, H, D, u, v, w, occ, 30, (1, 2, 4), closed, guiding=True)
+    ru, rv, rw = u.copy(), v.copy(), w.copy()
+    leaves, _ = O.octree_projection(ru, rv, rw, occ, 30, (1, 2, 4), closed, guiding=True)
+    scale = max(np.abs(ru).max(), np.abs(rv).max(), np.abs(rw).max())
+    if not closed:
+        mu, mv, mw = exterior_masks(leaves)
+        assert_same(g[0], ru, "u outside leaf interiors", mu)
+        assert_same(g[1], rv, "v outside leaf interiors", mv)
+        assert_same(g[2], rw, "w outside leaf interiors", mw)
+    for a, r, n in zip(g[:3], (ru, rv, rw), "uvw"):
+        np.testing.assert_allclose(a, r, rtol=0, atol=2e-5 * scale, err_msg=n)
+    # the step did something, and in the open scene the reference's labels are not a flood fill's
+    n, lab = O.octree_special_regions(*O.zeros_faces(W, H, D), leaves, 0)
+    if not closed:
+        import scipy.ndimage as ndi
+        assert n >= 2 and (lab != ndi.label(lab > 0)[0]).any()
     b = run_sim(W, H, D, u, v, w, occ, 30, (1, 2, 4), closed, guiding=False)
-    assert np.abs(a[0] - b[0]).max() > 1e-4
+    assert np.abs(g[0] - b[0]).max() > 1e-4
 
 
 def test_leaves_follow_a_changed_occupancy():

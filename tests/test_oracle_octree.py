@@ -287,12 +287,65 @@ def test_closed_mean_is_the_float_mean_over_fluid_cells():
     assert abs(got - d[occ == 0].mean()) < 1e-5
 
 
-def test_special_regions_are_scan_order_components_projected_alone():
-    """Octree guiding's special regions (ZQ_SmokeOctreeGrid3D.cpp:546-720): the 6-connected components
-    of the fine cells in level-0 and level-1 leaves, numbered by their first cell in scan order (as
-    scipy's labelling numbers them), each projected alone by the closed solver with the region's mean
-    divergence: converged, every region cell's divergence is that mean, and only faces between two
-    cells of one region change.  With zero iterations nothing changes."""
+def region_labels(cells, W=8, H=8, D=8):
+    """The oracle's special-region labels of a mask given as (x, y, z) cells (as level-0 leaves)."""
+    l0 = np.zeros((D, H, W), np.uint8)
+    for x, y, z in cells:
+        l0[z, y, x] = 1
+    leaves = [l0, np.zeros((D // 2, H // 2, W // 2), np.uint8)]
+    n, lab = O.octree_special_regions(*O.zeros_faces(W, H, D), leaves, 0)
+    return n, {c: int(lab[c[2], c[1], c[0]]) for c in cells}
+
+
+def test_special_region_labels_in_one_plane_are_scan_order_components():
+    """With every masked cell in one plane the reference's labelling (ZQ_SmokeOctreeGrid3D.cpp:
+    5169-5320) is a 4-connected flood fill numbered by each component's first cell in scan order,
+    which is how scipy numbers its components."""
+    rng = np.random.default_rng(8)
+    W, H, D = 24, 16, 8
+    plane = rng.random((H, W)) < 0.45
+    want, n = ndi.label(plane)  # 4-connectivity in 2D
+    l0 = np.zeros((D, H, W), np.uint8)
+    l0[5] = plane
+    leaves = [l0, np.zeros((D // 2, H // 2, W // 2), np.uint8)]
+    got, lab = O.octree_special_regions(*O.zeros_faces(W, H, D), leaves, 0)
+    assert got == n and n > 5
+    np.testing.assert_array_equal(lab[5], want)
+    assert not lab[np.arange(D) != 5].any()
+
+
+def test_special_region_labels_keep_the_reference_fill():
+    """Hand-worked cases of _find_special_label (ZQ_SmokeOctreeGrid3D.cpp:5169-5320).
+    - A cell reached only along z is queued but not labelled, so it seeds a region of its own: cells
+      (0,0,0), (1,0,0), (1,0,1) give (0,0,0) and the second plane's row-0 seed (1,0,1) label 1, and
+      (1,0,0) label 2, where a flood fill would give one region.
+    - After the first seed a pass also seeds the first cell of row 0 of every plane above, under the
+      same label: (2,3,0), (4,0,2), (5,0,2) are one region, not two; (4,2,2), off row 0, is not
+      seeded with (2,3,0).
+    - Two cells above each other reached only along z queue each other without end (the reference's
+      queue then overflows): a column of three cells off row 0 raises; a column of two gives two
+      regions."""
+    n, lab = region_labels([(0, 0, 0), (1, 0, 0), (1, 0, 1)])
+    assert n == 2 and lab == {(0, 0, 0): 1, (1, 0, 1): 1, (1, 0, 0): 2}
+    n, lab = region_labels([(2, 3, 0), (4, 0, 2), (5, 0, 2)])
+    assert n == 1 and set(lab.values()) == {1}
+    n, lab = region_labels([(2, 3, 0), (4, 2, 2)])  # above the first seed only row 0 is searched
+    assert n == 2 and lab == {(2, 3, 0): 1, (4, 2, 2): 2}
+    n, lab = region_labels([(1, 1, 0), (1, 1, 1)])
+    assert n == 2 and lab == {(1, 1, 0): 1, (1, 1, 1): 2}
+    with pytest.raises(RuntimeError):
+        region_labels([(1, 1, 0), (1, 1, 1), (1, 1, 2)])
+    # in row 0 every plane's cell is a seed of the first pass, so the same column ends
+    n, lab = region_labels([(1, 0, 0), (1, 0, 1), (1, 0, 2)])
+    assert n == 1 and set(lab.values()) == {1}
+
+
+def test_special_regions_are_projected_alone():
+    """Octree guiding's special regions (ZQ_SmokeOctreeGrid3D.cpp:546-720), each projected alone by the
+    closed solver with the region's mean divergence: converged, every region cell's divergence is that
+    mean, and only faces between two cells of one region change.  In this scene every region cell has a
+    neighbour in its plane and no region reaches row 0, so the reference's labels are the 6-connected
+    components in scan order (scipy's).  With zero iterations nothing changes."""
     W, H, D = 32, 24, 16
     occ = scene(W, H, D, 12, centres=[(7.0, 8.0, 8.0), (24.0, 15.0, 8.0)])
     leaves = O.octree_leaves(occ, (1, 2, 4))

@@ -40,26 +40,59 @@ int octree_special_regions(OctreeState& t, float* u, float* v, float* w, int ite
     ZQ_HIP(hipMemcpyAsync(hv.data(), v, z.Nv * 4, hipMemcpyDeviceToHost, s), where);
     ZQ_HIP(hipMemcpyAsync(hw.data(), w, z.Nw * 4, hipMemcpyDeviceToHost, s), where);
     ZQ_HIP(hipStreamSynchronize(s), where);
-    // regions: flood fill of the cells in level-0 and level-1 leaves, seeds in scan order
+    // regions: the labelling of _find_special_label (ZQ_SmokeOctreeGrid3D.cpp:5169-5320) over the
+    // cells in level-0 and level-1 leaves.  A pass seeds the first unlabelled cell of the lowest plane
+    // that has one and, in each plane above, the first unlabelled cell of row 0 only, all under the
+    // pass's label, then fills from the last seed: z neighbours are queued unlabelled, y and x
+    // neighbours labelled as they are queued (order z+1, z-1, y+1, y-1, x+1, x-1).  A cell reached
+    // only along z seeds a later pass; two such cells above each other queue each other without end,
+    // which overflows the reference's W*H*D queue: an error here.
+    const size_t plane = (size_t)W * H;
     std::vector<int32_t> lab(z.N, 0);
     for (size_t o = 0; o < z.N; o++)
         if (lev[o] == 0 || lev[o] == 1) lab[o] = -1;
     int regions = 0;
     std::vector<size_t> queue;
-    for (size_t o = 0; o < z.N; o++) {
-        if (lab[o] != -1) continue;
-        lab[o] = ++regions;
-        queue.assign(1, o);
+    for (;;) {
+        size_t seed = z.N;
+        bool seeded = false;
+        for (int k = 0; k < D; k++) {
+            const int rows = seeded ? 1 : H;
+            bool here = false;
+            for (int j = 0; j < rows && !here; j++)
+                for (int i = 0; i < W; i++) {
+                    const size_t c = (size_t)k * plane + (size_t)j * W + i;
+                    if (lab[c] == -1) {
+                        lab[c] = regions + 1;
+                        seed = c;
+                        here = true;
+                        break;
+                    }
+                }
+            seeded = seeded || here;
+        }
+        if (!seeded) break;
+        const int r = ++regions;
+        queue.assign(1, seed);
         for (size_t h = 0; h < queue.size(); h++) {
             const size_t c = queue[h];
-            const int x = (int)(c % W), y = (int)((c / W) % H), k = (int)(c / ((size_t)W * H));
-            const size_t nb[6] = {c + 1, c - 1, c + W, c - W, c + (size_t)W * H, c - (size_t)W * H};
-            const bool in[6] = {x + 1 < W, x > 0, y + 1 < H, y > 0, k + 1 < D, k > 0};
-            for (int q = 0; q < 6; q++)
-                if (in[q] && lab[nb[q]] == -1) {
-                    lab[nb[q]] = regions;
-                    queue.push_back(nb[q]);
-                }
+            const int x = (int)(c % W), y = (int)((c / W) % H), k = (int)(c / plane);
+            auto visit = [&](size_t n, bool label) {
+                if (lab[n] != -1) return;
+                if (label) lab[n] = r;
+                queue.push_back(n);
+            };
+            if (k + 1 < D) visit(c + plane, false);
+            if (k >= 1) visit(c - plane, false);
+            if (y + 1 < H) visit(c + W, true);
+            if (y >= 1) visit(c - W, true);
+            if (x + 1 < W) visit(c + 1, true);
+            if (x >= 1) visit(c - 1, true);
+            if (queue.size() > z.N) {
+                set_err(where, "octree guiding: the reference's special-region labelling does not end on these "
+                               "leaves (two cells above each other reached only along z)");
+                return ZQ3D_ERR_STATE;
+            }
         }
     }
     const size_t Wu = (size_t)W + 1, Hv = (size_t)H + 1;
