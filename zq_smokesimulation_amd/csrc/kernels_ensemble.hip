@@ -181,9 +181,11 @@ __global__ __launch_bounds__(NT) void ke_split_out(const EnsMember* __restrict__
 // tiles numbered member after member, handed out so each XCD gets a contiguous run); iteration it
 // reads pair it & 1 and writes the other; members whose own iteration count is reached are left
 // alone
+// 8-wave workgroups are held to six waves per SIMD (80 VGPRs), so three share a CU: the LDS rings of
+// three fit in 160 KB and the ensemble's tiles are too small to keep a CU busy two at a time.
 template <int TX, int TY, int NW>
-__global__ __launch_bounds__(64 * NW) void ke_sor_split_rb(const EnsMember* __restrict__ M, EnsTable T, int B,
-                                                           int zc, int it) {
+__global__ __launch_bounds__(64 * NW, NW == 8 ? 6 : 1) void ke_sor_split_rb(const EnsMember* __restrict__ M,
+                                                                            EnsTable T, int B, int zc, int it) {
     const int per = gridDim.x >> 3;
     const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (tile >= T.start[B]) return;  // spare block: uniform exit before any barrier

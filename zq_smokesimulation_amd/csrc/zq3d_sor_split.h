@@ -43,6 +43,23 @@ __device__ __forceinline__ int split_q(int Wh, int H, int xh, int y, int z) {
 // of a pass that starts from a zero correction).
 constexpr int SPLIT_TX = 64;  // tile width of the single-grid kernel
 
+// Raw buffer access with the hardware's range check (gfx9 buffer resource, byte offsets): a load at
+// an offset past the buffer's end returns 0 and a store there is dropped, so a lane with nothing to
+// load or store passes SPLIT_NONE instead of branching (no exec-mask juggling, 32-bit offsets).
+constexpr uint32_t SPLIT_NONE = 0x80000000u;  // past the end of every buffer the tile touches (< 2 GiB)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t split_buf(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float split_ldf(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ int split_ldb(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void split_stf(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+}
+
 struct SplitCellIn {
     float p, b;
     int n;  // -1: not a cell (outside the grid or past the row's end)
@@ -74,13 +91,17 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
     constexpr int NB = (TY * TX + NT - 1) / NT;  // black items per thread
     // old black planes t-1 .. t+2 (slot x0-2+c, row y0-2+r) and new red planes t-2 .. t (slot
     // x0-1+c, row y0-1+r); 1/n for n = 1..6
-    __shared__ float Ks[5 * KPL];
-    __shared__ float Rs[4 * RPL];
+    __shared__ float Ks[5 * KPL + 1];  // + 1: where items outside the tile's region write
+    __shared__ float Rs[4 * RPL + 1];
     __shared__ float rtab[8];
     // the tile's own width (the last tile of a row may be narrower); items are numbered row by row
     // over the tile's region so a wave reads contiguous runs of slots
     const int tw = min(TX, Wh - x0), kw = tw + 4, rw = tw + 2;
     const int SP = split_row(Wh), SZ = SP * (d.H + 2), HW = d.H * Wh;
+    const uint32_t PB = (uint32_t)SZ * (uint32_t)(d.D + 2) * 4u, CB = (uint32_t)HW * (uint32_t)d.D;
+    const __amdgpu_buffer_rsrc_t rs0 = split_buf(s0, PB), rs1 = split_buf(s1, PB), rd0 = split_buf(d0, PB),
+                                 rd1 = split_buf(d1, PB), rn0 = split_buf(n0, CB), rn1 = split_buf(n1, CB),
+                                 rb0 = split_buf(b0, 4u * CB), rb1 = split_buf(b1, 4u * CB);
 
     // ---- items, computed once ----
     // staged old black: KY rows x (tw + 4) columns, slot x0-2+c, row y0-2+r
@@ -107,7 +128,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         if (in && xr >= 1 && xr <= tw && yr >= 1 && yr <= TY) bits |= 4;
         if (ys & 1) bits |= 8;
         ri[i] = SplitItem{(ys + 1) * SP + SPLIT_XOFF + xs, ys * Wh + xs, mine ? yr * RX + xr : -1,
-                          mine ? (yr + 1) * KX + xr : 0, bits};
+                          mine ? (yr + 1) * KX + xr : KX, bits};
     }
     // black: TY rows x tw columns, slot x0+xb, row y0+yb
     SplitItem bi[NB];
@@ -132,13 +153,13 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const bool zin = z >= -1 && z <= d.D;
         const int zo = (z + 1) * SZ;
 #pragma unroll
-        for (int i = 0; i < NK; i++) o[i] = ZS ? 0.0f : s1[(zin && ki[i].bits) ? zo + ki[i].off : 0];
+        for (int i = 0; i < NK; i++)
+            o[i] = ZS ? 0.0f : split_ldf(rs1, (zin && ki[i].bits) ? 4u * (uint32_t)(zo + ki[i].off) : SPLIT_NONE);
     };
     auto store_k = [&](int z, const float (&o)[NK]) {
-        float* K = Ks + ((z + 10) % 5) * KPL;
+        const int K = ((z + 10) % 5) * KPL;
 #pragma unroll
-        for (int i = 0; i < NK; i++)
-            if (ki[i].lds >= 0) K[ki[i].lds] = o[i];
+        for (int i = 0; i < NK; i++) Ks[ki[i].lds >= 0 ? K + ki[i].lds : 5 * KPL] = o[i];
     };
     // a colour-c cell of row ys, plane t has parity (c + ys + t) & 1
     auto cell = [](const SplitItem& it, int par) { return (it.bits >> par) & 1; };
@@ -148,11 +169,10 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const bool ok = tin && cell(ri[i], ((ri[i].bits >> 3) ^ t) & 1);
-            const int oh = ok ? zh + ri[i].oh : 0;
-            const int q = ok ? zo + ri[i].off : 0;
-            const int n = n0[oh];
-            const float bv = b0[oh];
-            const float pv = ZS ? 0.0f : s0[q];
+            const uint32_t oh = (uint32_t)(zh + ri[i].oh);
+            const int n = split_ldb(rn0, ok ? oh : SPLIT_NONE);
+            const float bv = split_ldf(rb0, ok ? 4u * oh : SPLIT_NONE);
+            const float pv = ZS ? 0.0f : split_ldf(rs0, ok ? 4u * (uint32_t)(zo + ri[i].off) : SPLIT_NONE);
             o[i] = SplitCellIn{pv, bv, ok ? n : -1};
         }
     };
@@ -162,9 +182,9 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             const bool ok = tin && cell(bi[i], ((bi[i].bits >> 3) ^ t ^ 1) & 1);
-            const int oh = ok ? zh + bi[i].oh : 0;
-            const int n = n1[oh];
-            const float bv = b1[oh];
+            const uint32_t oh = (uint32_t)(zh + bi[i].oh);
+            const int n = split_ldb(rn1, ok ? oh : SPLIT_NONE);
+            const float bv = split_ldf(rb1, ok ? 4u * oh : SPLIT_NONE);
             o[i] = SplitCellIn{0.0f, bv, ok ? n : -1};
         }
     };
@@ -181,9 +201,10 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const float pg = (sum - bv) * rtab[c.n & 7];
         const float upd = omw * c.p + omega * pg;
         const float pc = c.n > 0 ? upd : c.p;
-        if (c.n >= 0 && own_plane && (it.bits & 4)) d0[(t + 1) * SZ + it.off] = pc;
-        if (GATE && c.n > 0 && own_plane && (it.bits & 4)) *dmax = max(*dmax, __float_as_uint(fabsf(pc - c.p)));
-        Rs[((t + 8) & 3) * RPL + it.lds] = c.n >= 0 ? pc : 0.0f;
+        const bool owned = own_plane && (it.bits & 4);
+        split_stf(rd0, c.n >= 0 && owned ? 4u * (uint32_t)((t + 1) * SZ + it.off) : SPLIT_NONE, pc);
+        if (GATE) *dmax = max(*dmax, c.n > 0 && owned ? __float_as_uint(fabsf(pc - c.p)) : 0u);
+        Rs[it.lds >= 0 ? ((t + 8) & 3) * RPL + it.lds : 4 * RPL] = c.n >= 0 ? pc : 0.0f;
     };
     auto black = [&](const SplitItem& it, int t, const SplitCellIn& c) {
         const int par = ((it.bits >> 3) ^ t ^ 1) & 1;
@@ -196,8 +217,8 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const float bv = c.b - m;
         const float pg = (sum - bv) * rtab[c.n & 7];
         const float upd = omw * p0 + omega * pg;
-        if (c.n >= 0) d1[(t + 1) * SZ + it.off] = c.n > 0 ? upd : p0;
-        if (GATE && c.n > 0) *dmax = max(*dmax, __float_as_uint(fabsf(upd - p0)));
+        split_stf(rd1, c.n >= 0 ? 4u * (uint32_t)((t + 1) * SZ + it.off) : SPLIT_NONE, c.n > 0 ? upd : p0);
+        if (GATE) *dmax = max(*dmax, c.n > 0 ? __float_as_uint(fabsf(upd - p0)) : 0u);
     };
 
     float kc[NK];
@@ -225,6 +246,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         load_black(t, bn);
         store_k(t + 2, kc);
         const bool own = t >= z0 && t < z1;
+        // whole waves whose items lie outside the tile's region (narrow tiles) skip them
 #pragma unroll
         for (int i = 0; i < NR; i++)
             if (ri[i].lds >= 0) red(ri[i], t, rc[i], own);
