@@ -50,19 +50,21 @@ constexpr uint32_t SPLIT_NONE = 0x80000000u;  // past the end of every buffer th
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t split_buf(const void* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float split_ldf(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+// off: the lane's byte offset (SPLIT_NONE: nothing); plane: a byte offset common to the wave (a
+// plane's start), added by the hardware outside the range check, which sees the lane's part only.
+__device__ __forceinline__ float split_ldf(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t plane = 0) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)plane, 0));
 }
-__device__ __forceinline__ int split_ldb(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+__device__ __forceinline__ int split_ldb(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t plane = 0) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, (int)plane, 0);
 }
-__device__ __forceinline__ void split_stf(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+__device__ __forceinline__ void split_stf(__amdgpu_buffer_rsrc_t r, uint32_t off, float v, uint32_t plane = 0) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, (int)plane, 0);
 }
 
 struct SplitCellIn {
     float p, b;
-    int n;  // -1: not a cell (outside the grid or past the row's end)
+    int n;  // negative: not a cell (outside the grid or past the row's end)
 };
 
 // Per-thread items of one plane: every index except the plane's own stride is fixed for the whole
@@ -154,7 +156,9 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const int zo = (z + 1) * SZ;
 #pragma unroll
         for (int i = 0; i < NK; i++)
-            o[i] = ZS ? 0.0f : split_ldf(rs1, (zin && ki[i].bits) ? 4u * (uint32_t)(zo + ki[i].off) : SPLIT_NONE);
+            o[i] = ZS ? 0.0f
+                      : split_ldf(rs1, (zin && ki[i].bits) ? 4u * (uint32_t)ki[i].off : SPLIT_NONE,
+                                  zin ? 4u * (uint32_t)zo : 0u);
     };
     auto store_k = [&](int z, const float (&o)[NK]) {
         const int K = ((z + 10) % 5) * KPL;
@@ -169,11 +173,14 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const bool ok = tin && cell(ri[i], ((ri[i].bits >> 3) ^ t) & 1);
-            const uint32_t oh = (uint32_t)(zh + ri[i].oh);
-            const int n = split_ldb(rn0, ok ? oh : SPLIT_NONE);
-            const float bv = split_ldf(rb0, ok ? 4u * oh : SPLIT_NONE);
-            const float pv = ZS ? 0.0f : split_ldf(rs0, ok ? 4u * (uint32_t)(zo + ri[i].off) : SPLIT_NONE);
-            o[i] = SplitCellIn{pv, bv, ok ? n : -1};
+            const uint32_t oh = (uint32_t)ri[i].oh;
+            const int n = split_ldb(rn0, ok ? oh : SPLIT_NONE, tin ? (uint32_t)zh : 0u);
+            const float bv = split_ldf(rb0, ok ? 4u * oh : SPLIT_NONE, tin ? 4u * (uint32_t)zh : 0u);
+            const float pv = ZS ? 0.0f
+                                : split_ldf(rs0, ok ? 4u * (uint32_t)ri[i].off : SPLIT_NONE,
+                                            tin ? 4u * (uint32_t)zo : 0u);
+            // not a cell: the sign bit (the load ran and read 0, so no branch is needed around it)
+            o[i] = SplitCellIn{pv, bv, (int)((uint32_t)n | (ok ? 0u : 0x80000000u))};
         }
     };
     auto load_black = [&](int t, SplitCellIn (&o)[NB]) {
@@ -182,10 +189,10 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             const bool ok = tin && cell(bi[i], ((bi[i].bits >> 3) ^ t ^ 1) & 1);
-            const uint32_t oh = (uint32_t)(zh + bi[i].oh);
-            const int n = split_ldb(rn1, ok ? oh : SPLIT_NONE);
-            const float bv = split_ldf(rb1, ok ? 4u * oh : SPLIT_NONE);
-            o[i] = SplitCellIn{0.0f, bv, ok ? n : -1};
+            const uint32_t oh = (uint32_t)bi[i].oh;
+            const int n = split_ldb(rn1, ok ? oh : SPLIT_NONE, tin ? (uint32_t)zh : 0u);
+            const float bv = split_ldf(rb1, ok ? 4u * oh : SPLIT_NONE, tin ? 4u * (uint32_t)zh : 0u);
+            o[i] = SplitCellIn{0.0f, bv, (int)((uint32_t)n | (ok ? 0u : 0x80000000u))};
         }
     };
     // The updates are evaluated for every item a lane holds and selected afterwards (n > 0:
@@ -202,7 +209,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const float upd = omw * c.p + omega * pg;
         const float pc = c.n > 0 ? upd : c.p;
         const bool owned = own_plane && (it.bits & 4);
-        split_stf(rd0, c.n >= 0 && owned ? 4u * (uint32_t)((t + 1) * SZ + it.off) : SPLIT_NONE, pc);
+        split_stf(rd0, c.n >= 0 && owned ? 4u * (uint32_t)it.off : SPLIT_NONE, pc, 4u * (uint32_t)((t + 1) * SZ));
         if (GATE) *dmax = max(*dmax, c.n > 0 && owned ? __float_as_uint(fabsf(pc - c.p)) : 0u);
         Rs[it.lds >= 0 ? ((t + 8) & 3) * RPL + it.lds : 4 * RPL] = c.n >= 0 ? pc : 0.0f;
     };
@@ -217,7 +224,7 @@ __device__ __forceinline__ void sor_split_rb_tile(const uint8_t* __restrict__ n0
         const float bv = c.b - m;
         const float pg = (sum - bv) * rtab[c.n & 7];
         const float upd = omw * p0 + omega * pg;
-        split_stf(rd1, c.n >= 0 ? 4u * (uint32_t)((t + 1) * SZ + it.off) : SPLIT_NONE, c.n > 0 ? upd : p0);
+        split_stf(rd1, c.n >= 0 ? 4u * (uint32_t)it.off : SPLIT_NONE, c.n > 0 ? upd : p0, 4u * (uint32_t)((t + 1) * SZ));
         if (GATE) *dmax = max(*dmax, c.n > 0 ? __float_as_uint(fabsf(upd - p0)) : 0u);
     };
 
