@@ -50,10 +50,10 @@ def initial_fields(W, H, D, closed=True, obstacle=False, seed=0):
 SOURCES = [[0.5, 0.2, 0.5, 0.12, 1.0, 1.0]]
 
 
-def make_engine(W, H, D, nsteps, K, dtype, closed=True, obstacle=False, adv_vel=0, adv_sca=2, seed=0):
+def make_engine(W, H, D, nsteps, K, dtype, closed=True, obstacle=False, adv_vel=0, adv_sca=2, seed=0, substeps=8):
     from zq_smokesimulation_amd.adjoint import SmokeRollout
     fp64 = dtype == torch.float64
-    eng = SmokeRollout(W, H, D, nsteps, K, dtype=dtype, voxel_size=1.0 / W, step_time=0.02, steps=8, Tamb=0.0,
+    eng = SmokeRollout(W, H, D, nsteps, K, dtype=dtype, voxel_size=1.0 / W, step_time=0.02, steps=substeps, Tamb=0.0,
                        buoyancy=0.5, gravity=0.05, confine=0.3, vel_atten=0.99, temp_atten=0.99, dens_atten=0.99,
                        adv_vel_type=adv_vel, adv_sca_type=adv_sca, sor_open_mask=0 if closed else O.ALL_FACES,
                        sor_closed_mean=1 if closed else 0, sor_omega=1.8, sor_iters=20,
@@ -156,11 +156,48 @@ def test_fd_initial_state_fp64(field):
     check_fd(fd, ad, "initial %s" % field)
 
 
-def test_fp32_matches_fp64_gradient():
+def test_fd_twenty_substeps_fp64():
+    """20 advection substeps, the workloads' count (the other tests take 8): backtrace paths of up to 20
+    points through the tiled advection adjoints.  Control forces (eps 1e-3) and the initial u (eps
+    1e-4) against central differences, as above."""
+    W, H, D, nsteps = 18, 20, 14, 3
+    print("fd 20 substeps", flush=True)
+    eng, f = make_engine(W, H, D, nsteps, 2, torch.float64, closed=True, obstacle=True, substeps=20)
+    gen = torch.Generator(device="cpu").manual_seed(6)
+    ctrl = (0.5 * torch.randn(eng.controls_shape(), generator=gen, dtype=torch.float64)).to(DEV)
+    target = torch.rand((D, H, W), generator=gen, dtype=torch.float64).to(DEV)
+    rho = eng.forward(ctrl)
+    g = eng.backward(ctrl, rho - target).cpu().numpy()
+    gu = eng.get("u", grad=True).cpu().numpy()
+    fd, ad = [], []
+    for idx in significant_indices(g, 10, 5):
+        pert = torch.zeros(ctrl.numel(), dtype=torch.float64)
+        pert[idx] = 1e-3
+        pert = pert.reshape(ctrl.shape).to(DEV)
+        fd.append((loss_of(eng, ctrl + pert, target) - loss_of(eng, ctrl - pert, target)) / 2e-3)
+        ad.append(g.reshape(-1)[idx])
+    check_fd(fd, ad, "controls, 20 substeps")
+    x = torch.from_numpy(f["u"].astype(np.float64)).to(DEV)
+    fd, ad = [], []
+    for idx in significant_indices(gu, 6, 9):
+        pert = torch.zeros(x.numel(), dtype=torch.float64)
+        pert[idx] = 1e-4
+        pert = pert.reshape(x.shape).to(DEV)
+        eng.set_initial(u=x + pert)
+        lp = loss_of(eng, ctrl, target)
+        eng.set_initial(u=x - pert)
+        lm = loss_of(eng, ctrl, target)
+        fd.append((lp - lm) / 2e-4)
+        ad.append(gu.reshape(-1)[idx])
+    check_fd(fd, ad, "initial u, 20 substeps")
+
+
+@pytest.mark.parametrize("substeps", [8, 20])
+def test_fp32_matches_fp64_gradient(substeps):
     W, H, D, nsteps = 20, 22, 16, 4
     gs = []
     for dt in (torch.float32, torch.float64):
-        eng, _ = make_engine(W, H, D, nsteps, 2, dt, closed=True, obstacle=True)
+        eng, _ = make_engine(W, H, D, nsteps, 2, dt, closed=True, obstacle=True, substeps=substeps)
         gen = torch.Generator(device="cpu").manual_seed(3)
         ctrl = (0.5 * torch.randn(eng.controls_shape(), generator=gen, dtype=torch.float64)).to(DEV, dt)
         target = torch.rand((D, H, W), generator=gen, dtype=torch.float64).to(DEV, dt)

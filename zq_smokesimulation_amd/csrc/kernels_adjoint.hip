@@ -365,6 +365,18 @@ __device__ __forceinline__ int trace_box(const Vel& V, const uint8_t* __restrict
     return n;
 }
 
+// rev.step on the stored points n-2 .. 0 (n >= 2), the next point's load issued before the current
+// step so its latency (the points live in scratch memory) overlaps the step's work
+template <class R, class Rev>
+__device__ __forceinline__ void reverse_points(Rev& rev, const R3<R>* q, int n) {
+    R3<R> cur = q[n - 2];
+    for (int m = n - 2; m >= 0; m--) {
+        const R3<R> nxt = q[m > 0 ? m - 1 : 0];
+        rev.step(cur);
+        cur = nxt;
+    }
+}
+
 // inRegular_outRegular velocity advection adjoint, tiled (see k_adj_advect_cells).  The FP32 build
 // is held to 80 VGPRs (one spilled), which lets six waves per SIMD run instead of five.
 template <class R, int CAP>
@@ -404,7 +416,7 @@ k_adj_advect_cells_tile(RegularVel<R> V, R* __restrict__ vb, const uint8_t* __re
         if (n >= 2) {
             const RegularVelAdj<R> A{V.vel4, nullptr, d};
             RevTile<R, 3, CAP> rev{V.vel4, d, &w, dtt / h, A.vjp(last.x, last.y, last.z, ab), {}, {}};
-            for (int m = n - 2; m >= 0; m--) rev.step(q[m]);
+            reverse_points(rev, q, n);
             rev.flush();
         }
     }
@@ -456,7 +468,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_adj_advect_scalar_tile(RegularVe
             const R3<R> gt = grad3<R>(Tin, d.W, d.H, t), gr = grad3<R>(Rin, d.W, d.H, t);
             const R3<R> g{tb * gt.x + rb * gr.x, tb * gt.y + rb * gr.y, tb * gt.z + rb * gr.z};
             RevTile<R, 5, CAP> rev{V.vel4, d, &w, dtt / h, g, {}, {}};
-            for (int m = n - 2; m >= 0; m--) rev.step(q[m]);
+            reverse_points(rev, q, n);
             rev.flush();
         }
     }
