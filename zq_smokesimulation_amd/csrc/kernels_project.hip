@@ -395,7 +395,7 @@ __global__ __launch_bounds__(BX* BY) void k_sor_split(const uint8_t* __restrict_
 // One red+black iteration per launch on split storage (sor_split_rb_tile); 1-D grid of tiles
 // (SPLIT_TX slots x TY rows x zc planes), numbered so each XCD (block b runs on XCD b % 8) receives
 // a contiguous run of tiles, which share their halos through that XCD's L2.
-template <int TY, int NW>
+template <int TY, int NW, bool ZS = false>
 __global__ __launch_bounds__(64 * NW) void k_sor_split_rb(const uint8_t* __restrict__ n0, const uint8_t* __restrict__ n1,
                                                           const float* __restrict__ b0, const float* __restrict__ b1,
                                                           const float* __restrict__ mean32,
@@ -409,8 +409,8 @@ __global__ __launch_bounds__(64 * NW) void k_sor_split_rb(const uint8_t* __restr
     const int by = rest % gy;
     const int z0 = (rest / gy) * zc;
     if (z0 >= d.D) return;  // spare block (the grid is a multiple of 8): uniform exit before any barrier
-    sor_split_rb_tile<SPLIT_TX, TY, NW>(n0, n1, b0, b1, *mean32, s0, s1, d0, d1, d, Wh, bx * SPLIT_TX, by * TY, z0,
-                              min(z0 + zc, d.D), omega, omw);
+    sor_split_rb_tile<SPLIT_TX, TY, NW, false, ZS>(n0, n1, b0, b1, *mean32, s0, s1, d0, d1, d, Wh, bx * SPLIT_TX,
+                                                  by * TY, z0, min(z0 + zc, d.D), omega, omw);
 }
 
 // Iteration `it` of a gated pass (k_sor_split_rb otherwise): it returns at once when the previous
@@ -745,8 +745,9 @@ __global__ __launch_bounds__(BX* BY) void k_residual64_z(const uint8_t* __restri
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = m;
 }
 
-// p64 += e, e = 0 on counted cells; SPLIT: e lives in the padded colour-split arrays e0/e1.
-template <bool SPLIT>
+// p64 += e, and with ZERO e = 0, on counted cells; SPLIT: e lives in the padded colour-split arrays
+// e0/e1.
+template <bool SPLIT, bool ZERO = true>
 __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict__ nc, float* __restrict__ e0,
                                                        float* __restrict__ e1, double* __restrict__ p, Dims d,
                                                        int Wh) {
@@ -758,7 +759,7 @@ __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict
     const int q = pidx(d, i, j, k);
     float* ep = SPLIT ? &(((i + j + k) & 1) ? e1 : e0)[split_q(Wh, d.H, i >> 1, j, k)] : &e0[q];
     p[q] = p[q] + (double)*ep;
-    *ep = 0.0f;
+    if (ZERO) *ep = 0.0f;
 }
 
 template <class F>
@@ -945,7 +946,7 @@ void launch_mean_finalize_mp(const Dims& d, const double* part, double* mean64, 
 size_t split_padded_size(const Dims& d) { return (size_t)split_row((d.W + 1) / 2) * (d.H + 2) * (d.D + 2); }
 size_t split_cell_size(const Dims& d) { return (size_t)((d.W + 1) / 2) * d.H * d.D; }
 
-template <int TY, int NW, bool TWO>
+template <int TY, int NW, bool TWO, bool ZS = false>
 void launch_split_rb(const Dims& d, const SplitBufs& sp, const float* const* src, float* const* dst,
                      const float* mean32, float omega, float omw, int zc, hipStream_t s) {
     const int Wh = (d.W + 1) / 2;
@@ -955,7 +956,7 @@ void launch_split_rb(const Dims& d, const SplitBufs& sp, const float* const* src
         hipLaunchKernelGGL((k_sor_split_rb2<TY, NW>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1], sp.b[0],
                            sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw);
     else
-        hipLaunchKernelGGL((k_sor_split_rb<TY, NW>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1], sp.b[0],
+        hipLaunchKernelGGL((k_sor_split_rb<TY, NW, ZS>), dim3(nb), dim3(64 * NW), 0, s, sp.n[0], sp.n[1], sp.b[0],
                            sp.b[1], mean32, src[0], src[1], dst[0], dst[1], d, Wh, gx, gy, zc, omega, omw);
 }
 
@@ -963,7 +964,7 @@ void launch_split_rb(const Dims& d, const SplitBufs& sp, const float* const* src
 inline int split_rows(SorShape sh) { return sh.ty <= -SOR_TWO ? -sh.ty - SOR_TWO : -sh.ty; }
 
 int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const float* mean32, float omega, int iters,
-                        SorShape shape, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                        SorShape shape, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, bool zero_start) {
     const float omw = 1.0f - omega;
     const int Wh = (d.W + 1) / 2;
     if (ev0) (void)hipEventRecord(ev0, s);
@@ -985,6 +986,12 @@ int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const floa
                 if (ty == 16) launch_split_rb<16, 8, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
                 else launch_split_rb<32, 16, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
                 it += 2;
+            } else if (zero_start && it == 0 && split_zero_start(shape)) {
+                if (ty == 4) launch_split_rb<4, 4, false, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else if (ty == 8) launch_split_rb<8, 8, false, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else if (ty == 16) launch_split_rb<16, 8, false, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                else launch_split_rb<32, 16, false, true>(d, sp, src, dst, mean32, omega, omw, zc, s);
+                it += 1;
             } else {
                 if (ty == 4) launch_split_rb<4, 4, false>(d, sp, src, dst, mean32, omega, omw, zc, s);
                 else if (ty == 8) launch_split_rb<8, 8, false>(d, sp, src, dst, mean32, omega, omw, zc, s);
@@ -1202,9 +1209,14 @@ void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p
                        d, 0);
 }
 
-void launch_update_p64_split(const Dims& d, const uint8_t* nc, float* const* e, double* p64_pad, hipStream_t s) {
-    hipLaunchKernelGGL(k_update_p64<true>, grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, e[0], e[1], p64_pad, d,
-                       (d.W + 1) / 2);
+void launch_update_p64_split(const Dims& d, const uint8_t* nc, float* const* e, double* p64_pad, hipStream_t s,
+                             bool zero_e) {
+    if (zero_e)
+        hipLaunchKernelGGL((k_update_p64<true, true>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, e[0], e[1],
+                           p64_pad, d, (d.W + 1) / 2);
+    else
+        hipLaunchKernelGGL((k_update_p64<true, false>), grid_for(d.W, d.H, d.D), dim3(BX, BY), 0, s, nc, e[0], e[1],
+                           p64_pad, d, (d.W + 1) / 2);
 }
 
 template <class F>

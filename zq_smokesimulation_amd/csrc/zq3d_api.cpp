@@ -275,9 +275,12 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
         if (split) {
             hipEvent_t ea = nullptr, eb = nullptr;
             if (pf) ZQ_HIP(pf->next(&ea, &eb), where);
-            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, n, B.shape, s, ea, eb);
+            // the one-launch shapes start each pass from zero without reading the pair, so the update
+            // need not clear it; the other split shapes read it and get it cleared
+            const bool zs = split_zero_start(B.shape);
+            pair = launch_split_sweeps(d, B.split, pair, &sc->zero32, o.omega, n, B.shape, s, ea, eb, zs);
             ZQ_HIP(pf ? account_sweeps(d, B.shape, n, pf) : hipGetLastError(), where);
-            launch_update_p64_split(d, B.nc, pair ? B.split.alt : B.split.p, B.p64, s);  // also zeroes e
+            launch_update_p64_split(d, B.nc, pair ? B.split.alt : B.split.p, B.p64, s, !zs);
         } else {
             float* res = nullptr;
             ZQ_HIP(timed_sor(d, B.nc, B.b, &sc->zero32, e, e_alt, o.omega, n, s, pf, &res, B.shape, &B.split), where);

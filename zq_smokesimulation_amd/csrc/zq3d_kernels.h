@@ -84,6 +84,8 @@ struct SorShape {
     int ty, zc;
 };
 inline bool is_split(SorShape s) { return s.ty <= SOR_SPLIT; }
+// the one-launch split shapes, whose first iteration can take the source pair as zeros unread
+inline bool split_zero_start(SorShape s) { return s.ty < SOR_SPLIT && s.ty > -SOR_TWO; }
 // Colour-split scratch: per colour a padded pressure (split_padded_size floats), a second padded
 // pressure the one-launch iteration ping-pongs with, right-hand side and counts (split_cell_size
 // each).  Padding slots and the slots past a row's end must hold 0 in p and alt.
@@ -125,9 +127,11 @@ float* launch_sor(const Dims& d, const uint8_t* nc, const float* b, const float*
                   const SplitBufs* split = nullptr, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 // iters red-black iterations on the split arrays as they stand (no layout conversion), starting
 // from the pressure pair `from` (0: sp.p, 1: sp.alt) with the split algorithm `shape`; returns the
-// pair holding the result.  Events (nullable) bracket the sweeps.
+// pair holding the result.  Events (nullable) bracket the sweeps.  zero_start (split_zero_start
+// shapes only): the start is all zeros and pair `from` is not read, so it need not hold zeros.
 int launch_split_sweeps(const Dims& d, const SplitBufs& sp, int from, const float* mean32, float omega, int iters,
-                        SorShape shape, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                        SorShape shape, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                        bool zero_start = false);
 // split counts n_0 / n_1 from the interleaved counts
 void launch_split_counts(const Dims& d, const uint8_t* nc, const SplitBufs& sp, hipStream_t s);
 // name of the sweep kernel a shape runs
@@ -154,8 +158,10 @@ void launch_residual64_split(const Dims& d, const uint8_t* nc, const double* b64
                              hipStream_t s);
 // p64 += e, e = 0 on counted cells
 void launch_update_p64(const Dims& d, const uint8_t* nc, float* e_pad, double* p64_pad, hipStream_t s);
-// the same with e in a colour-split pressure pair e[0] / e[1]
-void launch_update_p64_split(const Dims& d, const uint8_t* nc, float* const* e, double* p64_pad, hipStream_t s);
+// the same with e in a colour-split pressure pair e[0] / e[1]; zero_e false leaves e as it is (the
+// next pass starts with a zero_start sweep)
+void launch_update_p64_split(const Dims& d, const uint8_t* nc, float* const* e, double* p64_pad, hipStream_t s,
+                             bool zero_e = true);
 // max |div64 - mean64| over counted cells into *dmax (bits of a non-negative double)
 template <class F>
 void launch_max_div64(const Dims& d, const F* u, const F* v, const F* w, const uint8_t* nc, const double* mean64,
