@@ -722,23 +722,32 @@ __global__ __launch_bounds__(BX* BY) void k_residual64_z(const uint8_t* __restri
     if (i < d.W && j < d.H) {
         const int sy = d.W + 2, sz = (d.W + 2) * (d.H + 2);
         const double mean = *mean64;
-        int q = pidx(d, i, j, k0);
+        const int HW = d.W * d.H;
+        int q = pidx(d, i, j, k0), o = cidx(d, i, j, k0);
         double pm = p[q - sz], pc = p[q];
-        for (int k = k0; k < k1; k++, q += sz) {
-            const double pp = p[q + sz];
-            const int o = cidx(d, i, j, k);
-            const int n = nc[o];
+        // this plane's loads, then each step issues the next plane's before computing: every load is
+        // unconditional (padded pressure, all cells of b64 and nc) so it can be in flight early
+        double xl = p[q - 1], xr = p[q + 1], yl = p[q - sy], yr = p[q + sy], pp = p[q + sz], bv = b64[o];
+        int n = nc[o];
+        for (int k = k0; k < k1; k++) {
+            const bool more = k + 1 < k1;
+            const int qn = more ? q + sz : q, on = more ? o + HW : o;
+            const double xl2 = p[qn - 1], xr2 = p[qn + 1], yl2 = p[qn - sy], yr2 = p[qn + sy], pp2 = p[qn + sz];
+            const double bv2 = b64[on];
+            const int n2 = nc[on];
+            const double sum = xl + xr + yl + yr + pm + pp;
+            const double bb = bv - mean;
+            const double rr = bb - (sum - (double)n * pc);
             float r = 0.0f;
             if (n) {
-                const double sum = p[q - 1] + p[q + 1] + p[q - sy] + p[q + sy] + pm + pp;
-                const double bb = b64[o] - mean;
-                const double rr = bb - (sum - (double)n * pc);
                 r = (float)rr;
                 m = fmax(m, fabs(rr));
             }
             (((i + j + k) & 1) ? r1 : r0)[(k * d.H + j) * Wh + (i >> 1)] = r;
             pm = pc;
             pc = pp;
+            xl = xl2, xr = xr2, yl = yl2, yr = yr2, pp = pp2, bv = bv2, n = n2;
+            q = qn, o = on;
         }
     }
     m = block_reduce<true>(m, red);
@@ -755,10 +764,14 @@ __global__ __launch_bounds__(BX* BY) void k_update_p64(const uint8_t* __restrict
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.W || j >= d.H) return;
-    if (!nc[cidx(d, i, j, k)]) return;
     const int q = pidx(d, i, j, k);
     float* ep = SPLIT ? &(((i + j + k) & 1) ? e1 : e0)[split_q(Wh, d.H, i >> 1, j, k)] : &e0[q];
-    p[q] = p[q] + (double)*ep;
+    // the three loads together (the count no longer gates the other two), one round trip
+    const int n = nc[cidx(d, i, j, k)];
+    const double pv = p[q];
+    const float ev = *ep;
+    if (!n) return;
+    p[q] = pv + (double)ev;
     if (ZERO) *ep = 0.0f;
 }
 
