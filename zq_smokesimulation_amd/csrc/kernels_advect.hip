@@ -156,7 +156,8 @@ __global__ __launch_bounds__(BX* BY) void k_apply_faces(const R* __restrict__ sr
     const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
     const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
                         AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
-    const bool ok = (pos == 0 || pos == lim) ? (is_open && !occ[ci]) : (!occ[ci] && !occ[ci - step]);
+    const int oh = occ[ci], ol = occ[(pos > 0 && pos < lim) ? ci - step : ci];  // both load together
+    const bool ok = (pos == 0 || pos == lim) ? (is_open && !oh) : (!oh && !ol);
     if (ok) {
         const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
         face[fi] = src[fi];

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(BX* BY) void k_atten_faces(float* __restrict__ face
     const int ci = cidx(d, AXIS == 0 ? (i < d.W ? i : d.W - 1) : i, AXIS == 1 ? (j < d.H ? j : d.H - 1) : j,
                         AXIS == 2 ? (k < d.D ? k : d.D - 1) : k);
     const int fi = AXIS == 0 ? uidx(d, i, j, k) : (AXIS == 1 ? vidx(d, i, j, k) : widx(d, i, j, k));
-    const bool ok = (pos == 0 || pos == lim) ? !occ[ci] : (!occ[ci - step] && !occ[ci]);
+    const int oh = occ[ci], ol = occ[(pos > 0 && pos < lim) ? ci - step : ci];  // both load together
+    const bool ok = (pos == 0 || pos == lim) ? !oh : (!ol && !oh);
     if (ok) face[fi] = face[fi] * aV;
 }
 

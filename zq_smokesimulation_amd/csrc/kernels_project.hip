@@ -78,12 +78,16 @@ __global__ __launch_bounds__(BX* BY) void k_proj_setup_z(const float* __restrict
         for (int k = k0; k < k1; k++, o += HW) {
             const int oa = k < d.D - 1 ? occ[o + HW] : 1;  // plane above (not read at k = D-1)
             const bool fluid = !oc;
+            // the in-plane neighbours load together, whatever this cell is (an index off the grid is
+            // replaced by the cell's own and its value not used)
+            const int oxl = occ[i > 0 ? o - 1 : o], oxh = occ[i < d.W - 1 ? o + 1 : o];
+            const int oyl = occ[j > 0 ? o - d.W : o], oyh = occ[j < d.H - 1 ? o + d.W : o];
             int n = 0;
             if (fluid) {
-                n += (i > 0) ? !occ[o - 1] : ((open_mask & 1) != 0);
-                n += (i < d.W - 1) ? !occ[o + 1] : ((open_mask & 2) != 0);
-                n += (j > 0) ? !occ[o - d.W] : ((open_mask & 4) != 0);
-                n += (j < d.H - 1) ? !occ[o + d.W] : ((open_mask & 8) != 0);
+                n += (i > 0) ? !oxl : ((open_mask & 1) != 0);
+                n += (i < d.W - 1) ? !oxh : ((open_mask & 2) != 0);
+                n += (j > 0) ? !oyl : ((open_mask & 4) != 0);
+                n += (j < d.H - 1) ? !oyh : ((open_mask & 8) != 0);
                 n += (k > 0) ? !ob : ((open_mask & 16) != 0);
                 n += (k < d.D - 1) ? !oa : ((open_mask & 32) != 0);
             }
@@ -866,11 +870,11 @@ __global__ __launch_bounds__(BX* BY) void k_max_div64_pred(const F* __restrict__
                 wt = pred_face(w[widx(d, i, j, k + 1)], ok, pu, pc);
             }
             if (right_here) {
-                const bool ok = i + 1 == d.W ? ((open_mask & 2) && !oc) : (!oc && !occ[o + 1]);
+                const bool ok = i + 1 == d.W ? ((open_mask & 2) && !oc) : (!occ[o + 1] && !oc);
                 ur = pred_face(u[uidx(d, i + 1, j, k)], ok, p[q + 1], pc);
             }
             if (top_here) {
-                const bool ok = j + 1 == d.H ? ((open_mask & 8) && !oc) : (!oc && !occ[o + W]);
+                const bool ok = j + 1 == d.H ? ((open_mask & 8) && !oc) : (!occ[o + W] && !oc);
                 vt = pred_face(v[vidx(d, i, j + 1, k)], ok, p[q + sy], pc);
             }
         }

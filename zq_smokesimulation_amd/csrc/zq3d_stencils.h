@@ -132,20 +132,26 @@ __device__ __forceinline__ void st_force_face(R* __restrict__ face, const uint8_
     const int ci = face_cell<AXIS>(d, i, j, k);
     const int fi = face_index<AXIS>(d, i, j, k);
     const R* f = fc + (size_t)AXIS * d.W * d.H * d.D;
+    // both cells' occupancy and force load together (on a domain face the low index is the face's
+    // own cell and its values are not used)
+    const int cl = (pos > 0 && pos < lim) ? ci - step : ci;
+    const int oh = occ[ci], ol = occ[cl];
+    const R fh = f[ci], fl = f[cl];
     bool ok;
     R add;
     if (pos == 0) {
-        ok = !occ[ci];
-        add = dt * f[ci];
+        ok = !oh;
+        add = dt * fh;
     } else if (pos == lim) {
-        ok = !occ[ci];  // ci already points at the last cell
-        add = dt * f[ci];
+        ok = !oh;  // ci already points at the last cell
+        add = dt * fh;
     } else {
-        ok = !occ[ci - step] && !occ[ci];
-        add = dth * (f[ci] + f[ci - step]);
+        ok = !ol && !oh;
+        add = dth * (fh + fl);
     }
+    const R f0 = face[fi];  // loaded with the occupancy, not after it
     if (ok) {
-        R x = face[fi] + add;
+        R x = f0 + add;
         if (ATTEN) x = x * aV;
         face[fi] = x;
     }
@@ -202,14 +208,17 @@ __device__ __forceinline__ void st_cells_to_face(const R* __restrict__ adv, cons
     const int step = AXIS == 0 ? 1 : (AXIS == 1 ? d.W : d.W * d.H);
     const int ci = face_cell<AXIS>(d, i, j, k);
     const R* a = adv + (size_t)AXIS * d.W * d.H * d.D;
+    const int cl = (pos > 0 && pos < lim) ? ci - step : ci;  // as in st_force_face
+    const int oh = occ[ci], ol = occ[cl];
+    const R ah = a[ci], al = a[cl];
     bool ok;
     R val;
     if (pos == 0 || pos == lim) {
-        ok = is_open && !occ[ci];
-        val = a[ci];
+        ok = is_open && !oh;
+        val = ah;
     } else {
-        ok = !occ[ci] && !occ[ci - step];
-        val = R(0.5) * (a[ci] + a[ci - step]);
+        ok = !oh && !ol;
+        val = R(0.5) * (ah + al);
     }
     if (ok) face[face_index<AXIS>(d, i, j, k)] = val;
 }
@@ -353,15 +362,18 @@ __device__ __forceinline__ F st_corrected_face(const uint8_t* __restrict__ occ, 
     const int pstep = AXIS == 0 ? 1 : (AXIS == 1 ? d.W + 2 : (d.W + 2) * (d.H + 2));
     const int lo_bit = AXIS == 0 ? 1 : (AXIS == 1 ? 4 : 16);
     const int ci = face_cell<AXIS>(d, i, j, k);
+    const int oh = occ[ci], ol = occ[(pos > 0 && pos < lim) ? ci - step : ci];  // as in st_force_face
     bool ok;
-    if (pos == 0) ok = (open_mask & lo_bit) && !occ[ci];
-    else if (pos == lim) ok = (open_mask & (lo_bit << 1)) && !occ[ci];
-    else ok = !occ[ci - step] && !occ[ci];
+    if (pos == 0) ok = (open_mask & lo_bit) && !oh;
+    else if (pos == lim) ok = (open_mask & (lo_bit << 1)) && !oh;
+    else ok = !ol && !oh;
     const F f = face[face_index<AXIS>(d, i, j, k)];
-    if (!ok) return f;
-    // padded index of the cell on the high side of the face (ghost when pos == lim)
+    // padded index of the cell on the high side of the face (ghost when pos == lim); both pressures
+    // are loaded with the occupancy (every padded index is in range), not after the test
     const int qh = pidx(d, i, j, k);
-    return (F)((P)f - (p[qh] - p[qh - pstep]));
+    const P ph = p[qh], pl = p[qh - pstep];
+    if (!ok) return f;
+    return (F)((P)f - (ph - pl));
 }
 
 template <int AXIS, class P, class F>
