@@ -1132,6 +1132,10 @@ long long sor_launches(const Dims& d, SorShape shape, int iters) {
     return (fused_shape(d, shape).nblocks > 0 ? 1LL : 2LL) * iters;
 }
 
+bool split_buffers_fit(const Dims& d) {
+    return split_padded_size(d) * 4 <= 0x7FFFFFFFull && split_cell_size(d) * 4 <= 0x7FFFFFFFull;
+}
+
 int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
     static const int tys[] = {2, 4, 8};
     static const int zcs[] = {8, 16, 32, 64};
@@ -1144,9 +1148,10 @@ int sor_shape_candidates(const Dims& d, SorShape* out, int max) {
     if (n < max) out[n++] = SorShape{SOR_SPLIT, 0};
     static const int rb_rows[] = {4, 8, 16, 32};
     static const int rb_zc[] = {8, 16, 32};
-    for (int ty : rb_rows)
-        for (int zc : rb_zc)
-            if (n < max) out[n++] = SorShape{-ty, zc};
+    if (split_buffers_fit(d))
+        for (int ty : rb_rows)
+            for (int zc : rb_zc)
+                if (n < max) out[n++] = SorShape{-ty, zc};
     static const int rb2_rows[] = {16, 32};
     static const int rb2_zc[] = {8, 16, 32};
     for (int ty : rb2_rows)

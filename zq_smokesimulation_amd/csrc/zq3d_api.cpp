@@ -216,6 +216,10 @@ int refine_mixed(const Dims& d, const zq3d_sor_opts& o, double tol, const ProjBu
         set_err(where, "pass_rule 2 needs the colour-split buffers");
         return ZQ3D_ERR_ARG;
     }
+    if (gated && !split_buffers_fit(d)) {
+        set_err(where, "pass_rule 2 needs colour-split arrays under 2 GiB (grids up to about 1000^3)");
+        return ZQ3D_ERR_ARG;
+    }
     const bool split = gated || is_split(B.shape);
     int pair = 0;  // split pressure pair holding the correction
     float* e = B.p;

@@ -45,6 +45,13 @@ namespace {
 
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// whether the one-launch split sweeps can serve every member (split_buffers_fit)
+bool rb_fits(const zq3d_ens* e) {
+    for (const Dims& d : e->dims)
+        if (!split_buffers_fit(d)) return false;
+    return true;
+}
+
 void drop_graph(zq3d_ens* e) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -279,7 +286,9 @@ extern "C" int zq3d_ens_create(int count, const zq3d_config* cfgs, zq3d_ens** ou
     for (int sh = 0; sh < kEnsSweepShapeCount; sh++) L.half[sh] = table_at(e, flat, at[4 + sh]);
     for (int r = 0; r < kEnsRbShapeCount; r++) L.rb[r] = table_at(e, flat, at[4 + kEnsSweepShapeCount + r]);
     L.sweep_shape = 0;
-    L.sweep_rb = 4;  // one-launch iterations, 64 slots x 16 rows x 16 planes per workgroup
+    // one-launch iterations, 64 slots x 16 rows x 16 planes per workgroup, when every member's split
+    // arrays are small enough for them; else the half sweeps
+    L.sweep_rb = rb_fits(e) ? 4 : 0;
     L.blocks = 0;
     L.sweep_blocks = 0;
     L.p_pool = e->p;
@@ -403,7 +412,7 @@ extern "C" int zq3d_ens_set_sweep(zq3d_ens* e, int kind, int tbx, int tby, int b
                 L.sweep_blocks = blocks;
                 ok = true;
             }
-    } else if (kind == 1) {
+    } else if (kind == 1 && rb_fits(e)) {
         for (int r = 0; r < kEnsRbShapeCount; r++)
             if (kEnsRbShapes[r][0] == tbx && kEnsRbShapes[r][1] == tby && kEnsRbShapes[r][2] == blocks) {
                 L.sweep_rb = r + 1;
@@ -470,7 +479,7 @@ extern "C" int zq3d_ens_autotune(zq3d_ens* e, int iters, zq3d_ens_tune* rep) {
             const float ms = time_it([&] { ens_launch_sweeps(L, 0, iters, e->stream); });
             consider(kEnsSweepShapes[sh][0], kEnsSweepShapes[sh][1], blocks[bi], 1e3f * ms / iters, sh, blocks[bi], 0);
         }
-    for (int r = 0; r < kEnsRbShapeCount; r++) {
+    for (int r = 0; r < kEnsRbShapeCount && rb_fits(e); r++) {
         L.sweep_rb = r + 1;
         const float ms = time_it([&] { ens_launch_sweeps(L, 0, iters, e->stream); });
         consider(kEnsRbShapes[r][0], kEnsRbShapes[r][1], kEnsRbShapes[r][2], 1e3f * ms / iters, 0, 0, r + 1);

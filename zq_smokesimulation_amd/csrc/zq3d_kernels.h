@@ -86,6 +86,9 @@ struct SorShape {
 inline bool is_split(SorShape s) { return s.ty <= SOR_SPLIT; }
 // the one-launch split shapes, whose first iteration can take the source pair as zeros unread
 inline bool split_zero_start(SorShape s) { return s.ty < SOR_SPLIT && s.ty > -SOR_TWO; }
+// whether the one-launch split kernels (buffer access with 32-bit offsets) can run on the grid: every
+// colour-split array under 2 GiB (cubes up to about 1000 cells on a side)
+bool split_buffers_fit(const Dims& d);
 // Colour-split scratch: per colour a padded pressure (split_padded_size floats), a second padded
 // pressure the one-launch iteration ping-pongs with, right-hand side and counts (split_cell_size
 // each).  Padding slots and the slots past a row's end must hold 0 in p and alt.
