@@ -96,22 +96,33 @@ int octree_special_regions(OctreeState& t, float* u, float* v, float* w, int ite
         }
     }
     const size_t Wu = (size_t)W + 1, Hv = (size_t)H + 1;
+    // every region's cell count, float divergence sum and bounding box in one scan-order pass, where
+    // the reference scans the grid once per region (ZQ_SmokeOctreeGrid3D.cpp:561-590): each sum still
+    // takes its region's cells in scan order, and a region's solve changes only faces between two of
+    // its own cells, so the faces a later region sums are the ones before any solve
+    struct Box {
+        int x0, y0, z0, x1, y1, z1, count;
+        float sum;
+    };
+    std::vector<Box> boxes(regions + 1, Box{W, H, D, -1, -1, -1, 0, 0.0f});
+    for (int k = 0; k < D; k++)
+        for (int j = 0; j < H; j++)
+            for (int i = 0; i < W; i++) {
+                const int r = lab[((size_t)k * H + j) * W + i];
+                if (r <= 0) continue;
+                Box& b = boxes[r];
+                b.count++;
+                b.sum += hu[((size_t)k * H + j) * Wu + i + 1] - hu[((size_t)k * H + j) * Wu + i] +
+                         hv[((size_t)k * Hv + j + 1) * W + i] - hv[((size_t)k * Hv + j) * W + i] +
+                         hw[((size_t)(k + 1) * H + j) * W + i] - hw[((size_t)k * H + j) * W + i];
+                b.x0 = std::min(b.x0, i), b.y0 = std::min(b.y0, j), b.z0 = std::min(b.z0, k);
+                b.x1 = std::max(b.x1, i), b.y1 = std::max(b.y1, j), b.z1 = std::max(b.z1, k);
+            }
     for (int r = 1; r <= regions; r++) {
-        int x0 = W, y0 = H, z0 = D, x1 = -1, y1 = -1, z1 = -1, count = 0;
-        float mean = 0.0f;  // the reference's float sum in scan order (ZQ_SmokeOctreeGrid3D.cpp:575-580)
-        for (int k = 0; k < D; k++)
-            for (int j = 0; j < H; j++)
-                for (int i = 0; i < W; i++) {
-                    if (lab[((size_t)k * H + j) * W + i] != r) continue;
-                    count++;
-                    mean += hu[((size_t)k * H + j) * Wu + i + 1] - hu[((size_t)k * H + j) * Wu + i] +
-                            hv[((size_t)k * Hv + j + 1) * W + i] - hv[((size_t)k * Hv + j) * W + i] +
-                            hw[((size_t)(k + 1) * H + j) * W + i] - hw[((size_t)k * H + j) * W + i];
-                    x0 = std::min(x0, i), y0 = std::min(y0, j), z0 = std::min(z0, k);
-                    x1 = std::max(x1, i), y1 = std::max(y1, j), z1 = std::max(z1, k);
-                }
-        mean /= count;
-        const int sx = x1 - x0 + 1, sy = y1 - y0 + 1, sz = z1 - z0 + 1;
+        const Box& b = boxes[r];
+        const int x0 = b.x0, y0 = b.y0, z0 = b.z0, count = b.count;
+        const float mean = b.sum / count;
+        const int sx = b.x1 - x0 + 1, sy = b.y1 - y0 + 1, sz = b.z1 - z0 + 1;
         std::vector<uint8_t> bo((size_t)sx * sy * sz);
         std::vector<float> bu((size_t)(sx + 1) * sy * sz), bv((size_t)sx * (sy + 1) * sz), bw((size_t)sx * sy * (sz + 1));
         for (int k = 0; k < sz; k++)
