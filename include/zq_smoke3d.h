@@ -177,7 +177,7 @@ typedef struct zq3d_sor_opts {
                              once) have max|div - mean| <= tol */
     int pass_rule;        /* mixed: how long a refinement pass runs.  0: `iters` sweeps.  1: fitted, once a
                              pass has lowered max|r| (still above tol) the next runs only the sweeps that
-                             pass's per-sweep decay needs to reach tol, doubled (1 .. iters).  2: gated,
+                             pass's per-sweep decay needs to reach tol, times 1.25 (1 .. iters).  2: gated,
                              each pass sweeps until an iteration's largest update |new - old| is at most
                              0.75 tol omega / 6, or is no smaller than 16 iterations earlier (at most
                              `iters` <= 8192 sweeps); the iterations run back to back on the device,

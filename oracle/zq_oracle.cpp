@@ -797,12 +797,12 @@ void orc_project(int W, int H, int D, float* u, float* v, float* w, const uint8_
 
 /* The pass-length rule of the mixed-precision projection with fitted passes.  Over a pass of nprev
  * sweeps max|r| fell from rprev to r, a per-sweep factor rho = (r/rprev)^(1/nprev); reaching tol at
- * that rate takes log(tol/r)/log(rho) more sweeps.  The count is doubled because the decay slows as
- * the smooth error modes take over (a stationary iteration's early sweeps remove the rough modes
- * fastest).  Evaluated as 2 * nprev * (log(tol/r) / log(r/rprev)). */
+ * that rate takes log(tol/r)/log(rho) more sweeps.  The count is taken 1.25 times because the decay
+ * slows as the smooth error modes take over (a stationary iteration's early sweeps remove the rough
+ * modes fastest).  Evaluated as 1.25 * nprev * (log(tol/r) / log(r/rprev)). */
 int orc_fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol) {
     if (!fit || !(rprev > 0.0) || !(r < rprev) || !(r > tol) || nprev < 1) return iters;
-    const double need = 2.0 * (double)nprev * (std::log(tol / r) / std::log(r / rprev));
+    const double need = 1.25 * (double)nprev * (std::log(tol / r) / std::log(r / rprev));
     if (!(need < (double)iters)) return iters;
     const int n = (int)std::ceil(need);
     return n < 1 ? 1 : n;

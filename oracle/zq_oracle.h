@@ -114,7 +114,7 @@ int orc_project_mp(int W, int H, int D, float* u, float* v, float* w, const uint
 /* Sweeps of the next refinement pass when the pass lengths are fitted: r is the max|r| the last pass
  * left, rprev the one before it, nprev that pass's sweeps.  Once a pass has lowered max|r| (still
  * above tol), the next pass runs the sweeps that pass's per-sweep decay (r/rprev)^(1/nprev) needs to
- * reach tol, doubled, at least 1 and at most iters; otherwise iters. */
+ * reach tol, times 1.25, at least 1 and at most iters; otherwise iters. */
 int orc_fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol);
 /* Starting pressure extrapolated in time (the device's warm_start 2): over n padded cells,
  * p = p + beta (p - prev) and prev = the old p; valid == 0: prev = p only. */

@@ -247,7 +247,7 @@ def test_sor_iteration_equals_dense_red_black_sor(rng, shape, open_mask, frac, o
 
 def test_fitted_pass_length_rule():
     """orc_fitted_pass_length: a whole pass unless fitting applies; otherwise the sweeps that the last
-    pass's per-sweep decay needs to reach tol, doubled, at least 1 and at most a whole pass."""
+    pass's per-sweep decay needs to reach tol, times 1.25, at least 1 and at most a whole pass."""
     import math
     f = O.fitted_pass_length
     assert f(40, 0, 1e-5, 1e-4, 40, 1e-6) == 40    # fitting off
@@ -258,25 +258,28 @@ def test_fitted_pass_length_rule():
     # a geometric decay of rho per sweep over a pass of n sweeps
     rho, n, rprev = 0.9, 30, 1e-3
     r = rprev * rho ** n
-    for extra, want in ((4.3, 9), (2.2, 5), (0.01, 1), (19.6, 40), (200.0, 40)):
+    for extra, want in ((4.3, 6), (2.2, 3), (0.01, 1), (19.6, 25), (40.0, 40), (200.0, 40)):
         tol = r * rho ** extra      # reached after `extra` more sweeps at that rate
         assert f(40, 1, r, rprev, n, tol) == want, (extra, f(40, 1, r, rprev, n, tol))
-        assert want == min(40, max(1, math.ceil(2 * extra)))
+        assert want == min(40, max(1, math.ceil(1.25 * extra)))
 
 
 def test_fitted_passes_reach_tolerance_in_fewer_sweeps():
-    """With fitted pass lengths the projection still meets tol, and on this seeded case it stops sooner
-    than with whole passes (which overshoot by up to a pass): 206 sweeps against 240."""
+    """With fitted pass lengths the projection still meets tol.  A fitted pass is shorter than a whole
+    one, so the total usually drops (seed 2: 220 sweeps in 6 passes against 240 in 6), but a pass that
+    falls short of tol costs one more pass (seed 1: 244 in 7 against 240 in 6); the rule pays off
+    where a pass's residual and host wait cost more than the sweeps it saves."""
     W, H, D = 24, 20, 16
-    u, v, w = faces(np.random.default_rng(1), W, H, D, scale=0.05)
     occ = np.zeros((D, H, W), np.uint8)
-    res = {}
-    for fit in (False, True):
-        uu, vv, ww = u.copy(), v.copy(), w.copy()
-        outer, st, _ = O.project_mp(uu, vv, ww, occ, O.OPEN_YHI, False, 1.9, 40, 400, 1e-7, pass_rule=fit)
-        assert st[1] <= 1e-7 and st[2] <= 1e-7
-        res[fit] = int(st[0])
-    assert res[False] % 40 == 0 and res[True] < res[False], res
+    for seed, want in ((2, {False: (240, 6), True: (220, 6)}), (1, {False: (240, 6), True: (244, 7)})):
+        u, v, w = faces(np.random.default_rng(seed), W, H, D, scale=0.05)
+        res = {}
+        for fit in (False, True):
+            uu, vv, ww = u.copy(), v.copy(), w.copy()
+            outer, st, _ = O.project_mp(uu, vv, ww, occ, O.OPEN_YHI, False, 1.9, 40, 400, 1e-7, pass_rule=fit)
+            assert st[1] <= 1e-7 and st[2] <= 1e-7
+            res[fit] = (int(st[0]), outer)
+        assert res == want, (seed, res)
 
 
 def test_gated_sor_stops_after_the_first_small_iteration():

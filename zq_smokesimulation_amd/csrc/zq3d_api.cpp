@@ -200,8 +200,10 @@ int gated_count(const ProjBufs& B, hipStream_t s, const char* where, long long* 
 int fitted_pass_length(int iters, int fit, double r, double rprev, int nprev, double tol) {
     if (!fit || !(rprev > 0.0) || !(r < rprev) || !(r > tol) || nprev < 1) return iters;
     // per-sweep decay over the last pass, rho = (r / rprev)^(1/nprev); sweeps to tol at that rate,
-    // doubled because the decay slows as the smooth error modes take over
-    const double need = 2.0 * (double)nprev * (std::log(tol / r) / std::log(r / rprev));
+    // times 1.25 because the decay slows as the smooth error modes take over (on the 256^3 closed-box
+    // plume 1.25 ran fastest of 1.0, 1.1, 1.25, 1.5 and 2: a short pass costs another pass's residual
+    // and host wait, a long one overshoots)
+    const double need = 1.25 * (double)nprev * (std::log(tol / r) / std::log(r / rprev));
     if (!(need < (double)iters)) return iters;
     const int n = (int)std::ceil(need);
     return n < 1 ? 1 : n;
