@@ -17,22 +17,21 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-// The bodies live in zq3d_stencils.h (shared with the ensemble kernels).
-template <class R>
+// The bodies live in zq3d_stencils.h (shared with the ensemble kernels).  VO / VI: where the
+// vorticity records go and are read from (one 4-wide record per cell, or four planes).
+template <class R, class VO>
 __global__ __launch_bounds__(BX* BY) void k_vorticity(const R* __restrict__ u, const R* __restrict__ v,
-                                                      const R* __restrict__ w, vec4_t<R>* __restrict__ vort, Dims d,
-                                                      R two_h) {
-    st_vorticity<R>(u, v, w, vort, d, two_h, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
-                    blockIdx.z);
+                                                      const R* __restrict__ w, VO vort, Dims d, R two_h) {
+    st_vorticity_to<R>(u, v, w, vort, d, two_h, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y,
+                       blockIdx.z);
 }
 
-template <class R, bool ATTEN>
-__global__ __launch_bounds__(BX* BY) void k_force_cells(const vec4_t<R>* __restrict__ vort, R* __restrict__ T,
-                                                        R* __restrict__ rho, const R* __restrict__ extra,
-                                                        R* __restrict__ fcell, Dims d, R ch, R beta, R alpha, R Tamb,
-                                                        R aT, R aR) {
-    st_force_cell<R, ATTEN>(vort, T, rho, extra, fcell, d, ch, beta, alpha, Tamb, aT, aR,
-                            blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
+template <class R, bool ATTEN, class VI>
+__global__ __launch_bounds__(BX* BY) void k_force_cells(VI vort, R* __restrict__ T, R* __restrict__ rho,
+                                                        const R* __restrict__ extra, R* __restrict__ fcell, Dims d,
+                                                        R ch, R beta, R alpha, R Tamb, R aT, R aR) {
+    st_force_cell_from<R, ATTEN>(vort, T, rho, extra, fcell, d, ch, beta, alpha, Tamb, aT, aR,
+                                 blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
 }
 
 // All three face grids in one launch: thread (i, j) walks a chunk of FACE_RZ planes and updates each
@@ -101,27 +100,44 @@ void force_faces(const Dims& d, R* u, R* v, R* w, const uint8_t* occ, const R* f
 template <class R>
 void launch_vorticity(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vort, float h, hipStream_t s) {
     const R two_h = R(2) * (R)h;
-    hipLaunchKernelGGL(k_vorticity<R>, grid_for(d.W, d.H, d.D), dim3(BX, BY, 1), 0, s, u, v, w, vort, d, two_h);
+    hipLaunchKernelGGL((k_vorticity<R, VortCellsOut<R>>), grid_for(d.W, d.H, d.D), dim3(BX, BY, 1), 0, s, u, v, w,
+                       VortCellsOut<R>{vort}, d, two_h);
 }
 
-template <class R>
-void launch_force_atten(const Dims& d, R* u, R* v, R* w, R* T, R* rho, const uint8_t* occ, const R* extra,
-                        const ForceParams& fp, int atten, float aV, float aT, float aR, vec4_t<R>* vort, R* fcell,
-                        hipStream_t s) {
+namespace {
+template <class R, class VO, class VI>
+void force_atten(const Dims& d, R* u, R* v, R* w, R* T, R* rho, const uint8_t* occ, const R* extra,
+                 const ForceParams& fp, int atten, float aV, float aT, float aR, VO vo, VI vi, R* fcell,
+                 hipStream_t s) {
     const dim3 blk(BX, BY, 1);
-    launch_vorticity<R>(d, u, v, w, vort, fp.h, s);
+    hipLaunchKernelGGL((k_vorticity<R, VO>), grid_for(d.W, d.H, d.D), blk, 0, s, u, v, w, vo, d, R(2) * (R)fp.h);
     // the products are formed in float, as the reference's host code passes them
     const R ch = (R)(fp.eps * fp.h);
     const R dth = (R)(fp.dt * 0.5f);
     const R dt = (R)fp.dt;
     if (atten) {
-        hipLaunchKernelGGL((k_force_cells<R, true>), grid_for(d.W, d.H, d.D), blk, 0, s, vort, T, rho, extra, fcell, d,
-                           ch, (R)fp.beta, (R)fp.alpha, (R)fp.Tamb, (R)aT, (R)aR);
+        hipLaunchKernelGGL((k_force_cells<R, true, VI>), grid_for(d.W, d.H, d.D), blk, 0, s, vi, T, rho, extra, fcell,
+                           d, ch, (R)fp.beta, (R)fp.alpha, (R)fp.Tamb, (R)aT, (R)aR);
         force_faces<R, true>(d, u, v, w, occ, fcell, dt, dth, (R)aV, s);
     } else {
-        hipLaunchKernelGGL((k_force_cells<R, false>), grid_for(d.W, d.H, d.D), blk, 0, s, vort, T, rho, extra, fcell,
-                           d, ch, (R)fp.beta, (R)fp.alpha, (R)fp.Tamb, R(1), R(1));
+        hipLaunchKernelGGL((k_force_cells<R, false, VI>), grid_for(d.W, d.H, d.D), blk, 0, s, vi, T, rho, extra,
+                           fcell, d, ch, (R)fp.beta, (R)fp.alpha, (R)fp.Tamb, R(1), R(1));
         force_faces<R, false>(d, u, v, w, occ, fcell, dt, dth, R(1), s);
+    }
+}
+}  // namespace
+
+template <class R>
+void launch_force_atten(const Dims& d, R* u, R* v, R* w, R* T, R* rho, const uint8_t* occ, const R* extra,
+                        const ForceParams& fp, int atten, float aV, float aT, float aR, vec4_t<R>* vort, R* fcell,
+                        hipStream_t s, bool vort_planes) {
+    if (vort_planes) {
+        const int n = d.W * d.H * d.D;
+        force_atten<R>(d, u, v, w, T, rho, occ, extra, fp, atten, aV, aT, aR, VortPlanesOut<R>{(R*)vort, n},
+                       VortPlanesIn<R>{(const R*)vort, n}, fcell, s);
+    } else {
+        force_atten<R>(d, u, v, w, T, rho, occ, extra, fp, atten, aV, aT, aR, VortCellsOut<R>{vort},
+                       VortCellsIn<R>{vort}, fcell, s);
     }
 }
 
@@ -131,10 +147,10 @@ template void launch_vorticity<double>(const Dims&, const double*, const double*
                                        hipStream_t);
 template void launch_force_atten<float>(const Dims&, float*, float*, float*, float*, float*, const uint8_t*,
                                         const float*, const ForceParams&, int, float, float, float, float4*, float*,
-                                        hipStream_t);
+                                        hipStream_t, bool);
 template void launch_force_atten<double>(const Dims&, double*, double*, double*, double*, double*, const uint8_t*,
                                          const double*, const ForceParams&, int, float, float, float, double4*,
-                                         double*, hipStream_t);
+                                         double*, hipStream_t, bool);
 
 void launch_attenuation(const Dims& d, float* u, float* v, float* w, float* T, float* rho, const uint8_t* occ,
                         float aV, float aT, float aR, hipStream_t s) {

@@ -1119,8 +1119,9 @@ extern "C" int zq3d_add_force_attenuate(zq3d_sim* s, const float* extra) {
     const zq3d_config& c = s->cfg;
     const ForceParams fp{c.voxel_size, c.step_time, c.buoyancy, c.gravity, c.confine, c.Tamb};
     StageTimer t(s, &s->tm.ms_force);
+    // vel4 is scratch here (the advection stage fills it again), so the vorticity goes in planes
     launch_force_atten(s->d, s->u, s->v, s->w, s->T, s->R, s->occ, extra, fp, 1, c.vel_atten, c.temp_atten,
-                       c.dens_atten, s->vel4, s->scratch, s->stream);
+                       c.dens_atten, s->vel4, s->scratch, s->stream, true);
     ZQ_HIP(hipGetLastError(), "zq3d_add_force_attenuate");
     return t.done("zq3d_add_force_attenuate");
 }

@@ -19,10 +19,13 @@ struct ForceParams {
 // ---- force + attenuation (ZQ_CUDA_AddForce3D.cu, ZQ_CUDA_Attenuation3D.cu) ----
 // vort: N 4-wide scratch, fcell: 3N scratch.  extra (nullable): 3 values per cell.
 // atten != 0 fuses the attenuation stage (faces *= aV on the same mask, T *= aT, rho *= aR).
+// vort_planes: the vorticity records go to vort as four planes of N values (wx, wy, wz, |w|) instead
+// of one 4-wide record per cell (same bytes, same results; the force stage's |w| gradient then reads
+// dense neighbours).  Callers that read vort afterwards as records leave it false.
 template <class R>
 void launch_force_atten(const Dims& d, R* u, R* v, R* w, R* T, R* rho, const uint8_t* occ, const R* extra,
                         const ForceParams& fp, int atten, float aV, float aT, float aR, vec4_t<R>* vort, R* fcell,
-                        hipStream_t s);
+                        hipStream_t s, bool vort_planes = false);
 // (wx, wy, wz, |w|) per cell, zero on the border cells
 template <class R>
 void launch_vorticity(const Dims& d, const R* u, const R* v, const R* w, vec4_t<R>* vort, float h, hipStream_t s);

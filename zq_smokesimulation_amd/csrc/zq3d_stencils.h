@@ -44,10 +44,51 @@ __device__ __forceinline__ R st_cell_w(const R* __restrict__ w, const Dims& d, i
 
 // Compute_Vorticity_Vector_Scale_Kernel (ZQ_CUDA_AddForce3D.cu:11-34): central differences of the
 // cell velocity over 2h on interior cells, zero on the border (the reference memsets the arrays).
+// Where the vorticity stage keeps (wx, wy, wz, |w|) per cell.  Cells: one 4-wide record per cell.
+// Planes: four planes of N values (wx, wy, wz, |w|) in the same 4N-value buffer, so the force
+// stage's |w| gradient reads six dense neighbours instead of one lane of 16-byte records.  The
+// writers are the vorticity stage's, the readers the force stage's.
 template <class R>
-__device__ __forceinline__ void st_vorticity(const R* __restrict__ u, const R* __restrict__ v,
-                                             const R* __restrict__ w, vec4_t<R>* __restrict__ vort, const Dims& d,
-                                             R two_h, int i, int j, int k) {
+struct VortCellsOut {
+    vec4_t<R>* p;
+    __device__ __forceinline__ void store(int o, const vec4_t<R>& v) const { p[o] = v; }
+};
+template <class R>
+struct VortCellsIn {
+    const vec4_t<R>* p;
+    __device__ __forceinline__ R mag(int o) const { return p[o].w; }
+    __device__ __forceinline__ vec4_t<R> get(int o) const { return p[o]; }
+};
+template <class R>
+struct VortPlanesOut {
+    R* p;
+    int n;
+    __device__ __forceinline__ void store(int o, const vec4_t<R>& v) const {
+        p[o] = v.x;
+        p[n + o] = v.y;
+        p[2 * n + o] = v.z;
+        p[3 * n + o] = v.w;
+    }
+};
+template <class R>
+struct VortPlanesIn {
+    const R* p;
+    int n;
+    __device__ __forceinline__ R mag(int o) const { return p[3 * n + o]; }
+    __device__ __forceinline__ vec4_t<R> get(int o) const {
+        vec4_t<R> v;
+        v.x = p[o];
+        v.y = p[n + o];
+        v.z = p[2 * n + o];
+        v.w = p[3 * n + o];
+        return v;
+    }
+};
+
+template <class R, class VS>
+__device__ __forceinline__ void st_vorticity_to(const R* __restrict__ u, const R* __restrict__ v,
+                                                const R* __restrict__ w, const VS& vort, const Dims& d, R two_h,
+                                                int i, int j, int k) {
     if (i >= d.W || j >= d.H || k >= d.D) return;
     vec4_t<R> out;
     out.x = out.y = out.z = out.w = R(0);
@@ -63,38 +104,44 @@ __device__ __forceinline__ void st_vorticity(const R* __restrict__ u, const R* _
         out.z = wz;
         out.w = zsqrt(wx * wx + wy * wy + wz * wz);
     }
-    vort[cidx(d, i, j, k)] = out;
+    vort.store(cidx(d, i, j, k), out);
+}
+
+template <class R>
+__device__ __forceinline__ void st_vorticity(const R* __restrict__ u, const R* __restrict__ v,
+                                             const R* __restrict__ w, vec4_t<R>* __restrict__ vort, const Dims& d,
+                                             R two_h, int i, int j, int k) {
+    st_vorticity_to<R>(u, v, w, VortCellsOut<R>{vort}, d, two_h, i, j, k);
 }
 
 // Compute_Gradient_of_VorticityScale_Kernel (:64-111) + Compute_Force_Kernel (:161-184).
 // Output is SoA: fcell[0..N) = fx, [N..2N) = fy, [2N..3N) = fz.
-template <class R, bool ATTEN>
-__device__ __forceinline__ void st_force_cell(const vec4_t<R>* __restrict__ vort, R* __restrict__ T,
-                                              R* __restrict__ rho, const R* __restrict__ extra, R* __restrict__ fcell,
-                                              const Dims& d, R ch, R beta, R alpha, R Tamb, R aT, R aR, int i, int j,
-                                              int k) {
+template <class R, bool ATTEN, class VS>
+__device__ __forceinline__ void st_force_cell_from(const VS& vort, R* __restrict__ T, R* __restrict__ rho,
+                                                   const R* __restrict__ extra, R* __restrict__ fcell, const Dims& d,
+                                                   R ch, R beta, R alpha, R Tamb, R aT, R aR, int i, int j, int k) {
     if (i >= d.W || j >= d.H || k >= d.D) return;
     const int o = cidx(d, i, j, k);
     const int HW = d.H * d.W;
     const int N = HW * d.D;
-    const R s0 = vort[o].w;
+    const R s0 = vort.mag(o);
     R gx, gy, gz;
-    if (i == 0) gx = vort[o + 1].w - s0;
-    else if (i == d.W - 1) gx = s0 - vort[o - 1].w;
-    else gx = R(0.5) * (vort[o + 1].w - vort[o - 1].w);
-    if (j == 0) gy = vort[o + d.W].w - s0;
-    else if (j == d.H - 1) gy = s0 - vort[o - d.W].w;
-    else gy = R(0.5) * (vort[o + d.W].w - vort[o - d.W].w);
-    if (k == 0) gz = vort[o + HW].w - s0;
-    else if (k == d.D - 1) gz = s0 - vort[o - HW].w;
-    else gz = R(0.5) * (vort[o + HW].w - vort[o - HW].w);
+    if (i == 0) gx = vort.mag(o + 1) - s0;
+    else if (i == d.W - 1) gx = s0 - vort.mag(o - 1);
+    else gx = R(0.5) * (vort.mag(o + 1) - vort.mag(o - 1));
+    if (j == 0) gy = vort.mag(o + d.W) - s0;
+    else if (j == d.H - 1) gy = s0 - vort.mag(o - d.W);
+    else gy = R(0.5) * (vort.mag(o + d.W) - vort.mag(o - d.W));
+    if (k == 0) gz = vort.mag(o + HW) - s0;
+    else if (k == d.D - 1) gz = s0 - vort.mag(o - HW);
+    else gz = R(0.5) * (vort.mag(o + HW) - vort.mag(o - HW));
     const R len = zsqrt(gx * gx + gy * gy + gz * gz);
     if (len != R(0)) {
         gx = gx / len;
         gy = gy / len;
         gz = gz / len;
     }
-    const vec4_t<R> wv = vort[o];
+    const vec4_t<R> wv = vort.get(o);
     R fx = ch * (gy * wv.z - gz * wv.y);
     R fy = ch * (gz * wv.x - gx * wv.z);
     R fz = ch * (gx * wv.y - gy * wv.x);
@@ -114,6 +161,15 @@ __device__ __forceinline__ void st_force_cell(const vec4_t<R>* __restrict__ vort
         T[o] = t * aT;
         rho[o] = r * aR;
     }
+}
+
+template <class R, bool ATTEN>
+__device__ __forceinline__ void st_force_cell(const vec4_t<R>* __restrict__ vort, R* __restrict__ T,
+                                              R* __restrict__ rho, const R* __restrict__ extra, R* __restrict__ fcell,
+                                              const Dims& d, R ch, R beta, R alpha, R Tamb, R aT, R aR, int i, int j,
+                                              int k) {
+    st_force_cell_from<R, ATTEN>(VortCellsIn<R>{vort}, T, rho, extra, fcell, d, ch, beta, alpha, Tamb, aT, aR, i,
+                                 j, k);
 }
 
 // AddForce_{u,v,w}_Kernel (:211-420) fused with Atten_{u,v,w}_Kernel (ZQ_CUDA_Attenuation3D.cu:8-96):
