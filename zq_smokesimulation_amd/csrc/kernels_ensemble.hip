@@ -46,15 +46,18 @@ __device__ __forceinline__ void for_tiles(const EnsTable& T, int B, F&& f) {
 __global__ __launch_bounds__(NT) void ke_vorticity(const EnsMember* __restrict__ M, EnsTable T, int B) {
     for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
         const EnsMember& m = M[s];
-        st_vorticity<float>(m.u, m.v, m.w, m.vel4, m.d, m.two_h, i, j, k);
+        // four planes of the member's N values in its vel4 slot (the advection fills it again)
+        st_vorticity_to<float>(m.u, m.v, m.w, VortPlanesOut<float>{(float*)m.vel4, m.d.W * m.d.H * m.d.D}, m.d,
+                               m.two_h, i, j, k);
     });
 }
 
 __global__ __launch_bounds__(NT) void ke_force_cells(const EnsMember* __restrict__ M, EnsTable T, int B) {
     for_tiles<64, 4>(T, B, [&](int s, int, int i, int j, int k) {
         const EnsMember& m = M[s];
-        st_force_cell<float, true>(m.vel4, m.T, m.R, (const float*)nullptr, m.scratch, m.d, m.ch, m.beta, m.alpha,
-                                   m.Tamb, m.aT, m.aR, i, j, k);
+        st_force_cell_from<float, true>(VortPlanesIn<float>{(const float*)m.vel4, m.d.W * m.d.H * m.d.D}, m.T, m.R,
+                                        (const float*)nullptr, m.scratch, m.d, m.ch, m.beta, m.alpha, m.Tamb, m.aT,
+                                        m.aR, i, j, k);
     });
 }
 
